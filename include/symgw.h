@@ -1,0 +1,222 @@
+/*
+ * symgw.h — C ABI of the MI355X symmetry-adapted GW hot path.
+ *
+ * Every entry point returns an int status (SYMGW_OK == 0) and never throws; the
+ * message of the last failure on a context is returned by symgw_last_error().
+ * All complex arrays are interleaved complex128 (re, im) given as double*.
+ * Layout names follow SymmetryMBPT:
+ *   ts_kij  : [t][s][flat_ao(iBZ k)]           host G / Sigma (sc_loop_block_t)
+ *   k_ts_ij : [k][t][s][flat_ao(k)]            device G / Sigma
+ *   q_t_QP  : [q_ir][t][flat_aux(q_ir)]        P0 / P (row-major [Q][P] per block)
+ * A "flat" slice of one k is the concatenation of its row-major b x b irrep blocks.
+ *
+ * Reference interfaces replaced (paths relative to the CQMP/MBSymmetry tree):
+ *   cugw_routine_block<prec>   SymmetryMBPT/include/cuda/cugw_routines_block.h:22-114
+ *   cugw_solver_block_t        SymmetryMBPT/src/solver/cugw_solver_block.cpp:34-469
+ *   dyson_sc_block_t           SymmetryMBPT/src/scmbpt/sc_type_block.cpp:6-87
+ */
+#ifndef SYMGW_H
+#define SYMGW_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SYMGW_OK 0
+#define SYMGW_ERR_ARG 1        /* bad argument / layout inconsistency      */
+#define SYMGW_ERR_HIP 2        /* HIP runtime failure                      */
+#define SYMGW_ERR_NOMEM 3      /* device or host allocation failed          */
+#define SYMGW_ERR_STATE 4      /* call out of order (e.g. no G uploaded)   */
+#define SYMGW_ERR_NODEVICE 5   /* no usable gfx950 device                   */
+
+/* GEMM precision of the P0 and Sigma contractions.  Transforms, W, Dyson and
+ * the mu search always run in FP64. */
+#define SYMGW_PREC_FP64 0      /* plain: complex FP64                          */
+#define SYMGW_PREC_FP32 1      /* complex FP32 operands, FP32 accumulation   */
+
+/*
+ * Block-sparse layout of one system.  Mirrors the tables that
+ * symmetry_utils_t reads from symmetry_info.h5
+ * (SymmetryMBPT/src/common/symmetry_utils.cpp:6-218).  Offsets are derived
+ * inside the library (cumulative b*b sums), as the reference stores them.
+ *
+ * Orbital (ao) and auxiliary (aux) irrep blocks are given for every k of the
+ * full Brillouin zone.  The irreducible set and its maps follow
+ * symmetry_utils.h:171-187: irre_list[ik] is the full-BZ index of the
+ * ik-th stored point, index[k] the full-BZ index of the star representative
+ * of k, irre_index[k] the iBZ position of that representative and
+ * conj_index[k] the time-reversal partner (k itself when none).
+ * kpq[k1*nk + q] is the full-BZ index of k1 + q and kmq[k1*nk + q] that of
+ * k1 - q (momentum_conservation, symmetry_utils.cpp:315-341).
+ *
+ * Rotation blocks (orep / auxrep) are given per full-BZ k as a list of
+ * (ia1, ia2) pairs: block ia1 of k is  O * X(block ia2 of the iBZ point) * O^H
+ * with O a sizes_k(ia1) x sizes_ik(ia2) row-major matrix
+ * (cugw_routines_block.cu:147-179, cugw_qkpt_block.cu:302-332).
+ *
+ * Interaction tensor V: for each ordered k-pair p = k1*nk + k2 that is
+ * stored, kpair_nblocks[p] blocks (iq, ik1, ik2) with q x k1 x k2 row-major
+ * elements V[Q][a][b] (a in k1 block ik1, b in k2 block ik2).  The aux block
+ * index iq refers to the aux blocks of the momentum transfer of the pair.
+ * Pairs with kpair_nblocks[p] == 0 are not stored.
+ */
+typedef struct symgw_layout_desc {
+  int32_t nk;          /* k points in the full Brillouin zone            */
+  int32_t ink;         /* stored (irreducible) k points                  */
+  int32_t ns;          /* spins, 1 or 2                                  */
+  int32_t nts;         /* imaginary-time points incl. 0 and beta; even  */
+  int32_t nw;          /* fermionic Matsubara samples                    */
+  int32_t nw_b;        /* bosonic Matsubara samples                      */
+  const int32_t *kao_nblocks;    /* [nk]                                 */
+  const int32_t *ao_sizes;       /* [sum kao_nblocks]                    */
+  const int32_t *qaux_nblocks;   /* [nk]                                 */
+  const int32_t *aux_sizes;      /* [sum qaux_nblocks]                   */
+  const int32_t *irre_list;      /* [ink]                                */
+  const int32_t *index;          /* [nk]                                 */
+  const int32_t *irre_index;     /* [nk]                                 */
+  const int32_t *conj_index;     /* [nk]                                 */
+  const int32_t *kpq;            /* [nk*nk]                              */
+  const int32_t *kmq;            /* [nk*nk]                              */
+  const int32_t *orep_nblocks;   /* [nk]                                 */
+  const int32_t *orep_irreps;    /* [sum orep_nblocks][2]                */
+  const int32_t *auxrep_nblocks; /* [nk]                                 */
+  const int32_t *auxrep_irreps;  /* [sum auxrep_nblocks][2]              */
+  const int32_t *kpair_nblocks;  /* [nk*nk]                              */
+  const int32_t *V_irreps;       /* [sum kpair_nblocks][3]               */
+} symgw_layout_desc;
+
+typedef struct symgw_ctx symgw_ctx;
+
+/* Derived sizes of a layout (no device needed). */
+typedef struct symgw_layout_info {
+  int64_t flat_ao_size;        /* iBZ ts_kij slice length per (t,s)     */
+  int64_t flat_ao_size_full;   /* full-BZ flat ao length                */
+  int64_t flat_aux_size;       /* iBZ flat aux length (per t)           */
+  int64_t flat_aux_size_full;  /* full-BZ flat aux length (per t)       */
+  int64_t flat_V_size;         /* complex elements of all stored pairs  */
+  int64_t flat_orep_size;      /* complex elements of all orep blocks   */
+  int64_t flat_auxrep_size;    /* complex elements of all auxrep blocks */
+  int64_t n_P0_blocks;         /* V-block contractions of one P0 sweep  */
+  int64_t n_Sigma_blocks;      /* V-block contractions of one Sigma sweep */
+  double flop_P0;              /* 8*M*N*K FLOPs of one P0 sweep (t < nts/2, all spins) */
+  double flop_Sigma;           /* 8*M*N*K FLOPs of one Sigma sweep      */
+  int64_t device_bytes;        /* device footprint of a context          */
+} symgw_layout_info;
+
+/* GW_complexity_estimation_block (analysis.cpp:212-298) with the 8MNK
+ * convention, plus the footprint model used for admission. */
+int symgw_layout_query(const symgw_layout_desc *desc, symgw_layout_info *info);
+
+/* Offset of the flat V slice of ordered pair (k1, k2) in complex elements, or
+ * -1 when the pair is not stored (symmetry_utils.h:114-123). */
+int64_t symgw_kpair_offset(const symgw_ctx *ctx, int k1, int k2);
+
+/* cugw_routine_block ctor + allocate_memory (cugw_routines_block.cu:6-91):
+ * select `device`, allocate every device buffer of the system, upload the
+ * work lists.  precision: SYMGW_PREC_*.  The context owns its own stream. */
+int symgw_create(const symgw_layout_desc *desc, int device, int precision, symgw_ctx **out);
+int symgw_destroy(symgw_ctx *ctx);
+const char *symgw_last_error(const symgw_ctx *ctx);
+/* Message of the last symgw_create failure (no context exists then). */
+const char *symgw_create_error(void);
+int symgw_layout_info_ctx(const symgw_ctx *ctx, symgw_layout_info *info);
+
+/* V: the whole flat interaction tensor (df_integral_block_t::read_entire,
+ * df_integral_block_t.h:38-48), complex128, uploaded once and kept resident. */
+int symgw_set_V(symgw_ctx *ctx, const double *V_flat, int64_t n_complex);
+/* V generated on the device from a counter-based generator (no host copy):
+ * V[Q][a][b] = (g1 + i g2)/sqrt(2) * scale * exp(-|a_glob - b_glob|/4),
+ * g1, g2 unit-variance Irwin-Hall(4) deviates of Philox4x32-10(seed, element).
+ * decay[d] = exp(-d/4) for d < n_decay (host-computed so host == device). */
+int symgw_generate_V(symgw_ctx *ctx, uint64_t seed, double scale,
+                     const double *decay, int n_decay);
+/* Host twin of symgw_generate_V for one stored pair (k1, k2): writes the
+ * pair's flat slice (complex128) into out (kpair slice size elements). */
+int symgw_generate_V_pair_host(const symgw_layout_desc *desc, uint64_t seed, double scale,
+                               const double *decay, int n_decay, int k1, int k2, double *out);
+
+/* copy_ORep_host_to_device / copy_AuxRep_host_to_device
+ * (cugw_routines_block.cu:214-234): flat rotation blocks, complex128. */
+int symgw_set_reps(symgw_ctx *ctx, const double *orep_flat, const double *auxrep_flat);
+
+/* copy_G_host_to_device (cugw_routines_block.cu:100-201): G_ts_kij
+ * (nts, ns, flat_ao iBZ) complex128 -> device G over the full BZ (rotated,
+ * conjugated for time-reversal partners).  The iBZ G also stays resident. */
+int symgw_upload_G(symgw_ctx *ctx, const double *G_ts_kij);
+/* Device iBZ G -> host ts_kij (overwrite). */
+int symgw_download_G(symgw_ctx *ctx, double *G_ts_kij);
+
+/* cugw_routine_block::solve_P0 (cugw_routines_block.cu:276-294): P0 over every
+ * (q_ir, k) pair, t < nts/2, spin prefactor -2/ns applied, NOT divided by nk
+ * (the caller divides, cugw_solver_block.cpp:302-304). Deterministic. */
+int symgw_solve_P0(symgw_ctx *ctx);
+/* copy_P0_device_to_host (cugw_routines_block.cu:236-247): ACCUMULATES (+=)
+ * the device P0/P (q_t_QP, nts*flat_aux complex128) into host memory. */
+int symgw_copy_P0_to_host(const symgw_ctx *ctx, double *P0_q_t_QP);
+/* copy_P_host_to_device (cugw_routines_block.cu:203-212): overwrite device P. */
+int symgw_upload_P(symgw_ctx *ctx, const double *P_q_t_QP);
+
+/* Transform matrices (transformer_t.cpp:75-81), complex128 row-major:
+ * Tnt (nw x (nts-2)), Ttn (nts x nw), Tnt_BF (nw_b x (nts-2)),
+ * Ttn_FB (nts x nw_b); wsample_fermi: the nw values i*omega_n (complex). */
+int symgw_set_transforms(symgw_ctx *ctx, const double *Tnt, const double *Ttn,
+                         const double *Tnt_BF, const double *Ttn_FB,
+                         const double *iwn_fermi);
+
+/* cugw_solver_block_t::compute_Pq_from_P0q (cugw_solver_block.cpp:385-439) on
+ * the device: mirror P0(beta - t) = P0(t), scale by 1/nk, P0(iW) = Tnt_BF P0,
+ * per (q, n, block) Pt = [I - (P0 + P0^H)/2]^-1 (hermitized) * P0, back with
+ * Ttn_FB.  Overwrites the device P0 buffer with P (as the reference aliases). */
+int symgw_solve_W(symgw_ctx *ctx);
+
+/* cugw_routine_block::solve_Sigma (cugw_routines_block.cu:296-318): rotates P
+ * to every q of the full BZ and contracts Sigma for the iBZ k, all t;
+ * prefactor -1, NOT divided by nk (cugw_solver_block.cpp:50-52). */
+int symgw_solve_Sigma(symgw_ctx *ctx);
+/* copy_Sigma_device_to_host (cugw_routines_block.cu:249-274): ACCUMULATES
+ * (+=) into host Sigma_ts_kij (nts, ns, flat_ao iBZ). */
+int symgw_copy_Sigma_to_host(const symgw_ctx *ctx, double *Sigma_ts_kij);
+/* Overwrite the device Sigma (host ts_kij, already divided by nk). */
+int symgw_upload_Sigma(symgw_ctx *ctx, const double *Sigma_ts_kij);
+
+/* F and S (ns, flat_ao iBZ) complex128 (sc_loop_block_t::read_input_data,
+ * sc_loop_block_t.cpp:64-109 after the Bz shift and hermitization). */
+int symgw_set_fock(symgw_ctx *ctx, const double *F, const double *S);
+/* dyson_sc_block_t::compute_G (sc_type_block.cpp:6-87) on the device:
+ * Sigma(iw) = Tnt Sigma(t); per (s, k, w, block)
+ * G = [(iw + mu) S - F - Sigma(iw)]^-1; G(t) = Ttn G(iw).  The device Sigma
+ * used is the one left by symgw_solve_Sigma divided by nk (sigma_scale =
+ * 1/nk), or an uploaded one (sigma_scale = 1).  Result: device iBZ G and the
+ * full-BZ rotated G used by the next symgw_solve_P0. */
+int symgw_dyson(symgw_ctx *ctx, double mu, double sigma_scale);
+/* Electron count N(mu) = -(2/ns) * sum_{k,s} w_k Tr[G_k(beta) S_k] / nk of
+ * the device G (sc_loop_gf_block.cpp:18-20 density matrix, traced). */
+int symgw_density(symgw_ctx *ctx, const double *kweights, double *nel);
+/* Chemical-potential search (absent from the reference, whose mu is fixed):
+ * bisection on mu in [mu_lo, mu_hi] of N(mu) = nel_target through repeated
+ * device Dyson solves; leaves G of the final mu on the device. */
+int symgw_mu_search(symgw_ctx *ctx, double nel_target, const double *kweights,
+                    double sigma_scale, double mu_lo, double mu_hi, double tol,
+                    double *mu_out, double *nel_out);
+
+/* One GW iteration on the device: P0 -> W -> Sigma -> Dyson(mu), ending with
+ * the rotated full-BZ G ready for the next iteration.  use_graph != 0
+ * captures the launch sequence once as a hipGraph and replays it. */
+int symgw_gw_iteration(symgw_ctx *ctx, double mu, int use_graph);
+
+/* Wait for all work queued on the context's stream. */
+int symgw_sync(symgw_ctx *ctx);
+/* Average device time (ms) of the named kernel family over the last
+ * symgw_gw_iteration / solve calls measured with HIP events on the context
+ * stream: which = 0 P0, 1 W, 2 Sigma, 3 Dyson, 4 G rotation. */
+int symgw_last_times(const symgw_ctx *ctx, double *ms5);
+/* Enable/disable per-phase HIP event timing (off by default). */
+int symgw_set_timing(symgw_ctx *ctx, int on);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SYMGW_H */

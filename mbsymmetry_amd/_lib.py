@@ -1,0 +1,248 @@
+"""ctypes binding of libsymgw (include/symgw.h).
+
+The library is loaded from this package directory only; a missing library is
+an error (there is no CPU fallback for the device path).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsymgw.so")
+
+SYMGW_OK = 0
+PREC_FP64 = 0
+PREC_FP32 = 1
+
+_i32p = C.POINTER(C.c_int32)
+_dp = C.POINTER(C.c_double)
+
+
+class LayoutDesc(C.Structure):
+    _fields_ = [
+        ("nk", C.c_int32), ("ink", C.c_int32), ("ns", C.c_int32), ("nts", C.c_int32),
+        ("nw", C.c_int32), ("nw_b", C.c_int32),
+        ("kao_nblocks", _i32p), ("ao_sizes", _i32p), ("qaux_nblocks", _i32p), ("aux_sizes", _i32p),
+        ("irre_list", _i32p), ("index", _i32p), ("irre_index", _i32p), ("conj_index", _i32p),
+        ("kpq", _i32p), ("kmq", _i32p),
+        ("orep_nblocks", _i32p), ("orep_irreps", _i32p), ("auxrep_nblocks", _i32p), ("auxrep_irreps", _i32p),
+        ("kpair_nblocks", _i32p), ("V_irreps", _i32p),
+    ]
+
+
+class LayoutInfo(C.Structure):
+    _fields_ = [
+        ("flat_ao_size", C.c_int64), ("flat_ao_size_full", C.c_int64), ("flat_aux_size", C.c_int64),
+        ("flat_aux_size_full", C.c_int64), ("flat_V_size", C.c_int64), ("flat_orep_size", C.c_int64),
+        ("flat_auxrep_size", C.c_int64), ("n_P0_blocks", C.c_int64), ("n_Sigma_blocks", C.c_int64),
+        ("flop_P0", C.c_double), ("flop_Sigma", C.c_double), ("device_bytes", C.c_int64),
+    ]
+
+
+# every entry point declared in include/symgw.h: name -> (restype, argtypes)
+SIGNATURES = {
+    "symgw_layout_query": (C.c_int, [C.POINTER(LayoutDesc), C.POINTER(LayoutInfo)]),
+    "symgw_kpair_offset": (C.c_int64, [C.c_void_p, C.c_int, C.c_int]),
+    "symgw_create": (C.c_int, [C.POINTER(LayoutDesc), C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
+    "symgw_destroy": (C.c_int, [C.c_void_p]),
+    "symgw_last_error": (C.c_char_p, [C.c_void_p]),
+    "symgw_create_error": (C.c_char_p, []),
+    "symgw_layout_info_ctx": (C.c_int, [C.c_void_p, C.POINTER(LayoutInfo)]),
+    "symgw_set_V": (C.c_int, [C.c_void_p, _dp, C.c_int64]),
+    "symgw_generate_V": (C.c_int, [C.c_void_p, C.c_uint64, C.c_double, _dp, C.c_int]),
+    "symgw_generate_V_pair_host": (C.c_int, [C.POINTER(LayoutDesc), C.c_uint64, C.c_double, _dp, C.c_int,
+                                             C.c_int, C.c_int, _dp]),
+    "symgw_set_reps": (C.c_int, [C.c_void_p, _dp, _dp]),
+    "symgw_upload_G": (C.c_int, [C.c_void_p, _dp]),
+    "symgw_download_G": (C.c_int, [C.c_void_p, _dp]),
+    "symgw_solve_P0": (C.c_int, [C.c_void_p]),
+    "symgw_copy_P0_to_host": (C.c_int, [C.c_void_p, _dp]),
+    "symgw_upload_P": (C.c_int, [C.c_void_p, _dp]),
+    "symgw_set_transforms": (C.c_int, [C.c_void_p, _dp, _dp, _dp, _dp, _dp]),
+    "symgw_solve_W": (C.c_int, [C.c_void_p]),
+    "symgw_solve_Sigma": (C.c_int, [C.c_void_p]),
+    "symgw_copy_Sigma_to_host": (C.c_int, [C.c_void_p, _dp]),
+    "symgw_upload_Sigma": (C.c_int, [C.c_void_p, _dp]),
+    "symgw_set_fock": (C.c_int, [C.c_void_p, _dp, _dp]),
+    "symgw_dyson": (C.c_int, [C.c_void_p, C.c_double, C.c_double]),
+    "symgw_density": (C.c_int, [C.c_void_p, _dp, _dp]),
+    "symgw_mu_search": (C.c_int, [C.c_void_p, C.c_double, _dp, C.c_double, C.c_double, C.c_double, C.c_double,
+                                  _dp, _dp]),
+    "symgw_gw_iteration": (C.c_int, [C.c_void_p, C.c_double, C.c_int]),
+    "symgw_sync": (C.c_int, [C.c_void_p]),
+    "symgw_last_times": (C.c_int, [C.c_void_p, _dp]),
+    "symgw_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
+}
+
+_lib = None
+
+
+def load():
+    """Load libsymgw.so from the package directory (raises if it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built: run `python -m mbsymmetry_amd.build`")
+        lib = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def dptr(a: np.ndarray):
+    assert a.dtype in (np.float64, np.complex128) and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_dp)
+
+
+class SymGWError(RuntimeError):
+    pass
+
+
+class Context:
+    """One device-resident system (symgw_ctx)."""
+
+    def __init__(self, system, device: int = 0, precision: int = PREC_FP64):
+        self.lib = load()
+        self.system = system
+        self.desc = system.desc()
+        h = C.c_void_p()
+        rc = self.lib.symgw_create(C.byref(self.desc), device, precision, C.byref(h))
+        if rc != SYMGW_OK:
+            raise SymGWError(f"symgw_create failed ({rc}): {self.lib.symgw_create_error().decode()}")
+        self.h = h
+        self.precision = precision
+
+    def _chk(self, rc, what):
+        if rc != SYMGW_OK:
+            raise SymGWError(f"{what} failed ({rc}): {self.lib.symgw_last_error(self.h).decode()}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.symgw_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self) -> LayoutInfo:
+        inf = LayoutInfo()
+        self._chk(self.lib.symgw_layout_info_ctx(self.h, C.byref(inf)), "layout_info")
+        return inf
+
+    def set_V(self, V: np.ndarray):
+        V = np.ascontiguousarray(V, dtype=np.complex128)
+        self._chk(self.lib.symgw_set_V(self.h, dptr(V), V.size), "set_V")
+
+    def generate_V(self, seed: int, scale: float, decay: np.ndarray):
+        decay = np.ascontiguousarray(decay, dtype=np.float64)
+        self._chk(self.lib.symgw_generate_V(self.h, seed, scale, dptr(decay), decay.size), "generate_V")
+
+    def set_reps(self, orep: np.ndarray, auxrep: np.ndarray):
+        o = np.ascontiguousarray(orep, dtype=np.complex128)
+        a = np.ascontiguousarray(auxrep, dtype=np.complex128)
+        self._chk(self.lib.symgw_set_reps(self.h, dptr(o) if o.size else None, dptr(a) if a.size else None),
+                  "set_reps")
+
+    def upload_G(self, G):
+        G = np.ascontiguousarray(G, dtype=np.complex128)
+        self._chk(self.lib.symgw_upload_G(self.h, dptr(G)), "upload_G")
+
+    def download_G(self, shape):
+        out = np.zeros(shape, dtype=np.complex128)
+        self._chk(self.lib.symgw_download_G(self.h, dptr(out)), "download_G")
+        return out
+
+    def solve_P0(self):
+        self._chk(self.lib.symgw_solve_P0(self.h), "solve_P0")
+
+    def copy_P0_to_host(self, out: np.ndarray):
+        assert out.dtype == np.complex128 and out.flags["C_CONTIGUOUS"]
+        self._chk(self.lib.symgw_copy_P0_to_host(self.h, dptr(out)), "copy_P0_to_host")
+        return out
+
+    def upload_P(self, P):
+        P = np.ascontiguousarray(P, dtype=np.complex128)
+        self._chk(self.lib.symgw_upload_P(self.h, dptr(P)), "upload_P")
+
+    def set_transforms(self, tr):
+        arrs = [np.ascontiguousarray(a, dtype=np.complex128) for a in (tr.Tnt, tr.Ttn, tr.Tnt_BF, tr.Ttn_FB, tr.iwn)]
+        self._keep = arrs
+        self._chk(self.lib.symgw_set_transforms(self.h, *[dptr(a) for a in arrs]), "set_transforms")
+
+    def solve_W(self):
+        self._chk(self.lib.symgw_solve_W(self.h), "solve_W")
+
+    def solve_Sigma(self):
+        self._chk(self.lib.symgw_solve_Sigma(self.h), "solve_Sigma")
+
+    def copy_Sigma_to_host(self, out: np.ndarray):
+        assert out.dtype == np.complex128 and out.flags["C_CONTIGUOUS"]
+        self._chk(self.lib.symgw_copy_Sigma_to_host(self.h, dptr(out)), "copy_Sigma_to_host")
+        return out
+
+    def upload_Sigma(self, S):
+        S = np.ascontiguousarray(S, dtype=np.complex128)
+        self._chk(self.lib.symgw_upload_Sigma(self.h, dptr(S)), "upload_Sigma")
+
+    def set_fock(self, F, S):
+        F = np.ascontiguousarray(F, dtype=np.complex128)
+        S = np.ascontiguousarray(S, dtype=np.complex128)
+        self._chk(self.lib.symgw_set_fock(self.h, dptr(F), dptr(S)), "set_fock")
+
+    def dyson(self, mu: float, sigma_scale: float):
+        self._chk(self.lib.symgw_dyson(self.h, mu, sigma_scale), "dyson")
+
+    def density(self, kweights):
+        kw = np.ascontiguousarray(kweights, dtype=np.float64)
+        v = C.c_double()
+        self._chk(self.lib.symgw_density(self.h, dptr(kw), C.byref(v)), "density")
+        return v.value
+
+    def mu_search(self, nel, kweights, sigma_scale, lo, hi, tol):
+        kw = np.ascontiguousarray(kweights, dtype=np.float64)
+        mu, n = C.c_double(), C.c_double()
+        self._chk(self.lib.symgw_mu_search(self.h, nel, dptr(kw), sigma_scale, lo, hi, tol, C.byref(mu), C.byref(n)),
+                  "mu_search")
+        return mu.value, n.value
+
+    def gw_iteration(self, mu: float, use_graph: bool = True):
+        self._chk(self.lib.symgw_gw_iteration(self.h, mu, 1 if use_graph else 0), "gw_iteration")
+
+    def sync(self):
+        self._chk(self.lib.symgw_sync(self.h), "sync")
+
+    def set_timing(self, on: bool):
+        self._chk(self.lib.symgw_set_timing(self.h, 1 if on else 0), "set_timing")
+
+    def last_times(self):
+        out = np.zeros(5)
+        self._chk(self.lib.symgw_last_times(self.h, dptr(out)), "last_times")
+        return out
+
+
+def layout_query(system) -> LayoutInfo:
+    lib = load()
+    d = system.desc()
+    inf = LayoutInfo()
+    rc = lib.symgw_layout_query(C.byref(d), C.byref(inf))
+    if rc != SYMGW_OK:
+        raise SymGWError(f"symgw_layout_query failed: {lib.symgw_create_error().decode()}")
+    return inf
+
+
+def generate_V_pair_host(system, seed, scale, decay, k1, k2, n_complex):
+    lib = load()
+    d = system.desc()
+    out = np.zeros(n_complex, dtype=np.complex128)
+    decay = np.ascontiguousarray(decay, dtype=np.float64)
+    rc = lib.symgw_generate_V_pair_host(C.byref(d), seed, scale, dptr(decay), decay.size, k1, k2, dptr(out))
+    if rc != SYMGW_OK:
+        raise SymGWError("symgw_generate_V_pair_host failed")
+    return out

@@ -1,0 +1,875 @@
+// extern "C" entry points of libsymgw (include/symgw.h).
+//
+// A context is the device residency of one system: it plays the role of
+// cugw_routine_block<prec> (SymmetryMBPT/src/cuda/cugw_routines_block.cu:6-91)
+// but lives across GW iterations, owns one HIP stream and keeps G, P, Sigma
+// and V resident.  The host-side W and Dyson solves of the reference
+// (cugw_solver_block.cpp:385-439, sc_type_block.cpp:6-87) run on the device.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/symgw.h"
+#include "common.h"
+#include "kernels.h"
+#include "layout.h"
+
+using namespace symgw;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+};
+
+}  // namespace
+
+struct symgw_ctx {
+  Layout L;
+  WorkLists W;
+  int device = 0;
+  int prec = SYMGW_PREC_FP64;
+  int eb = 16;  // bytes per complex element in the contraction precision
+  hipStream_t st = nullptr;
+  std::string err;
+  std::vector<DevBuf> owned;
+
+  // resident data
+  void *V = nullptr;                   // flat V, contraction precision
+  cplx<double> *G_ibz = nullptr;       // ts_kij, FP64
+  void *G_full = nullptr;              // k_ts_ij full BZ, contraction precision
+  cplx<double> *P = nullptr;           // q_t_QP iBZ, FP64 (P0, then P)
+  void *P_full = nullptr;              // q_t_QP full BZ, contraction precision
+  cplx<double> *Sig = nullptr;         // k_ts_ij iBZ, FP64
+  cplx<double> *Pw = nullptr;          // [q_ir][n][qslice]
+  cplx<double> *SGw = nullptr;         // [k_ir][w][s][kslice] (Sigma(iw), then G(iw))
+  cplx<double> *F = nullptr, *S = nullptr;
+  cplx<double> *orep = nullptr, *auxrep = nullptr;
+  cplx<double> *Tnt = nullptr, *Ttn = nullptr, *Tnt_BF = nullptr, *Ttn_FB = nullptr, *iwn = nullptr;
+  cplx<double> *w_scratch = nullptr, *d_scratch = nullptr;
+  double *kw = nullptr, *nel_dev = nullptr;
+
+  // device tables
+  P0Item *d_p0_items = nullptr;
+  P0Blk *d_p0_blks = nullptr;
+  SigItem *d_sig_items = nullptr;
+  SigBlk *d_sig_blks = nullptr;
+  RotBlk *d_g_rot = nullptr, *d_p_rot = nullptr;
+  SolveBlk *d_w_blocks = nullptr, *d_d_blocks = nullptr;
+  int64_t *d_g_src_stride = nullptr, *d_g_src_slice = nullptr, *d_g_dst_slice = nullptr, *d_g_dst_size = nullptr;
+  int64_t *d_p_src_stride = nullptr, *d_p_src_slice = nullptr, *d_p_dst_slice = nullptr, *d_p_dst_size = nullptr;
+  int64_t *d_w_off = nullptr, *d_qslice_irre = nullptr, *d_qoff_irre = nullptr;
+  int64_t *d_d_off = nullptr, *d_kslice_irre = nullptr, *d_kao_off_irre = nullptr;
+  TBatch *d_tb_wf = nullptr, *d_tb_wb = nullptr, *d_tb_df = nullptr, *d_tb_db = nullptr;
+  int max_qslice = 0, max_kslice = 0;
+
+  bool have_V = false, have_G = false, have_reps = false, have_T = false, have_F = false, have_P = false,
+       have_Sig = false;
+  bool timing = false;
+  double last_ms[5] = {0, 0, 0, 0, 0};
+  hipEvent_t ev[6] = {};
+  hipGraphExec_t graph_exec = nullptr;
+  double graph_mu = 0.0;
+};
+
+namespace {
+
+int fail(symgw_ctx *c, int code, const std::string &msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+#define HIPCHK(ctx, call)                                                              \
+  do {                                                                                 \
+    hipError_t e_ = (call);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return fail(ctx, SYMGW_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+template <typename X>
+int dalloc(symgw_ctx *c, X **p, size_t count) {
+  *p = nullptr;
+  if (count == 0) count = 1;
+  void *q = nullptr;
+  hipError_t e = hipMalloc(&q, count * sizeof(X));
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(c, SYMGW_ERR_NOMEM, "hipMalloc of " + std::to_string(count * sizeof(X)) + " bytes failed");
+  }
+  c->owned.push_back(DevBuf{q, count * sizeof(X)});
+  *p = static_cast<X *>(q);
+  return SYMGW_OK;
+}
+int dalloc_bytes(symgw_ctx *c, void **p, size_t bytes) {
+  char *q = nullptr;
+  int r = dalloc(c, &q, bytes);
+  *p = q;
+  return r;
+}
+
+template <typename X>
+int upload(symgw_ctx *c, X **dst, const std::vector<X> &v) {
+  int r = dalloc(c, dst, v.size());
+  if (r) return r;
+  if (!v.empty()) HIPCHK(c, hipMemcpy(*dst, v.data(), v.size() * sizeof(X), hipMemcpyHostToDevice));
+  return SYMGW_OK;
+}
+
+int64_t footprint(const Layout &L, const WorkLists &W, int eb) {
+  int64_t b = 0;
+  b += L.flat_V * eb;
+  b += (int64_t)L.nts * L.ns * L.flat_ao * 16 * 2;       // G_ibz, Sig
+  b += (int64_t)L.nts * L.ns * L.flat_ao_full * eb;      // G_full
+  b += (int64_t)L.nts * L.flat_aux * 16;                 // P
+  b += (int64_t)L.nts * L.flat_aux_full * eb;            // P_full
+  if (eb != 16) b += (int64_t)L.nts * (L.flat_aux_full + L.ns * L.flat_ao_full) * 16;
+  b += (int64_t)L.nw_b * L.flat_aux * 16 + (int64_t)L.nw * L.ns * L.flat_ao * 16;
+  b += (int64_t)L.nw_b * W.w_scr_group * 16 + (int64_t)L.nw * L.ns * W.d_scr_group * 16;
+  b += (int64_t)(W.p0_blks.size() * sizeof(P0Blk) + W.sig_blks.size() * sizeof(SigBlk));
+  return b;
+}
+
+template <typename T>
+int run_rotate_G(symgw_ctx *c) {
+  const Layout &L = c->L;
+  hipError_t e = launch_rotate<T>(c->d_g_rot, (int)c->W.g_rot.size(), L.nts * L.ns, c->G_ibz, c->d_g_src_stride,
+                                  c->d_g_src_slice, c->orep, static_cast<cplx<T> *>(c->G_full), c->d_g_dst_slice,
+                                  c->d_g_dst_size, c->st);
+  if (e != hipSuccess) return fail(c, SYMGW_ERR_HIP, std::string("rotate G: ") + hipGetErrorString(e));
+  return SYMGW_OK;
+}
+
+int rotate_G(symgw_ctx *c) {
+  if (!c->have_reps && c->L.flat_orep > 0) return fail(c, SYMGW_ERR_STATE, "rotation matrices not set");
+  return c->prec == SYMGW_PREC_FP64 ? run_rotate_G<double>(c) : run_rotate_G<float>(c);
+}
+
+template <typename T>
+int run_rotate_P(symgw_ctx *c) {
+  const Layout &L = c->L;
+  hipError_t e = launch_rotate<T>(c->d_p_rot, (int)c->W.p_rot.size(), L.nts, c->P, c->d_p_src_stride,
+                                  c->d_p_src_slice, c->auxrep, static_cast<cplx<T> *>(c->P_full), c->d_p_dst_slice,
+                                  c->d_p_dst_size, c->st);
+  if (e != hipSuccess) return fail(c, SYMGW_ERR_HIP, std::string("rotate P: ") + hipGetErrorString(e));
+  return SYMGW_OK;
+}
+
+template <typename T>
+int run_p0(symgw_ctx *c) {
+  const Layout &L = c->L;
+  const double pref = L.ns == 1 ? -2.0 : -1.0;  // cugw_qkpt_block.cu:210
+  hipError_t e = launch_p0<T>(c->d_p0_items, c->d_p0_blks, (int)c->W.p0_items.size(), L.nts, L.ns,
+                              static_cast<const cplx<T> *>(c->V), static_cast<const cplx<T> *>(c->G_full), c->P, pref,
+                              c->W.p0_lds, c->st);
+  if (e != hipSuccess) return fail(c, SYMGW_ERR_HIP, std::string("P0 kernel: ") + hipGetErrorString(e));
+  return SYMGW_OK;
+}
+
+template <typename T>
+int run_sigma(symgw_ctx *c) {
+  const Layout &L = c->L;
+  hipError_t e = launch_sigma<T>(c->d_sig_items, c->d_sig_blks, (int)c->W.sig_items.size(), L.nts, L.ns,
+                                 static_cast<const cplx<T> *>(c->V), static_cast<const cplx<T> *>(c->G_full),
+                                 static_cast<const cplx<T> *>(c->P_full), c->Sig, -1.0, c->W.sig_lds, c->st);
+  if (e != hipSuccess) return fail(c, SYMGW_ERR_HIP, std::string("Sigma kernel: ") + hipGetErrorString(e));
+  return SYMGW_OK;
+}
+
+// Zero rows t >= nts/2 of every q slice of P (the reference's device P0 is
+// zero there: cugw_routines_block.cu:79 and only t < nts/2 is written).
+__global__ void zero_upper_kernel(cplx<double> *P, const int64_t *qoff, const int64_t *qsize, int nts) {
+  const int iq = blockIdx.y;
+  const int64_t sz = qsize[iq];
+  const int64_t base = (int64_t)nts * qoff[iq] + (int64_t)(nts / 2) * sz;
+  const int64_t n = (int64_t)(nts - nts / 2) * sz;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    P[base + i] = cplx<double>{0.0, 0.0};
+}
+
+int phase_P0(symgw_ctx *c) {
+  if (!c->have_V) return fail(c, SYMGW_ERR_STATE, "V not set");
+  if (!c->have_G) return fail(c, SYMGW_ERR_STATE, "G not uploaded");
+  const Layout &L = c->L;
+  int r = c->prec == SYMGW_PREC_FP64 ? run_p0<double>(c) : run_p0<float>(c);
+  if (r) return r;
+  hipLaunchKernelGGL(zero_upper_kernel, dim3(64, L.ink), dim3(256), 0, c->st, c->P, c->d_qoff_irre,
+                     c->d_qslice_irre, L.nts);
+  HIPCHK(c, hipGetLastError());
+  return SYMGW_OK;
+}
+
+int phase_W(symgw_ctx *c) {
+  if (!c->have_T) return fail(c, SYMGW_ERR_STATE, "transforms not set");
+  const Layout &L = c->L;
+  const int KT = L.nts - 2;
+  // P0(iW_n) = Tnt_BF * P0(t interior), mirrored, / nk (cugw_solver_block.cpp:395-400)
+  HIPCHK(c, launch_tgemm(c->d_tb_wf, L.ink, c->max_qslice, L.nw_b, KT, 1, c->Tnt_BF, c->P, c->Pw, 1, 1, L.nts,
+                         1.0 / L.nk, 0, 0, c->st));
+  HIPCHK(c, launch_wsolve(c->d_w_blocks, (int)c->W.w_blocks.size(), L.nw_b, c->Pw, c->d_w_off, c->d_qslice_irre,
+                          c->w_scratch, c->W.w_scr_group, c->st));
+  // P(t) = Ttn_FB * Pt(iW_n) (cugw_solver_block.cpp:428)
+  HIPCHK(c, launch_tgemm(c->d_tb_wb, L.ink, c->max_qslice, L.nts, L.nw_b, 1, c->Ttn_FB, c->Pw, c->P, 0, 0, L.nts,
+                         1.0, 0, 0, c->st));
+  c->have_P = true;
+  return SYMGW_OK;
+}
+
+int phase_Sigma(symgw_ctx *c) {
+  if (!c->have_V || !c->have_G) return fail(c, SYMGW_ERR_STATE, "V or G missing");
+  if (!c->have_P) return fail(c, SYMGW_ERR_STATE, "P not available");
+  if (!c->have_reps && c->L.flat_auxrep > 0) return fail(c, SYMGW_ERR_STATE, "rotation matrices not set");
+  int r = c->prec == SYMGW_PREC_FP64 ? run_rotate_P<double>(c) : run_rotate_P<float>(c);
+  if (r) return r;
+  r = c->prec == SYMGW_PREC_FP64 ? run_sigma<double>(c) : run_sigma<float>(c);
+  if (r) return r;
+  c->have_Sig = true;
+  return SYMGW_OK;
+}
+
+int phase_Dyson(symgw_ctx *c, double mu, double sigma_scale) {
+  if (!c->have_T || !c->have_F) return fail(c, SYMGW_ERR_STATE, "transforms or Fock not set");
+  const Layout &L = c->L;
+  const int KT = L.nts - 2;
+  if (c->have_Sig) {
+    // Sigma(iw) = Tnt * Sigma(t interior) (sc_type_block.cpp:150)
+    HIPCHK(c, launch_tgemm(c->d_tb_df, L.ink, c->max_kslice * L.ns, L.nw, KT, 1, c->Tnt, c->Sig, c->SGw, 0, 1,
+                           L.nts, 1.0, 0, 0, c->st));
+  } else {
+    sigma_scale = 0.0;  // Sigma = 0: Dyson of the bare Fock matrix (sc_loop init)
+  }
+  HIPCHK(c, launch_dyson(c->d_d_blocks, (int)c->W.d_blocks.size(), L.nw, L.ns, c->SGw, c->SGw, c->d_d_off,
+                         c->d_kslice_irre, c->d_kao_off_irre, c->F, c->S, L.flat_ao, c->iwn, mu, sigma_scale,
+                         c->d_scratch, c->W.d_scr_group, c->st));
+  // G(t) = Ttn * G(iw) -> ts_kij (sc_type_block.cpp:171)
+  HIPCHK(c, launch_tgemm(c->d_tb_db, L.ink, c->max_kslice * L.ns, L.nts, L.nw, L.ns, c->Ttn, c->SGw, c->G_ibz, 0, 0,
+                         L.nts, 1.0, 0, 0, c->st));
+  c->have_G = true;
+  return rotate_G(c);
+}
+
+void ev_record(symgw_ctx *c, int i) {
+  if (c->timing) (void)hipEventRecord(c->ev[i], c->st);
+}
+
+int iteration_body(symgw_ctx *c, double mu) {
+  int r;
+  ev_record(c, 0);
+  if ((r = phase_P0(c))) return r;
+  ev_record(c, 1);
+  if ((r = phase_W(c))) return r;
+  ev_record(c, 2);
+  if ((r = phase_Sigma(c))) return r;
+  ev_record(c, 3);
+  if ((r = phase_Dyson(c, mu, 1.0 / c->L.nk))) return r;
+  ev_record(c, 4);
+  return SYMGW_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *symgw_create_error(void) { return g_create_error.c_str(); }
+
+const char *symgw_last_error(const symgw_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int symgw_layout_query(const symgw_layout_desc *desc, symgw_layout_info *info) {
+  if (!info) return SYMGW_ERR_ARG;
+  Layout L;
+  std::string err;
+  if (!build_layout(desc, L, err)) {
+    g_create_error = err;
+    return SYMGW_ERR_ARG;
+  }
+  WorkLists W;
+  if (!build_worklists(L, 16, W, err)) {
+    g_create_error = err;
+    return SYMGW_ERR_ARG;
+  }
+  info->flat_ao_size = L.flat_ao;
+  info->flat_ao_size_full = L.flat_ao_full;
+  info->flat_aux_size = L.flat_aux;
+  info->flat_aux_size_full = L.flat_aux_full;
+  info->flat_V_size = L.flat_V;
+  info->flat_orep_size = L.flat_orep;
+  info->flat_auxrep_size = L.flat_auxrep;
+  info->n_P0_blocks = W.n_p0_contractions;
+  info->n_Sigma_blocks = W.n_sig_contractions;
+  info->flop_P0 = W.flop_p0;
+  info->flop_Sigma = W.flop_sig;
+  info->device_bytes = footprint(L, W, 16);
+  return SYMGW_OK;
+}
+
+int symgw_create(const symgw_layout_desc *desc, int device, int precision, symgw_ctx **out) {
+  if (!out) return SYMGW_ERR_ARG;
+  *out = nullptr;
+  if (precision != SYMGW_PREC_FP64 && precision != SYMGW_PREC_FP32) {
+    g_create_error = "unknown precision mode";
+    return SYMGW_ERR_ARG;
+  }
+  std::unique_ptr<symgw_ctx> c(new symgw_ctx());
+  c->prec = precision;
+  c->eb = precision == SYMGW_PREC_FP64 ? 16 : 8;
+  std::string err;
+  if (!build_layout(desc, c->L, err) || !build_worklists(c->L, c->eb, c->W, err)) {
+    g_create_error = err;
+    return SYMGW_ERR_ARG;
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+    (void)hipGetLastError();
+    g_create_error = "no HIP device";
+    return SYMGW_ERR_NODEVICE;
+  }
+  if (device < 0 || device >= ndev) {
+    g_create_error = "device index out of range";
+    return SYMGW_ERR_ARG;
+  }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess || std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+    g_create_error = std::string("device is not gfx950: ") + prop.gcnArchName;
+    return SYMGW_ERR_NODEVICE;
+  }
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
+    g_create_error = "stream creation failed";
+    return SYMGW_ERR_HIP;
+  }
+  for (auto &e : c->ev) (void)hipEventCreate(&e);
+  if (c->W.p0_lds > 160 * 1024 || c->W.sig_lds > 160 * 1024) {
+    g_create_error = "LDS plan exceeds 160 KiB";
+    return SYMGW_ERR_ARG;
+  }
+  // the W / Dyson solvers keep one pivot per row in fixed-size LDS arrays
+  if (c->L.max_ao_block > 512 || c->L.max_aux_block > 512) {
+    g_create_error = "irrep blocks larger than 512 are not supported by the solvers";
+    return SYMGW_ERR_ARG;
+  }
+  const Layout &L = c->L;
+  symgw_ctx *cp = c.get();
+  int r = 0;
+  const size_t eb = c->eb;
+  r = r ? r : dalloc_bytes(cp, &c->V, (size_t)L.flat_V * eb);
+  r = r ? r : dalloc(cp, &c->G_ibz, (size_t)L.nts * L.ns * L.flat_ao);
+  r = r ? r : dalloc_bytes(cp, &c->G_full, (size_t)L.nts * L.ns * L.flat_ao_full * eb);
+  r = r ? r : dalloc(cp, &c->P, (size_t)L.nts * L.flat_aux);
+  r = r ? r : dalloc_bytes(cp, &c->P_full, (size_t)L.nts * L.flat_aux_full * eb);
+  r = r ? r : dalloc(cp, &c->Sig, (size_t)L.nts * L.ns * L.flat_ao);
+  r = r ? r : dalloc(cp, &c->Pw, (size_t)std::max(1, L.nw_b) * L.flat_aux);
+  r = r ? r : dalloc(cp, &c->SGw, (size_t)std::max(1, L.nw) * L.ns * L.flat_ao);
+  r = r ? r : dalloc(cp, &c->F, (size_t)L.ns * L.flat_ao);
+  r = r ? r : dalloc(cp, &c->S, (size_t)L.ns * L.flat_ao);
+  r = r ? r : dalloc(cp, &c->orep, (size_t)L.flat_orep);
+  r = r ? r : dalloc(cp, &c->auxrep, (size_t)L.flat_auxrep);
+  r = r ? r : dalloc(cp, &c->Tnt, (size_t)L.nw * (L.nts - 2));
+  r = r ? r : dalloc(cp, &c->Ttn, (size_t)L.nts * L.nw);
+  r = r ? r : dalloc(cp, &c->Tnt_BF, (size_t)L.nw_b * (L.nts - 2));
+  r = r ? r : dalloc(cp, &c->Ttn_FB, (size_t)L.nts * L.nw_b);
+  r = r ? r : dalloc(cp, &c->iwn, (size_t)L.nw);
+  r = r ? r : dalloc(cp, &c->w_scratch, (size_t)std::max(1, L.nw_b) * c->W.w_scr_group);
+  r = r ? r : dalloc(cp, &c->d_scratch, (size_t)std::max(1, L.nw) * L.ns * c->W.d_scr_group);
+  r = r ? r : dalloc(cp, &c->kw, (size_t)L.ink);
+  r = r ? r : dalloc(cp, &c->nel_dev, 1);
+  // every data buffer starts at zero (outside any graph: plain memsets are fine here)
+  for (auto &b : c->owned)
+    if (!r && hipMemset(b.p, 0, b.bytes) != hipSuccess) r = fail(cp, SYMGW_ERR_HIP, "initial memset failed");
+  // work lists
+  r = r ? r : upload(cp, &c->d_p0_items, c->W.p0_items);
+  r = r ? r : upload(cp, &c->d_p0_blks, c->W.p0_blks);
+  r = r ? r : upload(cp, &c->d_sig_items, c->W.sig_items);
+  r = r ? r : upload(cp, &c->d_sig_blks, c->W.sig_blks);
+  r = r ? r : upload(cp, &c->d_g_rot, c->W.g_rot);
+  r = r ? r : upload(cp, &c->d_p_rot, c->W.p_rot);
+  r = r ? r : upload(cp, &c->d_w_blocks, c->W.w_blocks);
+  r = r ? r : upload(cp, &c->d_d_blocks, c->W.d_blocks);
+  // rotation source/destination slices, per full-BZ target
+  {
+    std::vector<int64_t> gs(L.nk), gsl(L.nk), gd(L.nk), gds(L.nk), ps(L.nk), psl(L.nk), pd(L.nk), pds(L.nk);
+    for (int k = 0; k < L.nk; ++k) {
+      int ick = L.irre_index[L.conj_index[k]];
+      gs[k] = L.ns * L.flat_ao;  // G_ibz is ts_kij: one (t,s) row holds every iBZ k
+      gsl[k] = L.kao_off_irre[ick];
+      gd[k] = L.kao_off[k];
+      gds[k] = L.kao_size[k];
+      ps[k] = L.qaux_size_irre[ick];  // P is q_t_QP
+      psl[k] = (int64_t)L.nts * L.qaux_off_irre[ick];
+      pd[k] = L.qaux_off[k];
+      pds[k] = L.qaux_size[k];
+    }
+    // G_ibz row stride is per (t, s): ts * flat_ao
+    for (int k = 0; k < L.nk; ++k) gs[k] = L.flat_ao;
+    r = r ? r : upload(cp, &c->d_g_src_stride, gs);
+    r = r ? r : upload(cp, &c->d_g_src_slice, gsl);
+    r = r ? r : upload(cp, &c->d_g_dst_slice, gd);
+    r = r ? r : upload(cp, &c->d_g_dst_size, gds);
+    r = r ? r : upload(cp, &c->d_p_src_stride, ps);
+    r = r ? r : upload(cp, &c->d_p_src_slice, psl);
+    r = r ? r : upload(cp, &c->d_p_dst_slice, pd);
+    r = r ? r : upload(cp, &c->d_p_dst_size, pds);
+  }
+  {
+    std::vector<int64_t> woff(L.ink), qsl(L.ink), qof(L.ink), doff(L.ink), ksl(L.ink), kof(L.ink);
+    std::vector<TBatch> wf(L.ink), wb(L.ink), df(L.ink), db(L.ink);
+    for (int i = 0; i < L.ink; ++i) {
+      woff[i] = (int64_t)L.nw_b * L.qaux_off_irre[i];
+      qsl[i] = L.qaux_size_irre[i];
+      qof[i] = L.qaux_off_irre[i];
+      doff[i] = (int64_t)L.nw * L.ns * L.kao_off_irre[i];
+      ksl[i] = L.kao_size_irre[i];
+      kof[i] = L.kao_off_irre[i];
+      c->max_qslice = std::max<int>(c->max_qslice, (int)qsl[i]);
+      c->max_kslice = std::max<int>(c->max_kslice, (int)ksl[i]);
+      // W forward: P (q_t_QP) -> Pw ([q][n][qslice])
+      wf[i] = TBatch{(int64_t)L.nts * L.qaux_off_irre[i], woff[i], qsl[i], qsl[i], 0, 0, (int32_t)qsl[i], 0};
+      // W back: Pw -> P
+      wb[i] = TBatch{woff[i], (int64_t)L.nts * L.qaux_off_irre[i], qsl[i], qsl[i], 0, 0, (int32_t)qsl[i], 0};
+      // Dyson forward: Sig (k_ts_ij) -> SGw ([k][w][s][kslice]); (s, x) contiguous
+      df[i] = TBatch{(int64_t)L.nts * L.ns * L.kao_off_irre[i], doff[i], L.ns * ksl[i], L.ns * ksl[i], 0, 0,
+                     (int32_t)(L.ns * ksl[i]), 0};
+      // Dyson back: SGw -> G_ibz (ts_kij): columns split per spin
+      db[i] = TBatch{doff[i], L.kao_off_irre[i], L.ns * ksl[i], L.ns * L.flat_ao, ksl[i], L.flat_ao,
+                     (int32_t)ksl[i], 0};
+    }
+    r = r ? r : upload(cp, &c->d_w_off, woff);
+    r = r ? r : upload(cp, &c->d_qslice_irre, qsl);
+    r = r ? r : upload(cp, &c->d_qoff_irre, qof);
+    r = r ? r : upload(cp, &c->d_d_off, doff);
+    r = r ? r : upload(cp, &c->d_kslice_irre, ksl);
+    r = r ? r : upload(cp, &c->d_kao_off_irre, kof);
+    r = r ? r : upload(cp, &c->d_tb_wf, wf);
+    r = r ? r : upload(cp, &c->d_tb_wb, wb);
+    r = r ? r : upload(cp, &c->d_tb_df, df);
+    r = r ? r : upload(cp, &c->d_tb_db, db);
+  }
+  if (r) {
+    g_create_error = c->err;
+    for (auto &b : c->owned) (void)hipFree(b.p);
+    for (auto &e : c->ev) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(c->st);
+    return r;
+  }
+  *out = c.release();
+  return SYMGW_OK;
+}
+
+int symgw_destroy(symgw_ctx *ctx) {
+  if (!ctx) return SYMGW_OK;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->st);
+  if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
+  for (auto &b : ctx->owned) (void)hipFree(b.p);
+  for (auto &e : ctx->ev) (void)hipEventDestroy(e);
+  (void)hipStreamDestroy(ctx->st);
+  delete ctx;
+  return SYMGW_OK;
+}
+
+int symgw_layout_info_ctx(const symgw_ctx *ctx, symgw_layout_info *info) {
+  if (!ctx || !info) return SYMGW_ERR_ARG;
+  const Layout &L = ctx->L;
+  info->flat_ao_size = L.flat_ao;
+  info->flat_ao_size_full = L.flat_ao_full;
+  info->flat_aux_size = L.flat_aux;
+  info->flat_aux_size_full = L.flat_aux_full;
+  info->flat_V_size = L.flat_V;
+  info->flat_orep_size = L.flat_orep;
+  info->flat_auxrep_size = L.flat_auxrep;
+  info->n_P0_blocks = ctx->W.n_p0_contractions;
+  info->n_Sigma_blocks = ctx->W.n_sig_contractions;
+  info->flop_P0 = ctx->W.flop_p0;
+  info->flop_Sigma = ctx->W.flop_sig;
+  info->device_bytes = footprint(L, ctx->W, ctx->eb);
+  return SYMGW_OK;
+}
+
+int64_t symgw_kpair_offset(const symgw_ctx *ctx, int k1, int k2) {
+  if (!ctx || k1 < 0 || k2 < 0 || k1 >= ctx->L.nk || k2 >= ctx->L.nk) return -1;
+  size_t p = (size_t)k1 * ctx->L.nk + k2;
+  return ctx->L.V_irreps[p].empty() ? -1 : ctx->L.kpair_off[p];
+}
+
+int symgw_set_V(symgw_ctx *ctx, const double *V_flat, int64_t n_complex) {
+  if (!ctx || !V_flat) return SYMGW_ERR_ARG;
+  if (n_complex != ctx->L.flat_V) return fail(ctx, SYMGW_ERR_ARG, "V length does not match the layout");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  if (ctx->prec == SYMGW_PREC_FP64) {
+    HIPCHK(ctx, hipMemcpy(ctx->V, V_flat, (size_t)n_complex * 16, hipMemcpyHostToDevice));
+  } else {
+    // staged conversion in 64 MiB chunks
+    const int64_t chunk = 4 << 20;
+    cplx<double> *stage = nullptr;
+    HIPCHK(ctx, hipMalloc(&stage, (size_t)chunk * 16));
+    for (int64_t o = 0; o < n_complex; o += chunk) {
+      int64_t n = std::min(chunk, n_complex - o);
+      hipError_t e = hipMemcpy(stage, V_flat + 2 * o, (size_t)n * 16, hipMemcpyHostToDevice);
+      if (e == hipSuccess) e = launch_convert<float>(stage, static_cast<cplx<float> *>(ctx->V) + o, n, ctx->st);
+      if (e == hipSuccess) e = hipStreamSynchronize(ctx->st);
+      if (e != hipSuccess) {
+        (void)hipFree(stage);
+        return fail(ctx, SYMGW_ERR_HIP, std::string("V upload: ") + hipGetErrorString(e));
+      }
+    }
+    (void)hipFree(stage);
+  }
+  ctx->have_V = true;
+  return SYMGW_OK;
+}
+
+static void gen_blocks(const Layout &L, std::vector<GenBlk> &out, int only_pair) {
+  for (size_t p = 0; p < L.V_irreps.size(); ++p) {
+    if (only_pair >= 0 && (int)p != only_pair) continue;
+    int k1 = (int)(p / L.nk), k2 = (int)(p % L.nk);
+    for (size_t b = 0; b < L.V_irreps[p].size(); ++b) {
+      auto &t = L.V_irreps[p][b];
+      GenBlk g;
+      g.v_off = L.kpair_off[p] + L.V_offsets[p][b];
+      g.local_off = L.V_offsets[p][b];
+      g.q = L.V_qsize[p][b];
+      g.k1 = L.ao_sizes[k1][t[1]];
+      g.k2 = L.ao_sizes[k2][t[2]];
+      g.a_base = L.ao_block_offsets[k1][t[1]];
+      g.b_base = L.ao_block_offsets[k2][t[2]];
+      g.pair = (int32_t)p;
+      out.push_back(g);
+    }
+  }
+}
+
+int symgw_generate_V(symgw_ctx *ctx, uint64_t seed, double scale, const double *decay, int n_decay) {
+  if (!ctx || !decay || n_decay <= 0) return SYMGW_ERR_ARG;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  std::vector<GenBlk> gb;
+  gen_blocks(ctx->L, gb, -1);
+  GenBlk *dgb = nullptr;
+  double *ddec = nullptr;
+  HIPCHK(ctx, hipMalloc(&dgb, std::max<size_t>(1, gb.size()) * sizeof(GenBlk)));
+  HIPCHK(ctx, hipMalloc(&ddec, (size_t)n_decay * sizeof(double)));
+  hipError_t e = hipMemcpy(dgb, gb.data(), gb.size() * sizeof(GenBlk), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(ddec, decay, (size_t)n_decay * sizeof(double), hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    e = ctx->prec == SYMGW_PREC_FP64
+            ? launch_genv<double>(dgb, (int)gb.size(), seed, scale, ddec, n_decay, static_cast<cplx<double> *>(ctx->V),
+                                  ctx->st)
+            : launch_genv<float>(dgb, (int)gb.size(), seed, scale, ddec, n_decay, static_cast<cplx<float> *>(ctx->V),
+                                 ctx->st);
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->st);
+  (void)hipFree(dgb);
+  (void)hipFree(ddec);
+  if (e != hipSuccess) return fail(ctx, SYMGW_ERR_HIP, std::string("V generation: ") + hipGetErrorString(e));
+  ctx->have_V = true;
+  return SYMGW_OK;
+}
+
+int symgw_generate_V_pair_host(const symgw_layout_desc *desc, uint64_t seed, double scale, const double *decay,
+                               int n_decay, int k1, int k2, double *out) {
+  if (!desc || !decay || n_decay <= 0 || !out) return SYMGW_ERR_ARG;
+  Layout L;
+  std::string err;
+  if (!build_layout(desc, L, err)) {
+    g_create_error = err;
+    return SYMGW_ERR_ARG;
+  }
+  if (k1 < 0 || k2 < 0 || k1 >= L.nk || k2 >= L.nk) return SYMGW_ERR_ARG;
+  std::vector<GenBlk> gb;
+  gen_blocks(L, gb, k1 * L.nk + k2);
+  for (const GenBlk &B : gb) {
+    const int64_t K = (int64_t)B.k1 * B.k2, n = (int64_t)B.q * K;
+    for (int64_t e = 0; e < n; ++e) {
+      const int64_t rem = e % K;
+      const int a = (int)(rem / B.k2), m = (int)(rem % B.k2);
+      int d = (B.a_base + a) - (B.b_base + m);
+      d = d < 0 ? -d : d;
+      d = d < n_decay ? d : n_decay - 1;
+      const double f = (scale * decay[d]) * 0.70710678118654752440;
+      double re, im;
+      gen_v_elem(seed, (uint32_t)B.pair, (uint64_t)(B.local_off + e), f, re, im);
+      out[2 * (B.local_off + e)] = re;
+      out[2 * (B.local_off + e) + 1] = im;
+    }
+  }
+  return SYMGW_OK;
+}
+
+int symgw_set_reps(symgw_ctx *ctx, const double *orep_flat, const double *auxrep_flat) {
+  if (!ctx) return SYMGW_ERR_ARG;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  if (ctx->L.flat_orep > 0) {
+    if (!orep_flat) return fail(ctx, SYMGW_ERR_ARG, "orep missing");
+    HIPCHK(ctx, hipMemcpy(ctx->orep, orep_flat, (size_t)ctx->L.flat_orep * 16, hipMemcpyHostToDevice));
+  }
+  if (ctx->L.flat_auxrep > 0) {
+    if (!auxrep_flat) return fail(ctx, SYMGW_ERR_ARG, "auxrep missing");
+    HIPCHK(ctx, hipMemcpy(ctx->auxrep, auxrep_flat, (size_t)ctx->L.flat_auxrep * 16, hipMemcpyHostToDevice));
+  }
+  ctx->have_reps = true;
+  return SYMGW_OK;
+}
+
+int symgw_upload_G(symgw_ctx *ctx, const double *G_ts_kij) {
+  if (!ctx || !G_ts_kij) return SYMGW_ERR_ARG;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  const Layout &L = ctx->L;
+  HIPCHK(ctx, hipMemcpy(ctx->G_ibz, G_ts_kij, (size_t)L.nts * L.ns * L.flat_ao * 16, hipMemcpyHostToDevice));
+  ctx->have_G = true;
+  int r = rotate_G(ctx);
+  if (r) return r;
+  HIPCHK(ctx, hipStreamSynchronize(ctx->st));
+  return SYMGW_OK;
+}
+
+int symgw_download_G(symgw_ctx *ctx, double *G_ts_kij) {
+  if (!ctx || !G_ts_kij) return SYMGW_ERR_ARG;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->st));
+  const Layout &L = ctx->L;
+  HIPCHK(ctx, hipMemcpy(G_ts_kij, ctx->G_ibz, (size_t)L.nts * L.ns * L.flat_ao * 16, hipMemcpyDeviceToHost));
+  return SYMGW_OK;
+}
+
+int symgw_solve_P0(symgw_ctx *ctx) {
+  if (!ctx) return SYMGW_ERR_ARG;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  ev_record(ctx, 0);
+  int r = phase_P0(ctx);
+  if (r) return r;
+  ev_record(ctx, 1);
+  HIPCHK(ctx, hipStreamSynchronize(ctx->st));
+  if (ctx->timing) {
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]);
+    ctx->last_ms[0] = ms;
+  }
+  ctx->have_P = false;
+  return SYMGW_OK;
+}
+
+int symgw_copy_P0_to_host(const symgw_ctx *ctx, double *P0_q_t_QP) {
+  if (!ctx || !P0_q_t_QP) return SYMGW_ERR_ARG;
+  symgw_ctx *c = const_cast<symgw_ctx *>(ctx);
+  HIPCHK(c, hipSetDevice(ctx->device));
+  HIPCHK(c, hipStreamSynchronize(ctx->st));
+  const size_t n = (size_t)ctx->L.nts * ctx->L.flat_aux;
+  std::vector<double> tmp(2 * n);
+  HIPCHK(c, hipMemcpy(tmp.data(), ctx->P, n * 16, hipMemcpyDeviceToHost));
+  for (size_t i = 0; i < 2 * n; ++i) P0_q_t_QP[i] += tmp[i];
+  return SYMGW_OK;
+}
+
+int symgw_upload_P(symgw_ctx *ctx, const double *P_q_t_QP) {
+  if (!ctx || !P_q_t_QP) return SYMGW_ERR_ARG;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  HIPCHK(ctx, hipMemcpy(ctx->P, P_q_t_QP, (size_t)ctx->L.nts * ctx->L.flat_aux * 16, hipMemcpyHostToDevice));
+  ctx->have_P = true;
+  return SYMGW_OK;
+}
+
+int symgw_set_transforms(symgw_ctx *ctx, const double *Tnt, const double *Ttn, const double *Tnt_BF,
+                         const double *Ttn_FB, const double *iwn_fermi) {
+  if (!ctx || !Tnt || !Ttn || !Tnt_BF || !Ttn_FB || !iwn_fermi) return SYMGW_ERR_ARG;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  const Layout &L = ctx->L;
+  HIPCHK(ctx, hipMemcpy(ctx->Tnt, Tnt, (size_t)L.nw * (L.nts - 2) * 16, hipMemcpyHostToDevice));
+  HIPCHK(ctx, hipMemcpy(ctx->Ttn, Ttn, (size_t)L.nts * L.nw * 16, hipMemcpyHostToDevice));
+  HIPCHK(ctx, hipMemcpy(ctx->Tnt_BF, Tnt_BF, (size_t)L.nw_b * (L.nts - 2) * 16, hipMemcpyHostToDevice));
+  HIPCHK(ctx, hipMemcpy(ctx->Ttn_FB, Ttn_FB, (size_t)L.nts * L.nw_b * 16, hipMemcpyHostToDevice));
+  HIPCHK(ctx, hipMemcpy(ctx->iwn, iwn_fermi, (size_t)L.nw * 16, hipMemcpyHostToDevice));
+  ctx->have_T = true;
+  return SYMGW_OK;
+}
+
+int symgw_solve_W(symgw_ctx *ctx) {
+  if (!ctx) return SYMGW_ERR_ARG;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  int r = phase_W(ctx);
+  if (r) return r;
+  HIPCHK(ctx, hipStreamSynchronize(ctx->st));
+  return SYMGW_OK;
+}
+
+int symgw_solve_Sigma(symgw_ctx *ctx) {
+  if (!ctx) return SYMGW_ERR_ARG;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  ev_record(ctx, 2);
+  int r = phase_Sigma(ctx);
+  if (r) return r;
+  ev_record(ctx, 3);
+  HIPCHK(ctx, hipStreamSynchronize(ctx->st));
+  if (ctx->timing) {
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]);
+    ctx->last_ms[2] = ms;
+  }
+  return SYMGW_OK;
+}
+
+int symgw_copy_Sigma_to_host(const symgw_ctx *ctx, double *Sigma_ts_kij) {
+  if (!ctx || !Sigma_ts_kij) return SYMGW_ERR_ARG;
+  symgw_ctx *c = const_cast<symgw_ctx *>(ctx);
+  HIPCHK(c, hipSetDevice(ctx->device));
+  HIPCHK(c, hipStreamSynchronize(ctx->st));
+  const Layout &L = ctx->L;
+  const size_t n = (size_t)L.nts * L.ns * L.flat_ao;
+  std::vector<double> tmp(2 * n);
+  HIPCHK(c, hipMemcpy(tmp.data(), ctx->Sig, n * 16, hipMemcpyDeviceToHost));
+  // k_ts_ij -> ts_kij with accumulation (cugw_routines_block.cu:257-273)
+  const int nts_ns = L.nts * L.ns;
+  for (int ik = 0; ik < L.ink; ++ik) {
+    const int64_t sz = L.kao_size_irre[ik], off = L.kao_off_irre[ik];
+    for (int ts = 0; ts < nts_ns; ++ts) {
+      const double *src = tmp.data() + 2 * (off * nts_ns + (int64_t)ts * sz);
+      double *dst = Sigma_ts_kij + 2 * ((int64_t)ts * L.flat_ao + off);
+      for (int64_t x = 0; x < 2 * sz; ++x) dst[x] += src[x];
+    }
+  }
+  return SYMGW_OK;
+}
+
+int symgw_upload_Sigma(symgw_ctx *ctx, const double *Sigma_ts_kij) {
+  if (!ctx || !Sigma_ts_kij) return SYMGW_ERR_ARG;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  const Layout &L = ctx->L;
+  const size_t n = (size_t)L.nts * L.ns * L.flat_ao;
+  std::vector<double> tmp(2 * n);
+  const int nts_ns = L.nts * L.ns;
+  for (int ik = 0; ik < L.ink; ++ik) {
+    const int64_t sz = L.kao_size_irre[ik], off = L.kao_off_irre[ik];
+    for (int ts = 0; ts < nts_ns; ++ts) {
+      const double *src = Sigma_ts_kij + 2 * ((int64_t)ts * L.flat_ao + off);
+      double *dst = tmp.data() + 2 * (off * nts_ns + (int64_t)ts * sz);
+      std::memcpy(dst, src, (size_t)sz * 16);
+    }
+  }
+  HIPCHK(ctx, hipMemcpy(ctx->Sig, tmp.data(), n * 16, hipMemcpyHostToDevice));
+  ctx->have_Sig = true;
+  return SYMGW_OK;
+}
+
+int symgw_set_fock(symgw_ctx *ctx, const double *F, const double *S) {
+  if (!ctx || !F || !S) return SYMGW_ERR_ARG;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  const size_t n = (size_t)ctx->L.ns * ctx->L.flat_ao;
+  HIPCHK(ctx, hipMemcpy(ctx->F, F, n * 16, hipMemcpyHostToDevice));
+  HIPCHK(ctx, hipMemcpy(ctx->S, S, n * 16, hipMemcpyHostToDevice));
+  ctx->have_F = true;
+  return SYMGW_OK;
+}
+
+int symgw_dyson(symgw_ctx *ctx, double mu, double sigma_scale) {
+  if (!ctx) return SYMGW_ERR_ARG;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  int r = phase_Dyson(ctx, mu, sigma_scale);
+  if (r) return r;
+  HIPCHK(ctx, hipStreamSynchronize(ctx->st));
+  return SYMGW_OK;
+}
+
+int symgw_density(symgw_ctx *ctx, const double *kweights, double *nel) {
+  if (!ctx || !kweights || !nel) return SYMGW_ERR_ARG;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  if (!ctx->have_G || !ctx->have_F) return fail(ctx, SYMGW_ERR_STATE, "G or S missing");
+  const Layout &L = ctx->L;
+  HIPCHK(ctx, hipMemcpyAsync(ctx->kw, kweights, (size_t)L.ink * sizeof(double), hipMemcpyHostToDevice, ctx->st));
+  const cplx<double> *Gbeta = ctx->G_ibz + (int64_t)(L.nts - 1) * L.ns * L.flat_ao;
+  HIPCHK(ctx, launch_density(ctx->d_d_blocks, (int)ctx->W.d_blocks.size(), L.ns, Gbeta, ctx->S, ctx->d_kao_off_irre,
+                             L.flat_ao, ctx->kw, ctx->nel_dev, ctx->st));
+  double v = 0;
+  HIPCHK(ctx, hipMemcpyAsync(&v, ctx->nel_dev, sizeof(double), hipMemcpyDeviceToHost, ctx->st));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->st));
+  // dmr = -G(beta) * (2 / ns) (sc_loop_gf_block.cpp:18-19), per k weighted by w_k / nk
+  *nel = -(2.0 / L.ns) * v / L.nk;
+  return SYMGW_OK;
+}
+
+int symgw_mu_search(symgw_ctx *ctx, double nel_target, const double *kweights, double sigma_scale, double mu_lo,
+                    double mu_hi, double tol, double *mu_out, double *nel_out) {
+  if (!ctx || !kweights || !mu_out || mu_hi <= mu_lo || tol <= 0) return SYMGW_ERR_ARG;
+  double lo = mu_lo, hi = mu_hi, mu = 0.5 * (lo + hi), n = 0;
+  int r;
+  for (int it = 0; it < 200; ++it) {
+    mu = 0.5 * (lo + hi);
+    if ((r = symgw_dyson(ctx, mu, sigma_scale))) return r;
+    if ((r = symgw_density(ctx, kweights, &n))) return r;
+    // N(mu) increases with mu
+    if (n < nel_target) lo = mu; else hi = mu;
+    if (std::fabs(n - nel_target) < tol || hi - lo < 1e-14) break;
+  }
+  *mu_out = mu;
+  if (nel_out) *nel_out = n;
+  return SYMGW_OK;
+}
+
+int symgw_gw_iteration(symgw_ctx *ctx, double mu, int use_graph) {
+  if (!ctx) return SYMGW_ERR_ARG;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  if (!ctx->have_V || !ctx->have_G || !ctx->have_T || !ctx->have_F)
+    return fail(ctx, SYMGW_ERR_STATE, "V, G, transforms and Fock must be set first");
+  ctx->have_P = true;   // produced inside the sequence
+  ctx->have_Sig = true;
+  if (!use_graph) {
+    int r = iteration_body(ctx, mu);
+    if (r) return r;
+  } else {
+    if (ctx->graph_exec && ctx->graph_mu != mu) {
+      (void)hipGraphExecDestroy(ctx->graph_exec);
+      ctx->graph_exec = nullptr;
+    }
+    if (!ctx->graph_exec) {
+      const bool tm = ctx->timing;
+      ctx->timing = false;
+      hipGraph_t g = nullptr;
+      HIPCHK(ctx, hipStreamBeginCapture(ctx->st, hipStreamCaptureModeThreadLocal));
+      int r = iteration_body(ctx, mu);
+      hipError_t e = hipStreamEndCapture(ctx->st, &g);
+      ctx->timing = tm;
+      if (r) {
+        if (g) (void)hipGraphDestroy(g);
+        return r;
+      }
+      if (e != hipSuccess) return fail(ctx, SYMGW_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+      e = hipGraphInstantiate(&ctx->graph_exec, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (e != hipSuccess) return fail(ctx, SYMGW_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+      ctx->graph_mu = mu;
+    }
+    HIPCHK(ctx, hipGraphLaunch(ctx->graph_exec, ctx->st));
+  }
+  return SYMGW_OK;
+}
+
+int symgw_sync(symgw_ctx *ctx) {
+  if (!ctx) return SYMGW_ERR_ARG;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->st));
+  if (ctx->timing) {
+    for (int i = 0; i < 4; ++i) {
+      float ms = 0;
+      if (hipEventElapsedTime(&ms, ctx->ev[i], ctx->ev[i + 1]) == hipSuccess) ctx->last_ms[i] = ms;
+      else (void)hipGetLastError();
+    }
+  }
+  return SYMGW_OK;
+}
+
+int symgw_last_times(const symgw_ctx *ctx, double *ms5) {
+  if (!ctx || !ms5) return SYMGW_ERR_ARG;
+  for (int i = 0; i < 5; ++i) ms5[i] = ctx->last_ms[i];
+  return SYMGW_OK;
+}
+
+int symgw_set_timing(symgw_ctx *ctx, int on) {
+  if (!ctx) return SYMGW_ERR_ARG;
+  ctx->timing = on != 0;
+  return SYMGW_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,62 @@
+// Launch interface of the device kernels (kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "common.h"
+#include "layout.h"
+
+namespace symgw {
+
+// One batch of the transform GEMM: rows of `in` start at in_off with stride
+// in_row_stride, rows of `out` at out_off with out_row_stride; xlen columns
+// per split group.
+struct TBatch {
+  int64_t in_off, out_off, in_row_stride, out_row_stride;
+  int64_t in_split, out_split;  // offset between column groups (nsplit > 1)
+  int32_t xlen, pad;
+};
+
+// One V block for the device generator.
+struct GenBlk {
+  int64_t v_off;      // absolute complex offset in the V buffer
+  int64_t local_off;  // offset inside the pair slice (generator counter base)
+  int32_t q, k1, k2;
+  int32_t a_base;     // dense orbital index of the block's first row (k1 side)
+  int32_t b_base;     // dense orbital index of the block's first column (k2 side)
+  int32_t pair;       // k1 * nk + k2
+};
+
+template <typename T>
+hipError_t launch_p0(const P0Item *items, const P0Blk *blks, int n_items, int nts, int ns, const cplx<T> *V,
+                     const cplx<T> *G, cplx<double> *P0, double pref, size_t lds, hipStream_t st);
+template <typename T>
+hipError_t launch_sigma(const SigItem *items, const SigBlk *blks, int n_items, int nts, int ns, const cplx<T> *V,
+                        const cplx<T> *G, const cplx<T> *Pf, cplx<double> *Sig, double pref, size_t lds,
+                        hipStream_t st);
+template <typename TO>
+hipError_t launch_rotate(const RotBlk *rb, int n_rb, int nts_ns, const cplx<double> *src, const int64_t *src_ts_stride,
+                         const int64_t *src_slice, const cplx<double> *rep, cplx<TO> *dst, const int64_t *dst_slice,
+                         const int64_t *dst_slice_size, hipStream_t st);
+hipError_t launch_tgemm(const TBatch *tb, int nbatch, int max_cols, int M, int KT, int nsplit,
+                        const cplx<double> *Tm, const cplx<double> *in, cplx<double> *out, int mode, int row_shift,
+                        int nts, double alpha, int64_t in_split_stride, int64_t out_split_stride, hipStream_t st);
+hipError_t launch_wsolve(const SolveBlk *blks, int nblk, int nw_b, cplx<double> *Pw, const int64_t *w_off,
+                         const int64_t *qslice, cplx<double> *scratch, int64_t scr_group, hipStream_t st);
+hipError_t launch_dyson(const SolveBlk *blks, int nblk, int nw, int ns, const cplx<double> *Sw, cplx<double> *Gw,
+                        const int64_t *d_off, const int64_t *kslice, const int64_t *f_off, const cplx<double> *F,
+                        const cplx<double> *S, int64_t flat_ao, const cplx<double> *iwn, double mu,
+                        double sigma_scale, cplx<double> *scratch, int64_t scr_group, hipStream_t st);
+hipError_t launch_density(const SolveBlk *blks, int nblk, int ns, const cplx<double> *Gbeta, const cplx<double> *S,
+                          const int64_t *f_off, int64_t flat_ao, const double *kw, double *out, hipStream_t st);
+template <typename T>
+hipError_t launch_genv(const GenBlk *gb, int ngb, uint64_t seed, double scale, const double *decay, int n_decay,
+                       cplx<T> *V, hipStream_t st);
+template <typename TO>
+hipError_t launch_convert(const cplx<double> *in, cplx<TO> *out, int64_t n, hipStream_t st);
+hipError_t launch_reorder(const cplx<double> *in, cplx<double> *out, const int64_t *koff, const int64_t *ksize,
+                          int ink, int nts_ns, int64_t max_ksize, int64_t flat_ao, int dir, double scale,
+                          hipStream_t st);
+
+}  // namespace symgw

@@ -1,0 +1,652 @@
+// Device kernels of the GW hot path (gfx950).
+//
+// Contraction kernels are "owner computes": every workgroup owns one output
+// slab and walks its V-block list in a fixed order, so results are bitwise
+// reproducible and no atomics or locks are needed (the reference serialises
+// per-q / per-k AXPYs behind spin locks, cugw_qkpt_block.cu:269-285, 414-436).
+#include <hip/hip_runtime.h>
+
+#include "common.h"
+#include "kernels.h"
+
+namespace symgw {
+
+static constexpr int NT = 256;  // threads per workgroup
+static constexpr int XT = 8;    // K tile of staged conj(V) rows in the P0 kernel
+
+template <typename T>
+__device__ inline cplx<T> czero() { return cplx<T>{T(0), T(0)}; }
+
+// ---------------------------------------------------------------------------
+// P0 contraction.  For one output slab (rows [Q0, Q0+Qc) of aux block iq of
+// iBZ point q) and time t < nts/2:
+//   P0[Q][P](t) = pref * sum_{k, blocks, s} sum_{p,n} C_Q[p][n] conj(V_P[p][n]),
+//   C_Q = G^k(beta - t) V_Q G^{k+q}(t)
+// which regroups the three GEMMs of cugw_qkpt_block.cu:231-260
+// (X1 = G V, X2 = conj(V) G^T, P0^T += X2 X1) around the single intermediate C.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(NT) void p0_kernel(const P0Item *__restrict__ items, const P0Blk *__restrict__ blks,
+                                               int64_t n_work, int nt_half, int nts, int ns,
+                                               const cplx<T> *__restrict__ V, const cplx<T> *__restrict__ G,
+                                               cplx<double> *__restrict__ P0, double pref) {
+  const int64_t w = xcd_logical(blockIdx.x, gridDim.x);
+  if (w >= n_work) return;
+  const int item_id = (int)(w / nt_half);
+  const int t = (int)(w % nt_half);
+  const P0Item it = items[item_id];
+  const int tid = threadIdx.x;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cplx<T> *bufA = reinterpret_cast<cplx<T> *>(smem_raw);  // V rows, then C
+  cplx<T> *bufB = bufA + it.Qc * it.kmax;                  // E = V G2
+  cplx<T> *bufV = bufB + it.Qc * it.kmax;                  // [XT][q] conj-V tile
+  const int q = it.q, Qc = it.Qc, nout = Qc * q;
+  cplx<T> acc[4];
+  int orow[4], ocol[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    acc[j] = czero<T>();
+    int o = tid + NT * j;
+    orow[j] = o < nout ? o / q : -1;
+    ocol[j] = o < nout ? o % q : 0;
+  }
+  for (int b = it.blk_begin; b < it.blk_end; ++b) {
+    const P0Blk B = blks[b];
+    const int k1 = B.k1, k2 = B.k2, K = k1 * k2;
+    const cplx<T> *Vb = V + B.v_off;
+    for (int s = 0; s < ns; ++s) {
+      const cplx<T> *G1 = G + B.g1_base + (int64_t)(nts - 1 - t) * B.g1_tstride + (int64_t)s * B.g1_sstride;
+      const cplx<T> *G2 = G + B.g2_base + (int64_t)t * B.g2_tstride + (int64_t)s * B.g2_sstride;
+      __syncthreads();
+      for (int i = tid; i < Qc * K; i += NT) bufA[i] = Vb[(int64_t)it.Q0 * K + i];
+      __syncthreads();
+      // E[r][a][n] = sum_m V[r][a][m] G2[m][n]
+      for (int i = tid; i < Qc * K; i += NT) {
+        const int r = i / K, rem = i - r * K, a = rem / k2, n = rem - a * k2;
+        const cplx<T> *vr = bufA + r * K + a * k2;
+        cplx<T> e = czero<T>();
+        for (int m = 0; m < k2; ++m) cfma(e, vr[m], G2[m * k2 + n]);
+        bufB[i] = e;
+      }
+      __syncthreads();
+      // C[r][p][n] = sum_a G1[p][a] E[r][a][n]
+      for (int i = tid; i < Qc * K; i += NT) {
+        const int r = i / K, rem = i - r * K, p = rem / k2, n = rem - p * k2;
+        const cplx<T> *er = bufB + r * K + n;
+        cplx<T> c = czero<T>();
+        for (int a = 0; a < k1; ++a) cfma(c, G1[p * k1 + a], er[a * k2]);
+        bufA[i] = c;
+      }
+      // acc[r][P] += sum_x C[r][x] conj(V[P][x])
+      for (int x0 = 0; x0 < K; x0 += XT) {
+        __syncthreads();
+        for (int i = tid; i < q * XT; i += NT) {
+          const int P = i / XT, xx = i - P * XT;
+          bufV[xx * q + P] = (x0 + xx < K) ? Vb[(int64_t)P * K + x0 + xx] : czero<T>();
+        }
+        __syncthreads();
+        const int xn = min(XT, K - x0);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          if (orow[j] < 0) continue;
+          const cplx<T> *cr = bufA + orow[j] * K + x0;
+          for (int xx = 0; xx < xn; ++xx) cfmac(acc[j], cr[xx], bufV[xx * q + ocol[j]]);
+        }
+      }
+    }
+  }
+  cplx<double> *out = P0 + it.out_base + (int64_t)t * it.out_tstride;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    int o = tid + NT * j;
+    if (o < nout) out[o] = cplx<double>{pref * (double)acc[j].re, pref * (double)acc[j].im};
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Sigma contraction.  For one output slab (rows [I0, I0+Ic) of ao block ik1 of
+// iBZ point k), time t and spin s:
+//   Sigma[i][j] = pref * sum_{q, blocks} sum_{Q,n} E_Q[i][n] conj(Z_Q[j][n]),
+//   E_Q = V_Q G^{k-q}(t),   Z_Q = sum_P conj(P_{QP}(t)) V_P
+// the same contraction as cugw_qkpt_block.cu:381-407 (Y1 = V G, Y2 = Y1 P,
+// Sigma += Y2 V^H) with the P-contraction moved onto V.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(NT) void sigma_kernel(const SigItem *__restrict__ items, const SigBlk *__restrict__ blks,
+                                                  int64_t n_work, int nts, int ns,
+                                                  const cplx<T> *__restrict__ V, const cplx<T> *__restrict__ G,
+                                                  const cplx<T> *__restrict__ Pf, cplx<double> *__restrict__ Sig,
+                                                  double pref) {
+  const int64_t w = xcd_logical(blockIdx.x, gridDim.x);
+  if (w >= n_work) return;
+  const int per_item = nts * ns;
+  const int item_id = (int)(w / per_item);
+  const int ts = (int)(w % per_item);
+  const int t = ts / ns, s = ts % ns;
+  const SigItem it = items[item_id];
+  const int tid = threadIdx.x;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cplx<T> *lds = reinterpret_cast<cplx<T> *>(smem_raw);
+  const int k1 = it.k1, Ic = it.Ic, nout = Ic * k1;
+  cplx<T> acc[4];
+  int oi[4], oj[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    acc[j] = czero<T>();
+    int o = tid + NT * j;
+    oi[j] = o < nout ? o / k1 : -1;
+    oj[j] = o < nout ? o % k1 : 0;
+  }
+  for (int b = it.blk_begin; b < it.blk_end; ++b) {
+    const SigBlk B = blks[b];
+    const int q = B.q, k2 = B.k2, K = k1 * k2, qc = B.qc;
+    const cplx<T> *Vb = V + B.v_off;
+    const cplx<T> *G2 = G + B.g2_base + (int64_t)t * B.g2_tstride + (int64_t)s * B.g2_sstride;
+    const cplx<T> *Pb = Pf + B.p_base + (int64_t)t * B.p_tstride;
+    cplx<T> *bufZ = lds;                      // [qc][K]
+    cplx<T> *bufE = bufZ + qc * K;            // [qc][Ic][k2]
+    cplx<T> *bufP = bufE + qc * Ic * k2;      // [qc][q] conj(P) rows
+    for (int Q0 = 0; Q0 < q; Q0 += qc) {
+      const int nr = min(qc, q - Q0);
+      __syncthreads();
+      for (int i = tid; i < nr * q; i += NT) {
+        const cplx<T> p = Pb[(int64_t)Q0 * q + i];
+        bufP[i] = cplx<T>{p.re, -p.im};
+      }
+      __syncthreads();
+      // Z[r][x] = sum_P conj(P[Q0+r][P]) V[P][x]; each thread owns columns x
+      for (int x = tid; x < K; x += NT) {
+        cplx<T> z[16];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) z[r] = czero<T>();
+        for (int P = 0; P < q; ++P) {
+          const cplx<T> v = Vb[(int64_t)P * K + x];
+#pragma unroll
+          for (int r = 0; r < 16; ++r)
+            if (r < nr) cfma(z[r], bufP[r * q + P], v);
+        }
+#pragma unroll
+        for (int r = 0; r < 16; ++r)
+          if (r < nr) bufZ[r * K + x] = z[r];
+      }
+      // E[r][i][n] = sum_m V[Q0+r][I0+i][m] G2[m][n]
+      for (int i = tid; i < nr * Ic * k2; i += NT) {
+        const int r = i / (Ic * k2), rem = i - r * (Ic * k2), ii = rem / k2, n = rem - ii * k2;
+        const cplx<T> *vr = Vb + (int64_t)(Q0 + r) * K + (it.I0 + ii) * k2;
+        cplx<T> e = czero<T>();
+        for (int m = 0; m < k2; ++m) cfma(e, vr[m], G2[m * k2 + n]);
+        bufE[i] = e;
+      }
+      __syncthreads();
+      // acc[i][j] += sum_{r,n} E[r][i][n] conj(Z[r][j][n])
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        if (oi[j] < 0) continue;
+        for (int r = 0; r < nr; ++r) {
+          const cplx<T> *er = bufE + (r * Ic + oi[j]) * k2;
+          const cplx<T> *zr = bufZ + r * K + oj[j] * k2;
+          for (int n = 0; n < k2; ++n) cfmac(acc[j], er[n], zr[n]);
+        }
+      }
+    }
+  }
+  cplx<double> *out = Sig + it.out_base + (int64_t)t * it.out_tstride + (int64_t)s * it.out_sstride;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    int o = tid + NT * j;
+    if (o < nout) out[o] = cplx<double>{pref * (double)acc[j].re, pref * (double)acc[j].im};
+  }
+}
+
+// ---------------------------------------------------------------------------
+// iBZ -> full-BZ rotation of one block for one (t, s) slice:
+//   Y = O X O^H (or a copy), conjugated for time-reversal partners.
+// src: iBZ array; the (t,s) slice of the representative of target k starts at
+// src_slice[k] + ts * src_ts_stride[k].  dst: k_ts_ij over the full BZ.
+// ---------------------------------------------------------------------------
+template <typename TO>
+__global__ __launch_bounds__(NT) void rotate_kernel(const RotBlk *__restrict__ rb, int n_rb, int nts_ns,
+                                                   const cplx<double> *__restrict__ src,
+                                                   const int64_t *__restrict__ src_ts_stride,
+                                                   const int64_t *__restrict__ src_slice,
+                                                   const cplx<double> *__restrict__ rep, cplx<TO> *__restrict__ dst,
+                                                   const int64_t *__restrict__ dst_slice,
+                                                   const int64_t *__restrict__ dst_slice_size) {
+  const int64_t w = blockIdx.x;
+  const int r = (int)(w / nts_ns);
+  const int ts = (int)(w % nts_ns);
+  if (r >= n_rb) return;
+  const RotBlk R = rb[r];
+  const int tid = threadIdx.x;
+  const cplx<double> *X = src + (int64_t)ts * src_ts_stride[R.target] + src_slice[R.target] + R.src_off;
+  cplx<TO> *Y = dst + dst_slice[R.target] * nts_ns + (int64_t)ts * dst_slice_size[R.target] + R.dst_off;
+  const int m = R.m, n = R.n;
+  if (R.rep_off < 0) {
+    for (int i = tid; i < m * m; i += NT) {
+      cplx<double> v = X[i];
+      if (R.conj) v.im = -v.im;
+      Y[i] = cplx<TO>{(TO)v.re, (TO)v.im};
+    }
+    return;
+  }
+  const cplx<double> *O = rep + R.rep_off;  // m x n
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cplx<double> *tmp = reinterpret_cast<cplx<double> *>(smem_raw);  // [n][jc]
+  const int jc_max = max(1, 4096 / n);
+  for (int j0 = 0; j0 < m; j0 += jc_max) {
+    const int jc = min(jc_max, m - j0);
+    __syncthreads();
+    // tmp[a][jj] = sum_b X[a][b] conj(O[j0+jj][b])
+    for (int i = tid; i < n * jc; i += NT) {
+      const int a = i / jc, jj = i - a * jc;
+      cplx<double> acc{0.0, 0.0};
+      for (int bb = 0; bb < n; ++bb) cfmac(acc, X[a * n + bb], O[(j0 + jj) * n + bb]);
+      tmp[i] = acc;
+    }
+    __syncthreads();
+    // Y[i][j0+jj] = sum_a O[i][a] tmp[a][jj]
+    for (int i = tid; i < m * jc; i += NT) {
+      const int ii = i / jc, jj = i - ii * jc;
+      cplx<double> acc{0.0, 0.0};
+      for (int a = 0; a < n; ++a) cfma(acc, O[ii * n + a], tmp[a * jc + jj]);
+      if (R.conj) acc.im = -acc.im;
+      Y[ii * m + j0 + jj] = cplx<TO>{(TO)acc.re, (TO)acc.im};
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Batched transform GEMM (FP64): out_b[r][c] = alpha * sum_j T[r][j] in_b[row(j)][c]
+// for r < M, j < KT, c < ncol_b.  Columns are split into nsplit groups of
+// xlen_b: column c -> (c / xlen) * split_b + c % xlen, with per-batch split
+// offsets on each side.
+// row(j) = j + row_shift (mode 0) or the P0(beta - t) = P0(t) mirror of
+// t = j + row_shift (mode 1).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(NT) void tgemm_kernel(const TBatch *__restrict__ tb, int M, int KT, int nsplit,
+                                                  const cplx<double> *__restrict__ Tm,
+                                                  const cplx<double> *__restrict__ in, cplx<double> *__restrict__ out,
+                                                  int mode, int row_shift, int nts, double alpha,
+                                                  int64_t in_split_stride, int64_t out_split_stride) {
+  const int b = blockIdx.y;
+  const TBatch B = tb[b];
+  const int ncol = B.xlen * nsplit;
+  const int c = blockIdx.x * NT + threadIdx.x;
+  const int r0 = blockIdx.z * 8;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cplx<double> *Ts = reinterpret_cast<cplx<double> *>(smem_raw);  // [8][KT]
+  const int nr = min(8, M - r0);
+  for (int i = threadIdx.x; i < nr * KT; i += NT) Ts[i] = Tm[(int64_t)r0 * KT + i];
+  __syncthreads();
+  if (c >= ncol) return;
+  const int sp = c / B.xlen, cx = c - sp * B.xlen;
+  const int64_t in_col = B.in_off + sp * B.in_split + cx;
+  cplx<double> acc[8];
+#pragma unroll
+  for (int r = 0; r < 8; ++r) acc[r] = cplx<double>{0.0, 0.0};
+  for (int j = 0; j < KT; ++j) {
+    int row = j + row_shift;
+    if (mode == 1 && row >= nts / 2) row = nts - 1 - row;
+    const cplx<double> v = in[in_col + (int64_t)row * B.in_row_stride];
+#pragma unroll
+    for (int r = 0; r < 8; ++r)
+      if (r < nr) cfma(acc[r], Ts[r * KT + j], v);
+  }
+  const int64_t out_col = B.out_off + sp * B.out_split + cx;
+#pragma unroll
+  for (int r = 0; r < 8; ++r)
+    if (r < nr) out[out_col + (int64_t)(r0 + r) * B.out_row_stride] = cplx<double>{alpha * acc[r].re, alpha * acc[r].im};
+}
+
+// ---------------------------------------------------------------------------
+// In-place Gauss-Jordan inversion with partial pivoting of an n x n complex
+// matrix A (row-major, any address space), by one workgroup.
+// ---------------------------------------------------------------------------
+__device__ void gj_invert(cplx<double> *A, int n, int *piv, cplx<double> *fcol, double *red_v, int *red_i) {
+  const int tid = threadIdx.x;
+  for (int k = 0; k < n; ++k) {
+    // pivot search over rows k..n-1
+    double best = -1.0;
+    int bi = k;
+    for (int i = k + tid; i < n; i += NT) {
+      cplx<double> a = A[i * n + k];
+      double v = a.re * a.re + a.im * a.im;
+      if (v > best) { best = v; bi = i; }
+    }
+    red_v[tid] = best;
+    red_i[tid] = bi;
+    __syncthreads();
+    for (int st = NT / 2; st > 0; st >>= 1) {
+      if (tid < st) {
+        double v2 = red_v[tid + st];
+        int i2 = red_i[tid + st];
+        if (v2 > red_v[tid] || (v2 == red_v[tid] && i2 < red_i[tid])) { red_v[tid] = v2; red_i[tid] = i2; }
+      }
+      __syncthreads();
+    }
+    const int p = red_i[0];
+    if (tid == 0) piv[k] = p;
+    if (p != k)
+      for (int j = tid; j < n; j += NT) {
+        cplx<double> x = A[k * n + j];
+        A[k * n + j] = A[p * n + j];
+        A[p * n + j] = x;
+      }
+    __syncthreads();
+    const cplx<double> d = A[k * n + k];
+    const double dn = d.re * d.re + d.im * d.im;
+    const cplx<double> inv{d.re / dn, -d.im / dn};
+    __syncthreads();
+    for (int j = tid; j < n; j += NT) {
+      if (j == k) A[k * n + k] = inv;
+      else A[k * n + j] = cmul(A[k * n + j], inv);
+    }
+    for (int i = tid; i < n; i += NT) fcol[i] = A[i * n + k];
+    __syncthreads();
+    for (int e = tid; e < n * n; e += NT) {
+      const int i = e / n, j = e - i * n;
+      if (i == k) continue;
+      const cplx<double> f = fcol[i];
+      const cplx<double> akj = A[k * n + j];
+      if (j == k) {
+        A[i * n + k] = cplx<double>{-(f.re * akj.re - f.im * akj.im), -(f.re * akj.im + f.im * akj.re)};
+      } else {
+        cplx<double> v = A[i * n + j];
+        v.re -= f.re * akj.re - f.im * akj.im;
+        v.im -= f.re * akj.im + f.im * akj.re;
+        A[i * n + j] = v;
+      }
+    }
+    __syncthreads();
+  }
+  for (int k = n - 1; k >= 0; --k) {
+    const int p = piv[k];
+    if (p != k)
+      for (int i = tid; i < n; i += NT) {
+        cplx<double> x = A[i * n + k];
+        A[i * n + k] = A[i * n + p];
+        A[i * n + p] = x;
+      }
+    __syncthreads();
+  }
+}
+
+// W solve per (q_ir, n, aux block) (cugw_solver_block.cpp:412-424):
+//   A = I - (X + X^H)/2,  A <- A^-1,  A <- (A + A^H)/2,  X <- A X
+// Pw layout: [q_ir][n][qslice], block at w_off[q_ir] + n*qslice + off.
+__global__ __launch_bounds__(NT) void wsolve_kernel(const SolveBlk *__restrict__ blks, int nblk, int nw_b,
+                                                   cplx<double> *__restrict__ Pw,
+                                                   const int64_t *__restrict__ w_off,
+                                                   const int64_t *__restrict__ qslice,
+                                                   cplx<double> *__restrict__ scratch, int64_t scr_group) {
+  const int b = blockIdx.x / nw_b, nn = blockIdx.x % nw_b;
+  if (b >= nblk) return;
+  const SolveBlk S = blks[b];
+  const int n = S.n, tid = threadIdx.x;
+  __shared__ int piv[512];
+  __shared__ cplx<double> fcol[512];
+  __shared__ double red_v[NT];
+  __shared__ int red_i[NT];
+  cplx<double> *X = Pw + w_off[S.slice] + (int64_t)nn * qslice[S.slice] + S.off;
+  cplx<double> *A = scratch + (int64_t)nn * scr_group + S.scr;
+  cplx<double> *Xc = A + (int64_t)n * n;
+  for (int e = tid; e < n * n; e += NT) {
+    const int i = e / n, j = e - i * n;
+    const cplx<double> x = X[e], xt = X[j * n + i];
+    Xc[e] = x;
+    A[e] = cplx<double>{(i == j ? 1.0 : 0.0) - 0.5 * (x.re + xt.re), -0.5 * (x.im - xt.im)};
+  }
+  __syncthreads();
+  gj_invert(A, n, piv, fcol, red_v, red_i);
+  // hermitize in place: pairs (i<j)
+  for (int e = tid; e < n * n; e += NT) {
+    const int i = e / n, j = e - i * n;
+    if (i > j) continue;
+    const cplx<double> a = A[i * n + j], bt = A[j * n + i];
+    const cplx<double> h{0.5 * (a.re + bt.re), 0.5 * (a.im - bt.im)};
+    A[i * n + j] = h;
+    A[j * n + i] = cplx<double>{h.re, -h.im};
+  }
+  __syncthreads();
+  for (int e = tid; e < n * n; e += NT) {
+    const int i = e / n, j = e - i * n;
+    cplx<double> acc{0.0, 0.0};
+    for (int kk = 0; kk < n; ++kk) cfma(acc, A[i * n + kk], Xc[kk * n + j]);
+    X[e] = acc;
+  }
+}
+
+// Dyson per (k_ir, w, s, ao block) (sc_type_block.cpp:159-168):
+//   G(iw) = [(iw + mu) S - F - sigma_scale * Sigma(iw)]^-1
+// Sw/Gw layout: [k_ir][w][s][kslice] at d_off[k_ir] + (w*ns + s)*kslice + off.
+__global__ __launch_bounds__(NT) void dyson_kernel(const SolveBlk *__restrict__ blks, int nblk, int nw, int ns,
+                                                  const cplx<double> *__restrict__ Sw, cplx<double> *__restrict__ Gw,
+                                                  const int64_t *__restrict__ d_off,
+                                                  const int64_t *__restrict__ kslice,
+                                                  const int64_t *__restrict__ f_off,
+                                                  const cplx<double> *__restrict__ F, const cplx<double> *__restrict__ S,
+                                                  int64_t flat_ao, const cplx<double> *__restrict__ iwn, double mu,
+                                                  double sigma_scale, cplx<double> *__restrict__ scratch,
+                                                  int64_t scr_group) {
+  const int per = nw * ns;
+  const int b = blockIdx.x / per, ws = blockIdx.x % per;
+  if (b >= nblk) return;
+  const int wi = ws / ns, s = ws % ns;
+  const SolveBlk B = blks[b];
+  const int n = B.n, tid = threadIdx.x;
+  __shared__ int piv[512];
+  __shared__ cplx<double> fcol[512];
+  __shared__ double red_v[NT];
+  __shared__ int red_i[NT];
+  const int64_t loc = d_off[B.slice] + (int64_t)ws * kslice[B.slice] + B.off;
+  const cplx<double> *Sig = Sw + loc;
+  const cplx<double> *Fb = F + (int64_t)s * flat_ao + f_off[B.slice] + B.off;
+  const cplx<double> *Sb = S + (int64_t)s * flat_ao + f_off[B.slice] + B.off;
+  const cplx<double> z{iwn[wi].re + mu, iwn[wi].im};
+  cplx<double> *A = scratch + (int64_t)ws * scr_group + B.scr;
+  for (int e = tid; e < n * n; e += NT) {
+    const cplx<double> zs = cmul(z, Sb[e]);
+    // sigma_scale == 0 means no self-energy: Sigma(iw) is not read at all
+    const cplx<double> sg = sigma_scale != 0.0 ? Sig[e] : cplx<double>{0.0, 0.0};
+    A[e] = cplx<double>{zs.re - Fb[e].re - sigma_scale * sg.re, zs.im - Fb[e].im - sigma_scale * sg.im};
+  }
+  __syncthreads();
+  gj_invert(A, n, piv, fcol, red_v, red_i);
+  cplx<double> *Go = Gw + loc;
+  for (int e = tid; e < n * n; e += NT) Go[e] = A[e];
+}
+
+// Electron count: sum over (s, k_ir, diagonal blocks) of w_k Tr[G(beta) S].
+// One workgroup, fixed-order reduction.
+__global__ __launch_bounds__(NT) void density_kernel(const SolveBlk *__restrict__ blks, int nblk, int ns,
+                                                    const cplx<double> *__restrict__ Gbeta,
+                                                    const cplx<double> *__restrict__ S,
+                                                    const int64_t *__restrict__ f_off, int64_t flat_ao,
+                                                    const double *__restrict__ kw, double *__restrict__ out) {
+  __shared__ double red[NT];
+  double part = 0.0;
+  for (int s = 0; s < ns; ++s)
+    for (int b = 0; b < nblk; ++b) {
+      const SolveBlk B = blks[b];
+      const int n = B.n;
+      const cplx<double> *Gb = Gbeta + (int64_t)s * flat_ao + f_off[B.slice] + B.off;
+      const cplx<double> *Sb = S + (int64_t)s * flat_ao + f_off[B.slice] + B.off;
+      for (int e = threadIdx.x; e < n * n; e += NT) {
+        const int i = e / n, j = e - i * n;
+        // Tr[G S] = sum_ij G[i][j] S[j][i] (real part)
+        const cplx<double> g = Gb[e], sv = Sb[j * n + i];
+        part += kw[B.slice] * (g.re * sv.re - g.im * sv.im);
+      }
+    }
+  red[threadIdx.x] = part;
+  __syncthreads();
+  for (int st = NT / 2; st > 0; st >>= 1) {
+    if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = red[0];
+}
+
+// Synthetic V on the device: one thread per complex element of each block.
+template <typename T>
+__global__ __launch_bounds__(NT) void genv_kernel(const GenBlk *__restrict__ gb, int ngb, uint64_t seed, double scale,
+                                                 const double *__restrict__ decay, int n_decay, cplx<T> *__restrict__ V) {
+  const int b = blockIdx.x;
+  if (b >= ngb) return;
+  const GenBlk B = gb[b];
+  const int64_t K = (int64_t)B.k1 * B.k2;
+  const int64_t n = (int64_t)B.q * K;
+  for (int64_t e = threadIdx.x; e < n; e += NT) {
+    const int64_t rem = e % K;
+    const int a = (int)(rem / B.k2), m = (int)(rem % B.k2);
+    int d = (B.a_base + a) - (B.b_base + m);
+    d = d < 0 ? -d : d;
+    d = d < n_decay ? d : n_decay - 1;
+    const double f = (scale * decay[d]) * 0.70710678118654752440;
+    double re, im;
+    gen_v_elem(seed, (uint32_t)B.pair, (uint64_t)(B.local_off + e), f, re, im);
+    V[B.v_off + e] = cplx<T>{(T)re, (T)im};
+  }
+}
+
+// Element-wise conversion (used for FP32 copies of FP64 data).
+template <typename TO>
+__global__ __launch_bounds__(NT) void convert_kernel(const cplx<double> *__restrict__ in, cplx<TO> *__restrict__ out,
+                                                    int64_t n) {
+  for (int64_t e = (int64_t)blockIdx.x * NT + threadIdx.x; e < n; e += (int64_t)gridDim.x * NT)
+    out[e] = cplx<TO>{(TO)in[e].re, (TO)in[e].im};
+}
+
+// Reorder k_ts_ij (iBZ, contraction output) <-> ts_kij: dir 0: k_ts -> ts_k.
+__global__ __launch_bounds__(NT) void reorder_kernel(const cplx<double> *__restrict__ in, cplx<double> *__restrict__ out,
+                                                    const int64_t *__restrict__ koff, const int64_t *__restrict__ ksize,
+                                                    int ink, int nts_ns, int64_t flat_ao, int dir, double scale) {
+  const int k = blockIdx.y;
+  const int ts = blockIdx.z;
+  if (k >= ink || ts >= nts_ns) return;
+  const int64_t sz = ksize[k];
+  for (int64_t x = (int64_t)blockIdx.x * NT + threadIdx.x; x < sz; x += (int64_t)gridDim.x * NT) {
+    const int64_t a = koff[k] * nts_ns + (int64_t)ts * sz + x;   // k_ts_ij
+    const int64_t bidx = (int64_t)ts * flat_ao + koff[k] + x;    // ts_kij
+    if (dir == 0) out[bidx] = cplx<double>{scale * in[a].re, scale * in[a].im};
+    else out[a] = cplx<double>{scale * in[bidx].re, scale * in[bidx].im};
+  }
+}
+
+// ---- launchers ---------------------------------------------------------------
+template <typename T>
+hipError_t launch_p0(const P0Item *items, const P0Blk *blks, int n_items, int nts, int ns, const cplx<T> *V,
+                     const cplx<T> *G, cplx<double> *P0, double pref, size_t lds, hipStream_t st) {
+  const int nt_half = nts / 2;
+  const int64_t n_work = (int64_t)n_items * nt_half;
+  if (n_work == 0) return hipSuccess;
+  (void)hipFuncSetAttribute((const void *)p0_kernel<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipLaunchKernelGGL((p0_kernel<T>), dim3((unsigned)xcd_grid(n_work)), dim3(NT), lds, st, items, blks, n_work,
+                     nt_half, nts, ns, V, G, P0, pref);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_sigma(const SigItem *items, const SigBlk *blks, int n_items, int nts, int ns, const cplx<T> *V,
+                        const cplx<T> *G, const cplx<T> *Pf, cplx<double> *Sig, double pref, size_t lds,
+                        hipStream_t st) {
+  const int64_t n_work = (int64_t)n_items * nts * ns;
+  if (n_work == 0) return hipSuccess;
+  (void)hipFuncSetAttribute((const void *)sigma_kernel<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipLaunchKernelGGL((sigma_kernel<T>), dim3((unsigned)xcd_grid(n_work)), dim3(NT), lds, st, items, blks, n_work,
+                     nts, ns, V, G, Pf, Sig, pref);
+  return hipGetLastError();
+}
+
+template <typename TO>
+hipError_t launch_rotate(const RotBlk *rb, int n_rb, int nts_ns, const cplx<double> *src, const int64_t *src_ts_stride,
+                         const int64_t *src_slice, const cplx<double> *rep, cplx<TO> *dst, const int64_t *dst_slice,
+                         const int64_t *dst_slice_size, hipStream_t st) {
+  if (n_rb == 0) return hipSuccess;
+  hipLaunchKernelGGL((rotate_kernel<TO>), dim3((unsigned)((int64_t)n_rb * nts_ns)), dim3(NT),
+                     4096 * sizeof(cplx<double>), st, rb, n_rb, nts_ns, src, src_ts_stride, src_slice, rep, dst,
+                     dst_slice, dst_slice_size);
+  return hipGetLastError();
+}
+
+hipError_t launch_tgemm(const TBatch *tb, int nbatch, int max_cols, int M, int KT, int nsplit,
+                        const cplx<double> *Tm, const cplx<double> *in, cplx<double> *out, int mode, int row_shift,
+                        int nts, double alpha, int64_t in_split_stride, int64_t out_split_stride, hipStream_t st) {
+  if (nbatch == 0 || M == 0) return hipSuccess;
+  dim3 grid((max_cols + NT - 1) / NT, nbatch, (M + 7) / 8);
+  hipLaunchKernelGGL(tgemm_kernel, grid, dim3(NT), (size_t)8 * KT * sizeof(cplx<double>), st, tb, M, KT, nsplit, Tm,
+                     in, out, mode, row_shift, nts, alpha, in_split_stride, out_split_stride);
+  return hipGetLastError();
+}
+
+hipError_t launch_wsolve(const SolveBlk *blks, int nblk, int nw_b, cplx<double> *Pw, const int64_t *w_off,
+                         const int64_t *qslice, cplx<double> *scratch, int64_t scr_group, hipStream_t st) {
+  if (nblk == 0 || nw_b == 0) return hipSuccess;
+  hipLaunchKernelGGL(wsolve_kernel, dim3(nblk * nw_b), dim3(NT), 0, st, blks, nblk, nw_b, Pw, w_off, qslice, scratch,
+                     scr_group);
+  return hipGetLastError();
+}
+
+hipError_t launch_dyson(const SolveBlk *blks, int nblk, int nw, int ns, const cplx<double> *Sw, cplx<double> *Gw,
+                        const int64_t *d_off, const int64_t *kslice, const int64_t *f_off, const cplx<double> *F,
+                        const cplx<double> *S, int64_t flat_ao, const cplx<double> *iwn, double mu,
+                        double sigma_scale, cplx<double> *scratch, int64_t scr_group, hipStream_t st) {
+  if (nblk == 0 || nw == 0) return hipSuccess;
+  hipLaunchKernelGGL(dyson_kernel, dim3(nblk * nw * ns), dim3(NT), 0, st, blks, nblk, nw, ns, Sw, Gw, d_off, kslice,
+                     f_off, F, S, flat_ao, iwn, mu, sigma_scale, scratch, scr_group);
+  return hipGetLastError();
+}
+
+hipError_t launch_density(const SolveBlk *blks, int nblk, int ns, const cplx<double> *Gbeta, const cplx<double> *S,
+                          const int64_t *f_off, int64_t flat_ao, const double *kw, double *out, hipStream_t st) {
+  hipLaunchKernelGGL(density_kernel, dim3(1), dim3(NT), 0, st, blks, nblk, ns, Gbeta, S, f_off, flat_ao, kw, out);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_genv(const GenBlk *gb, int ngb, uint64_t seed, double scale, const double *decay, int n_decay,
+                       cplx<T> *V, hipStream_t st) {
+  if (ngb == 0) return hipSuccess;
+  hipLaunchKernelGGL((genv_kernel<T>), dim3(ngb), dim3(NT), 0, st, gb, ngb, seed, scale, decay, n_decay, V);
+  return hipGetLastError();
+}
+
+template <typename TO>
+hipError_t launch_convert(const cplx<double> *in, cplx<TO> *out, int64_t n, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  int64_t blocks = std::min<int64_t>((n + NT - 1) / NT, 65536);
+  hipLaunchKernelGGL((convert_kernel<TO>), dim3((unsigned)blocks), dim3(NT), 0, st, in, out, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_reorder(const cplx<double> *in, cplx<double> *out, const int64_t *koff, const int64_t *ksize,
+                          int ink, int nts_ns, int64_t max_ksize, int64_t flat_ao, int dir, double scale,
+                          hipStream_t st) {
+  dim3 grid((unsigned)std::min<int64_t>((max_ksize + NT - 1) / NT, 1024), ink, nts_ns);
+  hipLaunchKernelGGL(reorder_kernel, grid, dim3(NT), 0, st, in, out, koff, ksize, ink, nts_ns, flat_ao, dir, scale);
+  return hipGetLastError();
+}
+
+template hipError_t launch_p0<double>(const P0Item *, const P0Blk *, int, int, int, const cplx<double> *,
+                                      const cplx<double> *, cplx<double> *, double, size_t, hipStream_t);
+template hipError_t launch_p0<float>(const P0Item *, const P0Blk *, int, int, int, const cplx<float> *,
+                                     const cplx<float> *, cplx<double> *, double, size_t, hipStream_t);
+template hipError_t launch_sigma<double>(const SigItem *, const SigBlk *, int, int, int, const cplx<double> *,
+                                         const cplx<double> *, const cplx<double> *, cplx<double> *, double, size_t,
+                                         hipStream_t);
+template hipError_t launch_sigma<float>(const SigItem *, const SigBlk *, int, int, int, const cplx<float> *,
+                                        const cplx<float> *, const cplx<float> *, cplx<double> *, double, size_t,
+                                        hipStream_t);
+template hipError_t launch_rotate<double>(const RotBlk *, int, int, const cplx<double> *, const int64_t *, const int64_t *,
+                                          const cplx<double> *, cplx<double> *, const int64_t *, const int64_t *,
+                                          hipStream_t);
+template hipError_t launch_rotate<float>(const RotBlk *, int, int, const cplx<double> *, const int64_t *, const int64_t *,
+                                         const cplx<double> *, cplx<float> *, const int64_t *, const int64_t *,
+                                         hipStream_t);
+template hipError_t launch_genv<double>(const GenBlk *, int, uint64_t, double, const double *, int, cplx<double> *,
+                                        hipStream_t);
+template hipError_t launch_genv<float>(const GenBlk *, int, uint64_t, double, const double *, int, cplx<float> *,
+                                       hipStream_t);
+template hipError_t launch_convert<float>(const cplx<double> *, cplx<float> *, int64_t, hipStream_t);
+
+}  // namespace symgw

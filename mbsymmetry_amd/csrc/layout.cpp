@@ -1,0 +1,396 @@
+// Derived block tables and device work lists.
+//
+// Table derivation follows symmetry_utils_t (SymmetryMBPT/src/common/
+// symmetry_utils.cpp:96-191: cumulative b*b flat offsets per k, iBZ slices in
+// irre_list order) and the kpair slice tables of df_integral_block_t
+// (SymmetryMBPT/include/common/df_integral_block_t.h:84-106).
+#include "layout.h"
+
+#include <algorithm>
+#include <sstream>
+
+namespace symgw {
+
+namespace {
+std::string fmt(const char *what, long a, long b = -1) {
+  std::ostringstream o;
+  o << what << " (" << a;
+  if (b >= 0) o << ", " << b;
+  o << ")";
+  return o.str();
+}
+}  // namespace
+
+bool build_layout(const symgw_layout_desc *d, Layout &L, std::string &err) {
+  if (!d) { err = "null layout"; return false; }
+  L = Layout();
+  L.nk = d->nk; L.ink = d->ink; L.ns = d->ns; L.nts = d->nts; L.nw = d->nw; L.nw_b = d->nw_b;
+  const int nk = L.nk, ink = L.ink;
+  if (nk <= 0 || ink <= 0 || ink > nk) { err = fmt("bad nk/ink", nk, ink); return false; }
+  if (L.ns != 1 && L.ns != 2) { err = fmt("ns must be 1 or 2", L.ns); return false; }
+  // odd nts is rejected by the reference (cugw_solver_block.h:39-40)
+  if (L.nts < 4 || (L.nts % 2) != 0) { err = fmt("nts must be even and >= 4", L.nts); return false; }
+  if (L.nw < 0 || L.nw_b < 0) { err = "negative nw"; return false; }
+  if (!d->kao_nblocks || !d->ao_sizes || !d->qaux_nblocks || !d->aux_sizes || !d->irre_list ||
+      !d->index || !d->irre_index || !d->conj_index || !d->kpq || !d->kmq || !d->kpair_nblocks) {
+    err = "missing layout array"; return false;
+  }
+  L.ao_sizes.resize(nk); L.aux_sizes.resize(nk);
+  L.ao_offsets.resize(nk); L.aux_offsets.resize(nk);
+  L.ao_block_offsets.resize(nk); L.aux_block_offsets.resize(nk);
+  L.kao_off.resize(nk); L.kao_size.resize(nk); L.qaux_off.resize(nk); L.qaux_size.resize(nk);
+  int64_t pa = 0, px = 0, ka = 0, kx = 0;
+  for (int k = 0; k < nk; ++k) {
+    int nb = d->kao_nblocks[k], nx = d->qaux_nblocks[k];
+    if (nb <= 0 || nx <= 0) { err = fmt("empty block list at k", k); return false; }
+    int64_t off = 0; int boff = 0;
+    for (int b = 0; b < nb; ++b) {
+      int s = d->ao_sizes[pa + b];
+      if (s <= 0) { err = fmt("non-positive ao block size at k", k, b); return false; }
+      L.ao_sizes[k].push_back(s); L.ao_offsets[k].push_back(off); L.ao_block_offsets[k].push_back(boff);
+      off += (int64_t)s * s; boff += s;
+      L.max_ao_block = std::max(L.max_ao_block, s);
+    }
+    pa += nb;
+    L.kao_off[k] = ka; L.kao_size[k] = off; ka += off;
+    off = 0; boff = 0;
+    for (int b = 0; b < nx; ++b) {
+      int s = d->aux_sizes[px + b];
+      if (s <= 0) { err = fmt("non-positive aux block size at q", k, b); return false; }
+      L.aux_sizes[k].push_back(s); L.aux_offsets[k].push_back(off); L.aux_block_offsets[k].push_back(boff);
+      off += (int64_t)s * s; boff += s;
+      L.max_aux_block = std::max(L.max_aux_block, s);
+    }
+    px += nx;
+    L.qaux_off[k] = kx; L.qaux_size[k] = off; kx += off;
+  }
+  L.flat_ao_full = ka; L.flat_aux_full = kx;
+
+  L.irre_list.assign(d->irre_list, d->irre_list + ink);
+  L.index.assign(d->index, d->index + nk);
+  L.irre_index.assign(d->irre_index, d->irre_index + nk);
+  L.conj_index.assign(d->conj_index, d->conj_index + nk);
+  L.kpq.assign(d->kpq, d->kpq + (size_t)nk * nk);
+  L.kmq.assign(d->kmq, d->kmq + (size_t)nk * nk);
+  for (int i = 0; i < ink; ++i)
+    if (L.irre_list[i] < 0 || L.irre_list[i] >= nk) { err = fmt("irre_list out of range", i); return false; }
+  for (int k = 0; k < nk; ++k) {
+    if (L.index[k] < 0 || L.index[k] >= nk || L.irre_index[k] < 0 || L.irre_index[k] >= ink ||
+        L.conj_index[k] < 0 || L.conj_index[k] >= nk) {
+      err = fmt("symmetry map out of range at k", k); return false;
+    }
+    // time-reversal partners share their block structure (symmetry_utils.cpp:122-136)
+    int ck = L.conj_index[k];
+    if (L.ao_sizes[k] != L.ao_sizes[ck] || L.aux_sizes[k] != L.aux_sizes[ck]) {
+      err = fmt("k and its conjugate partner differ in blocks", k, ck); return false;
+    }
+    for (int q = 0; q < nk; ++q) {
+      int a = L.kpq[(size_t)k * nk + q], b = L.kmq[(size_t)k * nk + q];
+      if (a < 0 || a >= nk || b < 0 || b >= nk) { err = fmt("kpq/kmq out of range", k, q); return false; }
+    }
+  }
+  // iBZ slices in irre_list order (symmetry_utils.cpp:174-191)
+  L.kao_off_irre.resize(ink); L.kao_size_irre.resize(ink);
+  L.qaux_off_irre.resize(ink); L.qaux_size_irre.resize(ink);
+  int64_t oa = 0, ox = 0;
+  for (int i = 0; i < ink; ++i) {
+    int k = L.irre_list[i];
+    L.kao_off_irre[i] = oa; L.kao_size_irre[i] = L.kao_size[k]; oa += L.kao_size[k];
+    L.qaux_off_irre[i] = ox; L.qaux_size_irre[i] = L.qaux_size[k]; ox += L.qaux_size[k];
+  }
+  L.flat_ao = oa; L.flat_aux = ox;
+
+  // rotation blocks
+  L.orep_irreps.resize(nk); L.auxrep_irreps.resize(nk);
+  L.orep_offsets.resize(nk); L.auxrep_offsets.resize(nk);
+  L.korep_off.resize(nk); L.korep_size.resize(nk); L.kauxrep_off.resize(nk); L.kauxrep_size.resize(nk);
+  int64_t po = 0, pp = 0, ro = 0, rx = 0;
+  for (int k = 0; k < nk; ++k) {
+    int nbo = d->orep_nblocks ? d->orep_nblocks[k] : 0;
+    int nbx = d->auxrep_nblocks ? d->auxrep_nblocks[k] : 0;
+    int ik = L.irre_index[k];
+    int kr = L.irre_list[ik];
+    int64_t off = 0;
+    for (int b = 0; b < nbo; ++b) {
+      int a1 = d->orep_irreps[2 * (po + b)], a2 = d->orep_irreps[2 * (po + b) + 1];
+      if (a1 < 0 || a1 >= (int)L.ao_sizes[k].size() || a2 < 0 || a2 >= (int)L.ao_sizes[kr].size()) {
+        err = fmt("orep block index out of range at k", k, b); return false;
+      }
+      L.orep_irreps[k].push_back({a1, a2}); L.orep_offsets[k].push_back(off);
+      off += (int64_t)L.ao_sizes[k][a1] * L.ao_sizes[kr][a2];
+    }
+    po += nbo;
+    L.korep_off[k] = ro; L.korep_size[k] = off; ro += off;
+    off = 0;
+    for (int b = 0; b < nbx; ++b) {
+      int a1 = d->auxrep_irreps[2 * (pp + b)], a2 = d->auxrep_irreps[2 * (pp + b) + 1];
+      if (a1 < 0 || a1 >= (int)L.aux_sizes[k].size() || a2 < 0 || a2 >= (int)L.aux_sizes[kr].size()) {
+        err = fmt("auxrep block index out of range at k", k, b); return false;
+      }
+      L.auxrep_irreps[k].push_back({a1, a2}); L.auxrep_offsets[k].push_back(off);
+      off += (int64_t)L.aux_sizes[k][a1] * L.aux_sizes[kr][a2];
+    }
+    pp += nbx;
+    L.kauxrep_off[k] = rx; L.kauxrep_size[k] = off; rx += off;
+  }
+  L.flat_orep = ro; L.flat_auxrep = rx;
+  // every k that is not a star representative must have rotation blocks
+  for (int k = 0; k < nk; ++k) {
+    int ck = L.conj_index[k];
+    if (L.index[ck] != ck) {
+      if (L.orep_irreps[ck].empty() || L.auxrep_irreps[ck].empty()) {
+        err = fmt("k needs rotation blocks but has none", ck); return false;
+      }
+    } else if (L.irre_list[L.irre_index[ck]] != ck) {
+      err = fmt("star representative not at its iBZ slot", ck); return false;
+    }
+  }
+
+  // V pairs (symmetry_utils.cpp:141-158)
+  const size_t np = (size_t)nk * nk;
+  L.kpair_off.assign(np, -1); L.kpair_size.assign(np, 0);
+  L.V_irreps.resize(np); L.V_offsets.resize(np); L.V_qsize.resize(np);
+  int64_t vo = 0, pv = 0;
+  for (size_t p = 0; p < np; ++p) {
+    int nb = d->kpair_nblocks[p];
+    if (nb < 0) { err = fmt("negative kpair_nblocks", (long)p); return false; }
+    if (nb == 0) continue;
+    int k1 = (int)(p / nk), k2 = (int)(p % nk);
+    int64_t off = 0;
+    for (int b = 0; b < nb; ++b) {
+      int iq = d->V_irreps[3 * (pv + b)], i1 = d->V_irreps[3 * (pv + b) + 1], i2 = d->V_irreps[3 * (pv + b) + 2];
+      if (i1 < 0 || i1 >= (int)L.ao_sizes[k1].size() || i2 < 0 || i2 >= (int)L.ao_sizes[k2].size() || iq < 0) {
+        err = fmt("V block index out of range at pair", (long)p, b); return false;
+      }
+      L.V_irreps[p].push_back({iq, i1, i2}); L.V_offsets[p].push_back(off);
+      // q size is checked when the pair is bound to a momentum transfer
+      off += (int64_t)L.ao_sizes[k1][i1] * L.ao_sizes[k2][i2];  // times q below
+    }
+    pv += nb;
+    L.kpair_off[p] = 0;  // filled after q sizes are known
+    L.kpair_size[p] = off;
+  }
+  // A V block's aux size is that of the pair's momentum transfer.  The P0
+  // sweep uses the pair (k, k+q) with aux blocks of q (cugw_qkpt_block.cu:108)
+  // and the Sigma sweep (k, k-q) with aux blocks of q (cugw_qkpt_block.cu:170);
+  // q(k1,k2) is found from the kpq table.
+  std::vector<int> pair_q(np, -1);
+  for (int k1 = 0; k1 < nk; ++k1)
+    for (int q = 0; q < nk; ++q) {
+      int k2 = L.kpq[(size_t)k1 * nk + q];
+      size_t p = (size_t)k1 * nk + k2;
+      if (pair_q[p] < 0) pair_q[p] = q;
+    }
+  for (size_t p = 0; p < np; ++p) {
+    if (L.V_irreps[p].empty()) continue;
+    int q = pair_q[p];
+    if (q < 0) { err = fmt("stored pair has no momentum transfer", (long)p); return false; }
+    int k1 = (int)(p / nk), k2 = (int)(p % nk);
+    int64_t off = 0;
+    for (size_t b = 0; b < L.V_irreps[p].size(); ++b) {
+      auto &t = L.V_irreps[p][b];
+      if (t[0] >= (int)L.aux_sizes[q].size()) { err = fmt("V aux block out of range at pair", (long)p, (long)b); return false; }
+      // the Sigma sweep reads the same pair with aux blocks of k1 - k2; they must agree
+      int qm = -1;
+      for (int qq = 0; qq < nk; ++qq) if (L.kmq[(size_t)k1 * nk + qq] == k2) { qm = qq; break; }
+      if (qm >= 0 && L.aux_sizes[qm] != L.aux_sizes[q]) {
+        err = fmt("aux blocks of q and -q differ for pair", (long)p); return false;
+      }
+      L.V_offsets[p][b] = off;
+      L.V_qsize[p].push_back(L.aux_sizes[q][t[0]]);
+      off += (int64_t)L.aux_sizes[q][t[0]] * L.ao_sizes[k1][t[1]] * L.ao_sizes[k2][t[2]];
+    }
+    L.kpair_off[p] = vo; L.kpair_size[p] = off; vo += off;
+  }
+  L.flat_V = vo;
+  return true;
+}
+
+void flop_model(const Layout &L, double &p0, double &sig, int64_t &nb_p0, int64_t &nb_sig) {
+  const int nk = L.nk;
+  p0 = 0; sig = 0; nb_p0 = 0; nb_sig = 0;
+  for (int iq = 0; iq < L.ink; ++iq) {
+    int q = L.irre_list[iq];
+    for (int k = 0; k < nk; ++k) {
+      int k2 = L.kpq[(size_t)k * nk + q];
+      size_t p = (size_t)k * nk + k2;
+      for (auto &t : L.V_irreps[p]) {
+        double qs = L.aux_sizes[q][t[0]], a = L.ao_sizes[k][t[1]], b = L.ao_sizes[k2][t[2]];
+        p0 += L.ns * (L.nts / 2) * 8.0 * (b * qs * a * a + qs * a * b * b + qs * qs * a * b);
+        nb_p0++;
+      }
+    }
+  }
+  for (int ik = 0; ik < L.ink; ++ik) {
+    int k = L.irre_list[ik];
+    for (int q = 0; q < nk; ++q) {
+      int k2 = L.kmq[(size_t)k * nk + q];
+      size_t p = (size_t)k * nk + k2;
+      for (auto &t : L.V_irreps[p]) {
+        double qs = L.aux_sizes[q][t[0]], a = L.ao_sizes[k][t[1]], b = L.ao_sizes[k2][t[2]];
+        sig += L.ns * L.nts * 8.0 * (qs * a * b * b + a * b * qs * qs + a * a * b * qs);
+        nb_sig++;
+      }
+    }
+  }
+}
+
+// LDS planning constants shared with the kernels (kernels.hip)
+static const int kThreads = 256;
+static const int kP0MaxOut = 4 * kThreads;     // outputs per P0 item
+static const int kSigMaxOut = 4 * kThreads;    // outputs per Sigma item
+static const int kXT = 8;                      // K tile of the staged conj(V) rows
+static const size_t kLdsBudget = 150 * 1024;
+
+bool build_worklists(const Layout &L, int eb, WorkLists &W, std::string &err) {
+  W = WorkLists();
+  const int nk = L.nk, nts = L.nts, ns = L.ns;
+  flop_model(L, W.flop_p0, W.flop_sig, W.n_p0_contractions, W.n_sig_contractions);
+
+  // ---- P0: items per (q_ir, iq, row chunk); blocks in (k, V-block) order ----
+  size_t p0_lds = 0;
+  for (int iqr = 0; iqr < L.ink; ++iqr) {
+    int q = L.irre_list[iqr];
+    for (size_t iq = 0; iq < L.aux_sizes[q].size(); ++iq) {
+      int qs = L.aux_sizes[q][iq];
+      int b0 = (int)W.p0_blks.size();
+      int kmax = 1;
+      for (int k = 0; k < nk; ++k) {
+        int k2 = L.kpq[(size_t)k * nk + q];
+        size_t p = (size_t)k * nk + k2;
+        for (size_t b = 0; b < L.V_irreps[p].size(); ++b) {
+          auto &t = L.V_irreps[p][b];
+          if (t[0] != (int)iq) continue;
+          P0Blk B;
+          B.v_off = L.kpair_off[p] + L.V_offsets[p][b];
+          B.k1 = L.ao_sizes[k][t[1]];
+          B.k2 = L.ao_sizes[k2][t[2]];
+          B.g1_base = L.kao_off[k] * ns * nts + L.ao_offsets[k][t[1]];
+          B.g2_base = L.kao_off[k2] * ns * nts + L.ao_offsets[k2][t[2]];
+          B.g1_tstride = (int32_t)(ns * L.kao_size[k]); B.g1_sstride = (int32_t)L.kao_size[k];
+          B.g2_tstride = (int32_t)(ns * L.kao_size[k2]); B.g2_sstride = (int32_t)L.kao_size[k2];
+          kmax = std::max(kmax, B.k1 * B.k2);
+          W.p0_blks.push_back(B);
+        }
+      }
+      int b1 = (int)W.p0_blks.size();
+      // rows per item: outputs <= kP0MaxOut and 2*Qc*kmax + XT*q elements in LDS
+      int qc = std::max(1, kP0MaxOut / qs);
+      while (qc > 1 && (size_t)(2 * qc * kmax + kXT * qs) * eb > kLdsBudget) --qc;
+      if ((size_t)(2 * kmax + kXT * qs) * eb > kLdsBudget) {
+        err = fmt("P0 block too large for the LDS plan (q, k1*k2)", qs, kmax); return false;
+      }
+      qc = std::min(qc, qs);
+      for (int Q0 = 0; Q0 < qs; Q0 += qc) {
+        P0Item it;
+        it.blk_begin = b0; it.blk_end = b1;
+        it.q = qs; it.Q0 = Q0; it.Qc = std::min(qc, qs - Q0); it.kmax = kmax;
+        it.out_base = (int64_t)nts * L.qaux_off_irre[iqr] + L.aux_offsets[q][iq] + (int64_t)Q0 * qs;
+        it.out_tstride = (int32_t)L.qaux_size_irre[iqr];
+        it.pad = 0;
+        W.p0_items.push_back(it);
+        p0_lds = std::max(p0_lds, (size_t)(2 * it.Qc * kmax + kXT * qs) * eb);
+      }
+    }
+  }
+  W.p0_lds = p0_lds;
+
+  // ---- Sigma: items per (k_ir, ik1, row chunk); blocks in (q, V-block) order ----
+  size_t sig_lds = 0;
+  for (int ikr = 0; ikr < L.ink; ++ikr) {
+    int k = L.irre_list[ikr];
+    for (size_t i1 = 0; i1 < L.ao_sizes[k].size(); ++i1) {
+      int k1 = L.ao_sizes[k][i1];
+      int b0 = (int)W.sig_blks.size();
+      size_t need_max = 0;
+      int ic = std::max(1, std::min(k1, kSigMaxOut / k1));
+      for (int q = 0; q < nk; ++q) {
+        int k2i = L.kmq[(size_t)k * nk + q];
+        size_t p = (size_t)k * nk + k2i;
+        for (size_t b = 0; b < L.V_irreps[p].size(); ++b) {
+          auto &t = L.V_irreps[p][b];
+          if (t[1] != (int)i1) continue;
+          SigBlk B;
+          B.v_off = L.kpair_off[p] + L.V_offsets[p][b];
+          B.q = L.aux_sizes[q][t[0]];
+          B.k2 = L.ao_sizes[k2i][t[2]];
+          B.g2_base = L.kao_off[k2i] * ns * nts + L.ao_offsets[k2i][t[2]];
+          B.g2_tstride = (int32_t)(ns * L.kao_size[k2i]); B.g2_sstride = (int32_t)L.kao_size[k2i];
+          B.p_base = L.qaux_off[q] * nts + L.aux_offsets[q][t[0]];
+          B.p_tstride = (int32_t)L.qaux_size[q];
+          int K = k1 * B.k2;
+          // Q rows per chunk: Z (qc*K) + E (qc*Ic*k2) + conj(P) rows (qc*q) in LDS,
+          // and qc * ceil(K/256) register accumulators per thread <= 16
+          int qc = std::max(1, std::min(B.q, 16 / std::max(1, (K + kThreads - 1) / kThreads)));
+          auto need = [&](int c) { return (size_t)(c * K + c * ic * B.k2 + c * B.q) * eb; };
+          while (qc > 1 && need(qc) > kLdsBudget) --qc;
+          if (need(1) > kLdsBudget) { err = fmt("Sigma block too large for the LDS plan (q, k1*k2)", B.q, K); return false; }
+          B.qc = qc;
+          need_max = std::max(need_max, need(qc));
+          W.sig_blks.push_back(B);
+        }
+      }
+      int b1 = (int)W.sig_blks.size();
+      for (int I0 = 0; I0 < k1; I0 += ic) {
+        SigItem it;
+        it.blk_begin = b0; it.blk_end = b1;
+        it.k1 = k1; it.I0 = I0; it.Ic = std::min(ic, k1 - I0); it.pad = 0;
+        it.out_base = L.kao_off_irre[ikr] * ns * nts + L.ao_offsets[k][i1] + (int64_t)I0 * k1;
+        it.out_tstride = (int32_t)(ns * L.kao_size_irre[ikr]);
+        it.out_sstride = (int32_t)L.kao_size_irre[ikr];
+        W.sig_items.push_back(it);
+      }
+      sig_lds = std::max(sig_lds, need_max);
+    }
+  }
+  W.sig_lds = sig_lds;
+
+  // ---- rotations of G (ao) and P (aux) to the full BZ ----
+  for (int which = 0; which < 2; ++which) {
+    auto &out = which == 0 ? W.g_rot : W.p_rot;
+    for (int k = 0; k < nk; ++k) {
+      int ck = L.conj_index[k];
+      int ick = L.irre_index[ck];
+      int kr = L.irre_list[ick];
+      const auto &sz = which == 0 ? L.ao_sizes[ck] : L.aux_sizes[ck];
+      const auto &so = which == 0 ? L.ao_offsets[ck] : L.aux_offsets[ck];
+      const auto &szr = which == 0 ? L.ao_sizes[kr] : L.aux_sizes[kr];
+      const auto &sor = which == 0 ? L.ao_offsets[kr] : L.aux_offsets[kr];
+      if (L.index[ck] == ck) {
+        for (size_t b = 0; b < sz.size(); ++b) {
+          RotBlk R{so[b], sor[b], -1, sz[b], szr[b], k, k != ck ? 1 : 0};
+          out.push_back(R);
+        }
+      } else {
+        const auto &irr = which == 0 ? L.orep_irreps[ck] : L.auxrep_irreps[ck];
+        const auto &off = which == 0 ? L.orep_offsets[ck] : L.auxrep_offsets[ck];
+        int64_t base = which == 0 ? L.korep_off[ck] : L.kauxrep_off[ck];
+        for (size_t b = 0; b < irr.size(); ++b) {
+          int a1 = irr[b][0], a2 = irr[b][1];
+          RotBlk R{so[a1], sor[a2], base + off[b], sz[a1], szr[a2], k, k != ck ? 1 : 0};
+          out.push_back(R);
+        }
+      }
+    }
+  }
+  // ---- W and Dyson solve blocks ----
+  for (int iqr = 0; iqr < L.ink; ++iqr) {
+    int q = L.irre_list[iqr];
+    for (size_t b = 0; b < L.aux_sizes[q].size(); ++b) {
+      int n = L.aux_sizes[q][b];
+      W.w_blocks.push_back(SolveBlk{L.aux_offsets[q][b], W.w_scr_group, n, iqr});
+      W.w_scr_group += 2 * (int64_t)n * n;
+    }
+  }
+  for (int ikr = 0; ikr < L.ink; ++ikr) {
+    int k = L.irre_list[ikr];
+    for (size_t b = 0; b < L.ao_sizes[k].size(); ++b) {
+      int n = L.ao_sizes[k][b];
+      W.d_blocks.push_back(SolveBlk{L.ao_offsets[k][b], W.d_scr_group, n, ikr});
+      W.d_scr_group += (int64_t)n * n;
+    }
+  }
+  return true;
+}
+
+}  // namespace symgw

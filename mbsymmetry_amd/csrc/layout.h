@@ -1,0 +1,131 @@
+// Host-side block-sparse layout of one GW system: the derived tables of
+// symmetry_utils_t (SymmetryMBPT/src/common/symmetry_utils.cpp:96-191) and the
+// per-contraction work lists that the device kernels walk.
+#pragma once
+
+#include <array>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/symgw.h"
+
+namespace symgw {
+
+struct Layout {
+  int nk = 0, ink = 0, ns = 0, nts = 0, nw = 0, nw_b = 0;
+
+  // full-BZ irrep blocks: sizes, flat offsets (cumulative b*b), offsets of the
+  // block inside the dense nao / naux index range (cumulative b)
+  std::vector<std::vector<int>> ao_sizes, aux_sizes;
+  std::vector<std::vector<int64_t>> ao_offsets, aux_offsets;
+  std::vector<std::vector<int>> ao_block_offsets, aux_block_offsets;
+  std::vector<int64_t> kao_off, kao_size, qaux_off, qaux_size;          // full BZ
+  std::vector<int64_t> kao_off_irre, kao_size_irre, qaux_off_irre, qaux_size_irre;
+
+  std::vector<int> irre_list, index, irre_index, conj_index, kpq, kmq;
+
+  // rotation blocks per full-BZ k (used through the TR partner ck)
+  std::vector<std::vector<std::array<int, 2>>> orep_irreps, auxrep_irreps;
+  std::vector<std::vector<int64_t>> orep_offsets, auxrep_offsets;
+  std::vector<int64_t> korep_off, korep_size, kauxrep_off, kauxrep_size;
+
+  // interaction tensor tables per ordered pair p = k1*nk + k2
+  std::vector<int64_t> kpair_off, kpair_size;           // -1 / 0 when absent
+  std::vector<std::vector<std::array<int, 3>>> V_irreps;
+  std::vector<std::vector<int64_t>> V_offsets;
+  std::vector<std::vector<int>> V_qsize;                // aux size of each block
+
+  int64_t flat_ao = 0, flat_ao_full = 0, flat_aux = 0, flat_aux_full = 0;
+  int64_t flat_V = 0, flat_orep = 0, flat_auxrep = 0;
+  int max_ao_block = 0, max_aux_block = 0;
+};
+
+// Build and validate; returns false with a message on inconsistent input.
+bool build_layout(const symgw_layout_desc *d, Layout &L, std::string &err);
+
+// ---- device work lists ------------------------------------------------------
+// One V-block contraction that feeds a P0 output slab (cugw_qkpt_block.cu:219-261).
+struct P0Blk {
+  int64_t v_off;       // V block start (complex elements)
+  int64_t g1_base;     // G_full offset of block ik1 of k      (t = 0, s = 0)
+  int64_t g2_base;     // G_full offset of block ik2 of k + q  (t = 0, s = 0)
+  int32_t g1_tstride;  // ns * kslice(k)
+  int32_t g1_sstride;  // kslice(k)
+  int32_t g2_tstride;
+  int32_t g2_sstride;
+  int32_t k1, k2;
+};
+// Rows [Q0, Q0 + Qc) of aux block iq of iBZ point q_ir, summed over all k.
+struct P0Item {
+  int32_t blk_begin, blk_end;
+  int32_t q, Q0, Qc, kmax;   // kmax = max k1*k2 over the item's blocks
+  int64_t out_base;          // q_t_QP offset of row Q0 at t = 0
+  int32_t out_tstride;       // qslice(q_ir)
+  int32_t pad;
+};
+
+// One V-block contraction that feeds a Sigma output slab
+// (cugw_qkpt_block.cu:369-407).
+struct SigBlk {
+  int64_t v_off;
+  int64_t g2_base;     // G_full offset of block ik2 of k - q (t = 0, s = 0)
+  int64_t p_base;      // P_full offset of aux block iq of q (t = 0)
+  int32_t g2_tstride, g2_sstride;
+  int32_t p_tstride;   // qslice(q)
+  int32_t q, k2;
+  int32_t qc;          // Q rows per chunk (LDS plan)
+};
+// Rows [I0, I0 + Ic) of ao block ik1 of iBZ point k_ir, summed over all q.
+struct SigItem {
+  int32_t blk_begin, blk_end;
+  int32_t k1, I0, Ic, pad;
+  int64_t out_base;          // k_ts_ij (iBZ) offset of row I0 at t = 0, s = 0
+  int32_t out_tstride;       // ns * kslice(k)
+  int32_t out_sstride;       // kslice(k)
+};
+
+// Rotation of one full-BZ k (G) or q (P) slice from its iBZ representative:
+// block ia1 of the target = O * X(block ia2 of the source) * O^H, or a copy
+// (cugw_routines_block.cu:134-186, cugw_qkpt_block.cu:288-340).
+struct RotBlk {
+  int64_t dst_off;     // target block offset inside the target slice
+  int64_t src_off;     // source block offset inside the source slice
+  int64_t rep_off;     // rotation block offset in the flat rep array, -1 => copy
+  int32_t m, n;        // target block m x m, source block n x n (O is m x n)
+  int32_t target;      // full-BZ index of the target
+  int32_t conj;        // 1 => complex-conjugate the result (k != conj_index[k])
+};
+
+// Contiguous aux-block of one W solve or ao-block of one Dyson solve.
+struct SolveBlk {
+  int64_t off;         // flat offset inside the slice
+  int64_t scr;         // offset of this block's scratch inside one frequency group
+  int32_t n;           // block size
+  int32_t slice;       // index of the k (Dyson) or q (W) slice
+};
+
+struct WorkLists {
+  std::vector<P0Blk> p0_blks;
+  std::vector<P0Item> p0_items;
+  std::vector<SigBlk> sig_blks;
+  std::vector<SigItem> sig_items;
+  std::vector<RotBlk> g_rot;      // targets over the full BZ (ao)
+  std::vector<RotBlk> p_rot;      // targets over the full BZ (aux)
+  std::vector<SolveBlk> w_blocks; // aux blocks of every iBZ q
+  std::vector<SolveBlk> d_blocks; // ao blocks of every iBZ k
+  int64_t w_scr_group = 0;        // scratch elements per bosonic frequency (2 n^2 per block)
+  int64_t d_scr_group = 0;        // scratch elements per (w, s) (n^2 per block)
+  int64_t n_p0_contractions = 0, n_sig_contractions = 0;
+  double flop_p0 = 0, flop_sig = 0;
+  size_t p0_lds = 0, sig_lds = 0; // dynamic LDS bytes of the contraction kernels
+};
+
+// elem_bytes: bytes of one complex element in the contraction precision.
+bool build_worklists(const Layout &L, int elem_bytes, WorkLists &W, std::string &err);
+
+// Per-iteration FLOPs of P0 and Sigma, 8*M*N*K per complex GEMM
+// (analysis.cpp:229-255 and :271-298 without the +12MN epilogue).
+void flop_model(const Layout &L, double &p0, double &sig, int64_t &nb_p0, int64_t &nb_sig);
+
+}  // namespace symgw

@@ -1,0 +1,192 @@
+"""GPU parity of the HIP path against the FP64 CPU oracle, through the C ABI.
+
+Tolerances follow the reference's own tests: P0 and Sigma 1e-10
+(test_cugw.cpp:98, :203), P (W) 1e-9 (:301), Dyson 1e-10 (test_dyson.cpp:82).
+"""
+import numpy as np
+import pytest
+
+from mbsymmetry_amd import _lib
+from mbsymmetry_amd import system as S
+from oracle.oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print("[case]", *a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def systems(lih, config1):
+    return {
+        "tiny_rot_ns2": S.tiny(ns=2, n=2, ink=4, ao=(3, 2), aux=(4, 3), ni=8, beta=10.0, seed=11),
+        "tiny_3cube": S.tiny(ns=1, n=3, ink=4, ao=(2, 3, 1), aux=(3, 2), ni=6, beta=8.0, seed=5),
+        "config1": config1,
+        "lih": lih,
+    }
+
+
+def _setup(sysm, precision=_lib.PREC_FP64, mu=0.0, device_V=False):
+    orc = Oracle(sysm)
+    ctx = _lib.Context(sysm, 0, precision)
+    if device_V:
+        ctx.generate_V(sysm.v_seed, sysm.v_scale, sysm.decay_table())
+    else:
+        ctx.set_V(sysm.host_V())
+    ctx.set_reps(sysm.orep, sysm.auxrep)
+    ctx.set_transforms(sysm.transforms)
+    ctx.set_fock(sysm.F, sysm.S)
+    G0 = orc.dyson(sysm.zeros_ts_kij(), mu)
+    ctx.upload_G(G0)
+    return orc, ctx, G0
+
+
+def _rel(a, b):
+    return np.abs(a - b).max() / max(1.0, np.abs(b).max())
+
+
+@pytest.mark.parametrize("name", ["tiny_rot_ns2", "tiny_3cube", "config1", "lih"])
+def test_P0_parity(name, systems):
+    sysm = systems[name]
+    _say("P0", name)
+    orc, ctx, G0 = _setup(sysm)
+    ctx.solve_P0()
+    got = ctx.copy_P0_to_host(sysm.zeros_q_t_QP())
+    ref = orc.P0(sysm.host_V(), orc.rotate_G(G0))
+    nts, h = sysm.nts, sysm.nts // 2
+    got_s = got / sysm.nk
+    qoff = sysm.qaux_off_irre()
+    for iqr, q in enumerate(sysm.irre_list):
+        qs = sysm.qslice(int(q))
+        base = nts * qoff[iqr]
+        a = got_s[base:base + h * qs]
+        b = ref[base:base + h * qs]
+        assert np.abs(a - b).max() < 1e-10 * max(1.0, np.abs(b).max()), (name, iqr)
+        # rows t >= nts/2 are zero, as in the reference's device buffer
+        assert np.all(got[base + h * qs:base + nts * qs] == 0)
+    ctx.close()
+
+
+@pytest.mark.parametrize("name", ["tiny_rot_ns2", "tiny_3cube", "config1", "lih"])
+def test_W_and_Sigma_parity(name, systems):
+    sysm = systems[name]
+    _say("W+Sigma", name)
+    orc, ctx, G0 = _setup(sysm)
+    V = sysm.host_V()
+    Gf = orc.rotate_G(G0)
+    P0 = orc.P0(V, Gf)
+    Pref = orc.W(P0)
+    ctx.solve_P0()
+    ctx.solve_W()
+    P = ctx.copy_P0_to_host(sysm.zeros_q_t_QP())
+    assert _rel(P, Pref) < 1e-9, name
+    # Sigma from the oracle's P (isolates the Sigma kernel)
+    ctx.upload_P(Pref)
+    ctx.solve_Sigma()
+    Sg = ctx.copy_Sigma_to_host(sysm.zeros_ts_kij()) / sysm.nk
+    Sref = orc.Sigma(V, Gf, orc.rotate_P(Pref))
+    assert _rel(Sg, Sref) < 1e-10, name
+    ctx.close()
+
+
+@pytest.mark.parametrize("name", ["tiny_rot_ns2", "config1", "lih"])
+def test_dyson_parity(name, systems):
+    sysm = systems[name]
+    _say("Dyson", name)
+    orc, ctx, G0 = _setup(sysm, mu=0.05)
+    # Sigma = 0 path (G0 itself)
+    assert _rel(ctx.download_G(G0.shape), G0) < 1e-13
+    ctx.dyson(0.05, 0.0)
+    assert _rel(ctx.download_G(G0.shape), G0) < 1e-10
+    rng = np.random.default_rng(4)
+    Sig = 0.05 * (rng.standard_normal(G0.shape) + 1j * rng.standard_normal(G0.shape))
+    ctx.upload_Sigma(Sig)
+    ctx.dyson(0.05, 1.0)
+    assert _rel(ctx.download_G(G0.shape), orc.dyson(Sig, 0.05)) < 1e-10, name
+    ctx.close()
+
+
+@pytest.mark.parametrize("name", ["tiny_rot_ns2", "config1"])
+def test_gw_iteration_parity_and_graph(name, systems):
+    sysm = systems[name]
+    _say("iteration", name)
+    mu = 0.0
+    orc, ctx, G0 = _setup(sysm, mu=mu)
+    ctx.gw_iteration(mu, use_graph=False)
+    ctx.sync()
+    G1 = ctx.download_G(G0.shape)
+    _, _, _, G1ref = orc.gw_iteration(G0, mu)
+    assert _rel(G1, G1ref) < 1e-10, name
+    # graph replay is bitwise identical to the eager sequence
+    ctx2 = _lib.Context(sysm)
+    ctx2.set_V(sysm.host_V()); ctx2.set_reps(sysm.orep, sysm.auxrep)
+    ctx2.set_transforms(sysm.transforms); ctx2.set_fock(sysm.F, sysm.S); ctx2.upload_G(G0)
+    eager = []
+    for _ in range(3):
+        ctx.gw_iteration(mu, use_graph=False); ctx.sync(); eager.append(ctx.download_G(G0.shape))
+    ctx2.gw_iteration(mu, use_graph=False); ctx2.sync()
+    for i in range(3):
+        ctx2.gw_iteration(mu, use_graph=True); ctx2.sync()
+        assert np.array_equal(ctx2.download_G(G0.shape), eager[i]), (name, i)
+    ctx.close(); ctx2.close()
+
+
+def test_device_generator_matches_host_bits(systems):
+    sysm = systems["config1"]
+    _say("generator", "config1")
+    _, c_host, _ = _setup(sysm)
+    _, c_dev, _ = _setup(sysm, device_V=True)
+    c_host.solve_P0(); c_dev.solve_P0()
+    a = c_host.copy_P0_to_host(sysm.zeros_q_t_QP())
+    b = c_dev.copy_P0_to_host(sysm.zeros_q_t_QP())
+    assert np.array_equal(a, b)
+    c_host.close(); c_dev.close()
+
+
+def test_contractions_are_deterministic(systems):
+    sysm = systems["config1"]
+    _say("determinism", "config1")
+    _, ctx, G0 = _setup(sysm)
+    outs = []
+    for _ in range(2):
+        ctx.solve_P0()
+        outs.append(ctx.copy_P0_to_host(sysm.zeros_q_t_QP()))
+    assert np.array_equal(outs[0], outs[1])
+    ctx.solve_W()
+    s = []
+    for _ in range(2):
+        ctx.solve_Sigma()
+        s.append(ctx.copy_Sigma_to_host(sysm.zeros_ts_kij()))
+    assert np.array_equal(s[0], s[1])
+    ctx.close()
+
+
+def test_fp32_mode_error_is_bounded(systems):
+    sysm = systems["config1"]
+    _say("fp32", "config1")
+    orc, ctx, G0 = _setup(sysm, precision=_lib.PREC_FP32)
+    ctx.solve_P0()
+    got = ctx.copy_P0_to_host(sysm.zeros_q_t_QP()) / sysm.nk
+    ref = orc.P0(sysm.host_V(), orc.rotate_G(G0))
+    h = sysm.nts // 2
+    mask = np.zeros_like(ref, dtype=bool)
+    qoff = sysm.qaux_off_irre()
+    for iqr, q in enumerate(sysm.irre_list):
+        qs = sysm.qslice(int(q))
+        mask[sysm.nts * qoff[iqr]:sysm.nts * qoff[iqr] + h * qs] = True
+    err = np.abs(got[mask] - ref[mask]).max() / np.abs(ref[mask]).max()
+    assert err < 1e-5, err
+    ctx.close()
+
+
+def test_density_and_mu_search(systems):
+    sysm = systems["config1"]
+    _say("density", "config1")
+    orc, ctx, G0 = _setup(sysm, mu=0.1)
+    n_dev = ctx.density(sysm.kweights)
+    assert abs(n_dev - orc.density(G0)) < 1e-11
+    target = n_dev
+    mu, n = ctx.mu_search(target, sysm.kweights, 0.0, -1.0, 1.0, 1e-10)
+    assert abs(n - target) < 1e-9 and abs(mu - 0.1) < 1e-6
+    ctx.close()
