@@ -183,26 +183,13 @@ int run_sigma(symgw_ctx *c) {
   return SYMGW_OK;
 }
 
-// Zero rows t >= nts/2 of every q slice of P (the reference's device P0 is
-// zero there: cugw_routines_block.cu:79 and only t < nts/2 is written).
-__global__ void zero_upper_kernel(cplx<double> *P, const int64_t *qoff, const int64_t *qsize, int nts) {
-  const int iq = blockIdx.y;
-  const int64_t sz = qsize[iq];
-  const int64_t base = (int64_t)nts * qoff[iq] + (int64_t)(nts / 2) * sz;
-  const int64_t n = (int64_t)(nts - nts / 2) * sz;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    P[base + i] = cplx<double>{0.0, 0.0};
-}
-
 int phase_P0(symgw_ctx *c) {
   if (!c->have_V) return fail(c, SYMGW_ERR_STATE, "V not set");
   if (!c->have_G) return fail(c, SYMGW_ERR_STATE, "G not uploaded");
   const Layout &L = c->L;
   int r = c->prec == SYMGW_PREC_FP64 ? run_p0<double>(c) : run_p0<float>(c);
   if (r) return r;
-  hipLaunchKernelGGL(zero_upper_kernel, dim3(64, L.ink), dim3(256), 0, c->st, c->P, c->d_qoff_irre,
-                     c->d_qslice_irre, L.nts);
-  HIPCHK(c, hipGetLastError());
+  (void)L;
   return SYMGW_OK;
 }
 

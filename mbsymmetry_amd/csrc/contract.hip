@@ -1,0 +1,367 @@
+// P0 and Sigma block contractions on the CDNA4 matrix cores.
+//
+// Every workgroup (4 waves) owns one output slab and walks its V-block list in
+// a fixed order ("owner computes"), so results are bitwise reproducible with
+// no atomics or locks (the reference serialises per-q / per-k AXPYs behind
+// spin locks, cugw_qkpt_block.cu:269-285 and :414-436).  All complex products
+// are 16x16 tiles built from four real MFMAs (v_mfma_f64_16x16x4f64 for the
+// plain path, v_mfma_f32_16x16x4f32 for the FP32 path); operands are staged in
+// LDS, intermediates never leave the workgroup.
+#include <hip/hip_runtime.h>
+
+#include "common.h"
+#include "kernels.h"
+
+namespace symgw {
+
+namespace {
+
+constexpr int NT = 256;
+
+template <typename T>
+struct Mfma;
+template <>
+struct Mfma<double> {
+  typedef double v4 __attribute__((ext_vector_type(4)));
+  __device__ static v4 op(double a, double b, v4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+  // result row held by register r of a lane (lane & 15 is the column)
+  __device__ static int row(int lane, int r) { return (lane >> 4) + 4 * r; }
+};
+template <>
+struct Mfma<float> {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  __device__ static v4 op(float a, float b, v4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+  __device__ static int row(int lane, int r) { return 4 * (lane >> 4) + r; }
+};
+
+template <typename T>
+struct Tile {
+  typename Mfma<T>::v4 re, im;
+};
+
+template <typename T>
+__device__ __forceinline__ void tile_zero(Tile<T> &t) {
+  t.re = typename Mfma<T>::v4{T(0), T(0), T(0), T(0)};
+  t.im = t.re;
+}
+
+template <typename T>
+__device__ __forceinline__ cplx<T> cz() { return cplx<T>{T(0), T(0)}; }
+
+// acc (16x16) += A (16 x K) * B (K x 16).  la(i, k) and lb(k, j) return the
+// operand element or zero outside the valid range.  Lane l holds A[l%16][k0+l/16]
+// and B[k0+l/16][l%16] (operand layout of the 16x16x4 MFMAs).
+template <typename T, typename LA, typename LB>
+__device__ __forceinline__ void tile_mma(Tile<T> &acc, int K, const LA &la, const LB &lb) {
+  const int lane = threadIdx.x & 63, i = lane & 15, kk = lane >> 4;
+  for (int k0 = 0; k0 < K; k0 += 4) {
+    const cplx<T> a = la(i, k0 + kk);
+    const cplx<T> b = lb(k0 + kk, i);
+    acc.re = Mfma<T>::op(a.re, b.re, acc.re);
+    acc.im = Mfma<T>::op(a.re, b.im, acc.im);
+    acc.re = Mfma<T>::op(-a.im, b.im, acc.re);
+    acc.im = Mfma<T>::op(a.im, b.re, acc.im);
+  }
+}
+
+// f(row, col, value) for the 4 elements a lane holds.
+template <typename T, typename F>
+__device__ __forceinline__ void tile_store(const Tile<T> &acc, const F &f) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) f(Mfma<T>::row(lane, r), lane & 15, cplx<T>{acc.re[r], acc.im[r]});
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// P0.  Slab = rows [Q0, Q0 + nband) (nband <= 16) of aux block iq of iBZ point
+// q, time t < nts/2.  For every V block (k, ik1, ik2) and spin:
+//   E_Q = V_Q G^{k+q}(t),  C_Q = G^k(beta - t) E_Q,  P0[Q][P] += sum_x C_Q[x] conj(V_P[x])
+// i.e. the three GEMMs of cugw_qkpt_block.cu:231-260 (X1 = G V, X2 = conj(V) G^T,
+// P0^T += X2 X1) regrouped around one intermediate per Q row, processed in row
+// chunks of qsub rows sized for the LDS.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(NT) void p0_mfma_kernel(const P0Item *__restrict__ items, const P0Blk *__restrict__ blks,
+                                                    int64_t n_work, int nt_half, int nts, int ns,
+                                                    const cplx<T> *__restrict__ V, const cplx<T> *__restrict__ G,
+                                                    cplx<double> *__restrict__ P0, double pref) {
+  const int64_t w = xcd_logical(blockIdx.x, gridDim.x);
+  if (w >= n_work) return;
+  const int t = (int)(w % nt_half);
+  const P0Item it = items[w / nt_half];
+  const int tid = threadIdx.x, wave = tid >> 6;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cplx<T> *lds = reinterpret_cast<cplx<T> *>(smem_raw);
+  const int q = it.q, nband = it.nband, nct = (q + 15) / 16;
+  Tile<T> acc[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) tile_zero(acc[c]);
+
+  for (int b = it.blk_begin; b < it.blk_end; ++b) {
+    const P0Blk B = blks[b];
+    const int k1 = B.k1, k2 = B.k2, K = k1 * k2, ldk = B.ldk, qsub = B.qsub;
+    const cplx<T> *Vb = V + B.v_off;
+    cplx<T> *bufV = lds;               // V rows, then C
+    cplx<T> *bufE = lds + qsub * ldk;  // E = V G2
+    for (int s = 0; s < ns; ++s) {
+      const cplx<T> *G1 = G + B.g1_base + (int64_t)(nts - 1 - t) * B.g1_tstride + (int64_t)s * B.g1_sstride;
+      const cplx<T> *G2 = G + B.g2_base + (int64_t)t * B.g2_tstride + (int64_t)s * B.g2_sstride;
+      for (int r0 = 0; r0 < nband; r0 += qsub) {
+        const int nr = min(qsub, nband - r0);
+        const cplx<T> *Vr = Vb + (int64_t)(it.Q0 + r0) * K;
+        __syncthreads();
+        for (int e = tid; e < nr * K; e += NT) {
+          const int r = e / K, x = e - r * K;
+          bufV[r * ldk + x] = Vr[e];
+        }
+        __syncthreads();
+        // E[(r,a)][n] = sum_m V[r][a][m] G2[m][n]
+        {
+          const int M = nr * k1, ntm = (M + 15) / 16, ntn = (k2 + 15) / 16;
+          for (int tt = wave; tt < ntm * ntn; tt += 4) {
+            const int m0 = (tt / ntn) * 16, n0 = (tt % ntn) * 16;
+            Tile<T> e;
+            tile_zero(e);
+            tile_mma(e, k2,
+                     [&](int i, int m) {
+                       const int row = m0 + i;
+                       if (row >= M || m >= k2) return cz<T>();
+                       const int r = row / k1, a = row - r * k1;
+                       return bufV[r * ldk + a * k2 + m];
+                     },
+                     [&](int m, int j) { return (m < k2 && n0 + j < k2) ? G2[m * k2 + n0 + j] : cz<T>(); });
+            tile_store(e, [&](int i, int j, cplx<T> v) {
+              const int row = m0 + i, n = n0 + j;
+              if (row < M && n < k2) {
+                const int r = row / k1, a = row - r * k1;
+                bufE[r * ldk + a * k2 + n] = v;
+              }
+            });
+          }
+        }
+        __syncthreads();
+        // C[r][p][n] = sum_a G1[p][a] E[r][a][n]  (into bufV)
+        {
+          const int N = nr * k2, ntm = (k1 + 15) / 16, ntn = (N + 15) / 16;
+          for (int tt = wave; tt < ntm * ntn; tt += 4) {
+            const int m0 = (tt / ntn) * 16, n0 = (tt % ntn) * 16;
+            Tile<T> c;
+            tile_zero(c);
+            tile_mma(c, k1, [&](int i, int a) { return (m0 + i < k1 && a < k1) ? G1[(m0 + i) * k1 + a] : cz<T>(); },
+                     [&](int a, int j) {
+                       const int col = n0 + j;
+                       if (a >= k1 || col >= N) return cz<T>();
+                       const int r = col / k2, n = col - r * k2;
+                       return bufE[r * ldk + a * k2 + n];
+                     });
+            tile_store(c, [&](int i, int j, cplx<T> v) {
+              const int p = m0 + i, col = n0 + j;
+              if (p < k1 && col < N) {
+                const int r = col / k2, n = col - r * k2;
+                bufV[r * ldk + p * k2 + n] = v;
+              }
+            });
+          }
+        }
+        __syncthreads();
+        // acc[r0 + r][P] += sum_x C[r][x] conj(V[P][x])
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          const int jt = wave + 4 * c;
+          if (jt < nct) {
+            const int p0 = jt * 16;
+            tile_mma(acc[c], K,
+                     [&](int i, int x) {
+                       const int rr = i - r0;
+                       return (rr >= 0 && rr < nr && x < K) ? bufV[rr * ldk + x] : cz<T>();
+                     },
+                     [&](int x, int j) {
+                       if (x >= K || p0 + j >= q) return cz<T>();
+                       const cplx<T> v = Vb[(int64_t)(p0 + j) * K + x];
+                       return cplx<T>{v.re, -v.im};
+                     });
+          }
+        }
+      }
+    }
+  }
+  cplx<double> *out = P0 + it.out_base + (int64_t)t * it.out_tstride;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const int jt = wave + 4 * c;
+    if (jt < nct)
+      tile_store(acc[c], [&](int i, int j, cplx<T> v) {
+        const int P = jt * 16 + j;
+        if (i < nband && P < q) out[i * q + P] = cplx<double>{pref * (double)v.re, pref * (double)v.im};
+      });
+  }
+  // rows t >= nts/2 of the reference's device P0 stay zero (cugw_routines_block.cu:79)
+  cplx<double> *mir = P0 + it.out_base + (int64_t)(nts - 1 - t) * it.out_tstride;
+  for (int e = tid; e < nband * q; e += NT) mir[e] = cplx<double>{0.0, 0.0};
+}
+
+// ---------------------------------------------------------------------------
+// Sigma.  Slab = rows [I0, I0+Ib) x columns [J0, J0+Jb) of ao block ik1 of iBZ
+// point k, time t, spin s.  For every V block (q, iq, ik2), in chunks of qsub
+// aux rows Q:
+//   Z_Q = sum_P conj(P_{QP}(t)) V_P,  E_Q = V_Q G^{k-q}(t),
+//   Sigma[i][j] += sum_{Q,n} E_Q[i][n] conj(Z_Q[j][n])
+// the contraction of cugw_qkpt_block.cu:381-407 (Y1 = V G, Y2 = Y1 P,
+// Sigma += Y2 V^H) with the aux contraction moved onto V so that the largest
+// GEMM (q x q x k1 k2) reads V straight from memory.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(NT) void sigma_mfma_kernel(const SigItem *__restrict__ items,
+                                                       const SigBlk *__restrict__ blks, int64_t n_work, int nts,
+                                                       int ns, const cplx<T> *__restrict__ V,
+                                                       const cplx<T> *__restrict__ G, const cplx<T> *__restrict__ Pf,
+                                                       cplx<double> *__restrict__ Sig, double pref) {
+  const int64_t w = xcd_logical(blockIdx.x, gridDim.x);
+  if (w >= n_work) return;
+  const int per = nts * ns;
+  const SigItem it = items[w / per];
+  const int ts = (int)(w % per), t = ts / ns, s = ts - (ts / ns) * ns;
+  const int tid = threadIdx.x, wave = tid >> 6;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cplx<T> *lds = reinterpret_cast<cplx<T> *>(smem_raw);
+  const int k1 = it.k1, I0 = it.I0, Ib = it.Ib, J0 = it.J0, Jb = it.Jb;
+  const int nti = (Ib + 15) / 16, ntj = (Jb + 15) / 16;
+  Tile<T> acc[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) tile_zero(acc[c]);
+
+  for (int b = it.blk_begin; b < it.blk_end; ++b) {
+    const SigBlk B = blks[b];
+    const int q = B.q, k2 = B.k2, K = k1 * k2, qsub = B.qsub, ldz = B.ldz, lde = B.lde;
+    const cplx<T> *Vb = V + B.v_off;
+    const cplx<T> *G2 = G + B.g2_base + (int64_t)t * B.g2_tstride + (int64_t)s * B.g2_sstride;
+    const cplx<T> *Pb = Pf + B.p_base + (int64_t)t * B.p_tstride;
+    cplx<T> *bufZ = lds;                 // [qsub][ldz]  Z rows, columns of the j band
+    cplx<T> *bufE = bufZ + qsub * ldz;   // [qsub][lde]  E rows, rows of the i band
+    cplx<T> *bufP = bufE + qsub * lde;   // [qsub][q]    conj(P) rows
+    const int zc = Jb * k2, ec = Ib * k2;
+    for (int Q0 = 0; Q0 < q; Q0 += qsub) {
+      const int nr = min(qsub, q - Q0);
+      __syncthreads();
+      for (int e = tid; e < nr * q; e += NT) {
+        const cplx<T> p = Pb[(int64_t)Q0 * q + e];
+        bufP[e] = cplx<T>{p.re, -p.im};
+      }
+      __syncthreads();
+      // Z[r][x] = sum_P conj(P[Q0+r][P]) V[P][J0*k2 + x],  x < Jb*k2
+      {
+        const int ntn = (zc + 15) / 16;
+        const cplx<T> *Vj = Vb + J0 * k2;
+        for (int nt = wave; nt < ntn; nt += 4) {
+          const int n0 = nt * 16;
+          Tile<T> z;
+          tile_zero(z);
+          tile_mma(z, q, [&](int i, int P) { return (i < nr && P < q) ? bufP[i * q + P] : cz<T>(); },
+                   [&](int P, int j) { return (P < q && n0 + j < zc) ? Vj[(int64_t)P * K + n0 + j] : cz<T>(); });
+          tile_store(z, [&](int i, int j, cplx<T> v) {
+            if (i < nr && n0 + j < zc) bufZ[i * ldz + n0 + j] = v;
+          });
+        }
+      }
+      // E[(r,i)][n] = sum_m V[Q0+r][I0+i][m] G2[m][n]
+      {
+        const int M = nr * Ib, ntm = (M + 15) / 16, ntn = (k2 + 15) / 16;
+        for (int tt = wave; tt < ntm * ntn; tt += 4) {
+          const int m0 = (tt / ntn) * 16, n0 = (tt % ntn) * 16;
+          Tile<T> e;
+          tile_zero(e);
+          tile_mma(e, k2,
+                   [&](int i, int m) {
+                     const int row = m0 + i;
+                     if (row >= M || m >= k2) return cz<T>();
+                     const int r = row / Ib, ii = row - r * Ib;
+                     return Vb[(int64_t)(Q0 + r) * K + (I0 + ii) * k2 + m];
+                   },
+                   [&](int m, int j) { return (m < k2 && n0 + j < k2) ? G2[m * k2 + n0 + j] : cz<T>(); });
+          tile_store(e, [&](int i, int j, cplx<T> v) {
+            const int row = m0 + i, n = n0 + j;
+            if (row < M && n < k2) {
+              const int r = row / Ib, ii = row - r * Ib;
+              bufE[r * lde + ii * k2 + n] = v;
+            }
+          });
+        }
+      }
+      __syncthreads();
+      // acc[i][j] += sum_{(r,n)} E[r][i][n] conj(Z[r][j][n])
+      const int Y = nr * k2;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const int tt = wave + 4 * c;
+        if (tt < nti * ntj) {
+          const int i0 = (tt / ntj) * 16, j0 = (tt % ntj) * 16;
+          tile_mma(acc[c], Y,
+                   [&](int i, int y) {
+                     if (i0 + i >= Ib || y >= Y) return cz<T>();
+                     const int r = y / k2, n = y - r * k2;
+                     return bufE[r * lde + (i0 + i) * k2 + n];
+                   },
+                   [&](int y, int j) {
+                     if (j0 + j >= Jb || y >= Y) return cz<T>();
+                     const int r = y / k2, n = y - r * k2;
+                     const cplx<T> z = bufZ[r * ldz + (j0 + j) * k2 + n];
+                     return cplx<T>{z.re, -z.im};
+                   });
+        }
+      }
+    }
+  }
+  cplx<double> *out = Sig + it.out_base + (int64_t)t * it.out_tstride + (int64_t)s * it.out_sstride;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const int tt = wave + 4 * c;
+    if (tt < nti * ntj) {
+      const int i0 = (tt / ntj) * 16, j0 = (tt % ntj) * 16;
+      tile_store(acc[c], [&](int i, int j, cplx<T> v) {
+        if (i0 + i < Ib && j0 + j < Jb)
+          out[(int64_t)(i0 + i) * k1 + J0 + j0 + j] = cplx<double>{pref * (double)v.re, pref * (double)v.im};
+      });
+    }
+  }
+  (void)I0;
+}
+
+template <typename T>
+hipError_t launch_p0(const P0Item *items, const P0Blk *blks, int n_items, int nts, int ns, const cplx<T> *V,
+                     const cplx<T> *G, cplx<double> *P0, double pref, size_t lds, hipStream_t st) {
+  const int nt_half = nts / 2;
+  const int64_t n_work = (int64_t)n_items * nt_half;
+  if (n_work == 0) return hipSuccess;
+  (void)hipFuncSetAttribute((const void *)p0_mfma_kernel<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipLaunchKernelGGL((p0_mfma_kernel<T>), dim3((unsigned)xcd_grid(n_work)), dim3(NT), lds, st, items, blks, n_work,
+                     nt_half, nts, ns, V, G, P0, pref);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_sigma(const SigItem *items, const SigBlk *blks, int n_items, int nts, int ns, const cplx<T> *V,
+                        const cplx<T> *G, const cplx<T> *Pf, cplx<double> *Sig, double pref, size_t lds,
+                        hipStream_t st) {
+  const int64_t n_work = (int64_t)n_items * nts * ns;
+  if (n_work == 0) return hipSuccess;
+  (void)hipFuncSetAttribute((const void *)sigma_mfma_kernel<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)lds);
+  hipLaunchKernelGGL((sigma_mfma_kernel<T>), dim3((unsigned)xcd_grid(n_work)), dim3(NT), lds, st, items, blks,
+                     n_work, nts, ns, V, G, Pf, Sig, pref);
+  return hipGetLastError();
+}
+
+template hipError_t launch_p0<double>(const P0Item *, const P0Blk *, int, int, int, const cplx<double> *,
+                                      const cplx<double> *, cplx<double> *, double, size_t, hipStream_t);
+template hipError_t launch_p0<float>(const P0Item *, const P0Blk *, int, int, int, const cplx<float> *,
+                                     const cplx<float> *, cplx<double> *, double, size_t, hipStream_t);
+template hipError_t launch_sigma<double>(const SigItem *, const SigBlk *, int, int, int, const cplx<double> *,
+                                         const cplx<double> *, const cplx<double> *, cplx<double> *, double, size_t,
+                                         hipStream_t);
+template hipError_t launch_sigma<float>(const SigItem *, const SigBlk *, int, int, int, const cplx<float> *,
+                                        const cplx<float> *, const cplx<float> *, cplx<double> *, double, size_t,
+                                        hipStream_t);
+
+}  // namespace symgw

@@ -1,9 +1,7 @@
 // Device kernels of the GW hot path (gfx950).
 //
-// Contraction kernels are "owner computes": every workgroup owns one output
-// slab and walks its V-block list in a fixed order, so results are bitwise
-// reproducible and no atomics or locks are needed (the reference serialises
-// per-q / per-k AXPYs behind spin locks, cugw_qkpt_block.cu:269-285, 414-436).
+// Rotations, transforms, W / Dyson solvers, density, generator.  The P0 and
+// Sigma contractions live in contract.hip.
 #include <hip/hip_runtime.h>
 
 #include "common.h"
@@ -12,191 +10,9 @@
 namespace symgw {
 
 static constexpr int NT = 256;  // threads per workgroup
-static constexpr int XT = 8;    // K tile of staged conj(V) rows in the P0 kernel
 
 template <typename T>
 __device__ inline cplx<T> czero() { return cplx<T>{T(0), T(0)}; }
-
-// ---------------------------------------------------------------------------
-// P0 contraction.  For one output slab (rows [Q0, Q0+Qc) of aux block iq of
-// iBZ point q) and time t < nts/2:
-//   P0[Q][P](t) = pref * sum_{k, blocks, s} sum_{p,n} C_Q[p][n] conj(V_P[p][n]),
-//   C_Q = G^k(beta - t) V_Q G^{k+q}(t)
-// which regroups the three GEMMs of cugw_qkpt_block.cu:231-260
-// (X1 = G V, X2 = conj(V) G^T, P0^T += X2 X1) around the single intermediate C.
-// ---------------------------------------------------------------------------
-template <typename T>
-__global__ __launch_bounds__(NT) void p0_kernel(const P0Item *__restrict__ items, const P0Blk *__restrict__ blks,
-                                               int64_t n_work, int nt_half, int nts, int ns,
-                                               const cplx<T> *__restrict__ V, const cplx<T> *__restrict__ G,
-                                               cplx<double> *__restrict__ P0, double pref) {
-  const int64_t w = xcd_logical(blockIdx.x, gridDim.x);
-  if (w >= n_work) return;
-  const int item_id = (int)(w / nt_half);
-  const int t = (int)(w % nt_half);
-  const P0Item it = items[item_id];
-  const int tid = threadIdx.x;
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  cplx<T> *bufA = reinterpret_cast<cplx<T> *>(smem_raw);  // V rows, then C
-  cplx<T> *bufB = bufA + it.Qc * it.kmax;                  // E = V G2
-  cplx<T> *bufV = bufB + it.Qc * it.kmax;                  // [XT][q] conj-V tile
-  const int q = it.q, Qc = it.Qc, nout = Qc * q;
-  cplx<T> acc[4];
-  int orow[4], ocol[4];
-#pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    acc[j] = czero<T>();
-    int o = tid + NT * j;
-    orow[j] = o < nout ? o / q : -1;
-    ocol[j] = o < nout ? o % q : 0;
-  }
-  for (int b = it.blk_begin; b < it.blk_end; ++b) {
-    const P0Blk B = blks[b];
-    const int k1 = B.k1, k2 = B.k2, K = k1 * k2;
-    const cplx<T> *Vb = V + B.v_off;
-    for (int s = 0; s < ns; ++s) {
-      const cplx<T> *G1 = G + B.g1_base + (int64_t)(nts - 1 - t) * B.g1_tstride + (int64_t)s * B.g1_sstride;
-      const cplx<T> *G2 = G + B.g2_base + (int64_t)t * B.g2_tstride + (int64_t)s * B.g2_sstride;
-      __syncthreads();
-      for (int i = tid; i < Qc * K; i += NT) bufA[i] = Vb[(int64_t)it.Q0 * K + i];
-      __syncthreads();
-      // E[r][a][n] = sum_m V[r][a][m] G2[m][n]
-      for (int i = tid; i < Qc * K; i += NT) {
-        const int r = i / K, rem = i - r * K, a = rem / k2, n = rem - a * k2;
-        const cplx<T> *vr = bufA + r * K + a * k2;
-        cplx<T> e = czero<T>();
-        for (int m = 0; m < k2; ++m) cfma(e, vr[m], G2[m * k2 + n]);
-        bufB[i] = e;
-      }
-      __syncthreads();
-      // C[r][p][n] = sum_a G1[p][a] E[r][a][n]
-      for (int i = tid; i < Qc * K; i += NT) {
-        const int r = i / K, rem = i - r * K, p = rem / k2, n = rem - p * k2;
-        const cplx<T> *er = bufB + r * K + n;
-        cplx<T> c = czero<T>();
-        for (int a = 0; a < k1; ++a) cfma(c, G1[p * k1 + a], er[a * k2]);
-        bufA[i] = c;
-      }
-      // acc[r][P] += sum_x C[r][x] conj(V[P][x])
-      for (int x0 = 0; x0 < K; x0 += XT) {
-        __syncthreads();
-        for (int i = tid; i < q * XT; i += NT) {
-          const int P = i / XT, xx = i - P * XT;
-          bufV[xx * q + P] = (x0 + xx < K) ? Vb[(int64_t)P * K + x0 + xx] : czero<T>();
-        }
-        __syncthreads();
-        const int xn = min(XT, K - x0);
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          if (orow[j] < 0) continue;
-          const cplx<T> *cr = bufA + orow[j] * K + x0;
-          for (int xx = 0; xx < xn; ++xx) cfmac(acc[j], cr[xx], bufV[xx * q + ocol[j]]);
-        }
-      }
-    }
-  }
-  cplx<double> *out = P0 + it.out_base + (int64_t)t * it.out_tstride;
-#pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    int o = tid + NT * j;
-    if (o < nout) out[o] = cplx<double>{pref * (double)acc[j].re, pref * (double)acc[j].im};
-  }
-}
-
-// ---------------------------------------------------------------------------
-// Sigma contraction.  For one output slab (rows [I0, I0+Ic) of ao block ik1 of
-// iBZ point k), time t and spin s:
-//   Sigma[i][j] = pref * sum_{q, blocks} sum_{Q,n} E_Q[i][n] conj(Z_Q[j][n]),
-//   E_Q = V_Q G^{k-q}(t),   Z_Q = sum_P conj(P_{QP}(t)) V_P
-// the same contraction as cugw_qkpt_block.cu:381-407 (Y1 = V G, Y2 = Y1 P,
-// Sigma += Y2 V^H) with the P-contraction moved onto V.
-// ---------------------------------------------------------------------------
-template <typename T>
-__global__ __launch_bounds__(NT) void sigma_kernel(const SigItem *__restrict__ items, const SigBlk *__restrict__ blks,
-                                                  int64_t n_work, int nts, int ns,
-                                                  const cplx<T> *__restrict__ V, const cplx<T> *__restrict__ G,
-                                                  const cplx<T> *__restrict__ Pf, cplx<double> *__restrict__ Sig,
-                                                  double pref) {
-  const int64_t w = xcd_logical(blockIdx.x, gridDim.x);
-  if (w >= n_work) return;
-  const int per_item = nts * ns;
-  const int item_id = (int)(w / per_item);
-  const int ts = (int)(w % per_item);
-  const int t = ts / ns, s = ts % ns;
-  const SigItem it = items[item_id];
-  const int tid = threadIdx.x;
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  cplx<T> *lds = reinterpret_cast<cplx<T> *>(smem_raw);
-  const int k1 = it.k1, Ic = it.Ic, nout = Ic * k1;
-  cplx<T> acc[4];
-  int oi[4], oj[4];
-#pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    acc[j] = czero<T>();
-    int o = tid + NT * j;
-    oi[j] = o < nout ? o / k1 : -1;
-    oj[j] = o < nout ? o % k1 : 0;
-  }
-  for (int b = it.blk_begin; b < it.blk_end; ++b) {
-    const SigBlk B = blks[b];
-    const int q = B.q, k2 = B.k2, K = k1 * k2, qc = B.qc;
-    const cplx<T> *Vb = V + B.v_off;
-    const cplx<T> *G2 = G + B.g2_base + (int64_t)t * B.g2_tstride + (int64_t)s * B.g2_sstride;
-    const cplx<T> *Pb = Pf + B.p_base + (int64_t)t * B.p_tstride;
-    cplx<T> *bufZ = lds;                      // [qc][K]
-    cplx<T> *bufE = bufZ + qc * K;            // [qc][Ic][k2]
-    cplx<T> *bufP = bufE + qc * Ic * k2;      // [qc][q] conj(P) rows
-    for (int Q0 = 0; Q0 < q; Q0 += qc) {
-      const int nr = min(qc, q - Q0);
-      __syncthreads();
-      for (int i = tid; i < nr * q; i += NT) {
-        const cplx<T> p = Pb[(int64_t)Q0 * q + i];
-        bufP[i] = cplx<T>{p.re, -p.im};
-      }
-      __syncthreads();
-      // Z[r][x] = sum_P conj(P[Q0+r][P]) V[P][x]; each thread owns columns x
-      for (int x = tid; x < K; x += NT) {
-        cplx<T> z[16];
-#pragma unroll
-        for (int r = 0; r < 16; ++r) z[r] = czero<T>();
-        for (int P = 0; P < q; ++P) {
-          const cplx<T> v = Vb[(int64_t)P * K + x];
-#pragma unroll
-          for (int r = 0; r < 16; ++r)
-            if (r < nr) cfma(z[r], bufP[r * q + P], v);
-        }
-#pragma unroll
-        for (int r = 0; r < 16; ++r)
-          if (r < nr) bufZ[r * K + x] = z[r];
-      }
-      // E[r][i][n] = sum_m V[Q0+r][I0+i][m] G2[m][n]
-      for (int i = tid; i < nr * Ic * k2; i += NT) {
-        const int r = i / (Ic * k2), rem = i - r * (Ic * k2), ii = rem / k2, n = rem - ii * k2;
-        const cplx<T> *vr = Vb + (int64_t)(Q0 + r) * K + (it.I0 + ii) * k2;
-        cplx<T> e = czero<T>();
-        for (int m = 0; m < k2; ++m) cfma(e, vr[m], G2[m * k2 + n]);
-        bufE[i] = e;
-      }
-      __syncthreads();
-      // acc[i][j] += sum_{r,n} E[r][i][n] conj(Z[r][j][n])
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        if (oi[j] < 0) continue;
-        for (int r = 0; r < nr; ++r) {
-          const cplx<T> *er = bufE + (r * Ic + oi[j]) * k2;
-          const cplx<T> *zr = bufZ + r * K + oj[j] * k2;
-          for (int n = 0; n < k2; ++n) cfmac(acc[j], er[n], zr[n]);
-        }
-      }
-    }
-  }
-  cplx<double> *out = Sig + it.out_base + (int64_t)t * it.out_tstride + (int64_t)s * it.out_sstride;
-#pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    int o = tid + NT * j;
-    if (o < nout) out[o] = cplx<double>{pref * (double)acc[j].re, pref * (double)acc[j].im};
-  }
-}
 
 // ---------------------------------------------------------------------------
 // iBZ -> full-BZ rotation of one block for one (t, s) slice:
@@ -534,30 +350,6 @@ __global__ __launch_bounds__(NT) void reorder_kernel(const cplx<double> *__restr
 }
 
 // ---- launchers ---------------------------------------------------------------
-template <typename T>
-hipError_t launch_p0(const P0Item *items, const P0Blk *blks, int n_items, int nts, int ns, const cplx<T> *V,
-                     const cplx<T> *G, cplx<double> *P0, double pref, size_t lds, hipStream_t st) {
-  const int nt_half = nts / 2;
-  const int64_t n_work = (int64_t)n_items * nt_half;
-  if (n_work == 0) return hipSuccess;
-  (void)hipFuncSetAttribute((const void *)p0_kernel<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  hipLaunchKernelGGL((p0_kernel<T>), dim3((unsigned)xcd_grid(n_work)), dim3(NT), lds, st, items, blks, n_work,
-                     nt_half, nts, ns, V, G, P0, pref);
-  return hipGetLastError();
-}
-
-template <typename T>
-hipError_t launch_sigma(const SigItem *items, const SigBlk *blks, int n_items, int nts, int ns, const cplx<T> *V,
-                        const cplx<T> *G, const cplx<T> *Pf, cplx<double> *Sig, double pref, size_t lds,
-                        hipStream_t st) {
-  const int64_t n_work = (int64_t)n_items * nts * ns;
-  if (n_work == 0) return hipSuccess;
-  (void)hipFuncSetAttribute((const void *)sigma_kernel<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  hipLaunchKernelGGL((sigma_kernel<T>), dim3((unsigned)xcd_grid(n_work)), dim3(NT), lds, st, items, blks, n_work,
-                     nts, ns, V, G, Pf, Sig, pref);
-  return hipGetLastError();
-}
-
 template <typename TO>
 hipError_t launch_rotate(const RotBlk *rb, int n_rb, int nts_ns, const cplx<double> *src, const int64_t *src_ts_stride,
                          const int64_t *src_slice, const cplx<double> *rep, cplx<TO> *dst, const int64_t *dst_slice,
@@ -627,16 +419,6 @@ hipError_t launch_reorder(const cplx<double> *in, cplx<double> *out, const int64
   return hipGetLastError();
 }
 
-template hipError_t launch_p0<double>(const P0Item *, const P0Blk *, int, int, int, const cplx<double> *,
-                                      const cplx<double> *, cplx<double> *, double, size_t, hipStream_t);
-template hipError_t launch_p0<float>(const P0Item *, const P0Blk *, int, int, int, const cplx<float> *,
-                                     const cplx<float> *, cplx<double> *, double, size_t, hipStream_t);
-template hipError_t launch_sigma<double>(const SigItem *, const SigBlk *, int, int, int, const cplx<double> *,
-                                         const cplx<double> *, const cplx<double> *, cplx<double> *, double, size_t,
-                                         hipStream_t);
-template hipError_t launch_sigma<float>(const SigItem *, const SigBlk *, int, int, int, const cplx<float> *,
-                                        const cplx<float> *, const cplx<float> *, cplx<double> *, double, size_t,
-                                        hipStream_t);
 template hipError_t launch_rotate<double>(const RotBlk *, int, int, const cplx<double> *, const int64_t *, const int64_t *,
                                           const cplx<double> *, cplx<double> *, const int64_t *, const int64_t *,
                                           hipStream_t);

@@ -235,26 +235,31 @@ void flop_model(const Layout &L, double &p0, double &sig, int64_t &nb_p0, int64_
   }
 }
 
-// LDS planning constants shared with the kernels (kernels.hip)
-static const int kThreads = 256;
-static const int kP0MaxOut = 4 * kThreads;     // outputs per P0 item
-static const int kSigMaxOut = 4 * kThreads;    // outputs per Sigma item
-static const int kXT = 8;                      // K tile of the staged conj(V) rows
-static const size_t kLdsBudget = 150 * 1024;
+namespace {
+// odd row stride (in complex elements) keeps the 16 rows an MFMA operand
+// fetch touches in distinct LDS banks
+int odd_ld(int n) { return n % 2 == 0 ? n + 1 : n; }
+// split n rows into the fewest chunks of at most cap rows, balanced
+int balanced(int n, int cap) {
+  int nch = (n + cap - 1) / cap;
+  return (n + nch - 1) / nch;
+}
+}  // namespace
 
 bool build_worklists(const Layout &L, int eb, WorkLists &W, std::string &err) {
   W = WorkLists();
   const int nk = L.nk, nts = L.nts, ns = L.ns;
+  const int budget = kContractLds / eb;  // complex elements of LDS per workgroup
   flop_model(L, W.flop_p0, W.flop_sig, W.n_p0_contractions, W.n_sig_contractions);
 
-  // ---- P0: items per (q_ir, iq, row chunk); blocks in (k, V-block) order ----
+  // ---- P0: items per (q_ir, iq, 16-row band); blocks in (k, V-block) order ----
   size_t p0_lds = 0;
   for (int iqr = 0; iqr < L.ink; ++iqr) {
     int q = L.irre_list[iqr];
     for (size_t iq = 0; iq < L.aux_sizes[q].size(); ++iq) {
       int qs = L.aux_sizes[q][iq];
+      if (qs > 256) { err = fmt("aux block larger than 256", qs); return false; }
       int b0 = (int)W.p0_blks.size();
-      int kmax = 1;
       for (int k = 0; k < nk; ++k) {
         int k2 = L.kpq[(size_t)k * nk + q];
         size_t p = (size_t)k * nk + k2;
@@ -269,41 +274,37 @@ bool build_worklists(const Layout &L, int eb, WorkLists &W, std::string &err) {
           B.g2_base = L.kao_off[k2] * ns * nts + L.ao_offsets[k2][t[2]];
           B.g1_tstride = (int32_t)(ns * L.kao_size[k]); B.g1_sstride = (int32_t)L.kao_size[k];
           B.g2_tstride = (int32_t)(ns * L.kao_size[k2]); B.g2_sstride = (int32_t)L.kao_size[k2];
-          kmax = std::max(kmax, B.k1 * B.k2);
+          B.ldk = odd_ld(B.k1 * B.k2);
+          // V rows and E rows of one chunk share the LDS budget
+          int cap = std::min(kP0Band, budget / (2 * B.ldk));
+          if (cap < 1) { err = fmt("P0 block too large for the LDS plan (k1, k2)", B.k1, B.k2); return false; }
+          B.qsub = balanced(std::min(kP0Band, qs), cap);
+          p0_lds = std::max(p0_lds, (size_t)(2 * B.qsub * B.ldk) * eb);
           W.p0_blks.push_back(B);
         }
       }
       int b1 = (int)W.p0_blks.size();
-      // rows per item: outputs <= kP0MaxOut and 2*Qc*kmax + XT*q elements in LDS
-      int qc = std::max(1, kP0MaxOut / qs);
-      while (qc > 1 && (size_t)(2 * qc * kmax + kXT * qs) * eb > kLdsBudget) --qc;
-      if ((size_t)(2 * kmax + kXT * qs) * eb > kLdsBudget) {
-        err = fmt("P0 block too large for the LDS plan (q, k1*k2)", qs, kmax); return false;
-      }
-      qc = std::min(qc, qs);
-      for (int Q0 = 0; Q0 < qs; Q0 += qc) {
+      for (int Q0 = 0; Q0 < qs; Q0 += kP0Band) {
         P0Item it;
         it.blk_begin = b0; it.blk_end = b1;
-        it.q = qs; it.Q0 = Q0; it.Qc = std::min(qc, qs - Q0); it.kmax = kmax;
+        it.q = qs; it.Q0 = Q0; it.nband = std::min(kP0Band, qs - Q0); it.pad = 0;
         it.out_base = (int64_t)nts * L.qaux_off_irre[iqr] + L.aux_offsets[q][iq] + (int64_t)Q0 * qs;
         it.out_tstride = (int32_t)L.qaux_size_irre[iqr];
-        it.pad = 0;
+        it.pad2 = 0;
         W.p0_items.push_back(it);
-        p0_lds = std::max(p0_lds, (size_t)(2 * it.Qc * kmax + kXT * qs) * eb);
       }
     }
   }
-  W.p0_lds = p0_lds;
+  W.p0_lds = std::max<size_t>(p0_lds, 16);
 
-  // ---- Sigma: items per (k_ir, ik1, row chunk); blocks in (q, V-block) order ----
+  // ---- Sigma: items per (k_ir, ik1, row band, column band); blocks in (q, V-block) order ----
   size_t sig_lds = 0;
   for (int ikr = 0; ikr < L.ink; ++ikr) {
     int k = L.irre_list[ikr];
     for (size_t i1 = 0; i1 < L.ao_sizes[k].size(); ++i1) {
       int k1 = L.ao_sizes[k][i1];
+      int band = std::min(k1, kSigBand);
       int b0 = (int)W.sig_blks.size();
-      size_t need_max = 0;
-      int ic = std::max(1, std::min(k1, kSigMaxOut / k1));
       for (int q = 0; q < nk; ++q) {
         int k2i = L.kmq[(size_t)k * nk + q];
         size_t p = (size_t)k * nk + k2i;
@@ -318,32 +319,33 @@ bool build_worklists(const Layout &L, int eb, WorkLists &W, std::string &err) {
           B.g2_tstride = (int32_t)(ns * L.kao_size[k2i]); B.g2_sstride = (int32_t)L.kao_size[k2i];
           B.p_base = L.qaux_off[q] * nts + L.aux_offsets[q][t[0]];
           B.p_tstride = (int32_t)L.qaux_size[q];
-          int K = k1 * B.k2;
-          // Q rows per chunk: Z (qc*K) + E (qc*Ic*k2) + conj(P) rows (qc*q) in LDS,
-          // and qc * ceil(K/256) register accumulators per thread <= 16
-          int qc = std::max(1, std::min(B.q, 16 / std::max(1, (K + kThreads - 1) / kThreads)));
-          auto need = [&](int c) { return (size_t)(c * K + c * ic * B.k2 + c * B.q) * eb; };
-          while (qc > 1 && need(qc) > kLdsBudget) --qc;
-          if (need(1) > kLdsBudget) { err = fmt("Sigma block too large for the LDS plan (q, k1*k2)", B.q, K); return false; }
-          B.qc = qc;
-          need_max = std::max(need_max, need(qc));
+          B.ldz = odd_ld(band * B.k2);
+          B.lde = odd_ld(band * B.k2);
+          B.pad = 0;
+          // Z, E and conj(P) rows of one chunk share the LDS budget
+          int per_row = B.q + B.ldz + B.lde;
+          int cap = std::min(16, budget / per_row);
+          if (cap < 1) { err = fmt("Sigma block too large for the LDS plan (q, k2)", B.q, B.k2); return false; }
+          B.qsub = balanced(B.q, cap);
+          sig_lds = std::max(sig_lds, (size_t)(B.qsub * per_row) * eb);
           W.sig_blks.push_back(B);
         }
       }
       int b1 = (int)W.sig_blks.size();
-      for (int I0 = 0; I0 < k1; I0 += ic) {
-        SigItem it;
-        it.blk_begin = b0; it.blk_end = b1;
-        it.k1 = k1; it.I0 = I0; it.Ic = std::min(ic, k1 - I0); it.pad = 0;
-        it.out_base = L.kao_off_irre[ikr] * ns * nts + L.ao_offsets[k][i1] + (int64_t)I0 * k1;
-        it.out_tstride = (int32_t)(ns * L.kao_size_irre[ikr]);
-        it.out_sstride = (int32_t)L.kao_size_irre[ikr];
-        W.sig_items.push_back(it);
-      }
-      sig_lds = std::max(sig_lds, need_max);
+      for (int I0 = 0; I0 < k1; I0 += band)
+        for (int J0 = 0; J0 < k1; J0 += band) {
+          SigItem it;
+          it.blk_begin = b0; it.blk_end = b1;
+          it.k1 = k1; it.I0 = I0; it.Ib = std::min(band, k1 - I0);
+          it.J0 = J0; it.Jb = std::min(band, k1 - J0); it.pad = 0;
+          it.out_base = L.kao_off_irre[ikr] * ns * nts + L.ao_offsets[k][i1] + (int64_t)I0 * k1;
+          it.out_tstride = (int32_t)(ns * L.kao_size_irre[ikr]);
+          it.out_sstride = (int32_t)L.kao_size_irre[ikr];
+          W.sig_items.push_back(it);
+        }
     }
   }
-  W.sig_lds = sig_lds;
+  W.sig_lds = std::max<size_t>(sig_lds, 16);
 
   // ---- rotations of G (ao) and P (aux) to the full BZ ----
   for (int which = 0; which < 2; ++which) {

@@ -45,6 +45,13 @@ struct Layout {
 bool build_layout(const symgw_layout_desc *d, Layout &L, std::string &err);
 
 // ---- device work lists ------------------------------------------------------
+// Dynamic LDS of one contraction workgroup (two workgroups fit one CU).
+constexpr int kContractLds = 80 * 1024;
+// Output rows of one P0 slab (one MFMA tile row).
+constexpr int kP0Band = 16;
+// Largest Sigma slab edge (4 x 4 tiles of 16, four per wave).
+constexpr int kSigBand = 64;
+
 // One V-block contraction that feeds a P0 output slab (cugw_qkpt_block.cu:219-261).
 struct P0Blk {
   int64_t v_off;       // V block start (complex elements)
@@ -55,14 +62,16 @@ struct P0Blk {
   int32_t g2_tstride;
   int32_t g2_sstride;
   int32_t k1, k2;
+  int32_t qsub;        // Q rows per LDS chunk
+  int32_t ldk;         // LDS row stride (k1*k2, odd)
 };
-// Rows [Q0, Q0 + Qc) of aux block iq of iBZ point q_ir, summed over all k.
+// Rows [Q0, Q0 + nband) of aux block iq of iBZ point q_ir, summed over all k.
 struct P0Item {
   int32_t blk_begin, blk_end;
-  int32_t q, Q0, Qc, kmax;   // kmax = max k1*k2 over the item's blocks
+  int32_t q, Q0, nband, pad;
   int64_t out_base;          // q_t_QP offset of row Q0 at t = 0
   int32_t out_tstride;       // qslice(q_ir)
-  int32_t pad;
+  int32_t pad2;
 };
 
 // One V-block contraction that feeds a Sigma output slab
@@ -74,12 +83,15 @@ struct SigBlk {
   int32_t g2_tstride, g2_sstride;
   int32_t p_tstride;   // qslice(q)
   int32_t q, k2;
-  int32_t qc;          // Q rows per chunk (LDS plan)
+  int32_t qsub;        // Q rows per LDS chunk
+  int32_t ldz, lde;    // LDS row strides of the Z and E chunks
+  int32_t pad;
 };
-// Rows [I0, I0 + Ic) of ao block ik1 of iBZ point k_ir, summed over all q.
+// Rows [I0, I0 + Ib) x columns [J0, J0 + Jb) of ao block ik1 of iBZ point
+// k_ir, summed over all q.
 struct SigItem {
   int32_t blk_begin, blk_end;
-  int32_t k1, I0, Ic, pad;
+  int32_t k1, I0, Ib, J0, Jb, pad;
   int64_t out_base;          // k_ts_ij (iBZ) offset of row I0 at t = 0, s = 0
   int32_t out_tstride;       // ns * kslice(k)
   int32_t out_sstride;       // kslice(k)
