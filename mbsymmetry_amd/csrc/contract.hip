@@ -51,16 +51,73 @@ __device__ __forceinline__ cplx<T> cz() { return cplx<T>{T(0), T(0)}; }
 // acc (16x16) += A (16 x K) * B (K x 16).  la(i, k) and lb(k, j) return the
 // operand element or zero outside the valid range.  Lane l holds A[l%16][k0+l/16]
 // and B[k0+l/16][l%16] (operand layout of the 16x16x4 MFMAs).
+template <typename T>
+__device__ __forceinline__ void mma4(Tile<T> &acc, const cplx<T> &a, const cplx<T> &b) {
+  acc.re = Mfma<T>::op(a.re, b.re, acc.re);
+  acc.im = Mfma<T>::op(a.re, b.im, acc.im);
+  acc.re = Mfma<T>::op(-a.im, b.im, acc.re);
+  acc.im = Mfma<T>::op(a.im, b.re, acc.im);
+}
+
+// Same over the K range [kb, ke) (kb a multiple of 4).  Operands are fetched
+// two k-steps ahead of the MFMAs that use them; the loaders return zero past
+// the end, so the look-ahead never reads out of range.
+template <typename T, typename LA, typename LB>
+__device__ __forceinline__ void tile_mma_range(Tile<T> &acc, int kb, int ke, const LA &la, const LB &lb) {
+  if (kb >= ke) return;
+  const int lane = threadIdx.x & 63, i = lane & 15, kk = lane >> 4;
+  cplx<T> a0 = la(i, kb + kk), b0 = lb(kb + kk, i);
+  cplx<T> a1 = la(i, kb + 4 + kk), b1 = lb(kb + 4 + kk, i);
+  for (int k0 = kb; k0 < ke; k0 += 4) {
+    const cplx<T> a2 = la(i, k0 + 8 + kk), b2 = lb(k0 + 8 + kk, i);
+    mma4(acc, a0, b0);
+    a0 = a1; b0 = b1;
+    a1 = a2; b1 = b2;
+  }
+}
+
 template <typename T, typename LA, typename LB>
 __device__ __forceinline__ void tile_mma(Tile<T> &acc, int K, const LA &la, const LB &lb) {
-  const int lane = threadIdx.x & 63, i = lane & 15, kk = lane >> 4;
-  for (int k0 = 0; k0 < K; k0 += 4) {
-    const cplx<T> a = la(i, k0 + kk);
-    const cplx<T> b = lb(k0 + kk, i);
-    acc.re = Mfma<T>::op(a.re, b.re, acc.re);
-    acc.im = Mfma<T>::op(a.re, b.im, acc.im);
-    acc.re = Mfma<T>::op(-a.im, b.im, acc.re);
-    acc.im = Mfma<T>::op(a.im, b.re, acc.im);
+  tile_mma_range(acc, 0, K, la, lb);
+}
+
+// Split-K partner of tile_mma_range: the K range of wave w out of 4.
+__device__ __forceinline__ void wave_krange(int K, int wave, int &kb, int &ke) {
+  const int q4 = ((K + 15) / 16) * 4;  // per-wave share, a multiple of 4
+  kb = min(K, wave * q4);
+  ke = min(K, kb + q4);
+}
+
+// Sum the four waves' copies of ntile accumulator tiles in wave order (fixed
+// order: bitwise reproducible) and hand the totals to wave 0.  Uses the LDS
+// at `red` (4 * ntile * 64 * 8 elements of T); callers sync before reuse.
+template <typename T>
+__device__ __forceinline__ void wave_reduce(Tile<T> (&acc)[4], int ntile, T *red) {
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < 4; ++c)
+    if (c < ntile)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        red[((wave * 4 + c) * 8 + r) * 64 + lane] = acc[c].re[r];
+        red[((wave * 4 + c) * 8 + 4 + r) * 64 + lane] = acc[c].im[r];
+      }
+  __syncthreads();
+  if (wave == 0) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+      if (c < ntile)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          T sre = red[((0 * 4 + c) * 8 + r) * 64 + lane], sim = red[((0 * 4 + c) * 8 + 4 + r) * 64 + lane];
+          for (int w2 = 1; w2 < 4; ++w2) {
+            sre += red[((w2 * 4 + c) * 8 + r) * 64 + lane];
+            sim += red[((w2 * 4 + c) * 8 + 4 + r) * 64 + lane];
+          }
+          acc[c].re[r] = sre;
+          acc[c].im[r] = sim;
+        }
   }
 }
 
@@ -95,6 +152,7 @@ __global__ __launch_bounds__(NT) void p0_mfma_kernel(const P0Item *__restrict__ 
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cplx<T> *lds = reinterpret_cast<cplx<T> *>(smem_raw);
   const int q = it.q, nband = it.nband, nct = (q + 15) / 16;
+  const bool split = nct <= 4;
   Tile<T> acc[4];
 #pragma unroll
   for (int c = 0; c < 4; ++c) tile_zero(acc[c]);
@@ -166,32 +224,36 @@ __global__ __launch_bounds__(NT) void p0_mfma_kernel(const P0Item *__restrict__ 
           }
         }
         __syncthreads();
-        // acc[r0 + r][P] += sum_x C[r][x] conj(V[P][x])
+        // acc[r0 + r][P] += sum_x C[r][x] conj(V[P][x]); with <= 4 column tiles every
+        // wave takes all of them over a quarter of x (split-K), else whole tiles
+        int kb = 0, ke = K;
+        if (split) wave_krange(K, wave, kb, ke);
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
-          const int jt = wave + 4 * c;
+          const int jt = split ? c : wave + 4 * c;
           if (jt < nct) {
             const int p0 = jt * 16;
-            tile_mma(acc[c], K,
-                     [&](int i, int x) {
-                       const int rr = i - r0;
-                       return (rr >= 0 && rr < nr && x < K) ? bufV[rr * ldk + x] : cz<T>();
-                     },
-                     [&](int x, int j) {
-                       if (x >= K || p0 + j >= q) return cz<T>();
-                       const cplx<T> v = Vb[(int64_t)(p0 + j) * K + x];
-                       return cplx<T>{v.re, -v.im};
-                     });
+            tile_mma_range(acc[c], kb, ke,
+                           [&](int i, int x) {
+                             const int rr = i - r0;
+                             return (rr >= 0 && rr < nr && x < ke) ? bufV[rr * ldk + x] : cz<T>();
+                           },
+                           [&](int x, int j) {
+                             if (x >= ke || p0 + j >= q) return cz<T>();
+                             const cplx<T> v = Vb[(int64_t)(p0 + j) * K + x];
+                             return cplx<T>{v.re, -v.im};
+                           });
           }
         }
       }
     }
   }
   cplx<double> *out = P0 + it.out_base + (int64_t)t * it.out_tstride;
+  if (split) wave_reduce(acc, nct, reinterpret_cast<T *>(lds));
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
-    const int jt = wave + 4 * c;
-    if (jt < nct)
+    const int jt = split ? c : wave + 4 * c;
+    if (jt < nct && (!split || wave == 0))
       tile_store(acc[c], [&](int i, int j, cplx<T> v) {
         const int P = jt * 16 + j;
         if (i < nband && P < q) out[i * q + P] = cplx<double>{pref * (double)v.re, pref * (double)v.im};
@@ -205,7 +267,7 @@ __global__ __launch_bounds__(NT) void p0_mfma_kernel(const P0Item *__restrict__ 
 // ---------------------------------------------------------------------------
 // Sigma.  Slab = rows [I0, I0+Ib) x columns [J0, J0+Jb) of ao block ik1 of iBZ
 // point k, time t, spin s.  For every V block (q, iq, ik2), in chunks of qsub
-// aux rows Q:
+// aux rows Q (E and Z stored with the (Q, n) reduction index contiguous):
 //   Z_Q = sum_P conj(P_{QP}(t)) V_P,  E_Q = V_Q G^{k-q}(t),
 //   Sigma[i][j] += sum_{Q,n} E_Q[i][n] conj(Z_Q[j][n])
 // the contraction of cugw_qkpt_block.cu:381-407 (Y1 = V G, Y2 = Y1 P,
@@ -228,6 +290,7 @@ __global__ __launch_bounds__(NT) void sigma_mfma_kernel(const SigItem *__restric
   cplx<T> *lds = reinterpret_cast<cplx<T> *>(smem_raw);
   const int k1 = it.k1, I0 = it.I0, Ib = it.Ib, J0 = it.J0, Jb = it.Jb;
   const int nti = (Ib + 15) / 16, ntj = (Jb + 15) / 16;
+  const bool split = nti * ntj <= 4;
   Tile<T> acc[4];
 #pragma unroll
   for (int c = 0; c < 4; ++c) tile_zero(acc[c]);
@@ -238,10 +301,11 @@ __global__ __launch_bounds__(NT) void sigma_mfma_kernel(const SigItem *__restric
     const cplx<T> *Vb = V + B.v_off;
     const cplx<T> *G2 = G + B.g2_base + (int64_t)t * B.g2_tstride + (int64_t)s * B.g2_sstride;
     const cplx<T> *Pb = Pf + B.p_base + (int64_t)t * B.p_tstride;
-    cplx<T> *bufZ = lds;                 // [qsub][ldz]  Z rows, columns of the j band
-    cplx<T> *bufE = bufZ + qsub * ldz;   // [qsub][lde]  E rows, rows of the i band
-    cplx<T> *bufP = bufE + qsub * lde;   // [qsub][q]    conj(P) rows
-    const int zc = Jb * k2, ec = Ib * k2;
+    // y = r*k2 + n indexes (aux row of the chunk, orbital n of k - q)
+    cplx<T> *bufZ = lds;                 // [Jb][ldz]  Z[r][j][n] at j*ldz + y
+    cplx<T> *bufE = bufZ + Jb * ldz;     // [Ib][lde]  E[r][i][n] at i*lde + y
+    cplx<T> *bufP = bufE + Ib * lde;     // [qsub][q]  conj(P) rows
+    const int zc = Jb * k2;
     for (int Q0 = 0; Q0 < q; Q0 += qsub) {
       const int nr = min(qsub, q - Q0);
       __syncthreads();
@@ -261,7 +325,11 @@ __global__ __launch_bounds__(NT) void sigma_mfma_kernel(const SigItem *__restric
           tile_mma(z, q, [&](int i, int P) { return (i < nr && P < q) ? bufP[i * q + P] : cz<T>(); },
                    [&](int P, int j) { return (P < q && n0 + j < zc) ? Vj[(int64_t)P * K + n0 + j] : cz<T>(); });
           tile_store(z, [&](int i, int j, cplx<T> v) {
-            if (i < nr && n0 + j < zc) bufZ[i * ldz + n0 + j] = v;
+            const int x = n0 + j;
+            if (i < nr && x < zc) {
+              const int jj = x / k2, n = x - jj * k2;
+              bufZ[jj * ldz + i * k2 + n] = v;
+            }
           });
         }
       }
@@ -284,40 +352,41 @@ __global__ __launch_bounds__(NT) void sigma_mfma_kernel(const SigItem *__restric
             const int row = m0 + i, n = n0 + j;
             if (row < M && n < k2) {
               const int r = row / Ib, ii = row - r * Ib;
-              bufE[r * lde + ii * k2 + n] = v;
+              bufE[ii * lde + r * k2 + n] = v;
             }
           });
         }
       }
       __syncthreads();
-      // acc[i][j] += sum_{(r,n)} E[r][i][n] conj(Z[r][j][n])
+      // acc[i][j] += sum_y E[i][y] conj(Z[j][y]); with <= 4 tiles every wave
+      // takes all of them over a quarter of y (split-K), else whole tiles
       const int Y = nr * k2;
+      int yb = 0, ye = Y;
+      if (split) wave_krange(Y, wave, yb, ye);
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
-        const int tt = wave + 4 * c;
+        const int tt = split ? c : wave + 4 * c;
         if (tt < nti * ntj) {
           const int i0 = (tt / ntj) * 16, j0 = (tt % ntj) * 16;
-          tile_mma(acc[c], Y,
-                   [&](int i, int y) {
-                     if (i0 + i >= Ib || y >= Y) return cz<T>();
-                     const int r = y / k2, n = y - r * k2;
-                     return bufE[r * lde + (i0 + i) * k2 + n];
-                   },
-                   [&](int y, int j) {
-                     if (j0 + j >= Jb || y >= Y) return cz<T>();
-                     const int r = y / k2, n = y - r * k2;
-                     const cplx<T> z = bufZ[r * ldz + (j0 + j) * k2 + n];
-                     return cplx<T>{z.re, -z.im};
-                   });
+          tile_mma_range(acc[c], yb, ye,
+                         [&](int i, int y) {
+                           return (i0 + i < Ib && y < ye) ? bufE[(i0 + i) * lde + y] : cz<T>();
+                         },
+                         [&](int y, int j) {
+                           if (j0 + j >= Jb || y >= ye) return cz<T>();
+                           const cplx<T> z = bufZ[(j0 + j) * ldz + y];
+                           return cplx<T>{z.re, -z.im};
+                         });
         }
       }
     }
   }
   cplx<double> *out = Sig + it.out_base + (int64_t)t * it.out_tstride + (int64_t)s * it.out_sstride;
+  if (split) wave_reduce(acc, nti * ntj, reinterpret_cast<T *>(lds));
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
-    const int tt = wave + 4 * c;
-    if (tt < nti * ntj) {
+    const int tt = split ? c : wave + 4 * c;
+    if (tt < nti * ntj && (!split || wave == 0)) {
       const int i0 = (tt / ntj) * 16, j0 = (tt % ntj) * 16;
       tile_store(acc[c], [&](int i, int j, cplx<T> v) {
         if (i0 + i < Ib && j0 + j < Jb)

@@ -295,7 +295,9 @@ bool build_worklists(const Layout &L, int eb, WorkLists &W, std::string &err) {
       }
     }
   }
-  W.p0_lds = std::max<size_t>(p0_lds, 16);
+  // the split-K epilogue reduces 4 waves x 4 tiles x 64 lanes x 8 reals through LDS
+  const size_t red_bytes = (size_t)4 * 4 * 64 * 8 * (eb / 2);
+  W.p0_lds = std::max(p0_lds, red_bytes);
 
   // ---- Sigma: items per (k_ir, ik1, row band, column band); blocks in (q, V-block) order ----
   size_t sig_lds = 0;
@@ -319,15 +321,16 @@ bool build_worklists(const Layout &L, int eb, WorkLists &W, std::string &err) {
           B.g2_tstride = (int32_t)(ns * L.kao_size[k2i]); B.g2_sstride = (int32_t)L.kao_size[k2i];
           B.p_base = L.qaux_off[q] * nts + L.aux_offsets[q][t[0]];
           B.p_tstride = (int32_t)L.qaux_size[q];
-          B.ldz = odd_ld(band * B.k2);
-          B.lde = odd_ld(band * B.k2);
           B.pad = 0;
-          // Z, E and conj(P) rows of one chunk share the LDS budget
-          int per_row = B.q + B.ldz + B.lde;
-          int cap = std::min(16, budget / per_row);
+          // Z ([band][qsub*k2]), E ([band][qsub*k2]) and conj(P) rows ([qsub][q])
+          // of one chunk share the LDS budget (+2 elements of row padding)
+          int per_row = B.q + 2 * band * B.k2;
+          int cap = std::min(16, (budget - 2 * band) / per_row);
           if (cap < 1) { err = fmt("Sigma block too large for the LDS plan (q, k2)", B.q, B.k2); return false; }
           B.qsub = balanced(B.q, cap);
-          sig_lds = std::max(sig_lds, (size_t)(B.qsub * per_row) * eb);
+          B.ldz = odd_ld(B.qsub * B.k2);
+          B.lde = odd_ld(B.qsub * B.k2);
+          sig_lds = std::max(sig_lds, (size_t)(band * (B.ldz + B.lde) + B.qsub * B.q) * eb);
           W.sig_blks.push_back(B);
         }
       }
@@ -345,7 +348,7 @@ bool build_worklists(const Layout &L, int eb, WorkLists &W, std::string &err) {
         }
     }
   }
-  W.sig_lds = std::max<size_t>(sig_lds, 16);
+  W.sig_lds = std::max(sig_lds, red_bytes);
 
   // ---- rotations of G (ao) and P (aux) to the full BZ ----
   for (int which = 0; which < 2; ++which) {
