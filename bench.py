@@ -144,7 +144,19 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mu", type=float, default=0.0)
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="run only a 4 GiB device-to-device copy kernel (scale reference for FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
+    if args.copy_calib:
+        import torch
+        n = 4 << 30
+        a = torch.empty(n // 8, dtype=torch.float64, device="cuda").fill_(1.0)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": n, "copy_bytes_written": n}))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 (the k-pair sharding "
                                    "over GPUs is a design, not part of this build)"}))
