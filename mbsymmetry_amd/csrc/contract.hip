@@ -1,12 +1,13 @@
 // P0 and Sigma block contractions on the CDNA4 matrix cores.
 //
-// Every workgroup (4 waves) owns one output slab and walks its V-block list in
+// Every workgroup owns one output slab and walks its V-block list in
 // a fixed order ("owner computes"), so results are bitwise reproducible with
 // no atomics or locks (the reference serialises per-q / per-k AXPYs behind
 // spin locks, cugw_qkpt_block.cu:269-285 and :414-436).  All complex products
 // are 16x16 tiles built from four real MFMAs (v_mfma_f64_16x16x4f64 for the
 // plain path, v_mfma_f32_16x16x4f32 for the FP32 path); operands are staged in
-// LDS, intermediates never leave the workgroup.
+// LDS, intermediates never leave the workgroup.  A wave works on up to four
+// tiles at a time that share one operand fragment per k-step.
 #include <hip/hip_runtime.h>
 
 #include "common.h"
@@ -16,7 +17,14 @@ namespace symgw {
 
 namespace {
 
-constexpr int NT = 256;
+// Waves per workgroup: FP64 runs 8 waves on the full 160 KiB LDS of a CU,
+// FP32 runs 4 waves on 80 KiB (two workgroups per CU).  Either way two waves
+// share each SIMD.
+template <typename T>
+struct Cfg {
+  static constexpr int NW = sizeof(T) == 8 ? 8 : 4;
+  static constexpr int NT = 64 * NW;
+};
 
 template <typename T>
 struct Mfma;
@@ -38,6 +46,12 @@ template <typename T>
 struct Tile {
   typename Mfma<T>::v4 re, im;
 };
+// Tiles a wave advances together, sharing one operand fragment per k-step:
+// 2 for FP64 and 4 for FP32 (two waves per SIMD within the register file).
+template <typename T>
+struct Group { static constexpr int n = sizeof(T) == 8 ? 2 : 4; };
+template <typename T>
+using Tiles = Tile<T>[4];
 
 template <typename T>
 __device__ __forceinline__ void tile_zero(Tile<T> &t) {
@@ -48,9 +62,8 @@ __device__ __forceinline__ void tile_zero(Tile<T> &t) {
 template <typename T>
 __device__ __forceinline__ cplx<T> cz() { return cplx<T>{T(0), T(0)}; }
 
-// acc (16x16) += A (16 x K) * B (K x 16).  la(i, k) and lb(k, j) return the
-// operand element or zero outside the valid range.  Lane l holds A[l%16][k0+l/16]
-// and B[k0+l/16][l%16] (operand layout of the 16x16x4 MFMAs).
+// acc += a * b for one 16x16x4 complex step (four real MFMAs).  Lane l holds
+// A[l%16][k0+l/16] and B[k0+l/16][l%16] (operand layout of the 16x16x4 MFMAs).
 template <typename T>
 __device__ __forceinline__ void mma4(Tile<T> &acc, const cplx<T> &a, const cplx<T> &b) {
   acc.re = Mfma<T>::op(a.re, b.re, acc.re);
@@ -59,40 +72,120 @@ __device__ __forceinline__ void mma4(Tile<T> &acc, const cplx<T> &a, const cplx<
   acc.im = Mfma<T>::op(a.im, b.re, acc.im);
 }
 
-// Same over the K range [kb, ke) (kb a multiple of 4).  Operands are fetched
-// two k-steps ahead of the MFMAs that use them; the loaders return zero past
-// the end, so the look-ahead never reads out of range.
-template <typename T, typename LA, typename LB>
-__device__ __forceinline__ void tile_mma_range(Tile<T> &acc, int kb, int ke, const LA &la, const LB &lb) {
-  if (kb >= ke) return;
+// The loaders below return the operand element or zero outside the valid
+// range, so the one-step look-ahead never reads out of bounds.
+
+// acc[C0+c] += A (16 x K) * B_c (K x 16) for c < NG with C0 + c < n, over k in
+// [kb, ke): one A fragment per k-step shared by the tiles.  la(i, k), lb(c, k, j)
+// with c the absolute tile index.
+template <int C0, int NG, typename T, typename LA, typename LB>
+__device__ __forceinline__ void mma_rowshare(Tiles<T> &acc, int n, int kb, int ke, const LA &la, const LB &lb) {
+  if (kb >= ke || C0 >= n) return;
   const int lane = threadIdx.x & 63, i = lane & 15, kk = lane >> 4;
-  cplx<T> a0 = la(i, kb + kk), b0 = lb(kb + kk, i);
-  cplx<T> a1 = la(i, kb + 4 + kk), b1 = lb(kb + 4 + kk, i);
+  cplx<T> a0 = la(i, kb + kk), b0[NG];
+#pragma unroll
+  for (int c = 0; c < NG; ++c) b0[c] = C0 + c < n ? lb(C0 + c, kb + kk, i) : cz<T>();
   for (int k0 = kb; k0 < ke; k0 += 4) {
-    const cplx<T> a2 = la(i, k0 + 8 + kk), b2 = lb(k0 + 8 + kk, i);
-    mma4(acc, a0, b0);
-    a0 = a1; b0 = b1;
-    a1 = a2; b1 = b2;
+    const cplx<T> a1 = la(i, k0 + 4 + kk);
+    cplx<T> b1[NG];
+#pragma unroll
+    for (int c = 0; c < NG; ++c) b1[c] = C0 + c < n ? lb(C0 + c, k0 + 4 + kk, i) : cz<T>();
+#pragma unroll
+    for (int c = 0; c < NG; ++c)
+      if (C0 + c < n) mma4(acc[C0 + c], a0, b0[c]);
+    a0 = a1;
+#pragma unroll
+    for (int c = 0; c < NG; ++c) b0[c] = b1[c];
   }
 }
 
-template <typename T, typename LA, typename LB>
-__device__ __forceinline__ void tile_mma(Tile<T> &acc, int K, const LA &la, const LB &lb) {
-  tile_mma_range(acc, 0, K, la, lb);
+// acc[C0+c] += A_c (16 x K) * B (K x 16): one B fragment shared.  la(c, i, k), lb(k, j).
+template <int C0, int NG, typename T, typename LA, typename LB>
+__device__ __forceinline__ void mma_colshare(Tiles<T> &acc, int n, int kb, int ke, const LA &la, const LB &lb) {
+  if (kb >= ke || C0 >= n) return;
+  const int lane = threadIdx.x & 63, i = lane & 15, kk = lane >> 4;
+  cplx<T> b0 = lb(kb + kk, i), a0[NG];
+#pragma unroll
+  for (int c = 0; c < NG; ++c) a0[c] = C0 + c < n ? la(C0 + c, i, kb + kk) : cz<T>();
+  for (int k0 = kb; k0 < ke; k0 += 4) {
+    const cplx<T> b1 = lb(k0 + 4 + kk, i);
+    cplx<T> a1[NG];
+#pragma unroll
+    for (int c = 0; c < NG; ++c) a1[c] = C0 + c < n ? la(C0 + c, i, k0 + 4 + kk) : cz<T>();
+#pragma unroll
+    for (int c = 0; c < NG; ++c)
+      if (C0 + c < n) mma4(acc[C0 + c], a0[c], b0);
+    b0 = b1;
+#pragma unroll
+    for (int c = 0; c < NG; ++c) a0[c] = a1[c];
+  }
 }
 
-// Split-K partner of tile_mma_range: the K range of wave w out of 4.
+// acc[C0+c] += A_c * B_c with independent operands.  la(c, i, k), lb(c, k, j).
+template <int C0, int NG, typename T, typename LA, typename LB>
+__device__ __forceinline__ void mma_multi(Tiles<T> &acc, int n, int kb, int ke, const LA &la, const LB &lb) {
+  if (kb >= ke || C0 >= n) return;
+  const int lane = threadIdx.x & 63, i = lane & 15, kk = lane >> 4;
+  cplx<T> a0[NG], b0[NG];
+#pragma unroll
+  for (int c = 0; c < NG; ++c) {
+    a0[c] = C0 + c < n ? la(C0 + c, i, kb + kk) : cz<T>();
+    b0[c] = C0 + c < n ? lb(C0 + c, kb + kk, i) : cz<T>();
+  }
+  for (int k0 = kb; k0 < ke; k0 += 4) {
+    cplx<T> a1[NG], b1[NG];
+#pragma unroll
+    for (int c = 0; c < NG; ++c) {
+      a1[c] = C0 + c < n ? la(C0 + c, i, k0 + 4 + kk) : cz<T>();
+      b1[c] = C0 + c < n ? lb(C0 + c, k0 + 4 + kk, i) : cz<T>();
+    }
+#pragma unroll
+    for (int c = 0; c < NG; ++c)
+      if (C0 + c < n) mma4(acc[C0 + c], a0[c], b0[c]);
+#pragma unroll
+    for (int c = 0; c < NG; ++c) {
+      a0[c] = a1[c];
+      b0[c] = b1[c];
+    }
+  }
+}
+
+// All four persistent tiles, in groups of Group<T>::n.
+template <typename T, typename LA, typename LB>
+__device__ __forceinline__ void mma_rowshare4(Tiles<T> &acc, int n, int kb, int ke, const LA &la, const LB &lb) {
+  constexpr int NG = Group<T>::n;
+  mma_rowshare<0, NG>(acc, n, kb, ke, la, lb);
+  if (NG == 2) mma_rowshare<2, 2>(acc, n, kb, ke, la, lb);
+}
+template <typename T, typename LA, typename LB>
+__device__ __forceinline__ void mma_multi4(Tiles<T> &acc, int n, int kb, int ke, const LA &la, const LB &lb) {
+  constexpr int NG = Group<T>::n;
+  mma_multi<0, NG>(acc, n, kb, ke, la, lb);
+  if (NG == 2) mma_multi<2, 2>(acc, n, kb, ke, la, lb);
+}
+
+// Split-K share of wave w out of NW over [0, K) (boundaries multiples of 4).
+template <int NW>
 __device__ __forceinline__ void wave_krange(int K, int wave, int &kb, int &ke) {
-  const int q4 = ((K + 15) / 16) * 4;  // per-wave share, a multiple of 4
+  const int q4 = ((K + 4 * NW - 1) / (4 * NW)) * 4;
   kb = min(K, wave * q4);
   ke = min(K, kb + q4);
 }
 
-// Sum the four waves' copies of ntile accumulator tiles in wave order (fixed
-// order: bitwise reproducible) and hand the totals to wave 0.  Uses the LDS
-// at `red` (4 * ntile * 64 * 8 elements of T); callers sync before reuse.
+// Contiguous share [b, e) of n items for wave w out of NW.
+template <int NW>
+__device__ __forceinline__ void wave_share(int n, int wave, int &b, int &e) {
+  const int per = (n + NW - 1) / NW;
+  b = min(n, wave * per);
+  e = min(n, b + per);
+}
+
+// Sum the NW waves' copies of ntile accumulator tiles in wave order (fixed
+// order: bitwise reproducible) and leave the totals in wave 0.  Uses the LDS at
+// `red` (NW * 4 * 8 * 64 elements of T); callers sync before reusing it.
 template <typename T>
-__device__ __forceinline__ void wave_reduce(Tile<T> (&acc)[4], int ntile, T *red) {
+__device__ __forceinline__ void wave_reduce(Tiles<T> &acc, int ntile, T *red) {
+  constexpr int NW = Cfg<T>::NW;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   __syncthreads();
 #pragma unroll
@@ -111,7 +204,7 @@ __device__ __forceinline__ void wave_reduce(Tile<T> (&acc)[4], int ntile, T *red
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           T sre = red[((0 * 4 + c) * 8 + r) * 64 + lane], sim = red[((0 * 4 + c) * 8 + 4 + r) * 64 + lane];
-          for (int w2 = 1; w2 < 4; ++w2) {
+          for (int w2 = 1; w2 < NW; ++w2) {
             sre += red[((w2 * 4 + c) * 8 + r) * 64 + lane];
             sim += red[((w2 * 4 + c) * 8 + 4 + r) * 64 + lane];
           }
@@ -129,6 +222,66 @@ __device__ __forceinline__ void tile_store(const Tile<T> &acc, const F &f) {
   for (int r = 0; r < 4; ++r) f(Mfma<T>::row(lane, r), lane & 15, cplx<T>{acc.re[r], acc.im[r]});
 }
 
+// C (M x N) = A (M x K) B (K x N) into LDS by the waves, for products with
+// at most two column tiles (N <= 32): row tiles are split among the waves and
+// processed in groups of four sharing the B fragment.  la(i_row, k), lb(k, j_col),
+// st(row, col, value) (called only for row < M, col < N).
+template <typename T, typename LA, typename LB, typename ST>
+__device__ __forceinline__ void wg_gemm_rows(int M, int N, int K, const LA &la, const LB &lb, const ST &st) {
+  const int wave = threadIdx.x >> 6;
+  const int ntm = (M + 15) / 16, ntn = (N + 15) / 16;
+  int rb, re;
+  wave_share<Cfg<T>::NW>(ntm, wave, rb, re);
+  for (int nt = 0; nt < ntn; ++nt) {
+    const int n0 = nt * 16;
+    constexpr int NG = Group<T>::n;
+    for (int g = rb; g < re; g += NG) {
+      const int n = min(NG, re - g);
+      Tiles<T> acc;
+#pragma unroll
+      for (int c = 0; c < NG; ++c) tile_zero(acc[c]);
+      mma_colshare<0, NG>(acc, n, 0, K, [&](int c, int i, int k) { return la((g + c) * 16 + i, k); },
+                          [&](int k, int j) { return lb(k, n0 + j); });
+#pragma unroll
+      for (int c = 0; c < NG; ++c)
+        if (c < n)
+          tile_store(acc[c], [&](int i, int j, cplx<T> v) {
+            const int row = (g + c) * 16 + i, col = n0 + j;
+            if (row < M && col < N) st(row, col, v);
+          });
+    }
+  }
+}
+
+// Same for products with few row tiles and many column tiles: column tiles
+// split among the waves, groups of four sharing the A fragment.
+template <typename T, typename LA, typename LB, typename ST>
+__device__ __forceinline__ void wg_gemm_cols(int M, int N, int K, const LA &la, const LB &lb, const ST &st) {
+  const int wave = threadIdx.x >> 6;
+  const int ntm = (M + 15) / 16, ntn = (N + 15) / 16;
+  int cb, ce;
+  wave_share<Cfg<T>::NW>(ntn, wave, cb, ce);
+  for (int mt = 0; mt < ntm; ++mt) {
+    const int m0 = mt * 16;
+    constexpr int NG = Group<T>::n;
+    for (int g = cb; g < ce; g += NG) {
+      const int n = min(NG, ce - g);
+      Tiles<T> acc;
+#pragma unroll
+      for (int c = 0; c < NG; ++c) tile_zero(acc[c]);
+      mma_rowshare<0, NG>(acc, n, 0, K, [&](int i, int k) { return la(m0 + i, k); },
+                          [&](int c, int k, int j) { return lb(k, (g + c) * 16 + j); });
+#pragma unroll
+      for (int c = 0; c < NG; ++c)
+        if (c < n)
+          tile_store(acc[c], [&](int i, int j, cplx<T> v) {
+            const int row = m0 + i, col = (g + c) * 16 + j;
+            if (row < M && col < N) st(row, col, v);
+          });
+    }
+  }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -140,7 +293,7 @@ __device__ __forceinline__ void tile_store(const Tile<T> &acc, const F &f) {
 // chunks of qsub rows sized for the LDS.
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ __launch_bounds__(NT) void p0_mfma_kernel(const P0Item *__restrict__ items, const P0Blk *__restrict__ blks,
+__global__ __launch_bounds__(Cfg<T>::NT) void p0_mfma_kernel(const P0Item *__restrict__ items, const P0Blk *__restrict__ blks,
                                                     int64_t n_work, int nt_half, int nts, int ns,
                                                     const cplx<T> *__restrict__ V, const cplx<T> *__restrict__ G,
                                                     cplx<double> *__restrict__ P0, double pref) {
@@ -148,12 +301,16 @@ __global__ __launch_bounds__(NT) void p0_mfma_kernel(const P0Item *__restrict__ 
   if (w >= n_work) return;
   const int t = (int)(w % nt_half);
   const P0Item it = items[w / nt_half];
+  constexpr int NW = Cfg<T>::NW, NT = Cfg<T>::NT;
   const int tid = threadIdx.x, wave = tid >> 6;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cplx<T> *lds = reinterpret_cast<cplx<T> *>(smem_raw);
   const int q = it.q, nband = it.nband, nct = (q + 15) / 16;
+  // <= 4 column tiles: every wave takes all of them over 1/NW of x
+  // (split-K); otherwise wave w owns column tiles w, w+NW, ...
   const bool split = nct <= 4;
-  Tile<T> acc[4];
+  const int nmine = split ? nct : (nct - wave + NW - 1) / NW;
+  Tiles<T> acc;
 #pragma unroll
   for (int c = 0; c < 4; ++c) tile_zero(acc[c]);
 
@@ -176,75 +333,47 @@ __global__ __launch_bounds__(NT) void p0_mfma_kernel(const P0Item *__restrict__ 
         }
         __syncthreads();
         // E[(r,a)][n] = sum_m V[r][a][m] G2[m][n]
-        {
-          const int M = nr * k1, ntm = (M + 15) / 16, ntn = (k2 + 15) / 16;
-          for (int tt = wave; tt < ntm * ntn; tt += 4) {
-            const int m0 = (tt / ntn) * 16, n0 = (tt % ntn) * 16;
-            Tile<T> e;
-            tile_zero(e);
-            tile_mma(e, k2,
-                     [&](int i, int m) {
-                       const int row = m0 + i;
-                       if (row >= M || m >= k2) return cz<T>();
-                       const int r = row / k1, a = row - r * k1;
-                       return bufV[r * ldk + a * k2 + m];
-                     },
-                     [&](int m, int j) { return (m < k2 && n0 + j < k2) ? G2[m * k2 + n0 + j] : cz<T>(); });
-            tile_store(e, [&](int i, int j, cplx<T> v) {
-              const int row = m0 + i, n = n0 + j;
-              if (row < M && n < k2) {
-                const int r = row / k1, a = row - r * k1;
-                bufE[r * ldk + a * k2 + n] = v;
-              }
+        wg_gemm_rows<T>(
+            nr * k1, k2, k2,
+            [&](int row, int m) {
+              if (row >= nr * k1 || m >= k2) return cz<T>();
+              const int r = row / k1, a = row - r * k1;
+              return bufV[r * ldk + a * k2 + m];
+            },
+            [&](int m, int n) { return (m < k2 && n < k2) ? G2[m * k2 + n] : cz<T>(); },
+            [&](int row, int n, cplx<T> v) {
+              const int r = row / k1, a = row - r * k1;
+              bufE[r * ldk + a * k2 + n] = v;
             });
-          }
-        }
         __syncthreads();
         // C[r][p][n] = sum_a G1[p][a] E[r][a][n]  (into bufV)
-        {
-          const int N = nr * k2, ntm = (k1 + 15) / 16, ntn = (N + 15) / 16;
-          for (int tt = wave; tt < ntm * ntn; tt += 4) {
-            const int m0 = (tt / ntn) * 16, n0 = (tt % ntn) * 16;
-            Tile<T> c;
-            tile_zero(c);
-            tile_mma(c, k1, [&](int i, int a) { return (m0 + i < k1 && a < k1) ? G1[(m0 + i) * k1 + a] : cz<T>(); },
-                     [&](int a, int j) {
-                       const int col = n0 + j;
-                       if (a >= k1 || col >= N) return cz<T>();
-                       const int r = col / k2, n = col - r * k2;
-                       return bufE[r * ldk + a * k2 + n];
-                     });
-            tile_store(c, [&](int i, int j, cplx<T> v) {
-              const int p = m0 + i, col = n0 + j;
-              if (p < k1 && col < N) {
-                const int r = col / k2, n = col - r * k2;
-                bufV[r * ldk + p * k2 + n] = v;
-              }
+        wg_gemm_cols<T>(
+            k1, nr * k2, k1, [&](int p, int a) { return (p < k1 && a < k1) ? G1[p * k1 + a] : cz<T>(); },
+            [&](int a, int col) {
+              if (a >= k1 || col >= nr * k2) return cz<T>();
+              const int r = col / k2, n = col - r * k2;
+              return bufE[r * ldk + a * k2 + n];
+            },
+            [&](int p, int col, cplx<T> v) {
+              const int r = col / k2, n = col - r * k2;
+              bufV[r * ldk + p * k2 + n] = v;
             });
-          }
-        }
         __syncthreads();
-        // acc[r0 + r][P] += sum_x C[r][x] conj(V[P][x]); with <= 4 column tiles every
-        // wave takes all of them over a quarter of x (split-K), else whole tiles
+        // acc[r0 + r][P] += sum_x C[r][x] conj(V[P][x])
         int kb = 0, ke = K;
-        if (split) wave_krange(K, wave, kb, ke);
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-          const int jt = split ? c : wave + 4 * c;
-          if (jt < nct) {
-            const int p0 = jt * 16;
-            tile_mma_range(acc[c], kb, ke,
-                           [&](int i, int x) {
-                             const int rr = i - r0;
-                             return (rr >= 0 && rr < nr && x < ke) ? bufV[rr * ldk + x] : cz<T>();
-                           },
-                           [&](int x, int j) {
-                             if (x >= ke || p0 + j >= q) return cz<T>();
-                             const cplx<T> v = Vb[(int64_t)(p0 + j) * K + x];
-                             return cplx<T>{v.re, -v.im};
-                           });
-          }
-        }
+        if (split) wave_krange<NW>(K, wave, kb, ke);
+        mma_rowshare4(
+            acc, nmine, kb, ke,
+            [&](int i, int x) {
+              const int rr = i - r0;
+              return (rr >= 0 && rr < nr && x < ke) ? bufV[rr * ldk + x] : cz<T>();
+            },
+            [&](int c, int x, int j) {
+              const int P = (split ? c : wave + NW * c) * 16 + j;
+              if (x >= ke || P >= q) return cz<T>();
+              const cplx<T> v = Vb[(int64_t)P * K + x];
+              return cplx<T>{v.re, -v.im};
+            });
       }
     }
   }
@@ -252,8 +381,8 @@ __global__ __launch_bounds__(NT) void p0_mfma_kernel(const P0Item *__restrict__ 
   if (split) wave_reduce(acc, nct, reinterpret_cast<T *>(lds));
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
-    const int jt = split ? c : wave + 4 * c;
-    if (jt < nct && (!split || wave == 0))
+    const int jt = split ? c : wave + NW * c;
+    if (c < nmine && (!split || wave == 0))
       tile_store(acc[c], [&](int i, int j, cplx<T> v) {
         const int P = jt * 16 + j;
         if (i < nband && P < q) out[i * q + P] = cplx<double>{pref * (double)v.re, pref * (double)v.im};
@@ -275,7 +404,7 @@ __global__ __launch_bounds__(NT) void p0_mfma_kernel(const P0Item *__restrict__ 
 // GEMM (q x q x k1 k2) reads V straight from memory.
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ __launch_bounds__(NT) void sigma_mfma_kernel(const SigItem *__restrict__ items,
+__global__ __launch_bounds__(Cfg<T>::NT) void sigma_mfma_kernel(const SigItem *__restrict__ items,
                                                        const SigBlk *__restrict__ blks, int64_t n_work, int nts,
                                                        int ns, const cplx<T> *__restrict__ V,
                                                        const cplx<T> *__restrict__ G, const cplx<T> *__restrict__ Pf,
@@ -285,13 +414,17 @@ __global__ __launch_bounds__(NT) void sigma_mfma_kernel(const SigItem *__restric
   const int per = nts * ns;
   const SigItem it = items[w / per];
   const int ts = (int)(w % per), t = ts / ns, s = ts - (ts / ns) * ns;
+  constexpr int NW = Cfg<T>::NW, NT = Cfg<T>::NT;
   const int tid = threadIdx.x, wave = tid >> 6;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cplx<T> *lds = reinterpret_cast<cplx<T> *>(smem_raw);
   const int k1 = it.k1, I0 = it.I0, Ib = it.Ib, J0 = it.J0, Jb = it.Jb;
-  const int nti = (Ib + 15) / 16, ntj = (Jb + 15) / 16;
-  const bool split = nti * ntj <= 4;
-  Tile<T> acc[4];
+  const int nti = (Ib + 15) / 16, ntj = (Jb + 15) / 16, ntile = nti * ntj;
+  // <= 4 output tiles: every wave takes all of them over 1/NW of the (Q, n)
+  // reduction (split-K); otherwise wave w owns tiles w, w+NW, ...
+  const bool split = ntile <= 4;
+  const int nmine = split ? ntile : (ntile - wave + NW - 1) / NW;
+  Tiles<T> acc;
 #pragma unroll
   for (int c = 0; c < 4; ++c) tile_zero(acc[c]);
 
@@ -306,6 +439,7 @@ __global__ __launch_bounds__(NT) void sigma_mfma_kernel(const SigItem *__restric
     cplx<T> *bufE = bufZ + Jb * ldz;     // [Ib][lde]  E[r][i][n] at i*lde + y
     cplx<T> *bufP = bufE + Ib * lde;     // [qsub][q]  conj(P) rows
     const int zc = Jb * k2;
+    const cplx<T> *Vj = Vb + J0 * k2;
     for (int Q0 = 0; Q0 < q; Q0 += qsub) {
       const int nr = min(qsub, q - Q0);
       __syncthreads();
@@ -315,78 +449,51 @@ __global__ __launch_bounds__(NT) void sigma_mfma_kernel(const SigItem *__restric
       }
       __syncthreads();
       // Z[r][x] = sum_P conj(P[Q0+r][P]) V[P][J0*k2 + x],  x < Jb*k2
-      {
-        const int ntn = (zc + 15) / 16;
-        const cplx<T> *Vj = Vb + J0 * k2;
-        for (int nt = wave; nt < ntn; nt += 4) {
-          const int n0 = nt * 16;
-          Tile<T> z;
-          tile_zero(z);
-          tile_mma(z, q, [&](int i, int P) { return (i < nr && P < q) ? bufP[i * q + P] : cz<T>(); },
-                   [&](int P, int j) { return (P < q && n0 + j < zc) ? Vj[(int64_t)P * K + n0 + j] : cz<T>(); });
-          tile_store(z, [&](int i, int j, cplx<T> v) {
-            const int x = n0 + j;
-            if (i < nr && x < zc) {
-              const int jj = x / k2, n = x - jj * k2;
-              bufZ[jj * ldz + i * k2 + n] = v;
-            }
+      wg_gemm_cols<T>(
+          nr, zc, q, [&](int r, int P) { return (r < nr && P < q) ? bufP[r * q + P] : cz<T>(); },
+          [&](int P, int x) { return (P < q && x < zc) ? Vj[(int64_t)P * K + x] : cz<T>(); },
+          [&](int r, int x, cplx<T> v) {
+            const int jj = x / k2, n = x - jj * k2;
+            bufZ[jj * ldz + r * k2 + n] = v;
           });
-        }
-      }
       // E[(r,i)][n] = sum_m V[Q0+r][I0+i][m] G2[m][n]
-      {
-        const int M = nr * Ib, ntm = (M + 15) / 16, ntn = (k2 + 15) / 16;
-        for (int tt = wave; tt < ntm * ntn; tt += 4) {
-          const int m0 = (tt / ntn) * 16, n0 = (tt % ntn) * 16;
-          Tile<T> e;
-          tile_zero(e);
-          tile_mma(e, k2,
-                   [&](int i, int m) {
-                     const int row = m0 + i;
-                     if (row >= M || m >= k2) return cz<T>();
-                     const int r = row / Ib, ii = row - r * Ib;
-                     return Vb[(int64_t)(Q0 + r) * K + (I0 + ii) * k2 + m];
-                   },
-                   [&](int m, int j) { return (m < k2 && n0 + j < k2) ? G2[m * k2 + n0 + j] : cz<T>(); });
-          tile_store(e, [&](int i, int j, cplx<T> v) {
-            const int row = m0 + i, n = n0 + j;
-            if (row < M && n < k2) {
-              const int r = row / Ib, ii = row - r * Ib;
-              bufE[ii * lde + r * k2 + n] = v;
-            }
+      wg_gemm_rows<T>(
+          nr * Ib, k2, k2,
+          [&](int row, int m) {
+            if (row >= nr * Ib || m >= k2) return cz<T>();
+            const int r = row / Ib, ii = row - r * Ib;
+            return Vb[(int64_t)(Q0 + r) * K + (I0 + ii) * k2 + m];
+          },
+          [&](int m, int n) { return (m < k2 && n < k2) ? G2[m * k2 + n] : cz<T>(); },
+          [&](int row, int n, cplx<T> v) {
+            const int r = row / Ib, ii = row - r * Ib;
+            bufE[ii * lde + r * k2 + n] = v;
           });
-        }
-      }
       __syncthreads();
-      // acc[i][j] += sum_y E[i][y] conj(Z[j][y]); with <= 4 tiles every wave
-      // takes all of them over a quarter of y (split-K), else whole tiles
+      // acc[i][j] += sum_y E[i][y] conj(Z[j][y])
       const int Y = nr * k2;
       int yb = 0, ye = Y;
-      if (split) wave_krange(Y, wave, yb, ye);
-#pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        const int tt = split ? c : wave + 4 * c;
-        if (tt < nti * ntj) {
-          const int i0 = (tt / ntj) * 16, j0 = (tt % ntj) * 16;
-          tile_mma_range(acc[c], yb, ye,
-                         [&](int i, int y) {
-                           return (i0 + i < Ib && y < ye) ? bufE[(i0 + i) * lde + y] : cz<T>();
-                         },
-                         [&](int y, int j) {
-                           if (j0 + j >= Jb || y >= ye) return cz<T>();
-                           const cplx<T> z = bufZ[(j0 + j) * ldz + y];
-                           return cplx<T>{z.re, -z.im};
-                         });
-        }
-      }
+      if (split) wave_krange<NW>(Y, wave, yb, ye);
+      mma_multi4(
+          acc, nmine, yb, ye,
+          [&](int c, int i, int y) {
+            const int i0 = ((split ? c : wave + NW * c) / ntj) * 16;
+            return (i0 + i < Ib && y < ye) ? bufE[(i0 + i) * lde + y] : cz<T>();
+          },
+          [&](int c, int y, int j) {
+            const int j0 = ((split ? c : wave + NW * c) % ntj) * 16;
+            if (j0 + j >= Jb || y >= ye) return cz<T>();
+            const cplx<T> z = bufZ[(j0 + j) * ldz + y];
+            return cplx<T>{z.re, -z.im};
+          });
     }
   }
   cplx<double> *out = Sig + it.out_base + (int64_t)t * it.out_tstride + (int64_t)s * it.out_sstride;
-  if (split) wave_reduce(acc, nti * ntj, reinterpret_cast<T *>(lds));
+  if (split) wave_reduce(acc, ntile, reinterpret_cast<T *>(lds));
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
-    const int tt = split ? c : wave + 4 * c;
-    if (tt < nti * ntj && (!split || wave == 0)) {
+    const int tt = split ? c : wave + NW * c;
+    if (c < nmine && (!split || wave == 0)) {
       const int i0 = (tt / ntj) * 16, j0 = (tt % ntj) * 16;
       tile_store(acc[c], [&](int i, int j, cplx<T> v) {
         if (i0 + i < Ib && j0 + j < Jb)
@@ -404,7 +511,8 @@ hipError_t launch_p0(const P0Item *items, const P0Blk *blks, int n_items, int nt
   const int64_t n_work = (int64_t)n_items * nt_half;
   if (n_work == 0) return hipSuccess;
   (void)hipFuncSetAttribute((const void *)p0_mfma_kernel<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  hipLaunchKernelGGL((p0_mfma_kernel<T>), dim3((unsigned)xcd_grid(n_work)), dim3(NT), lds, st, items, blks, n_work,
+  hipLaunchKernelGGL((p0_mfma_kernel<T>), dim3((unsigned)xcd_grid(n_work)), dim3(Cfg<T>::NT), lds, st, items, blks,
+                     n_work,
                      nt_half, nts, ns, V, G, P0, pref);
   return hipGetLastError();
 }
@@ -417,7 +525,7 @@ hipError_t launch_sigma(const SigItem *items, const SigBlk *blks, int n_items, i
   if (n_work == 0) return hipSuccess;
   (void)hipFuncSetAttribute((const void *)sigma_mfma_kernel<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)lds);
-  hipLaunchKernelGGL((sigma_mfma_kernel<T>), dim3((unsigned)xcd_grid(n_work)), dim3(NT), lds, st, items, blks,
+  hipLaunchKernelGGL((sigma_mfma_kernel<T>), dim3((unsigned)xcd_grid(n_work)), dim3(Cfg<T>::NT), lds, st, items, blks,
                      n_work, nts, ns, V, G, Pf, Sig, pref);
   return hipGetLastError();
 }

@@ -249,7 +249,7 @@ int balanced(int n, int cap) {
 bool build_worklists(const Layout &L, int eb, WorkLists &W, std::string &err) {
   W = WorkLists();
   const int nk = L.nk, nts = L.nts, ns = L.ns;
-  const int budget = kContractLds / eb;  // complex elements of LDS per workgroup
+  const int budget = contract_lds(eb) / eb;  // complex elements of LDS per workgroup
   flop_model(L, W.flop_p0, W.flop_sig, W.n_p0_contractions, W.n_sig_contractions);
 
   // ---- P0: items per (q_ir, iq, 16-row band); blocks in (k, V-block) order ----
@@ -295,8 +295,8 @@ bool build_worklists(const Layout &L, int eb, WorkLists &W, std::string &err) {
       }
     }
   }
-  // the split-K epilogue reduces 4 waves x 4 tiles x 64 lanes x 8 reals through LDS
-  const size_t red_bytes = (size_t)4 * 4 * 64 * 8 * (eb / 2);
+  // the split-K epilogue reduces (waves x 4 tiles x 64 lanes x 8 reals) through LDS
+  const size_t red_bytes = (size_t)contract_waves(eb) * 4 * 64 * 8 * (eb / 2);
   W.p0_lds = std::max(p0_lds, red_bytes);
 
   // ---- Sigma: items per (k_ir, ik1, row band, column band); blocks in (q, V-block) order ----
