@@ -60,49 +60,54 @@ def setup_ctx(sysm, precision, mu):
 
 
 def run_path(sysm, precision, steps, warmup, mu, G0):
-    """Time `steps` graph-replayed GW iterations; return stats and Sigma of the first iteration."""
+    """Time `steps` GW iterations replayed from per-phase hipGraphs.
+
+    The first iteration from G0 is untimed (it is warmup step 1) and gives the
+    Sigma and G compared across precisions.  In the timed region HIP events sit
+    between the phase graphs on the context stream, so the per-phase device
+    times are averages over exactly the timed iterations."""
     ctx = setup_ctx(sysm, precision, mu)
     ctx.upload_G(G0)
-    # first iteration eagerly with per-phase HIP-event timing (kernel times on the context stream)
-    ctx.set_timing(True)
-    ctx.gw_iteration(mu, use_graph=False)
+    ctx.gw_iteration(mu, use_graph=True)
     ctx.sync()
-    phase_ms = ctx.last_times()
-    ctx.set_timing(False)
     sig = ctx.copy_Sigma_to_host(sysm.zeros_ts_kij()) / sysm.nk
     G1 = ctx.download_G(G0.shape)
-    ctx.upload_G(G0)
-    for i in range(warmup):
+    log("warmup 1 done (iteration from G0)")
+    for i in range(1, warmup):
         ctx.gw_iteration(mu, use_graph=True)
         ctx.sync()
         log(f"warmup {i + 1}/{warmup} done")
+    ctx.set_timing(True)
     ctx.sync()
+    phase = np.zeros(4)
+    step_s = []
     t0 = time.perf_counter()
     for i in range(steps):
+        ts = time.perf_counter()
         ctx.gw_iteration(mu, use_graph=True)
-    ctx.sync()
+        ctx.sync()
+        step_s.append(time.perf_counter() - ts)
+        phase += np.asarray(ctx.last_times()[:4])
     dt = time.perf_counter() - t0
     log(f"timed {steps} iterations: {dt:.3f} s")
     info = ctx.info()
     ctx.close()
-    return dict(sec_per_iter=dt / steps, phase_ms=phase_ms, flop_p0=info.flop_P0, flop_sig=info.flop_Sigma,
-                sigma=sig, G1=G1)
+    return dict(sec_per_iter=dt / steps, phase_ms=phase / steps, step_s=step_s, flop_p0=info.flop_P0,
+                flop_sig=info.flop_Sigma, sigma=sig, G1=G1)
 
 
-def cpu_baseline(sysm, G0, budget_s=20.0):
+def cpu_baseline(sysm, G0, budget_s=15.0):
     """The FP64 CPU oracle (the reference's CPU loop nest restated) timed on a
-    bounded sample of the same workload: P0 for the first iBZ q, the first
-    k points and one tau point, on every host core; scaled to GW iterations/s
-    with the FLOP model."""
+    bounded sample of the same workload: P0 for min(cores, ink) iBZ q points
+    (one per thread), the first nkk k points and the first ntau tau points,
+    sized to about `budget_s` seconds; scaled to GW iterations/s with the FLOP
+    model."""
     from oracle.oracle import Oracle
     cores = len(os.sched_getaffinity(0))
     orc = Oracle(sysm)
     nk = sysm.nk
-    q = int(sysm.irre_list[0])
-    # sample: k < nkk for q_ir = 0, t = 0
-    offs = sysm.kpair_offsets()
 
-    def flops(nkk):
+    def flops(q, nkk):
         f = 0.0
         for k in range(nkk):
             p = k * nk + int(sysm.kpq[k, q])
@@ -110,29 +115,29 @@ def cpu_baseline(sysm, G0, budget_s=20.0):
                 f += sysm.ns * 8.0 * (a2 * qs * a1 * a1 + qs * a1 * a2 * a2 + qs * qs * a1 * a2)
         return f
 
-    pairs = {k * nk + int(sysm.kpq[k, q]) for k in range(nk)}
+    nq = min(cores, sysm.ink)
+    qs_ = [int(sysm.irre_list[i]) for i in range(nq)]
+    # at most ~256 k-pairs of V are generated on the host (V of config 2 is 8 MiB per pair)
+    nkk = min(nk, max(1, 256 // nq))
+    pairs = {k * nk + int(sysm.kpq[k, q]) for q in qs_ for k in range(nkk)}
     V = sysm.host_V(pairs=pairs)
     Gf = orc.rotate_G(G0)
-    # calibrate on 1 k, then size the sample to ~budget_s (single q -> one thread busy)
+    log(f"cpu baseline: host V of {len(pairs)} k-pairs ready")
+    # calibrate on one k of the first q (one thread), then size the sample
     t0 = time.perf_counter()
     orc.P0_sample(V, Gf, 0, 1, 1, 1, nthreads=1)
-    t1 = time.perf_counter() - t0
-    rate1 = flops(1) / max(t1, 1e-6)
-    nkk = int(min(nk, max(1, budget_s * rate1 / (flops(nk) / nk))))
-    # all cores: the sample spans min(cores, ink) q points of one tau each
-    nq = min(cores, sysm.ink)
+    rate1 = flops(qs_[0], 1) / max(time.perf_counter() - t0, 1e-6)
+    per_tau = max(flops(q, nkk) for q in qs_) / rate1      # seconds for nkk k, one tau, per thread
+    if per_tau >= budget_s:
+        nkk, ntau = max(1, int(nkk * budget_s / per_tau)), 1
+    else:
+        ntau = int(min(sysm.nts // 2, max(1, budget_s // per_tau)))
+    log(f"cpu baseline: timing {nq} q x {nkk} k x {ntau} tau on {nq} threads")
     t0 = time.perf_counter()
-    orc.P0_sample(V, Gf, 0, 1, nq, nkk, nthreads=nq)
+    orc.P0_sample(V, Gf, 0, ntau, nq, nkk, nthreads=nq)
     dt = time.perf_counter() - t0
-    fl = 0.0
-    for iq in range(nq):
-        qq = int(sysm.irre_list[iq])
-        for k in range(nkk):
-            p = k * nk + int(sysm.kpq[k, qq])
-            for (qs, a1, a2) in sysm.V_block_sizes(p):
-                fl += sysm.ns * 8.0 * (a2 * qs * a1 * a1 + qs * a1 * a2 * a2 + qs * qs * a1 * a2)
-    del offs
-    return dict(rate=fl / dt, seconds=dt, cores=nq, nkk=nkk, nq=nq, flops=fl)
+    fl = ntau * sum(flops(q, nkk) for q in qs_)
+    return dict(rate=fl / dt, seconds=dt, cores=nq, nkk=nkk, nq=nq, ntau=ntau, flops=fl)
 
 
 def main():
@@ -153,7 +158,7 @@ def main():
         a = torch.empty(n // 8, dtype=torch.float64, device="cuda").fill_(1.0)
         b = torch.empty_like(a)
         torch.cuda.synchronize()
-        b.copy_(a)
+        torch.mul(a, 1.0, out=b)  # one elementwise kernel: reads 4 GiB, writes 4 GiB
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_read": n, "copy_bytes_written": n}))
         return
@@ -180,12 +185,20 @@ def main():
         mixed = run_path(sysm, _lib.PREC_FP32, args.steps, args.warmup, args.mu, G0)
         log("fp32", mixed["sec_per_iter"], "s/iter; phases ms", mixed["phase_ms"][:4])
 
-    p0_ms, sig_ms = plain["phase_ms"][0], plain["phase_ms"][2]
+    p0_ms, sig_ms = float(plain["phase_ms"][0]), float(plain["phase_ms"][2])
     dom = "Sigma" if sig_ms >= p0_ms else "P0"
     dom_ms = max(sig_ms, p0_ms)
     dom_flop = plain["flop_sig"] if dom == "Sigma" else plain["flop_p0"]
     achieved = dom_flop / (dom_ms * 1e-3) / 1e12
     flops_iter = plain["flop_p0"] + plain["flop_sig"]
+    dom_kernel = ("sigma" if dom == "Sigma" else "p0") + "_mfma_kernel<double>"
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"traffic_{dom.lower()}_{args.config}_fp64.json")
+    if os.path.exists(tpath):  # HBM bytes per launch from the FETCH_SIZE / WRITE_SIZE runs
+        with open(tpath) as f:
+            tr = json.load(f)
+        if tr.get("kernel") == dom_kernel and tr.get("workload") == args.config:
+            traffic = tr["bytes_per_launch"]
     out = {
         "metric": "GW iter/s & P0+Sigma TFLOP/s vs plain FP64 GPU at max|dSigma|<=1e-8 Ha; p99 latency",
         "value": 1.0 / plain["sec_per_iter"],
@@ -194,6 +207,7 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": plain["sec_per_iter"] * 1e3,
+        "ms_per_step_median": float(np.median(plain["step_s"])) * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -201,9 +215,10 @@ def main():
         "data": "synthetic (seeded device-generated V, Haar-random rotations and Fock blocks)",
         "config": {"workload": workload_name(sysm), "flop_P0": plain["flop_p0"], "flop_Sigma": plain["flop_sig"]},
         "p0_sigma_tflops": flops_iter / (1e-3 * (p0_ms + sig_ms)) / 1e12,
-        "phase_ms": {"P0": p0_ms, "W": plain["phase_ms"][1], "Sigma": sig_ms, "Dyson+rotate": plain["phase_ms"][3]},
-        "roofline": {"bound": "mfma", "kernel": f"{dom.lower()}_kernel<double>", "achieved": achieved,
-                     "peak": PEAK["fp64"], "unit": "TFLOP/s", "frac": achieved / PEAK["fp64"], "traffic": None},
+        "phase_ms": {"P0_kernel": p0_ms, "W+P_rotation": float(plain["phase_ms"][1]), "Sigma_kernel": sig_ms,
+                     "Dyson+G_rotation": float(plain["phase_ms"][3])},
+        "roofline": {"bound": "mfma", "kernel": dom_kernel, "achieved": achieved, "peak": PEAK["fp64"],
+                     "unit": "TFLOP/s", "frac": achieved / PEAK["fp64"], "traffic": traffic},
     }
     if mixed is not None:
         dsig = float(np.abs(mixed["sigma"] - plain["sigma"]).max())
@@ -217,7 +232,7 @@ def main():
         out["cpu_baseline"] = {"value": cb["rate"] / flops_iter, "unit": "GW iterations/s", "cores": cb["cores"],
                                "kind": "port",
                                "sample": f"FP64 oracle P0 loop nest (gw_block.cpp:6-84 restated), {cb['nq']} iBZ q x "
-                                         f"{cb['nkk']} k x 1 tau, {cb['flops'] / 1e9:.1f} GFLOP in {cb['seconds']:.1f} s "
+                                         f"{cb['nkk']} k x {cb['ntau']} tau, {cb['flops'] / 1e9:.1f} GFLOP in {cb['seconds']:.1f} s "
                                          f"({cb['rate'] / 1e9:.2f} GFLOP/s), scaled by the FLOP model"}
     print(json.dumps(out), flush=True)
 

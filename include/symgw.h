@@ -204,14 +204,19 @@ int symgw_mu_search(symgw_ctx *ctx, double nel_target, const double *kweights,
 
 /* One GW iteration on the device: P0 -> W -> Sigma -> Dyson(mu), ending with
  * the rotated full-BZ G ready for the next iteration.  use_graph != 0
- * captures the launch sequence once as a hipGraph and replays it. */
+ * captures the launch sequence of each phase once as a hipGraph (re-captured
+ * when mu changes) and replays the four graphs; with timing on, HIP events
+ * between the replays give the per-phase device times. */
 int symgw_gw_iteration(symgw_ctx *ctx, double mu, int use_graph);
 
 /* Wait for all work queued on the context's stream. */
 int symgw_sync(symgw_ctx *ctx);
-/* Average device time (ms) of the named kernel family over the last
- * symgw_gw_iteration / solve calls measured with HIP events on the context
- * stream: which = 0 P0, 1 W, 2 Sigma, 3 Dyson, 4 G rotation. */
+/* Device time (ms) of the phases of the last timed symgw_gw_iteration,
+ * from HIP events on the context stream: ms5[0] P0 contraction kernel,
+ * [1] W (transforms, solves, rotation of P to the full BZ), [2] Sigma
+ * contraction kernel, [3] Dyson (transforms, solves, rotation of G), [4] the
+ * whole iteration.  symgw_solve_P0 / symgw_solve_Sigma with timing on set
+ * [0] / [2]. */
 int symgw_last_times(const symgw_ctx *ctx, double *ms5);
 /* Enable/disable per-phase HIP event timing (off by default). */
 int symgw_set_timing(symgw_ctx *ctx, int on);

@@ -73,10 +73,13 @@ struct symgw_ctx {
 
   bool have_V = false, have_G = false, have_reps = false, have_T = false, have_F = false, have_P = false,
        have_Sig = false;
+  bool p_rotated = false;  // P_full holds the rotation of the current P
   bool timing = false;
   double last_ms[5] = {0, 0, 0, 0, 0};
   hipEvent_t ev[6] = {};
-  hipGraphExec_t graph_exec = nullptr;
+  // one hipGraph per phase (P0, W, Sigma, Dyson), replayed back to back on the
+  // context stream; with timing on, events sit between the replays
+  hipGraphExec_t graph_exec[4] = {};
   double graph_mu = 0.0;
 };
 
@@ -183,6 +186,15 @@ int run_sigma(symgw_ctx *c) {
   return SYMGW_OK;
 }
 
+// P (iBZ q) -> P_full (every q of the full BZ).
+int rotate_P(symgw_ctx *c) {
+  if (!c->have_reps && c->L.flat_auxrep > 0) return fail(c, SYMGW_ERR_STATE, "rotation matrices not set");
+  int r = c->prec == SYMGW_PREC_FP64 ? run_rotate_P<double>(c) : run_rotate_P<float>(c);
+  if (r) return r;
+  c->p_rotated = true;
+  return SYMGW_OK;
+}
+
 int phase_P0(symgw_ctx *c) {
   if (!c->have_V) return fail(c, SYMGW_ERR_STATE, "V not set");
   if (!c->have_G) return fail(c, SYMGW_ERR_STATE, "G not uploaded");
@@ -190,6 +202,7 @@ int phase_P0(symgw_ctx *c) {
   int r = c->prec == SYMGW_PREC_FP64 ? run_p0<double>(c) : run_p0<float>(c);
   if (r) return r;
   (void)L;
+  c->p_rotated = false;
   return SYMGW_OK;
 }
 
@@ -206,14 +219,14 @@ int phase_W(symgw_ctx *c) {
   HIPCHK(c, launch_tgemm(c->d_tb_wb, L.ink, c->max_qslice, L.nts, L.nw_b, 1, c->Ttn_FB, c->Pw, c->P, 0, 0, L.nts,
                          1.0, 0, 0, c->st));
   c->have_P = true;
-  return SYMGW_OK;
+  // P of every full-BZ q for the Sigma contraction (cugw_qkpt_block.cu:287-340)
+  return rotate_P(c);
 }
 
 int phase_Sigma(symgw_ctx *c) {
   if (!c->have_V || !c->have_G) return fail(c, SYMGW_ERR_STATE, "V or G missing");
   if (!c->have_P) return fail(c, SYMGW_ERR_STATE, "P not available");
-  if (!c->have_reps && c->L.flat_auxrep > 0) return fail(c, SYMGW_ERR_STATE, "rotation matrices not set");
-  int r = c->prec == SYMGW_PREC_FP64 ? run_rotate_P<double>(c) : run_rotate_P<float>(c);
+  int r = c->p_rotated ? SYMGW_OK : rotate_P(c);
   if (r) return r;
   r = c->prec == SYMGW_PREC_FP64 ? run_sigma<double>(c) : run_sigma<float>(c);
   if (r) return r;
@@ -246,18 +259,30 @@ void ev_record(symgw_ctx *c, int i) {
   if (c->timing) (void)hipEventRecord(c->ev[i], c->st);
 }
 
+int run_phase(symgw_ctx *c, int phase, double mu) {
+  switch (phase) {
+    case 0: return phase_P0(c);
+    case 1: return phase_W(c);
+    case 2: return phase_Sigma(c);
+    default: return phase_Dyson(c, mu, 1.0 / c->L.nk);
+  }
+}
+
 int iteration_body(symgw_ctx *c, double mu) {
   int r;
   ev_record(c, 0);
-  if ((r = phase_P0(c))) return r;
-  ev_record(c, 1);
-  if ((r = phase_W(c))) return r;
-  ev_record(c, 2);
-  if ((r = phase_Sigma(c))) return r;
-  ev_record(c, 3);
-  if ((r = phase_Dyson(c, mu, 1.0 / c->L.nk))) return r;
-  ev_record(c, 4);
+  for (int ph = 0; ph < 4; ++ph) {
+    if ((r = run_phase(c, ph, mu))) return r;
+    ev_record(c, ph + 1);
+  }
   return SYMGW_OK;
+}
+
+void drop_graphs(symgw_ctx *c) {
+  for (auto &g : c->graph_exec) {
+    if (g) (void)hipGraphExecDestroy(g);
+    g = nullptr;
+  }
 }
 
 }  // namespace
@@ -452,7 +477,7 @@ int symgw_destroy(symgw_ctx *ctx) {
   if (!ctx) return SYMGW_OK;
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->st);
-  if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
+  drop_graphs(ctx);
   for (auto &b : ctx->owned) (void)hipFree(b.p);
   for (auto &e : ctx->ev) (void)hipEventDestroy(e);
   (void)hipStreamDestroy(ctx->st);
@@ -599,6 +624,7 @@ int symgw_set_reps(symgw_ctx *ctx, const double *orep_flat, const double *auxrep
     HIPCHK(ctx, hipMemcpy(ctx->auxrep, auxrep_flat, (size_t)ctx->L.flat_auxrep * 16, hipMemcpyHostToDevice));
   }
   ctx->have_reps = true;
+  ctx->p_rotated = false;
   return SYMGW_OK;
 }
 
@@ -657,6 +683,7 @@ int symgw_upload_P(symgw_ctx *ctx, const double *P_q_t_QP) {
   HIPCHK(ctx, hipSetDevice(ctx->device));
   HIPCHK(ctx, hipMemcpy(ctx->P, P_q_t_QP, (size_t)ctx->L.nts * ctx->L.flat_aux * 16, hipMemcpyHostToDevice));
   ctx->have_P = true;
+  ctx->p_rotated = false;
   return SYMGW_OK;
 }
 
@@ -806,29 +833,36 @@ int symgw_gw_iteration(symgw_ctx *ctx, double mu, int use_graph) {
     int r = iteration_body(ctx, mu);
     if (r) return r;
   } else {
-    if (ctx->graph_exec && ctx->graph_mu != mu) {
-      (void)hipGraphExecDestroy(ctx->graph_exec);
-      ctx->graph_exec = nullptr;
-    }
-    if (!ctx->graph_exec) {
+    if (ctx->graph_exec[0] && ctx->graph_mu != mu) drop_graphs(ctx);
+    if (!ctx->graph_exec[0]) {
       const bool tm = ctx->timing;
       ctx->timing = false;
-      hipGraph_t g = nullptr;
-      HIPCHK(ctx, hipStreamBeginCapture(ctx->st, hipStreamCaptureModeThreadLocal));
-      int r = iteration_body(ctx, mu);
-      hipError_t e = hipStreamEndCapture(ctx->st, &g);
-      ctx->timing = tm;
-      if (r) {
+      for (int ph = 0; ph < 4; ++ph) {
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamBeginCapture(ctx->st, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) {
+          ctx->timing = tm;
+          drop_graphs(ctx);
+          return fail(ctx, SYMGW_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+        }
+        int r = run_phase(ctx, ph, mu);
+        e = hipStreamEndCapture(ctx->st, &g);
+        if (!r && e == hipSuccess) e = hipGraphInstantiate(&ctx->graph_exec[ph], g, nullptr, nullptr, 0);
         if (g) (void)hipGraphDestroy(g);
-        return r;
+        if (r || e != hipSuccess) {
+          ctx->timing = tm;
+          drop_graphs(ctx);
+          return r ? r : fail(ctx, SYMGW_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+        }
       }
-      if (e != hipSuccess) return fail(ctx, SYMGW_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
-      e = hipGraphInstantiate(&ctx->graph_exec, g, nullptr, nullptr, 0);
-      (void)hipGraphDestroy(g);
-      if (e != hipSuccess) return fail(ctx, SYMGW_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+      ctx->timing = tm;
       ctx->graph_mu = mu;
     }
-    HIPCHK(ctx, hipGraphLaunch(ctx->graph_exec, ctx->st));
+    ev_record(ctx, 0);
+    for (int ph = 0; ph < 4; ++ph) {
+      HIPCHK(ctx, hipGraphLaunch(ctx->graph_exec[ph], ctx->st));
+      ev_record(ctx, ph + 1);
+    }
   }
   return SYMGW_OK;
 }
@@ -838,9 +872,10 @@ int symgw_sync(symgw_ctx *ctx) {
   HIPCHK(ctx, hipSetDevice(ctx->device));
   HIPCHK(ctx, hipStreamSynchronize(ctx->st));
   if (ctx->timing) {
-    for (int i = 0; i < 4; ++i) {
+    for (int i = 0; i < 5; ++i) {
       float ms = 0;
-      if (hipEventElapsedTime(&ms, ctx->ev[i], ctx->ev[i + 1]) == hipSuccess) ctx->last_ms[i] = ms;
+      const int a = i < 4 ? i : 0, b = i < 4 ? i + 1 : 4;
+      if (hipEventElapsedTime(&ms, ctx->ev[a], ctx->ev[b]) == hipSuccess) ctx->last_ms[i] = ms;
       else (void)hipGetLastError();
     }
   }
