@@ -257,11 +257,67 @@ def synthetic(name: str, n: int, ink: int, ao_sizes: List[int], aux_sizes: List[
             irre_index[k] = i
     conj_index = np.arange(nk, dtype=np.int32)
     kweights = np.array([len(st) for st in stars], dtype=np.float64)
+    return _assemble(name, rng, kpq, kmq, irre_list, index, irre_index, conj_index, kweights, ao_sizes, aux_sizes,
+                     ns, transforms, seed, v_scale, all_pairs)
+
+
+def time_reversal_square(ns: int = 1, ao=(3, 2), aux=(4, 3), ni: int = 8, beta: float = 10.0, seed: int = 13,
+                         all_pairs: bool = False) -> System:
+    """3x3 two-dimensional mesh whose symmetry is the x <-> y mirror plus time
+    reversal k -> -k.  iBZ = {(0,0), (1,0), (1,1), (1,2)}; the other points
+    cover every way the reference derives a full-BZ slice
+    (cugw_routines_block.cu:124-186): rotation of a representative ((0,1),
+    (2,1)), complex conjugate of a representative ((2,0), (2,2)), and complex
+    conjugate of a rotated point ((0,2) = -(0,1)).  Every k carries the same
+    block sizes and each rotation maps block b to block b."""
+    from .transforms import chebyshev
+    n = 3
+    pts = np.array([(i, j) for i in range(n) for j in range(n)], dtype=np.int64)
+    nk = len(pts)
+
+    def idx(v):
+        v = np.mod(v, n)
+        return int(v[0] * n + v[1])
+
+    kpq = np.array([[idx(a + b) for b in pts] for a in pts], dtype=np.int32)
+    kmq = np.array([[idx(a - b) for b in pts] for a in pts], dtype=np.int32)
+    reps = [idx((0, 0)), idx((1, 0)), idx((1, 1)), idx((1, 2))]
+    irre_list = np.array(reps, dtype=np.int32)
+    index = np.zeros(nk, dtype=np.int32)
+    conj_index = np.arange(nk, dtype=np.int32)
+    for k, v in enumerate(pts):
+        mir = idx(v[::-1])
+        if k in reps:
+            index[k] = k
+        elif mir in reps:                     # reached by the mirror
+            index[k] = mir
+        else:                                 # reached by time reversal (after the mirror or not)
+            ck = idx(-v)
+            conj_index[k] = ck
+            index[k] = ck if ck in reps else idx(pts[ck][::-1])
+            assert index[k] in reps
+    for k in range(nk):                       # the TR partner's own representative
+        ck = conj_index[k]
+        if ck != k:
+            assert conj_index[ck] == ck
+    irre_index = np.array([reps.index(int(index[k])) for k in range(nk)], dtype=np.int32)
+    kweights = np.array([float(np.sum(irre_index == i)) for i in range(len(reps))])
+    rng = np.random.default_rng(seed)
+    return _assemble("tr_square", rng, kpq, kmq, irre_list, index, irre_index, conj_index, kweights, list(ao),
+                     list(aux), ns, chebyshev(ni, beta), seed, 1.0, all_pairs)
+
+
+def _assemble(name, rng, kpq, kmq, irre_list, index, irre_index, conj_index, kweights, ao_sizes, aux_sizes, ns,
+              transforms, seed, v_scale, all_pairs) -> System:
+    """Rotation blocks, V pair list, Fock matrices and the System record of a
+    mesh whose points all carry the same block sizes."""
+    nk = len(index)
+    ink = len(irre_list)
     aos = [list(ao_sizes) for _ in range(nk)]
     auxs = [list(aux_sizes) for _ in range(nk)]
     orep_irreps, auxrep_irreps, orep, auxrep = [], [], [], []
     for k in range(nk):
-        if index[k] == k:
+        if index[k] == k or conj_index[k] != k:   # representative, or derived through its TR partner
             orep_irreps.append([])
             auxrep_irreps.append([])
             continue

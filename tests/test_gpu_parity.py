@@ -22,6 +22,7 @@ def systems(lih, config1):
     return {
         "tiny_rot_ns2": S.tiny(ns=2, n=2, ink=4, ao=(3, 2), aux=(4, 3), ni=8, beta=10.0, seed=11),
         "tiny_3cube": S.tiny(ns=1, n=3, ink=4, ao=(2, 3, 1), aux=(3, 2), ni=6, beta=8.0, seed=5),
+        "tr_square": S.time_reversal_square(ns=2, ao=(5, 3, 2), aux=(12, 8, 6), ni=10, beta=12.0, seed=13),
         "config1": config1,
         "lih": lih,
     }
@@ -46,7 +47,7 @@ def _rel(a, b):
     return np.abs(a - b).max() / max(1.0, np.abs(b).max())
 
 
-@pytest.mark.parametrize("name", ["tiny_rot_ns2", "tiny_3cube", "config1", "lih"])
+@pytest.mark.parametrize("name", ["tiny_rot_ns2", "tiny_3cube", "tr_square", "config1", "lih"])
 def test_P0_parity(name, systems):
     sysm = systems[name]
     _say("P0", name)
@@ -68,7 +69,7 @@ def test_P0_parity(name, systems):
     ctx.close()
 
 
-@pytest.mark.parametrize("name", ["tiny_rot_ns2", "tiny_3cube", "config1", "lih"])
+@pytest.mark.parametrize("name", ["tiny_rot_ns2", "tiny_3cube", "tr_square", "config1", "lih"])
 def test_W_and_Sigma_parity(name, systems):
     sysm = systems[name]
     _say("W+Sigma", name)
@@ -90,7 +91,7 @@ def test_W_and_Sigma_parity(name, systems):
     ctx.close()
 
 
-@pytest.mark.parametrize("name", ["tiny_rot_ns2", "config1", "lih"])
+@pytest.mark.parametrize("name", ["tiny_rot_ns2", "tr_square", "config1", "lih"])
 def test_dyson_parity(name, systems):
     sysm = systems[name]
     _say("Dyson", name)
@@ -107,7 +108,7 @@ def test_dyson_parity(name, systems):
     ctx.close()
 
 
-@pytest.mark.parametrize("name", ["tiny_rot_ns2", "config1"])
+@pytest.mark.parametrize("name", ["tiny_rot_ns2", "tr_square", "config1"])
 def test_gw_iteration_parity_and_graph(name, systems):
     sysm = systems[name]
     _say("iteration", name)
@@ -190,3 +191,29 @@ def test_density_and_mu_search(systems):
     mu, n = ctx.mu_search(target, sysm.kweights, 0.0, -1.0, 1.0, 1e-10)
     assert abs(n - target) < 1e-9 and abs(mu - 0.1) < 1e-6
     ctx.close()
+
+
+@pytest.mark.parametrize("name", ["lih", "config1"])
+def test_iteration_matches_golden(name, systems):
+    """Sigma and the new G of one device GW iteration against the committed
+    oracle regression vectors (tests/golden/oracle_golden.json)."""
+    import json
+    import os
+    from tests.golden.make_oracle_golden import projections
+    sysm = systems[name]
+    _say("golden", name)
+    with open(os.path.join(os.path.dirname(__file__), "golden", "oracle_golden.json")) as f:
+        gold = json.load(f)
+    mu = gold["mu"]
+    order = sorted(["G1", "P", "P0", "Sigma"])
+    _, ctx, G0 = _setup(sysm, mu=mu)
+    ctx.gw_iteration(mu, use_graph=True)
+    ctx.sync()
+    got = {"Sigma": ctx.copy_Sigma_to_host(sysm.zeros_ts_kij()) / sysm.nk, "G1": ctx.download_G(G0.shape)}
+    ctx.close()
+    for k, v in got.items():
+        g = projections(v, seed=1000 + order.index(k))
+        ref = gold["systems"][name][k]
+        tol = 1e-10 * ref["norm"]
+        assert abs(g["norm"] - ref["norm"]) < tol, (name, k)
+        assert np.abs(np.array(g["proj"]) - np.array(ref["proj"])).max() < 10 * tol, (name, k)

@@ -21,13 +21,19 @@ def tiny_big():
     return S.tiny(ns=1, n=3, ink=4, ao=(2, 3, 1), aux=(3, 2), ni=6, beta=8.0, seed=5)
 
 
+@pytest.fixture(scope="module")
+def tiny_tr():
+    # 3x3 mesh with mirror + time reversal: conjugate partners, ns = 2
+    return S.time_reversal_square(ns=2, ao=(3, 2), aux=(4, 3), ni=8, beta=10.0, seed=13)
+
+
 def _G_full(orc, sysm, seed=3):
     rng = np.random.default_rng(seed)
     G = rng.standard_normal((sysm.nts, sysm.ns, sysm.flat_ao)) + 1j * rng.standard_normal((sysm.nts, sysm.ns, sysm.flat_ao))
     return G, orc.rotate_G(G)
 
 
-@pytest.mark.parametrize("name", ["tiny_rot", "tiny_big"])
+@pytest.mark.parametrize("name", ["tiny_rot", "tiny_big", "tiny_tr"])
 def test_P0_matches_bruteforce(name, request):
     sysm = request.getfixturevalue(name)
     orc = Oracle(sysm)
@@ -39,7 +45,7 @@ def test_P0_matches_bruteforce(name, request):
     assert np.abs(got - ref).max() < 1e-11 * max(1.0, np.abs(ref).max())
 
 
-@pytest.mark.parametrize("name", ["tiny_rot", "tiny_big"])
+@pytest.mark.parametrize("name", ["tiny_rot", "tiny_big", "tiny_tr"])
 def test_Sigma_matches_bruteforce(name, request):
     sysm = request.getfixturevalue(name)
     orc = Oracle(sysm)
@@ -76,6 +82,67 @@ def test_rotation_is_O_G_OH(tiny_rot):
         if sysm.index[k] == k:
             sl = slice(koffi[ik], koffi[ik] + sysm.kslice(k))
             assert np.array_equal(Gf[:, :, koff[k]:koff[k] + sysm.kslice(k)], G[:, :, sl])
+
+
+def _expected_full(sysm, X_ibz, lead, sizes, block_offs, slice_len, off_irre, reps, rep_irreps, off_full):
+    """Full-BZ slices from the definition (cugw_routines_block.cu:124-186,
+    cugw_qkpt_block.cu:288-340): the TR partner ck of k is copied from its
+    iBZ slice when it is a representative, else rotated block by block
+    (O X O^H); the slice of k is the complex conjugate when k != ck."""
+    out = np.zeros(lead + (off_full[-1] + slice_len(sysm.nk - 1),), dtype=np.complex128)
+    rep_base = np.cumsum([0] + [sum(sizes(k)[a1] * sizes(int(sysm.index[k]))[a2] for a1, a2 in rep_irreps[k])
+                                for k in range(sysm.nk)])
+    for k in range(sysm.nk):
+        ck = int(sysm.conj_index[k])
+        ik = int(sysm.irre_index[ck])
+        kr = int(sysm.irre_list[ik])
+        src = X_ibz[..., off_irre[ik]:off_irre[ik] + slice_len(kr)]
+        if int(sysm.index[ck]) == ck:
+            Y = src.copy()
+        else:
+            Y = np.zeros(lead + (slice_len(ck),), dtype=np.complex128)
+            ro = rep_base[ck]
+            for a1, a2 in rep_irreps[ck]:
+                m, n = sizes(ck)[a1], sizes(kr)[a2]
+                O = reps[ro:ro + m * n].reshape(m, n)
+                ro += m * n
+                so, do = block_offs(kr)[a2], block_offs(ck)[a1]
+                Xb = src[..., so:so + n * n].reshape(lead + (n, n))
+                Y[..., do:do + m * m] = (O @ Xb @ O.conj().T).reshape(lead + (m * m,))
+        if ck != k:
+            Y = Y.conj()
+        out[..., off_full[k]:off_full[k] + slice_len(k)] = Y
+    return out
+
+
+def test_rotation_time_reversal(tiny_tr):
+    """Full-BZ G and P of a system with rotations and time-reversal partners
+    against the definition, slice by slice; the system covers all four ways
+    a slice is derived."""
+    sysm = tiny_tr
+    cases = set()
+    for k in range(sysm.nk):
+        ck = int(sysm.conj_index[k])
+        cases.add((ck != k, int(sysm.index[ck]) != ck))
+    assert cases == {(False, False), (False, True), (True, False), (True, True)}
+    orc = Oracle(sysm)
+    G, Gf = _G_full(orc, sysm)
+    ref = _expected_full(sysm, G, (sysm.nts, sysm.ns), lambda k: sysm.ao_sizes[k], sysm.ao_block_offsets,
+                         sysm.kslice, sysm.kao_off_irre(), sysm.orep, sysm.orep_irreps, H.kao_off_full(sysm))
+    assert np.abs(Gf - ref).max() < 1e-13
+    rng = np.random.default_rng(4)
+    nts = sysm.nts
+    Pi = rng.standard_normal(nts * sysm.flat_aux) + 1j * rng.standard_normal(nts * sysm.flat_aux)
+    Pf = orc.rotate_P(Pi)
+    # q_t_QP: [q][t][slice] -> per-q views with a leading t axis
+    qoi, qof = sysm.qaux_off_irre(), H.qaux_off_full(sysm)
+    Pi_t = np.concatenate([Pi[nts * qoi[i]:nts * (qoi[i] + sysm.qslice(int(q)))].reshape(nts, -1)
+                           for i, q in enumerate(sysm.irre_list)], axis=1)
+    Pf_t = np.concatenate([Pf[nts * qof[q]:nts * (qof[q] + sysm.qslice(q))].reshape(nts, -1)
+                           for q in range(sysm.nk)], axis=1)
+    ref = _expected_full(sysm, Pi_t, (nts,), lambda q: sysm.aux_sizes[q], sysm.aux_block_offsets, sysm.qslice,
+                         qoi, sysm.auxrep, sysm.auxrep_irreps, qof)
+    assert np.abs(Pf_t - ref).max() < 1e-13
 
 
 def test_P0_lindhard_closed_form(tiny_big):
@@ -274,3 +341,21 @@ def test_ir_fixture_tau_mesh(ir_npz):
     tr = ir_from_fixture(ir_npz, 5.0)
     assert tr.nts == 74 and tr.nw == 72 and tr.nw_b == 71
     assert tr.tau[0] == 0.0 and tr.tau[-1] == 5.0 and np.all(np.diff(tr.tau) > 0)
+
+
+@pytest.mark.parametrize("name", ["lih", "config1"])
+def test_oracle_matches_golden(name, request):
+    """The oracle's GW iteration still reproduces the committed regression
+    vectors (tests/golden/make_oracle_golden.py)."""
+    import json
+    import os
+    from tests.golden.make_oracle_golden import iterate, projections
+    with open(os.path.join(os.path.dirname(__file__), "golden", "oracle_golden.json")) as f:
+        gold = json.load(f)["systems"][name]
+    res = iterate(request.getfixturevalue(name))
+    for i, (k, v) in enumerate(sorted(res.items())):
+        got = projections(v, seed=1000 + i)
+        ref = gold[k]
+        tol = 1e-11 * ref["norm"]
+        assert abs(got["norm"] - ref["norm"]) < tol, (name, k)
+        assert np.abs(np.array(got["proj"]) - np.array(ref["proj"])).max() < tol * 10, (name, k)
