@@ -140,6 +140,33 @@ def cpu_baseline(sysm, G0, budget_s=15.0):
     return dict(rate=fl / dt, seconds=dt, cores=nq, nkk=nkk, nq=nq, ntau=ntau, flops=fl)
 
 
+def serving_run(n_requests, rate, with_fp32):
+    """The serving loop on a seeded request mix: Poisson arrivals, HBM-footprint
+    admission, continuous batching over per-system contexts; latency from
+    arrival to the final G on the host."""
+    from mbsymmetry_amd import _lib, serving as SV
+    from mbsymmetry_amd import system as S
+    mix = S.serving_mix(n_requests)
+    arr = SV.poisson_arrivals(len(mix), rate, seed=S.SEED_BASE + 5)
+    reqs = [SV.Request(i, s, float(arr[i]), iterations=it) for i, (s, it) in enumerate(mix)]
+    flops = sum(it * (_lib.layout_query(s).flop_P0 + _lib.layout_query(s).flop_Sigma) for s, it in mix)
+    out = {"requests": len(reqs), "arrival_rate_per_s": rate,
+           "mix": "synthetic systems on 2^3 / 4^3 meshes, nao in {26, 52, 104} (naux = 4 nao), nts 32, "
+                  "2 GW iterations each",
+           "P0_Sigma_TFLOP": flops / 1e12}
+    for name, prec in (("fp64", _lib.PREC_FP64), ("fp32", _lib.PREC_FP32)):
+        if name == "fp32" and not with_fp32:
+            continue
+        r = SV.serve(reqs, device=0, precision=prec, max_active=8)
+        sm = r["summary"]
+        sm["peak_active"] = r["peak_active"]
+        sm["peak_footprint_GiB"] = r["peak_bytes"] / 2 ** 30
+        out[name] = sm
+        log(f"serving {name}: p50 {sm['p50_latency_s']:.3f} s  p99 {sm['p99_latency_s']:.3f} s  "
+            f"{sm['gw_iter_per_s']:.2f} iter/s")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -149,6 +176,9 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mu", type=float, default=0.0)
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serving-requests", type=int, default=32)
+    ap.add_argument("--serving-rate", type=float, default=50.0, help="Poisson arrival rate (requests/s)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="run only a 4 GiB device-to-device copy kernel (scale reference for FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -227,6 +257,8 @@ def main():
                              plain["sec_per_iter"] / mixed["sec_per_iter"], "max_abs_dSigma_vs_plain": dsig,
                              "max_abs_dG_vs_plain": dg, "meets_1e-8": dsig <= 1e-8,
                              "phase_ms": list(map(float, mixed["phase_ms"][:4]))}
+    if not args.no_serving:
+        out["serving"] = serving_run(args.serving_requests, args.serving_rate, not args.no_mixed)
     if not args.no_cpu:
         cb = cpu_baseline(sysm, G0)
         out["cpu_baseline"] = {"value": cb["rate"] / flops_iter, "unit": "GW iterations/s", "cores": cb["cores"],

@@ -110,6 +110,17 @@ typedef struct symgw_layout_info {
  * convention, plus the footprint model used for admission. */
 int symgw_layout_query(const symgw_layout_desc *desc, symgw_layout_info *info);
 
+/* Device bytes a context of this layout allocates in the given precision
+ * (SYMGW_PREC_*): the footprint model behind admission control.  Counterpart
+ * of gw_qkpt_block::size (SymmetryMBPT/include/cuda/cugw_qkpt_block.h:45-53),
+ * for the whole resident system instead of one stream's scratch. */
+int symgw_layout_footprint(const symgw_layout_desc *desc, int precision, int64_t *bytes);
+
+/* Free and total memory of a device (hipMemGetInfo); the check
+ * GW_check_devices_free_space (SymmetryMBPT/src/solver/cugw_solver_block.cpp:471-493)
+ * makes before sizing its streams. */
+int symgw_device_memory(int device, int64_t *free_bytes, int64_t *total_bytes);
+
 /* Offset of the flat V slice of ordered pair (k1, k2) in complex elements, or
  * -1 when the pair is not stored (symmetry_utils.h:114-123). */
 int64_t symgw_kpair_offset(const symgw_ctx *ctx, int k1, int k2);
@@ -120,7 +131,9 @@ int64_t symgw_kpair_offset(const symgw_ctx *ctx, int k1, int k2);
 int symgw_create(const symgw_layout_desc *desc, int device, int precision, symgw_ctx **out);
 int symgw_destroy(symgw_ctx *ctx);
 const char *symgw_last_error(const symgw_ctx *ctx);
-/* Message of the last symgw_create failure (no context exists then). */
+/* Message of the last failure on the calling thread of symgw_create,
+ * symgw_layout_query, symgw_layout_footprint or symgw_device_memory (no
+ * context exists then). */
 const char *symgw_create_error(void);
 int symgw_layout_info_ctx(const symgw_ctx *ctx, symgw_layout_info *info);
 

@@ -44,6 +44,8 @@ class LayoutInfo(C.Structure):
 SIGNATURES = {
     "symgw_layout_query": (C.c_int, [C.POINTER(LayoutDesc), C.POINTER(LayoutInfo)]),
     "symgw_kpair_offset": (C.c_int64, [C.c_void_p, C.c_int, C.c_int]),
+    "symgw_layout_footprint": (C.c_int, [C.POINTER(LayoutDesc), C.c_int, C.POINTER(C.c_int64)]),
+    "symgw_device_memory": (C.c_int, [C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "symgw_create": (C.c_int, [C.POINTER(LayoutDesc), C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
     "symgw_destroy": (C.c_int, [C.c_void_p]),
     "symgw_last_error": (C.c_char_p, [C.c_void_p]),
@@ -235,6 +237,27 @@ def layout_query(system) -> LayoutInfo:
     if rc != SYMGW_OK:
         raise SymGWError(f"symgw_layout_query failed: {lib.symgw_create_error().decode()}")
     return inf
+
+
+def layout_footprint(system, precision=PREC_FP64) -> int:
+    """Device bytes a context of `system` allocates in `precision`."""
+    lib = load()
+    d = system.desc()
+    out = C.c_int64(0)
+    rc = lib.symgw_layout_footprint(C.byref(d), precision, C.byref(out))
+    if rc != SYMGW_OK:
+        raise SymGWError(f"symgw_layout_footprint failed: {lib.symgw_create_error().decode()}")
+    return int(out.value)
+
+
+def device_memory(device=0):
+    """(free, total) bytes of `device`."""
+    lib = load()
+    fr, tot = C.c_int64(0), C.c_int64(0)
+    rc = lib.symgw_device_memory(device, C.byref(fr), C.byref(tot))
+    if rc != SYMGW_OK:
+        raise SymGWError(f"symgw_device_memory failed: {lib.symgw_create_error().decode()}")
+    return int(fr.value), int(tot.value)
 
 
 def generate_V_pair_host(system, seed, scale, decay, k1, k2, n_complex):

@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -97,12 +98,35 @@ int fail(symgw_ctx *c, int code, const std::string &msg) {
       return fail(ctx, SYMGW_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
+// Allocation, release and graph capture are serialised over all contexts of
+// the process: hipMalloc / hipFree must not run on one thread while another
+// thread captures a graph.  Copies and memsets go through the context's own
+// non-blocking stream; null-stream operations would have to wait on (and so
+// break) a capture running on another context's thread.
+std::mutex g_dev_mu;
+
+hipError_t copy_sync(symgw_ctx *c, void *dst, const void *src, size_t bytes, hipMemcpyKind kind) {
+  hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, c->st);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+  return e;
+}
+
+hipError_t dev_malloc(void **p, size_t bytes) {
+  std::lock_guard<std::mutex> lk(g_dev_mu);
+  return hipMalloc(p, bytes);
+}
+
+void dev_free(void *p) {
+  std::lock_guard<std::mutex> lk(g_dev_mu);
+  (void)hipFree(p);
+}
+
 template <typename X>
 int dalloc(symgw_ctx *c, X **p, size_t count) {
   *p = nullptr;
   if (count == 0) count = 1;
   void *q = nullptr;
-  hipError_t e = hipMalloc(&q, count * sizeof(X));
+  hipError_t e = dev_malloc(&q, count * sizeof(X));
   if (e != hipSuccess) {
     (void)hipGetLastError();
     return fail(c, SYMGW_ERR_NOMEM, "hipMalloc of " + std::to_string(count * sizeof(X)) + " bytes failed");
@@ -122,7 +146,7 @@ template <typename X>
 int upload(symgw_ctx *c, X **dst, const std::vector<X> &v) {
   int r = dalloc(c, dst, v.size());
   if (r) return r;
-  if (!v.empty()) HIPCHK(c, hipMemcpy(*dst, v.data(), v.size() * sizeof(X), hipMemcpyHostToDevice));
+  if (!v.empty()) HIPCHK(c, copy_sync(c, *dst, v.data(), v.size() * sizeof(X), hipMemcpyHostToDevice));
   return SYMGW_OK;
 }
 
@@ -321,6 +345,46 @@ int symgw_layout_query(const symgw_layout_desc *desc, symgw_layout_info *info) {
   return SYMGW_OK;
 }
 
+int symgw_layout_footprint(const symgw_layout_desc *desc, int precision, int64_t *bytes) {
+  if (!desc || !bytes) return SYMGW_ERR_ARG;
+  if (precision != SYMGW_PREC_FP64 && precision != SYMGW_PREC_FP32) {
+    g_create_error = "unknown precision mode";
+    return SYMGW_ERR_ARG;
+  }
+  const int eb = precision == SYMGW_PREC_FP64 ? 16 : 8;
+  Layout L;
+  WorkLists W;
+  std::string err;
+  if (!build_layout(desc, L, err) || !build_worklists(L, eb, W, err)) {
+    g_create_error = err;
+    return SYMGW_ERR_ARG;
+  }
+  *bytes = footprint(L, W, eb);
+  return SYMGW_OK;
+}
+
+int symgw_device_memory(int device, int64_t *free_bytes, int64_t *total_bytes) {
+  if (!free_bytes || !total_bytes) return SYMGW_ERR_ARG;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+    (void)hipGetLastError();
+    g_create_error = "no HIP device";
+    return SYMGW_ERR_NODEVICE;
+  }
+  if (device < 0 || device >= ndev) {
+    g_create_error = "device index out of range";
+    return SYMGW_ERR_ARG;
+  }
+  size_t fr = 0, tot = 0;
+  if (hipSetDevice(device) != hipSuccess || hipMemGetInfo(&fr, &tot) != hipSuccess) {
+    g_create_error = "hipMemGetInfo failed";
+    return SYMGW_ERR_HIP;
+  }
+  *free_bytes = (int64_t)fr;
+  *total_bytes = (int64_t)tot;
+  return SYMGW_OK;
+}
+
 int symgw_create(const symgw_layout_desc *desc, int device, int precision, symgw_ctx **out) {
   if (!out) return SYMGW_ERR_ARG;
   *out = nullptr;
@@ -352,11 +416,6 @@ int symgw_create(const symgw_layout_desc *desc, int device, int precision, symgw
     return SYMGW_ERR_NODEVICE;
   }
   c->device = device;
-  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
-    g_create_error = "stream creation failed";
-    return SYMGW_ERR_HIP;
-  }
-  for (auto &e : c->ev) (void)hipEventCreate(&e);
   if (c->W.p0_lds > 160 * 1024 || c->W.sig_lds > 160 * 1024) {
     g_create_error = "LDS plan exceeds 160 KiB";
     return SYMGW_ERR_ARG;
@@ -365,6 +424,14 @@ int symgw_create(const symgw_layout_desc *desc, int device, int precision, symgw
   if (c->L.max_ao_block > 512 || c->L.max_aux_block > 512) {
     g_create_error = "irrep blocks larger than 512 are not supported by the solvers";
     return SYMGW_ERR_ARG;
+  }
+  {
+    std::lock_guard<std::mutex> lk(g_dev_mu);
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
+      g_create_error = "stream creation failed";
+      return SYMGW_ERR_HIP;
+    }
+    for (auto &e : c->ev) (void)hipEventCreate(&e);
   }
   const Layout &L = c->L;
   symgw_ctx *cp = c.get();
@@ -393,7 +460,8 @@ int symgw_create(const symgw_layout_desc *desc, int device, int precision, symgw
   r = r ? r : dalloc(cp, &c->nel_dev, 1);
   // every data buffer starts at zero (outside any graph: plain memsets are fine here)
   for (auto &b : c->owned)
-    if (!r && hipMemset(b.p, 0, b.bytes) != hipSuccess) r = fail(cp, SYMGW_ERR_HIP, "initial memset failed");
+    if (!r && hipMemsetAsync(b.p, 0, b.bytes, c->st) != hipSuccess) r = fail(cp, SYMGW_ERR_HIP, "initial memset failed");
+  if (!r && hipStreamSynchronize(c->st) != hipSuccess) r = fail(cp, SYMGW_ERR_HIP, "initial memset failed");
   // work lists
   r = r ? r : upload(cp, &c->d_p0_items, c->W.p0_items);
   r = r ? r : upload(cp, &c->d_p0_blks, c->W.p0_blks);
@@ -464,6 +532,7 @@ int symgw_create(const symgw_layout_desc *desc, int device, int precision, symgw
   }
   if (r) {
     g_create_error = c->err;
+    std::lock_guard<std::mutex> lk(g_dev_mu);
     for (auto &b : c->owned) (void)hipFree(b.p);
     for (auto &e : c->ev) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(c->st);
@@ -477,10 +546,13 @@ int symgw_destroy(symgw_ctx *ctx) {
   if (!ctx) return SYMGW_OK;
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->st);
-  drop_graphs(ctx);
-  for (auto &b : ctx->owned) (void)hipFree(b.p);
-  for (auto &e : ctx->ev) (void)hipEventDestroy(e);
-  (void)hipStreamDestroy(ctx->st);
+  {
+    std::lock_guard<std::mutex> lk(g_dev_mu);
+    drop_graphs(ctx);
+    for (auto &b : ctx->owned) (void)hipFree(b.p);
+    for (auto &e : ctx->ev) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(ctx->st);
+  }
   delete ctx;
   return SYMGW_OK;
 }
@@ -514,23 +586,23 @@ int symgw_set_V(symgw_ctx *ctx, const double *V_flat, int64_t n_complex) {
   if (n_complex != ctx->L.flat_V) return fail(ctx, SYMGW_ERR_ARG, "V length does not match the layout");
   HIPCHK(ctx, hipSetDevice(ctx->device));
   if (ctx->prec == SYMGW_PREC_FP64) {
-    HIPCHK(ctx, hipMemcpy(ctx->V, V_flat, (size_t)n_complex * 16, hipMemcpyHostToDevice));
+    HIPCHK(ctx, copy_sync(ctx, ctx->V, V_flat, (size_t)n_complex * 16, hipMemcpyHostToDevice));
   } else {
     // staged conversion in 64 MiB chunks
     const int64_t chunk = 4 << 20;
     cplx<double> *stage = nullptr;
-    HIPCHK(ctx, hipMalloc(&stage, (size_t)chunk * 16));
+    HIPCHK(ctx, dev_malloc((void **)&stage, (size_t)chunk * 16));
     for (int64_t o = 0; o < n_complex; o += chunk) {
       int64_t n = std::min(chunk, n_complex - o);
-      hipError_t e = hipMemcpy(stage, V_flat + 2 * o, (size_t)n * 16, hipMemcpyHostToDevice);
+      hipError_t e = copy_sync(ctx, stage, V_flat + 2 * o, (size_t)n * 16, hipMemcpyHostToDevice);
       if (e == hipSuccess) e = launch_convert<float>(stage, static_cast<cplx<float> *>(ctx->V) + o, n, ctx->st);
       if (e == hipSuccess) e = hipStreamSynchronize(ctx->st);
       if (e != hipSuccess) {
-        (void)hipFree(stage);
+        dev_free(stage);
         return fail(ctx, SYMGW_ERR_HIP, std::string("V upload: ") + hipGetErrorString(e));
       }
     }
-    (void)hipFree(stage);
+    dev_free(stage);
   }
   ctx->have_V = true;
   return SYMGW_OK;
@@ -563,10 +635,14 @@ int symgw_generate_V(symgw_ctx *ctx, uint64_t seed, double scale, const double *
   gen_blocks(ctx->L, gb, -1);
   GenBlk *dgb = nullptr;
   double *ddec = nullptr;
-  HIPCHK(ctx, hipMalloc(&dgb, std::max<size_t>(1, gb.size()) * sizeof(GenBlk)));
-  HIPCHK(ctx, hipMalloc(&ddec, (size_t)n_decay * sizeof(double)));
-  hipError_t e = hipMemcpy(dgb, gb.data(), gb.size() * sizeof(GenBlk), hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(ddec, decay, (size_t)n_decay * sizeof(double), hipMemcpyHostToDevice);
+  HIPCHK(ctx, dev_malloc((void **)&dgb, std::max<size_t>(1, gb.size()) * sizeof(GenBlk)));
+  hipError_t e0 = dev_malloc((void **)&ddec, (size_t)n_decay * sizeof(double));
+  if (e0 != hipSuccess) {
+    dev_free(dgb);
+    return fail(ctx, SYMGW_ERR_NOMEM, "V generation: allocation failed");
+  }
+  hipError_t e = copy_sync(ctx, dgb, gb.data(), gb.size() * sizeof(GenBlk), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = copy_sync(ctx, ddec, decay, (size_t)n_decay * sizeof(double), hipMemcpyHostToDevice);
   if (e == hipSuccess) {
     e = ctx->prec == SYMGW_PREC_FP64
             ? launch_genv<double>(dgb, (int)gb.size(), seed, scale, ddec, n_decay, static_cast<cplx<double> *>(ctx->V),
@@ -575,8 +651,8 @@ int symgw_generate_V(symgw_ctx *ctx, uint64_t seed, double scale, const double *
                                  ctx->st);
   }
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->st);
-  (void)hipFree(dgb);
-  (void)hipFree(ddec);
+  dev_free(dgb);
+  dev_free(ddec);
   if (e != hipSuccess) return fail(ctx, SYMGW_ERR_HIP, std::string("V generation: ") + hipGetErrorString(e));
   ctx->have_V = true;
   return SYMGW_OK;
@@ -617,11 +693,11 @@ int symgw_set_reps(symgw_ctx *ctx, const double *orep_flat, const double *auxrep
   HIPCHK(ctx, hipSetDevice(ctx->device));
   if (ctx->L.flat_orep > 0) {
     if (!orep_flat) return fail(ctx, SYMGW_ERR_ARG, "orep missing");
-    HIPCHK(ctx, hipMemcpy(ctx->orep, orep_flat, (size_t)ctx->L.flat_orep * 16, hipMemcpyHostToDevice));
+    HIPCHK(ctx, copy_sync(ctx, ctx->orep, orep_flat, (size_t)ctx->L.flat_orep * 16, hipMemcpyHostToDevice));
   }
   if (ctx->L.flat_auxrep > 0) {
     if (!auxrep_flat) return fail(ctx, SYMGW_ERR_ARG, "auxrep missing");
-    HIPCHK(ctx, hipMemcpy(ctx->auxrep, auxrep_flat, (size_t)ctx->L.flat_auxrep * 16, hipMemcpyHostToDevice));
+    HIPCHK(ctx, copy_sync(ctx, ctx->auxrep, auxrep_flat, (size_t)ctx->L.flat_auxrep * 16, hipMemcpyHostToDevice));
   }
   ctx->have_reps = true;
   ctx->p_rotated = false;
@@ -632,7 +708,7 @@ int symgw_upload_G(symgw_ctx *ctx, const double *G_ts_kij) {
   if (!ctx || !G_ts_kij) return SYMGW_ERR_ARG;
   HIPCHK(ctx, hipSetDevice(ctx->device));
   const Layout &L = ctx->L;
-  HIPCHK(ctx, hipMemcpy(ctx->G_ibz, G_ts_kij, (size_t)L.nts * L.ns * L.flat_ao * 16, hipMemcpyHostToDevice));
+  HIPCHK(ctx, copy_sync(ctx, ctx->G_ibz, G_ts_kij, (size_t)L.nts * L.ns * L.flat_ao * 16, hipMemcpyHostToDevice));
   ctx->have_G = true;
   int r = rotate_G(ctx);
   if (r) return r;
@@ -645,7 +721,7 @@ int symgw_download_G(symgw_ctx *ctx, double *G_ts_kij) {
   HIPCHK(ctx, hipSetDevice(ctx->device));
   HIPCHK(ctx, hipStreamSynchronize(ctx->st));
   const Layout &L = ctx->L;
-  HIPCHK(ctx, hipMemcpy(G_ts_kij, ctx->G_ibz, (size_t)L.nts * L.ns * L.flat_ao * 16, hipMemcpyDeviceToHost));
+  HIPCHK(ctx, copy_sync(ctx, G_ts_kij, ctx->G_ibz, (size_t)L.nts * L.ns * L.flat_ao * 16, hipMemcpyDeviceToHost));
   return SYMGW_OK;
 }
 
@@ -673,7 +749,7 @@ int symgw_copy_P0_to_host(const symgw_ctx *ctx, double *P0_q_t_QP) {
   HIPCHK(c, hipStreamSynchronize(ctx->st));
   const size_t n = (size_t)ctx->L.nts * ctx->L.flat_aux;
   std::vector<double> tmp(2 * n);
-  HIPCHK(c, hipMemcpy(tmp.data(), ctx->P, n * 16, hipMemcpyDeviceToHost));
+  HIPCHK(c, copy_sync(c, tmp.data(), ctx->P, n * 16, hipMemcpyDeviceToHost));
   for (size_t i = 0; i < 2 * n; ++i) P0_q_t_QP[i] += tmp[i];
   return SYMGW_OK;
 }
@@ -681,7 +757,7 @@ int symgw_copy_P0_to_host(const symgw_ctx *ctx, double *P0_q_t_QP) {
 int symgw_upload_P(symgw_ctx *ctx, const double *P_q_t_QP) {
   if (!ctx || !P_q_t_QP) return SYMGW_ERR_ARG;
   HIPCHK(ctx, hipSetDevice(ctx->device));
-  HIPCHK(ctx, hipMemcpy(ctx->P, P_q_t_QP, (size_t)ctx->L.nts * ctx->L.flat_aux * 16, hipMemcpyHostToDevice));
+  HIPCHK(ctx, copy_sync(ctx, ctx->P, P_q_t_QP, (size_t)ctx->L.nts * ctx->L.flat_aux * 16, hipMemcpyHostToDevice));
   ctx->have_P = true;
   ctx->p_rotated = false;
   return SYMGW_OK;
@@ -692,11 +768,11 @@ int symgw_set_transforms(symgw_ctx *ctx, const double *Tnt, const double *Ttn, c
   if (!ctx || !Tnt || !Ttn || !Tnt_BF || !Ttn_FB || !iwn_fermi) return SYMGW_ERR_ARG;
   HIPCHK(ctx, hipSetDevice(ctx->device));
   const Layout &L = ctx->L;
-  HIPCHK(ctx, hipMemcpy(ctx->Tnt, Tnt, (size_t)L.nw * (L.nts - 2) * 16, hipMemcpyHostToDevice));
-  HIPCHK(ctx, hipMemcpy(ctx->Ttn, Ttn, (size_t)L.nts * L.nw * 16, hipMemcpyHostToDevice));
-  HIPCHK(ctx, hipMemcpy(ctx->Tnt_BF, Tnt_BF, (size_t)L.nw_b * (L.nts - 2) * 16, hipMemcpyHostToDevice));
-  HIPCHK(ctx, hipMemcpy(ctx->Ttn_FB, Ttn_FB, (size_t)L.nts * L.nw_b * 16, hipMemcpyHostToDevice));
-  HIPCHK(ctx, hipMemcpy(ctx->iwn, iwn_fermi, (size_t)L.nw * 16, hipMemcpyHostToDevice));
+  HIPCHK(ctx, copy_sync(ctx, ctx->Tnt, Tnt, (size_t)L.nw * (L.nts - 2) * 16, hipMemcpyHostToDevice));
+  HIPCHK(ctx, copy_sync(ctx, ctx->Ttn, Ttn, (size_t)L.nts * L.nw * 16, hipMemcpyHostToDevice));
+  HIPCHK(ctx, copy_sync(ctx, ctx->Tnt_BF, Tnt_BF, (size_t)L.nw_b * (L.nts - 2) * 16, hipMemcpyHostToDevice));
+  HIPCHK(ctx, copy_sync(ctx, ctx->Ttn_FB, Ttn_FB, (size_t)L.nts * L.nw_b * 16, hipMemcpyHostToDevice));
+  HIPCHK(ctx, copy_sync(ctx, ctx->iwn, iwn_fermi, (size_t)L.nw * 16, hipMemcpyHostToDevice));
   ctx->have_T = true;
   return SYMGW_OK;
 }
@@ -734,7 +810,7 @@ int symgw_copy_Sigma_to_host(const symgw_ctx *ctx, double *Sigma_ts_kij) {
   const Layout &L = ctx->L;
   const size_t n = (size_t)L.nts * L.ns * L.flat_ao;
   std::vector<double> tmp(2 * n);
-  HIPCHK(c, hipMemcpy(tmp.data(), ctx->Sig, n * 16, hipMemcpyDeviceToHost));
+  HIPCHK(c, copy_sync(c, tmp.data(), ctx->Sig, n * 16, hipMemcpyDeviceToHost));
   // k_ts_ij -> ts_kij with accumulation (cugw_routines_block.cu:257-273)
   const int nts_ns = L.nts * L.ns;
   for (int ik = 0; ik < L.ink; ++ik) {
@@ -763,7 +839,7 @@ int symgw_upload_Sigma(symgw_ctx *ctx, const double *Sigma_ts_kij) {
       std::memcpy(dst, src, (size_t)sz * 16);
     }
   }
-  HIPCHK(ctx, hipMemcpy(ctx->Sig, tmp.data(), n * 16, hipMemcpyHostToDevice));
+  HIPCHK(ctx, copy_sync(ctx, ctx->Sig, tmp.data(), n * 16, hipMemcpyHostToDevice));
   ctx->have_Sig = true;
   return SYMGW_OK;
 }
@@ -772,8 +848,8 @@ int symgw_set_fock(symgw_ctx *ctx, const double *F, const double *S) {
   if (!ctx || !F || !S) return SYMGW_ERR_ARG;
   HIPCHK(ctx, hipSetDevice(ctx->device));
   const size_t n = (size_t)ctx->L.ns * ctx->L.flat_ao;
-  HIPCHK(ctx, hipMemcpy(ctx->F, F, n * 16, hipMemcpyHostToDevice));
-  HIPCHK(ctx, hipMemcpy(ctx->S, S, n * 16, hipMemcpyHostToDevice));
+  HIPCHK(ctx, copy_sync(ctx, ctx->F, F, n * 16, hipMemcpyHostToDevice));
+  HIPCHK(ctx, copy_sync(ctx, ctx->S, S, n * 16, hipMemcpyHostToDevice));
   ctx->have_F = true;
   return SYMGW_OK;
 }
@@ -833,6 +909,7 @@ int symgw_gw_iteration(symgw_ctx *ctx, double mu, int use_graph) {
     int r = iteration_body(ctx, mu);
     if (r) return r;
   } else {
+    std::unique_lock<std::mutex> lk(g_dev_mu);
     if (ctx->graph_exec[0] && ctx->graph_mu != mu) drop_graphs(ctx);
     if (!ctx->graph_exec[0]) {
       const bool tm = ctx->timing;
@@ -858,6 +935,7 @@ int symgw_gw_iteration(symgw_ctx *ctx, double mu, int use_graph) {
       ctx->timing = tm;
       ctx->graph_mu = mu;
     }
+    lk.unlock();
     ev_record(ctx, 0);
     for (int ph = 0; ph < 4; ++ph) {
       HIPCHK(ctx, hipGraphLaunch(ctx->graph_exec[ph], ctx->st));

@@ -446,3 +446,27 @@ def tiny(ns: int = 1, n: int = 2, ink: int = 4, ao=(3, 2), aux=(4, 3), ni: int =
     """Small synthetic system for brute-force checks."""
     from .transforms import chebyshev
     return synthetic("tiny", n, ink, list(ao), list(aux), ns, chebyshev(ni, beta), seed, all_pairs=all_pairs)
+
+
+def serving_mix(n_requests: int = 32, seed: int = SEED_BASE + 5, meshes=(2, 4), naos=(26, 52, 104),
+                iterations: int = 2):
+    """Seeded request mix for the serving loop: each request is a synthetic
+    system on a Gamma-centred n^3 mesh (n from `meshes`, reduced to 4 / 8 / 16
+    iBZ points for n = 2 / 4 / 6) with nao from `naos` split into
+    max(2, min(8, nao // 13)) blocks and naux = 4 nao in as many blocks;
+    beta = 50 on a 30-node Chebyshev grid, ns = 1.  Returns a list of
+    (system, iterations)."""
+    from .transforms import chebyshev
+    rng = np.random.default_rng(seed)
+    tr = chebyshev(30, 50.0)
+    out = []
+    for i in range(n_requests):
+        n = int(rng.choice(meshes))
+        nao = int(rng.choice(naos))
+        nb = max(2, min(8, nao // 13))
+        ao = split_sizes(rng, nao, nb, max(2, nao // (2 * nb)), max(4, 2 * nao // nb))
+        aux = split_sizes(rng, 4 * nao, nb, max(4, 4 * nao // (2 * nb)), max(8, 8 * nao // nb))
+        ink = {2: 4, 4: 8, 6: 16}[n]
+        sysm = synthetic(f"mix{i}-n{n}-nao{nao}", n, ink, ao, aux, 1, tr, seed * 100 + i)
+        out.append((sysm, iterations))
+    return out
