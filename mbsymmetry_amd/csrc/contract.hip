@@ -164,49 +164,60 @@ __device__ __forceinline__ void mma_multi4(Tiles<T> &acc, int n, int kb, int ke,
   if (NG == 2) mma_multi<2, 2>(acc, n, kb, ke, la, lb);
 }
 
-// Split-K share of wave w out of NW over [0, K) (boundaries multiples of 4).
-template <int NW>
-__device__ __forceinline__ void wave_krange(int K, int wave, int &kb, int &ke) {
-  const int q4 = ((K + 4 * NW - 1) / (4 * NW)) * 4;
-  kb = min(K, wave * q4);
+// Split of K into S parts for part p (boundaries multiples of 4).
+__device__ __forceinline__ void k_part(int K, int S, int p, int &kb, int &ke) {
+  const int q4 = ((K + 4 * S - 1) / (4 * S)) * 4;
+  kb = min(K, p * q4);
   ke = min(K, kb + q4);
 }
 
-// Contiguous share [b, e) of n items for wave w out of NW.
+// Work split of nt accumulator tiles over the NW waves: S waves share each
+// tile over S parts of the reduction (split-K), so nt tiles are spread over
+// NW / S tile groups with at most four tiles per wave.  S minimises the
+// per-wave work ceil(nt / (NW / S)) / S; ties keep the smaller S.
 template <int NW>
-__device__ __forceinline__ void wave_share(int n, int wave, int &b, int &e) {
-  const int per = (n + NW - 1) / NW;
-  b = min(n, wave * per);
-  e = min(n, b + per);
+__device__ __forceinline__ int choose_split(int nt) {
+  int best = 1, num = 1 << 30, den = 1;
+  for (int s = 1; s <= NW; s *= 2) {
+    const int g = NW / s, tpw = (nt + g - 1) / g;
+    if (tpw > 4) continue;
+    if (tpw * den < num * s) {
+      num = tpw;
+      den = s;
+      best = s;
+    }
+  }
+  return best;
 }
 
-// Sum the NW waves' copies of ntile accumulator tiles in wave order (fixed
-// order: bitwise reproducible) and leave the totals in wave 0.  Uses the LDS at
+// Sum the S split-K partial copies of each wave's tiles (waves g*S .. g*S+S-1
+// hold the parts of tile group g) in part order, so the result is bitwise
+// reproducible; part 0 of each group ends with the totals.  Uses the LDS at
 // `red` (NW * 4 * 8 * 64 elements of T); callers sync before reusing it.
 template <typename T>
-__device__ __forceinline__ void wave_reduce(Tiles<T> &acc, int ntile, T *red) {
+__device__ __forceinline__ void split_reduce(Tiles<T> &acc, int nmine, int S, T *red) {
   constexpr int NW = Cfg<T>::NW;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   __syncthreads();
 #pragma unroll
   for (int c = 0; c < 4; ++c)
-    if (c < ntile)
+    if (c < nmine)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         red[((wave * 4 + c) * 8 + r) * 64 + lane] = acc[c].re[r];
         red[((wave * 4 + c) * 8 + 4 + r) * 64 + lane] = acc[c].im[r];
       }
   __syncthreads();
-  if (wave == 0) {
+  if (wave % S == 0) {
 #pragma unroll
     for (int c = 0; c < 4; ++c)
-      if (c < ntile)
+      if (c < nmine)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          T sre = red[((0 * 4 + c) * 8 + r) * 64 + lane], sim = red[((0 * 4 + c) * 8 + 4 + r) * 64 + lane];
-          for (int w2 = 1; w2 < NW; ++w2) {
-            sre += red[((w2 * 4 + c) * 8 + r) * 64 + lane];
-            sim += red[((w2 * 4 + c) * 8 + 4 + r) * 64 + lane];
+          T sre = acc[c].re[r], sim = acc[c].im[r];
+          for (int p = 1; p < S; ++p) {
+            sre += red[(((wave + p) * 4 + c) * 8 + r) * 64 + lane];
+            sim += red[(((wave + p) * 4 + c) * 8 + 4 + r) * 64 + lane];
           }
           acc[c].re[r] = sre;
           acc[c].im[r] = sim;
@@ -222,64 +233,84 @@ __device__ __forceinline__ void tile_store(const Tile<T> &acc, const F &f) {
   for (int r = 0; r < 4; ++r) f(Mfma<T>::row(lane, r), lane & 15, cplx<T>{acc.re[r], acc.im[r]});
 }
 
-// C (M x N) = A (M x K) B (K x N) into LDS by the waves, for products with
-// at most two column tiles (N <= 32): row tiles are split among the waves and
-// processed in groups of four sharing the B fragment.  la(i_row, k), lb(k, j_col),
-// st(row, col, value) (called only for row < M, col < N).
-template <typename T, typename LA, typename LB, typename ST>
-__device__ __forceinline__ void wg_gemm_rows(int M, int N, int K, const LA &la, const LB &lb, const ST &st) {
-  const int wave = threadIdx.x >> 6;
-  const int ntm = (M + 15) / 16, ntn = (N + 15) / 16;
-  int rb, re;
-  wave_share<Cfg<T>::NW>(ntm, wave, rb, re);
-  for (int nt = 0; nt < ntn; ++nt) {
-    const int n0 = nt * 16;
-    constexpr int NG = Group<T>::n;
-    for (int g = rb; g < re; g += NG) {
-      const int n = min(NG, re - g);
-      Tiles<T> acc;
+// Copy n contiguous elements src[0, n) to dst[ (e / len) * ld + e % len ]
+// (rows of length len, row stride ld) through f: all U loads of a thread are
+// issued before its stores, so U * NT loads are in flight per round.
+template <typename T, int U, typename F>
+__device__ __forceinline__ void stage_rows(cplx<T> *dst, int ld, const cplx<T> *__restrict__ src, int n, int len,
+                                           const F &f) {
+  constexpr int NT = Cfg<T>::NT;
+  const int tid = threadIdx.x;
+  for (int base = 0; base < n; base += U * NT) {
+    cplx<T> v[U];
 #pragma unroll
-      for (int c = 0; c < NG; ++c) tile_zero(acc[c]);
-      mma_colshare<0, NG>(acc, n, 0, K, [&](int c, int i, int k) { return la((g + c) * 16 + i, k); },
-                          [&](int k, int j) { return lb(k, n0 + j); });
+    for (int u = 0; u < U; ++u) {
+      const int e = base + u * NT + tid;
+      v[u] = e < n ? src[e] : cz<T>();
+    }
 #pragma unroll
-      for (int c = 0; c < NG; ++c)
-        if (c < n)
-          tile_store(acc[c], [&](int i, int j, cplx<T> v) {
-            const int row = (g + c) * 16 + i, col = n0 + j;
-            if (row < M && col < N) st(row, col, v);
-          });
+    for (int u = 0; u < U; ++u) {
+      const int e = base + u * NT + tid;
+      if (e < n) {
+        const int r = e / len, x = e - r * len;
+        dst[r * ld + x] = f(v[u]);
+      }
     }
   }
 }
 
-// Same for products with few row tiles and many column tiles: column tiles
-// split among the waves, groups of four sharing the A fragment.
+// C (M x N) = A (M x K) B (K x N) into LDS as independent jobs dealt to the
+// waves round-robin, job j to wave (first + j) % NW; returns the job count so
+// that a following product can continue the deal.  Row-share jobs are
+// (row tile, group of NG column tiles) sharing the A fragment of each k-step;
+// column-share jobs are (group of NG row tiles, column tile) sharing B.
+// la(row, k), lb(k, col), st(row, col, value) (st only for row < M, col < N).
 template <typename T, typename LA, typename LB, typename ST>
-__device__ __forceinline__ void wg_gemm_cols(int M, int N, int K, const LA &la, const LB &lb, const ST &st) {
+__device__ __forceinline__ int gemm_jobs_rs(int M, int N, int K, int first, const LA &la, const LB &lb,
+                                            const ST &st) {
+  constexpr int NW = Cfg<T>::NW, NG = Group<T>::n;
   const int wave = threadIdx.x >> 6;
-  const int ntm = (M + 15) / 16, ntn = (N + 15) / 16;
-  int cb, ce;
-  wave_share<Cfg<T>::NW>(ntn, wave, cb, ce);
-  for (int mt = 0; mt < ntm; ++mt) {
-    const int m0 = mt * 16;
-    constexpr int NG = Group<T>::n;
-    for (int g = cb; g < ce; g += NG) {
-      const int n = min(NG, ce - g);
-      Tiles<T> acc;
+  const int ntm = (M + 15) / 16, ntn = (N + 15) / 16, ngr = (ntn + NG - 1) / NG, njobs = ntm * ngr;
+  for (int j = (wave - first % NW + NW) % NW; j < njobs; j += NW) {
+    const int mt = j / ngr, g = (j - mt * ngr) * NG, n = min(NG, ntn - g), m0 = mt * 16;
+    Tiles<T> acc;
 #pragma unroll
-      for (int c = 0; c < NG; ++c) tile_zero(acc[c]);
-      mma_rowshare<0, NG>(acc, n, 0, K, [&](int i, int k) { return la(m0 + i, k); },
-                          [&](int c, int k, int j) { return lb(k, (g + c) * 16 + j); });
+    for (int c = 0; c < NG; ++c) tile_zero(acc[c]);
+    mma_rowshare<0, NG>(acc, n, 0, K, [&](int i, int k) { return la(m0 + i, k); },
+                        [&](int c, int k, int jj) { return lb(k, (g + c) * 16 + jj); });
 #pragma unroll
-      for (int c = 0; c < NG; ++c)
-        if (c < n)
-          tile_store(acc[c], [&](int i, int j, cplx<T> v) {
-            const int row = m0 + i, col = (g + c) * 16 + j;
-            if (row < M && col < N) st(row, col, v);
-          });
-    }
+    for (int c = 0; c < NG; ++c)
+      if (c < n)
+        tile_store(acc[c], [&](int i, int jj, cplx<T> v) {
+          const int row = m0 + i, col = (g + c) * 16 + jj;
+          if (row < M && col < N) st(row, col, v);
+        });
   }
+  return njobs;
+}
+
+template <typename T, typename LA, typename LB, typename ST>
+__device__ __forceinline__ int gemm_jobs_cs(int M, int N, int K, int first, const LA &la, const LB &lb,
+                                            const ST &st) {
+  constexpr int NW = Cfg<T>::NW, NG = Group<T>::n;
+  const int wave = threadIdx.x >> 6;
+  const int ntm = (M + 15) / 16, ntn = (N + 15) / 16, ngr = (ntm + NG - 1) / NG, njobs = ngr * ntn;
+  for (int j = (wave - first % NW + NW) % NW; j < njobs; j += NW) {
+    const int gt = j / ntn, nt = j - gt * ntn, g = gt * NG, n = min(NG, ntm - g), n0 = nt * 16;
+    Tiles<T> acc;
+#pragma unroll
+    for (int c = 0; c < NG; ++c) tile_zero(acc[c]);
+    mma_colshare<0, NG>(acc, n, 0, K, [&](int c, int i, int k) { return la((g + c) * 16 + i, k); },
+                        [&](int k, int jj) { return lb(k, n0 + jj); });
+#pragma unroll
+    for (int c = 0; c < NG; ++c)
+      if (c < n)
+        tile_store(acc[c], [&](int i, int jj, cplx<T> v) {
+          const int row = (g + c) * 16 + i, col = n0 + jj;
+          if (row < M && col < N) st(row, col, v);
+        });
+  }
+  return njobs;
 }
 
 }  // namespace
@@ -306,10 +337,9 @@ __global__ __launch_bounds__(Cfg<T>::NT) void p0_mfma_kernel(const P0Item *__res
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cplx<T> *lds = reinterpret_cast<cplx<T> *>(smem_raw);
   const int q = it.q, nband = it.nband, nct = (q + 15) / 16;
-  // <= 4 column tiles: every wave takes all of them over 1/NW of x
-  // (split-K); otherwise wave w owns column tiles w, w+NW, ...
-  const bool split = nct <= 4;
-  const int nmine = split ? nct : (nct - wave + NW - 1) / NW;
+  // the nct column tiles of the slab: S waves share a tile over S parts of x
+  const int S = choose_split<NW>(nct), TG = NW / S, kp = wave % S, tg = wave / S;
+  const int nmine = nct > tg ? (nct - tg + TG - 1) / TG : 0;  // tiles tg, tg + TG, ...
   Tiles<T> acc;
 #pragma unroll
   for (int c = 0; c < 4; ++c) tile_zero(acc[c]);
@@ -320,6 +350,8 @@ __global__ __launch_bounds__(Cfg<T>::NT) void p0_mfma_kernel(const P0Item *__res
     const cplx<T> *Vb = V + B.v_off;
     cplx<T> *bufV = lds;               // V rows, then C
     cplx<T> *bufE = lds + qsub * ldk;  // E = V G2
+    int kb, ke;
+    k_part(K, S, kp, kb, ke);
     for (int s = 0; s < ns; ++s) {
       const cplx<T> *G1 = G + B.g1_base + (int64_t)(nts - 1 - t) * B.g1_tstride + (int64_t)s * B.g1_sstride;
       const cplx<T> *G2 = G + B.g2_base + (int64_t)t * B.g2_tstride + (int64_t)s * B.g2_sstride;
@@ -327,28 +359,27 @@ __global__ __launch_bounds__(Cfg<T>::NT) void p0_mfma_kernel(const P0Item *__res
         const int nr = min(qsub, nband - r0);
         const cplx<T> *Vr = Vb + (int64_t)(it.Q0 + r0) * K;
         __syncthreads();
-        for (int e = tid; e < nr * K; e += NT) {
-          const int r = e / K, x = e - r * K;
-          bufV[r * ldk + x] = Vr[e];
-        }
+        stage_rows<T, 8>(bufV, ldk, Vr, nr * K, K, [](cplx<T> v) { return v; });
         __syncthreads();
         // E[(r,a)][n] = sum_m V[r][a][m] G2[m][n]
-        wg_gemm_rows<T>(
-            nr * k1, k2, k2,
-            [&](int row, int m) {
-              if (row >= nr * k1 || m >= k2) return cz<T>();
-              const int r = row / k1, a = row - r * k1;
-              return bufV[r * ldk + a * k2 + m];
-            },
-            [&](int m, int n) { return (m < k2 && n < k2) ? G2[m * k2 + n] : cz<T>(); },
-            [&](int row, int n, cplx<T> v) {
-              const int r = row / k1, a = row - r * k1;
-              bufE[r * ldk + a * k2 + n] = v;
-            });
+        {
+          auto la = [&](int row, int m) {
+            if (row >= nr * k1 || m >= k2) return cz<T>();
+            const int r = row / k1, a = row - r * k1;
+            return bufV[r * ldk + a * k2 + m];
+          };
+          auto lb = [&](int m, int n) { return (m < k2 && n < k2) ? G2[m * k2 + n] : cz<T>(); };
+          auto st = [&](int row, int n, cplx<T> v) {
+            const int r = row / k1, a = row - r * k1;
+            bufE[r * ldk + a * k2 + n] = v;
+          };
+          if (k2 > 16) gemm_jobs_rs<T>(nr * k1, k2, k2, 0, la, lb, st);
+          else gemm_jobs_cs<T>(nr * k1, k2, k2, 0, la, lb, st);
+        }
         __syncthreads();
         // C[r][p][n] = sum_a G1[p][a] E[r][a][n]  (into bufV)
-        wg_gemm_cols<T>(
-            k1, nr * k2, k1, [&](int p, int a) { return (p < k1 && a < k1) ? G1[p * k1 + a] : cz<T>(); },
+        gemm_jobs_rs<T>(
+            k1, nr * k2, k1, 0, [&](int p, int a) { return (p < k1 && a < k1) ? G1[p * k1 + a] : cz<T>(); },
             [&](int a, int col) {
               if (a >= k1 || col >= nr * k2) return cz<T>();
               const int r = col / k2, n = col - r * k2;
@@ -359,9 +390,7 @@ __global__ __launch_bounds__(Cfg<T>::NT) void p0_mfma_kernel(const P0Item *__res
               bufV[r * ldk + p * k2 + n] = v;
             });
         __syncthreads();
-        // acc[r0 + r][P] += sum_x C[r][x] conj(V[P][x])
-        int kb = 0, ke = K;
-        if (split) wave_krange<NW>(K, wave, kb, ke);
+        // acc[r0 + r][P] += sum_x C[r][x] conj(V[P][x]) over this wave's part of x
         mma_rowshare4(
             acc, nmine, kb, ke,
             [&](int i, int x) {
@@ -369,7 +398,7 @@ __global__ __launch_bounds__(Cfg<T>::NT) void p0_mfma_kernel(const P0Item *__res
               return (rr >= 0 && rr < nr && x < ke) ? bufV[rr * ldk + x] : cz<T>();
             },
             [&](int c, int x, int j) {
-              const int P = (split ? c : wave + NW * c) * 16 + j;
+              const int P = (tg + TG * c) * 16 + j;
               if (x >= ke || P >= q) return cz<T>();
               const cplx<T> v = Vb[(int64_t)P * K + x];
               return cplx<T>{v.re, -v.im};
@@ -378,15 +407,17 @@ __global__ __launch_bounds__(Cfg<T>::NT) void p0_mfma_kernel(const P0Item *__res
     }
   }
   cplx<double> *out = P0 + it.out_base + (int64_t)t * it.out_tstride;
-  if (split) wave_reduce(acc, nct, reinterpret_cast<T *>(lds));
+  if (S > 1) split_reduce(acc, nmine, S, reinterpret_cast<T *>(lds));
+  if (kp == 0) {
 #pragma unroll
-  for (int c = 0; c < 4; ++c) {
-    const int jt = split ? c : wave + NW * c;
-    if (c < nmine && (!split || wave == 0))
-      tile_store(acc[c], [&](int i, int j, cplx<T> v) {
-        const int P = jt * 16 + j;
-        if (i < nband && P < q) out[i * q + P] = cplx<double>{pref * (double)v.re, pref * (double)v.im};
-      });
+    for (int c = 0; c < 4; ++c) {
+      const int jt = tg + TG * c;
+      if (c < nmine)
+        tile_store(acc[c], [&](int i, int j, cplx<T> v) {
+          const int P = jt * 16 + j;
+          if (i < nband && P < q) out[i * q + P] = cplx<double>{pref * (double)v.re, pref * (double)v.im};
+        });
+    }
   }
   // rows t >= nts/2 of the reference's device P0 stay zero (cugw_routines_block.cu:79)
   cplx<double> *mir = P0 + it.out_base + (int64_t)(nts - 1 - t) * it.out_tstride;
@@ -420,10 +451,9 @@ __global__ __launch_bounds__(Cfg<T>::NT) void sigma_mfma_kernel(const SigItem *_
   cplx<T> *lds = reinterpret_cast<cplx<T> *>(smem_raw);
   const int k1 = it.k1, I0 = it.I0, Ib = it.Ib, J0 = it.J0, Jb = it.Jb;
   const int nti = (Ib + 15) / 16, ntj = (Jb + 15) / 16, ntile = nti * ntj;
-  // <= 4 output tiles: every wave takes all of them over 1/NW of the (Q, n)
-  // reduction (split-K); otherwise wave w owns tiles w, w+NW, ...
-  const bool split = ntile <= 4;
-  const int nmine = split ? ntile : (ntile - wave + NW - 1) / NW;
+  // output tiles: S waves share a tile over S parts of the (Q, n) reduction
+  const int S = choose_split<NW>(ntile), TG = NW / S, kp = wave % S, tg = wave / S;
+  const int nmine = ntile > tg ? (ntile - tg + TG - 1) / TG : 0;  // tiles tg, tg + TG, ...
   Tiles<T> acc;
 #pragma unroll
   for (int c = 0; c < 4; ++c) tile_zero(acc[c]);
@@ -443,45 +473,49 @@ __global__ __launch_bounds__(Cfg<T>::NT) void sigma_mfma_kernel(const SigItem *_
     for (int Q0 = 0; Q0 < q; Q0 += qsub) {
       const int nr = min(qsub, q - Q0);
       __syncthreads();
-      for (int e = tid; e < nr * q; e += NT) {
-        const cplx<T> p = Pb[(int64_t)Q0 * q + e];
-        bufP[e] = cplx<T>{p.re, -p.im};
-      }
+      stage_rows<T, 4>(bufP, q, Pb + (int64_t)Q0 * q, nr * q, q, [](cplx<T> p) { return cplx<T>{p.re, -p.im}; });
       __syncthreads();
       // Z[r][x] = sum_P conj(P[Q0+r][P]) V[P][J0*k2 + x],  x < Jb*k2
-      wg_gemm_cols<T>(
-          nr, zc, q, [&](int r, int P) { return (r < nr && P < q) ? bufP[r * q + P] : cz<T>(); },
-          [&](int P, int x) { return (P < q && x < zc) ? Vj[(int64_t)P * K + x] : cz<T>(); },
-          [&](int r, int x, cplx<T> v) {
-            const int jj = x / k2, n = x - jj * k2;
-            bufZ[jj * ldz + r * k2 + n] = v;
-          });
-      // E[(r,i)][n] = sum_m V[Q0+r][I0+i][m] G2[m][n]
-      wg_gemm_rows<T>(
-          nr * Ib, k2, k2,
-          [&](int row, int m) {
-            if (row >= nr * Ib || m >= k2) return cz<T>();
-            const int r = row / Ib, ii = row - r * Ib;
-            return Vb[(int64_t)(Q0 + r) * K + (I0 + ii) * k2 + m];
-          },
-          [&](int m, int n) { return (m < k2 && n < k2) ? G2[m * k2 + n] : cz<T>(); },
-          [&](int row, int n, cplx<T> v) {
-            const int r = row / Ib, ii = row - r * Ib;
-            bufE[ii * lde + r * k2 + n] = v;
-          });
+      int nj;
+      {
+        auto la = [&](int r, int P) { return (r < nr && P < q) ? bufP[r * q + P] : cz<T>(); };
+        auto lb = [&](int P, int x) { return (P < q && x < zc) ? Vj[(int64_t)P * K + x] : cz<T>(); };
+        auto st = [&](int r, int x, cplx<T> v) {
+          const int jj = x / k2, n = x - jj * k2;
+          bufZ[jj * ldz + r * k2 + n] = v;
+        };
+        // several row tiles: share each V fragment among them
+        if (nr > 16) nj = gemm_jobs_cs<T>(nr, zc, q, 0, la, lb, st);
+        else nj = gemm_jobs_rs<T>(nr, zc, q, 0, la, lb, st);
+      }
+      // E[(r,i)][n] = sum_m V[Q0+r][I0+i][m] G2[m][n]  (jobs dealt on after Z's)
+      {
+        auto la = [&](int row, int m) {
+          if (row >= nr * Ib || m >= k2) return cz<T>();
+          const int r = row / Ib, ii = row - r * Ib;
+          return Vb[(int64_t)(Q0 + r) * K + (I0 + ii) * k2 + m];
+        };
+        auto lb = [&](int m, int n) { return (m < k2 && n < k2) ? G2[m * k2 + n] : cz<T>(); };
+        auto st = [&](int row, int n, cplx<T> v) {
+          const int r = row / Ib, ii = row - r * Ib;
+          bufE[ii * lde + r * k2 + n] = v;
+        };
+        if (k2 > 16) gemm_jobs_rs<T>(nr * Ib, k2, k2, nj, la, lb, st);
+        else gemm_jobs_cs<T>(nr * Ib, k2, k2, nj, la, lb, st);
+      }
       __syncthreads();
-      // acc[i][j] += sum_y E[i][y] conj(Z[j][y])
+      // acc[i][j] += sum_y E[i][y] conj(Z[j][y]) over this wave's part of y
       const int Y = nr * k2;
-      int yb = 0, ye = Y;
-      if (split) wave_krange<NW>(Y, wave, yb, ye);
+      int yb, ye;
+      k_part(Y, S, kp, yb, ye);
       mma_multi4(
           acc, nmine, yb, ye,
           [&](int c, int i, int y) {
-            const int i0 = ((split ? c : wave + NW * c) / ntj) * 16;
+            const int i0 = ((tg + TG * c) / ntj) * 16;
             return (i0 + i < Ib && y < ye) ? bufE[(i0 + i) * lde + y] : cz<T>();
           },
           [&](int c, int y, int j) {
-            const int j0 = ((split ? c : wave + NW * c) % ntj) * 16;
+            const int j0 = ((tg + TG * c) % ntj) * 16;
             if (j0 + j >= Jb || y >= ye) return cz<T>();
             const cplx<T> z = bufZ[(j0 + j) * ldz + y];
             return cplx<T>{z.re, -z.im};
@@ -489,19 +523,20 @@ __global__ __launch_bounds__(Cfg<T>::NT) void sigma_mfma_kernel(const SigItem *_
     }
   }
   cplx<double> *out = Sig + it.out_base + (int64_t)t * it.out_tstride + (int64_t)s * it.out_sstride;
-  if (split) wave_reduce(acc, ntile, reinterpret_cast<T *>(lds));
+  if (S > 1) split_reduce(acc, nmine, S, reinterpret_cast<T *>(lds));
+  if (kp == 0) {
 #pragma unroll
-  for (int c = 0; c < 4; ++c) {
-    const int tt = split ? c : wave + NW * c;
-    if (c < nmine && (!split || wave == 0)) {
-      const int i0 = (tt / ntj) * 16, j0 = (tt % ntj) * 16;
-      tile_store(acc[c], [&](int i, int j, cplx<T> v) {
-        if (i0 + i < Ib && j0 + j < Jb)
-          out[(int64_t)(i0 + i) * k1 + J0 + j0 + j] = cplx<double>{pref * (double)v.re, pref * (double)v.im};
-      });
+    for (int c = 0; c < 4; ++c) {
+      const int tt = tg + TG * c;
+      if (c < nmine) {
+        const int i0 = (tt / ntj) * 16, j0 = (tt % ntj) * 16;
+        tile_store(acc[c], [&](int i, int j, cplx<T> v) {
+          if (i0 + i < Ib && j0 + j < Jb)
+            out[(int64_t)(i0 + i) * k1 + J0 + j0 + j] = cplx<double>{pref * (double)v.re, pref * (double)v.im};
+        });
+      }
     }
   }
-  (void)I0;
 }
 
 template <typename T>

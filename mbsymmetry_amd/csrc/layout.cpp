@@ -325,7 +325,7 @@ bool build_worklists(const Layout &L, int eb, WorkLists &W, std::string &err) {
           // Z ([band][qsub*k2]), E ([band][qsub*k2]) and conj(P) rows ([qsub][q])
           // of one chunk share the LDS budget (+2 elements of row padding)
           int per_row = B.q + 2 * band * B.k2;
-          int cap = std::min(16, (budget - 2 * band) / per_row);
+          int cap = std::min(kSigRows, (budget - 2 * band) / per_row);
           if (cap < 1) { err = fmt("Sigma block too large for the LDS plan (q, k2)", B.q, B.k2); return false; }
           B.qsub = balanced(B.q, cap);
           B.ldz = odd_ld(B.qsub * B.k2);

@@ -53,8 +53,10 @@ constexpr int contract_lds(int elem_bytes) { return elem_bytes == 16 ? 160 * 102
 constexpr int contract_waves(int elem_bytes) { return elem_bytes == 16 ? 8 : 4; }
 // Output rows of one P0 slab (one MFMA tile row).
 constexpr int kP0Band = 16;
-// Largest Sigma slab edge (4 x 4 tiles of 16, four per wave).
+// Largest Sigma slab edge (4 x 4 tiles of 16).
 constexpr int kSigBand = 64;
+// Most aux rows Q of one Sigma LDS chunk (four row tiles of the Z product).
+constexpr int kSigRows = 64;
 
 // One V-block contraction that feeds a P0 output slab (cugw_qkpt_block.cu:219-261).
 struct P0Blk {
