@@ -17,13 +17,15 @@ namespace symgw {
 
 namespace {
 
-// Waves per workgroup: FP64 runs 8 waves on the full 160 KiB LDS of a CU,
-// FP32 runs 4 waves on 80 KiB (two workgroups per CU).  Either way two waves
-// share each SIMD.
+// Waves per workgroup: FP64 runs 12 waves (three per SIMD, at most 168
+// VGPRs each) on the full 160 KiB LDS of a CU, FP32 runs 4 waves on 80 KiB
+// (two workgroups per CU).
 template <typename T>
 struct Cfg {
-  static constexpr int NW = sizeof(T) == 8 ? 8 : 4;
+  static constexpr int NW = sizeof(T) == 8 ? 12 : 4;
   static constexpr int NT = 64 * NW;
+  // accumulator tiles a wave keeps across the whole slab
+  static constexpr int MT = sizeof(T) == 8 ? 2 : 4;
 };
 
 template <typename T>
@@ -52,6 +54,8 @@ template <typename T>
 struct Group { static constexpr int n = sizeof(T) == 8 ? 2 : 4; };
 template <typename T>
 using Tiles = Tile<T>[4];
+template <typename T>
+using Acc = Tile<T>[Cfg<T>::MT];
 
 template <typename T>
 __device__ __forceinline__ void tile_zero(Tile<T> &t) {
@@ -78,8 +82,9 @@ __device__ __forceinline__ void mma4(Tile<T> &acc, const cplx<T> &a, const cplx<
 // acc[C0+c] += A (16 x K) * B_c (K x 16) for c < NG with C0 + c < n, over k in
 // [kb, ke): one A fragment per k-step shared by the tiles.  la(i, k), lb(c, k, j)
 // with c the absolute tile index.
-template <int C0, int NG, typename T, typename LA, typename LB>
-__device__ __forceinline__ void mma_rowshare(Tiles<T> &acc, int n, int kb, int ke, const LA &la, const LB &lb) {
+template <int C0, int NG, typename T, int N, typename LA, typename LB>
+__device__ __forceinline__ void mma_rowshare(Tile<T> (&acc)[N], int n, int kb, int ke, const LA &la, const LB &lb) {
+  static_assert(C0 + NG <= N, "tile group outside the accumulator array");
   if (kb >= ke || C0 >= n) return;
   const int lane = threadIdx.x & 63, i = lane & 15, kk = lane >> 4;
   cplx<T> a0 = la(i, kb + kk), b0[NG];
@@ -100,8 +105,9 @@ __device__ __forceinline__ void mma_rowshare(Tiles<T> &acc, int n, int kb, int k
 }
 
 // acc[C0+c] += A_c (16 x K) * B (K x 16): one B fragment shared.  la(c, i, k), lb(k, j).
-template <int C0, int NG, typename T, typename LA, typename LB>
-__device__ __forceinline__ void mma_colshare(Tiles<T> &acc, int n, int kb, int ke, const LA &la, const LB &lb) {
+template <int C0, int NG, typename T, int N, typename LA, typename LB>
+__device__ __forceinline__ void mma_colshare(Tile<T> (&acc)[N], int n, int kb, int ke, const LA &la, const LB &lb) {
+  static_assert(C0 + NG <= N, "tile group outside the accumulator array");
   if (kb >= ke || C0 >= n) return;
   const int lane = threadIdx.x & 63, i = lane & 15, kk = lane >> 4;
   cplx<T> b0 = lb(kb + kk, i), a0[NG];
@@ -122,8 +128,9 @@ __device__ __forceinline__ void mma_colshare(Tiles<T> &acc, int n, int kb, int k
 }
 
 // acc[C0+c] += A_c * B_c with independent operands.  la(c, i, k), lb(c, k, j).
-template <int C0, int NG, typename T, typename LA, typename LB>
-__device__ __forceinline__ void mma_multi(Tiles<T> &acc, int n, int kb, int ke, const LA &la, const LB &lb) {
+template <int C0, int NG, typename T, int N, typename LA, typename LB>
+__device__ __forceinline__ void mma_multi(Tile<T> (&acc)[N], int n, int kb, int ke, const LA &la, const LB &lb) {
+  static_assert(C0 + NG <= N, "tile group outside the accumulator array");
   if (kb >= ke || C0 >= n) return;
   const int lane = threadIdx.x & 63, i = lane & 15, kk = lane >> 4;
   cplx<T> a0[NG], b0[NG];
@@ -150,18 +157,18 @@ __device__ __forceinline__ void mma_multi(Tiles<T> &acc, int n, int kb, int ke, 
   }
 }
 
-// All four persistent tiles, in groups of Group<T>::n.
+// All persistent tiles of a wave, in groups of Group<T>::n.
 template <typename T, typename LA, typename LB>
-__device__ __forceinline__ void mma_rowshare4(Tiles<T> &acc, int n, int kb, int ke, const LA &la, const LB &lb) {
-  constexpr int NG = Group<T>::n;
-  mma_rowshare<0, NG>(acc, n, kb, ke, la, lb);
-  if (NG == 2) mma_rowshare<2, 2>(acc, n, kb, ke, la, lb);
+__device__ __forceinline__ void mma_rowshare_all(Acc<T> &acc, int n, int kb, int ke, const LA &la, const LB &lb) {
+  constexpr int NG = Group<T>::n, MT = Cfg<T>::MT;
+  mma_rowshare<0, NG < MT ? NG : MT>(acc, n, kb, ke, la, lb);
+  if (NG < MT) mma_rowshare<NG < MT ? NG : 0, NG < MT ? MT - NG : 1>(acc, n, kb, ke, la, lb);
 }
 template <typename T, typename LA, typename LB>
-__device__ __forceinline__ void mma_multi4(Tiles<T> &acc, int n, int kb, int ke, const LA &la, const LB &lb) {
-  constexpr int NG = Group<T>::n;
-  mma_multi<0, NG>(acc, n, kb, ke, la, lb);
-  if (NG == 2) mma_multi<2, 2>(acc, n, kb, ke, la, lb);
+__device__ __forceinline__ void mma_multi_all(Acc<T> &acc, int n, int kb, int ke, const LA &la, const LB &lb) {
+  constexpr int NG = Group<T>::n, MT = Cfg<T>::MT;
+  mma_multi<0, NG < MT ? NG : MT>(acc, n, kb, ke, la, lb);
+  if (NG < MT) mma_multi<NG < MT ? NG : 0, NG < MT ? MT - NG : 1>(acc, n, kb, ke, la, lb);
 }
 
 // Split of K into S parts for part p (boundaries multiples of 4).
@@ -173,14 +180,15 @@ __device__ __forceinline__ void k_part(int K, int S, int p, int &kb, int &ke) {
 
 // Work split of nt accumulator tiles over the NW waves: S waves share each
 // tile over S parts of the reduction (split-K), so nt tiles are spread over
-// NW / S tile groups with at most four tiles per wave.  S minimises the
+// NW / S tile groups with at most MT tiles per wave.  S minimises the
 // per-wave work ceil(nt / (NW / S)) / S; ties keep the smaller S.
-template <int NW>
+template <int NW, int MT>
 __device__ __forceinline__ int choose_split(int nt) {
   int best = 1, num = 1 << 30, den = 1;
   for (int s = 1; s <= NW; s *= 2) {
+    if (NW % s) continue;
     const int g = NW / s, tpw = (nt + g - 1) / g;
-    if (tpw > 4) continue;
+    if (tpw > MT) continue;
     if (tpw * den < num * s) {
       num = tpw;
       den = s;
@@ -191,37 +199,38 @@ __device__ __forceinline__ int choose_split(int nt) {
 }
 
 // Sum the S split-K partial copies of each wave's tiles (waves g*S .. g*S+S-1
-// hold the parts of tile group g) in part order, so the result is bitwise
-// reproducible; part 0 of each group ends with the totals.  Uses the LDS at
-// `red` (NW * 4 * 8 * 64 elements of T); callers sync before reusing it.
-template <typename T>
-__device__ __forceinline__ void split_reduce(Tiles<T> &acc, int nmine, int S, T *red) {
-  constexpr int NW = Cfg<T>::NW;
-  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  __syncthreads();
+// hold the parts of tile group g) as a fixed pairwise tree (parts 0+1, 2+3,
+// then (0+1)+(2+3), ...), so the result is bitwise reproducible; part 0 of
+// each group ends with the totals.  Writers of a round use LDS slot wave/2 at
+// `red` (NW/2 * 4 * 8 * 64 elements of T); callers sync before reusing it.
+template <typename T, int N>
+__device__ __forceinline__ void split_reduce(Tile<T> (&acc)[N], int nmine, int S, T *red) {
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, kp = wave % S;
+  for (int st = 1; st < S; st *= 2) {
+    __syncthreads();
+    if (kp % (2 * st) == st) {
+      T *slot = red + (wave >> 1) * (4 * 8 * 64);
 #pragma unroll
-  for (int c = 0; c < 4; ++c)
-    if (c < nmine)
+      for (int c = 0; c < N; ++c)
+        if (c < nmine)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        red[((wave * 4 + c) * 8 + r) * 64 + lane] = acc[c].re[r];
-        red[((wave * 4 + c) * 8 + 4 + r) * 64 + lane] = acc[c].im[r];
-      }
-  __syncthreads();
-  if (wave % S == 0) {
-#pragma unroll
-    for (int c = 0; c < 4; ++c)
-      if (c < nmine)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          T sre = acc[c].re[r], sim = acc[c].im[r];
-          for (int p = 1; p < S; ++p) {
-            sre += red[(((wave + p) * 4 + c) * 8 + r) * 64 + lane];
-            sim += red[(((wave + p) * 4 + c) * 8 + 4 + r) * 64 + lane];
+          for (int r = 0; r < 4; ++r) {
+            slot[(c * 8 + r) * 64 + lane] = acc[c].re[r];
+            slot[(c * 8 + 4 + r) * 64 + lane] = acc[c].im[r];
           }
-          acc[c].re[r] = sre;
-          acc[c].im[r] = sim;
-        }
+    }
+    __syncthreads();
+    if (kp % (2 * st) == 0) {
+      const T *slot = red + ((wave + st) >> 1) * (4 * 8 * 64);
+#pragma unroll
+      for (int c = 0; c < N; ++c)
+        if (c < nmine)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            acc[c].re[r] += slot[(c * 8 + r) * 64 + lane];
+            acc[c].im[r] += slot[(c * 8 + 4 + r) * 64 + lane];
+          }
+    }
   }
 }
 
@@ -332,17 +341,17 @@ __global__ __launch_bounds__(Cfg<T>::NT) void p0_mfma_kernel(const P0Item *__res
   if (w >= n_work) return;
   const int t = (int)(w % nt_half);
   const P0Item it = items[w / nt_half];
-  constexpr int NW = Cfg<T>::NW, NT = Cfg<T>::NT;
+  constexpr int NW = Cfg<T>::NW, NT = Cfg<T>::NT, MT = Cfg<T>::MT;
   const int tid = threadIdx.x, wave = tid >> 6;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cplx<T> *lds = reinterpret_cast<cplx<T> *>(smem_raw);
   const int q = it.q, nband = it.nband, nct = (q + 15) / 16;
   // the nct column tiles of the slab: S waves share a tile over S parts of x
-  const int S = choose_split<NW>(nct), TG = NW / S, kp = wave % S, tg = wave / S;
+  const int S = choose_split<NW, MT>(nct), TG = NW / S, kp = wave % S, tg = wave / S;
   const int nmine = nct > tg ? (nct - tg + TG - 1) / TG : 0;  // tiles tg, tg + TG, ...
-  Tiles<T> acc;
+  Acc<T> acc;
 #pragma unroll
-  for (int c = 0; c < 4; ++c) tile_zero(acc[c]);
+  for (int c = 0; c < MT; ++c) tile_zero(acc[c]);
 
   for (int b = it.blk_begin; b < it.blk_end; ++b) {
     const P0Blk B = blks[b];
@@ -391,7 +400,7 @@ __global__ __launch_bounds__(Cfg<T>::NT) void p0_mfma_kernel(const P0Item *__res
             });
         __syncthreads();
         // acc[r0 + r][P] += sum_x C[r][x] conj(V[P][x]) over this wave's part of x
-        mma_rowshare4(
+        mma_rowshare_all(
             acc, nmine, kb, ke,
             [&](int i, int x) {
               const int rr = i - r0;
@@ -410,7 +419,7 @@ __global__ __launch_bounds__(Cfg<T>::NT) void p0_mfma_kernel(const P0Item *__res
   if (S > 1) split_reduce(acc, nmine, S, reinterpret_cast<T *>(lds));
   if (kp == 0) {
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
+    for (int c = 0; c < MT; ++c) {
       const int jt = tg + TG * c;
       if (c < nmine)
         tile_store(acc[c], [&](int i, int j, cplx<T> v) {
@@ -445,18 +454,18 @@ __global__ __launch_bounds__(Cfg<T>::NT) void sigma_mfma_kernel(const SigItem *_
   const int per = nts * ns;
   const SigItem it = items[w / per];
   const int ts = (int)(w % per), t = ts / ns, s = ts - (ts / ns) * ns;
-  constexpr int NW = Cfg<T>::NW, NT = Cfg<T>::NT;
+  constexpr int NW = Cfg<T>::NW, NT = Cfg<T>::NT, MT = Cfg<T>::MT;
   const int tid = threadIdx.x, wave = tid >> 6;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cplx<T> *lds = reinterpret_cast<cplx<T> *>(smem_raw);
   const int k1 = it.k1, I0 = it.I0, Ib = it.Ib, J0 = it.J0, Jb = it.Jb;
   const int nti = (Ib + 15) / 16, ntj = (Jb + 15) / 16, ntile = nti * ntj;
   // output tiles: S waves share a tile over S parts of the (Q, n) reduction
-  const int S = choose_split<NW>(ntile), TG = NW / S, kp = wave % S, tg = wave / S;
+  const int S = choose_split<NW, MT>(ntile), TG = NW / S, kp = wave % S, tg = wave / S;
   const int nmine = ntile > tg ? (ntile - tg + TG - 1) / TG : 0;  // tiles tg, tg + TG, ...
-  Tiles<T> acc;
+  Acc<T> acc;
 #pragma unroll
-  for (int c = 0; c < 4; ++c) tile_zero(acc[c]);
+  for (int c = 0; c < MT; ++c) tile_zero(acc[c]);
 
   for (int b = it.blk_begin; b < it.blk_end; ++b) {
     const SigBlk B = blks[b];
@@ -508,7 +517,7 @@ __global__ __launch_bounds__(Cfg<T>::NT) void sigma_mfma_kernel(const SigItem *_
       const int Y = nr * k2;
       int yb, ye;
       k_part(Y, S, kp, yb, ye);
-      mma_multi4(
+      mma_multi_all(
           acc, nmine, yb, ye,
           [&](int c, int i, int y) {
             const int i0 = ((tg + TG * c) / ntj) * 16;
@@ -526,7 +535,7 @@ __global__ __launch_bounds__(Cfg<T>::NT) void sigma_mfma_kernel(const SigItem *_
   if (S > 1) split_reduce(acc, nmine, S, reinterpret_cast<T *>(lds));
   if (kp == 0) {
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
+    for (int c = 0; c < MT; ++c) {
       const int tt = tg + TG * c;
       if (c < nmine) {
         const int i0 = (tt / ntj) * 16, j0 = (tt % ntj) * 16;
