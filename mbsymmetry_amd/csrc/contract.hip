@@ -17,15 +17,14 @@ namespace symgw {
 
 namespace {
 
-// Waves per workgroup: FP64 runs 12 waves (three per SIMD, at most 168
-// VGPRs each) on the full 160 KiB LDS of a CU, FP32 runs 4 waves on 80 KiB
-// (two workgroups per CU).
+// Waves per workgroup: 12 (three per SIMD, at most 168 VGPRs each) on the
+// full 160 KiB LDS of a CU, in both precisions.
 template <typename T>
 struct Cfg {
-  static constexpr int NW = sizeof(T) == 8 ? 12 : 4;
+  static constexpr int NW = 12;
   static constexpr int NT = 64 * NW;
   // accumulator tiles a wave keeps across the whole slab
-  static constexpr int MT = sizeof(T) == 8 ? 2 : 4;
+  static constexpr int MT = 2;
 };
 
 template <typename T>

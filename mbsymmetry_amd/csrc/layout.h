@@ -45,12 +45,11 @@ struct Layout {
 bool build_layout(const symgw_layout_desc *d, Layout &L, std::string &err);
 
 // ---- device work lists ------------------------------------------------------
-// Contraction workgroup shape per complex element size: FP64 (16 B) runs 12
-// waves (three per SIMD) on the whole 160 KiB LDS of a CU, so large blocks
-// still get aux-row chunks that fill the 16-row MFMA tiles; FP32 (8 B) runs 4
-// waves on 80 KiB, two workgroups per CU.
-constexpr int contract_lds(int elem_bytes) { return elem_bytes == 16 ? 160 * 1024 : 80 * 1024; }
-constexpr int contract_waves(int elem_bytes) { return elem_bytes == 16 ? 12 : 4; }
+// Contraction workgroup shape: 12 waves (three per SIMD) on the whole
+// 160 KiB LDS of a CU, so large blocks still get aux-row chunks that fill the
+// 16-row MFMA tiles.  Same shape for both element sizes.
+constexpr int contract_lds(int elem_bytes) { return elem_bytes > 0 ? 160 * 1024 : 0; }
+constexpr int contract_waves(int elem_bytes) { return elem_bytes > 0 ? 12 : 0; }
 // Output rows of one P0 slab (one MFMA tile row).
 constexpr int kP0Band = 16;
 // Largest Sigma slab edge (4 x 4 tiles of 16).
