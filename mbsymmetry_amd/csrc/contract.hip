@@ -21,10 +21,10 @@ namespace {
 // full 160 KiB LDS of a CU, in both precisions.
 template <typename T>
 struct Cfg {
-  static constexpr int NW = 12;
+  static constexpr int NW = contract_waves(2 * (int)sizeof(T));
   static constexpr int NT = 64 * NW;
   // accumulator tiles a wave keeps across the whole slab
-  static constexpr int MT = 2;
+  static constexpr int MT = kContractTiles;
 };
 
 template <typename T>
@@ -179,12 +179,12 @@ __device__ __forceinline__ void k_part(int K, int S, int p, int &kb, int &ke) {
 
 // Work split of nt accumulator tiles over the NW waves: S waves share each
 // tile over S parts of the reduction (split-K), so nt tiles are spread over
-// NW / S tile groups with at most MT tiles per wave.  S minimises the
-// per-wave work ceil(nt / (NW / S)) / S; ties keep the smaller S.
+// NW / S tile groups with at most MT tiles per wave.  S (a divisor of NW)
+// minimises the per-wave work ceil(nt / (NW / S)) / S; ties keep the smaller S.
 template <int NW, int MT>
 __device__ __forceinline__ int choose_split(int nt) {
   int best = 1, num = 1 << 30, den = 1;
-  for (int s = 1; s <= NW; s *= 2) {
+  for (int s = 1; s <= NW; ++s) {
     if (NW % s) continue;
     const int g = NW / s, tpw = (nt + g - 1) / g;
     if (tpw > MT) continue;
@@ -198,29 +198,29 @@ __device__ __forceinline__ int choose_split(int nt) {
 }
 
 // Sum the S split-K partial copies of each wave's tiles (waves g*S .. g*S+S-1
-// hold the parts of tile group g) as a fixed pairwise tree (parts 0+1, 2+3,
-// then (0+1)+(2+3), ...), so the result is bitwise reproducible; part 0 of
-// each group ends with the totals.  Writers of a round use LDS slot wave/2 at
-// `red` (NW/2 * 4 * 8 * 64 elements of T); callers sync before reusing it.
+// hold the parts of tile group g) in part order 0, 1, ..., S-1, so the result
+// is bitwise reproducible; part 0 of each group ends with the totals.  Parts
+// 1.. S-1 pass through LDS slot `wave` at `red` (NW * N * 8 * 64 elements of
+// T); callers sync before reusing it.
 template <typename T, int N>
 __device__ __forceinline__ void split_reduce(Tile<T> (&acc)[N], int nmine, int S, T *red) {
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, kp = wave % S;
-  for (int st = 1; st < S; st *= 2) {
-    __syncthreads();
-    if (kp % (2 * st) == st) {
-      T *slot = red + (wave >> 1) * (4 * 8 * 64);
+  __syncthreads();
+  if (kp != 0) {
+    T *slot = red + wave * (N * 8 * 64);
 #pragma unroll
-      for (int c = 0; c < N; ++c)
-        if (c < nmine)
+    for (int c = 0; c < N; ++c)
+      if (c < nmine)
 #pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            slot[(c * 8 + r) * 64 + lane] = acc[c].re[r];
-            slot[(c * 8 + 4 + r) * 64 + lane] = acc[c].im[r];
-          }
-    }
-    __syncthreads();
-    if (kp % (2 * st) == 0) {
-      const T *slot = red + ((wave + st) >> 1) * (4 * 8 * 64);
+        for (int r = 0; r < 4; ++r) {
+          slot[(c * 8 + r) * 64 + lane] = acc[c].re[r];
+          slot[(c * 8 + 4 + r) * 64 + lane] = acc[c].im[r];
+        }
+  }
+  __syncthreads();
+  if (kp == 0) {
+    for (int p = 1; p < S; ++p) {
+      const T *slot = red + (wave + p) * (N * 8 * 64);
 #pragma unroll
       for (int c = 0; c < N; ++c)
         if (c < nmine)
@@ -273,10 +273,10 @@ __device__ __forceinline__ void stage_rows(cplx<T> *dst, int ld, const cplx<T> *
 // (row tile, group of NG column tiles) sharing the A fragment of each k-step;
 // column-share jobs are (group of NG row tiles, column tile) sharing B.
 // la(row, k), lb(k, col), st(row, col, value) (st only for row < M, col < N).
-template <typename T, typename LA, typename LB, typename ST>
-__device__ __forceinline__ int gemm_jobs_rs(int M, int N, int K, int first, const LA &la, const LB &lb,
-                                            const ST &st) {
-  constexpr int NW = Cfg<T>::NW, NG = Group<T>::n;
+template <typename T, int NG, typename LA, typename LB, typename ST>
+__device__ __forceinline__ int gemm_jobs_rs_g(int M, int N, int K, int first, const LA &la, const LB &lb,
+                                              const ST &st) {
+  constexpr int NW = Cfg<T>::NW;
   const int wave = threadIdx.x >> 6;
   const int ntm = (M + 15) / 16, ntn = (N + 15) / 16, ngr = (ntn + NG - 1) / NG, njobs = ntm * ngr;
   for (int j = (wave - first % NW + NW) % NW; j < njobs; j += NW) {
@@ -297,10 +297,10 @@ __device__ __forceinline__ int gemm_jobs_rs(int M, int N, int K, int first, cons
   return njobs;
 }
 
-template <typename T, typename LA, typename LB, typename ST>
-__device__ __forceinline__ int gemm_jobs_cs(int M, int N, int K, int first, const LA &la, const LB &lb,
-                                            const ST &st) {
-  constexpr int NW = Cfg<T>::NW, NG = Group<T>::n;
+template <typename T, int NG, typename LA, typename LB, typename ST>
+__device__ __forceinline__ int gemm_jobs_cs_g(int M, int N, int K, int first, const LA &la, const LB &lb,
+                                              const ST &st) {
+  constexpr int NW = Cfg<T>::NW;
   const int wave = threadIdx.x >> 6;
   const int ntm = (M + 15) / 16, ntn = (N + 15) / 16, ngr = (ntm + NG - 1) / NG, njobs = ngr * ntn;
   for (int j = (wave - first % NW + NW) % NW; j < njobs; j += NW) {
@@ -319,6 +319,25 @@ __device__ __forceinline__ int gemm_jobs_cs(int M, int N, int K, int first, cons
         });
   }
   return njobs;
+}
+
+// Grouped jobs when there are enough of them to keep every wave busy, else
+// single-tile jobs (less operand sharing, shorter critical path).
+template <typename T, typename LA, typename LB, typename ST>
+__device__ __forceinline__ int gemm_jobs_rs(int M, int N, int K, int first, const LA &la, const LB &lb,
+                                            const ST &st) {
+  constexpr int NG = Group<T>::n;
+  const int ntm = (M + 15) / 16, ntn = (N + 15) / 16;
+  if (ntm * ((ntn + NG - 1) / NG) < Cfg<T>::NW) return gemm_jobs_rs_g<T, 1>(M, N, K, first, la, lb, st);
+  return gemm_jobs_rs_g<T, NG>(M, N, K, first, la, lb, st);
+}
+template <typename T, typename LA, typename LB, typename ST>
+__device__ __forceinline__ int gemm_jobs_cs(int M, int N, int K, int first, const LA &la, const LB &lb,
+                                            const ST &st) {
+  constexpr int NG = Group<T>::n;
+  const int ntm = (M + 15) / 16, ntn = (N + 15) / 16;
+  if (((ntm + NG - 1) / NG) * ntn < Cfg<T>::NW) return gemm_jobs_cs_g<T, 1>(M, N, K, first, la, lb, st);
+  return gemm_jobs_cs_g<T, NG>(M, N, K, first, la, lb, st);
 }
 
 }  // namespace

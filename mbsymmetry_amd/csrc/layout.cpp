@@ -295,9 +295,9 @@ bool build_worklists(const Layout &L, int eb, WorkLists &W, std::string &err) {
       }
     }
   }
-  // the split-K epilogue reduces pairwise through LDS: (waves / 2) slots of
-  // 4 tiles x 64 lanes x 8 reals
-  const size_t red_bytes = (size_t)(contract_waves(eb) / 2) * 4 * 64 * 8 * (eb / 2);
+  // the split-K epilogue passes partial tiles through LDS: one slot per wave of
+  // kContractTiles tiles x 64 lanes x 8 reals
+  const size_t red_bytes = (size_t)contract_waves(eb) * kContractTiles * 64 * 8 * (eb / 2);
   W.p0_lds = std::max(p0_lds, red_bytes);
 
   // ---- Sigma: items per (k_ir, ik1, row band, column band); blocks in (q, V-block) order ----

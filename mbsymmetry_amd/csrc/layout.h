@@ -50,6 +50,8 @@ bool build_layout(const symgw_layout_desc *d, Layout &L, std::string &err);
 // 16-row MFMA tiles.  Same shape for both element sizes.
 constexpr int contract_lds(int elem_bytes) { return elem_bytes > 0 ? 160 * 1024 : 0; }
 constexpr int contract_waves(int elem_bytes) { return elem_bytes > 0 ? 12 : 0; }
+// Accumulator tiles each contraction wave keeps across a slab.
+constexpr int kContractTiles = 2;
 // Output rows of one P0 slab (one MFMA tile row).
 constexpr int kP0Band = 16;
 // Largest Sigma slab edge (4 x 4 tiles of 16).
