@@ -65,109 +65,146 @@ __device__ __forceinline__ void tile_zero(Tile<T> &t) {
 template <typename T>
 __device__ __forceinline__ cplx<T> cz() { return cplx<T>{T(0), T(0)}; }
 
-// acc += a * b for one 16x16x4 complex step (four real MFMAs).  Lane l holds
-// A[l%16][k0+l/16] and B[k0+l/16][l%16] (operand layout of the 16x16x4 MFMAs).
-template <typename T>
+// acc += a * b (CB: acc += a * conj(b)) for one 16x16x4 complex step, four
+// real MFMAs.  Lane l holds A[l%16][k0+l/16] and B[k0+l/16][l%16] (operand
+// layout of the 16x16x4 MFMAs).  The conjugate of B is folded into the signs
+// so that a freshly loaded operand is never touched before its MFMA.
+template <bool CB, typename T>
 __device__ __forceinline__ void mma4(Tile<T> &acc, const cplx<T> &a, const cplx<T> &b) {
-  acc.re = Mfma<T>::op(a.re, b.re, acc.re);
-  acc.im = Mfma<T>::op(a.re, b.im, acc.im);
-  acc.re = Mfma<T>::op(-a.im, b.im, acc.re);
-  acc.im = Mfma<T>::op(a.im, b.re, acc.im);
+  if (CB) {
+    acc.re = Mfma<T>::op(a.re, b.re, acc.re);
+    acc.im = Mfma<T>::op(-a.re, b.im, acc.im);
+    acc.re = Mfma<T>::op(a.im, b.im, acc.re);
+    acc.im = Mfma<T>::op(a.im, b.re, acc.im);
+  } else {
+    acc.re = Mfma<T>::op(a.re, b.re, acc.re);
+    acc.im = Mfma<T>::op(a.re, b.im, acc.im);
+    acc.re = Mfma<T>::op(-a.im, b.im, acc.re);
+    acc.im = Mfma<T>::op(a.im, b.re, acc.im);
+  }
 }
 
-// The loaders below return the operand element or zero outside the valid
-// range, so the one-step look-ahead never reads out of bounds.
+// The k-loops below keep one k-step of operands in flight: the fragments of
+// step k+1 are loaded while the MFMAs of step k run, in two register sets
+// used alternately (no copies), and every load is unconditional.  Loaders
+// therefore clamp their indices and return some element of their operand for
+// any row, column or k; garbage in rows or columns outside the product only
+// reaches accumulator rows or columns that are never stored, and the A
+// operand is zeroed where k >= ke or the lane's A row is masked (arow false)
+// at the moment it is used.
+
+template <typename T>
+__device__ __forceinline__ cplx<T> amask(const cplx<T> &a, bool ok) {
+  return cplx<T>{ok ? a.re : T(0), ok ? a.im : T(0)};
+}
 
 // acc[C0+c] += A (16 x K) * B_c (K x 16) for c < NG with C0 + c < n, over k in
 // [kb, ke): one A fragment per k-step shared by the tiles.  la(i, k), lb(c, k, j)
-// with c the absolute tile index.
-template <int C0, int NG, typename T, int N, typename LA, typename LB>
-__device__ __forceinline__ void mma_rowshare(Tile<T> (&acc)[N], int n, int kb, int ke, const LA &la, const LB &lb) {
+// with c the absolute tile index.  n must be uniform across the wave.
+template <int C0, int NG, bool CB, typename T, int N, typename LA, typename LB>
+__device__ __forceinline__ void mma_rowshare(Tile<T> (&acc)[N], int n, int kb, int ke, bool arow, const LA &la,
+                                             const LB &lb) {
   static_assert(C0 + NG <= N, "tile group outside the accumulator array");
   if (kb >= ke || C0 >= n) return;
   const int lane = threadIdx.x & 63, i = lane & 15, kk = lane >> 4;
-  cplx<T> a0 = la(i, kb + kk), b0[NG];
+  cplx<T> a[2], b[2][NG];
+  a[0] = la(i, kb + kk);
 #pragma unroll
-  for (int c = 0; c < NG; ++c) b0[c] = C0 + c < n ? lb(C0 + c, kb + kk, i) : cz<T>();
-  for (int k0 = kb; k0 < ke; k0 += 4) {
-    const cplx<T> a1 = la(i, k0 + 4 + kk);
-    cplx<T> b1[NG];
+  for (int c = 0; c < NG; ++c)
+    if (C0 + c < n) b[0][c] = lb(C0 + c, kb + kk, i);
+  int k0 = kb;
+  while (true) {
 #pragma unroll
-    for (int c = 0; c < NG; ++c) b1[c] = C0 + c < n ? lb(C0 + c, k0 + 4 + kk, i) : cz<T>();
+    for (int h = 0; h < 2; ++h) {
+      a[h ^ 1] = la(i, k0 + 4 + kk);
 #pragma unroll
-    for (int c = 0; c < NG; ++c)
-      if (C0 + c < n) mma4(acc[C0 + c], a0, b0[c]);
-    a0 = a1;
+      for (int c = 0; c < NG; ++c)
+        if (C0 + c < n) b[h ^ 1][c] = lb(C0 + c, k0 + 4 + kk, i);
+      const cplx<T> am = amask(a[h], arow && k0 + kk < ke);
 #pragma unroll
-    for (int c = 0; c < NG; ++c) b0[c] = b1[c];
+      for (int c = 0; c < NG; ++c)
+        if (C0 + c < n) mma4<CB>(acc[C0 + c], am, b[h][c]);
+      k0 += 4;
+      if (k0 >= ke) return;
+    }
   }
 }
 
 // acc[C0+c] += A_c (16 x K) * B (K x 16): one B fragment shared.  la(c, i, k), lb(k, j).
-template <int C0, int NG, typename T, int N, typename LA, typename LB>
+template <int C0, int NG, bool CB, typename T, int N, typename LA, typename LB>
 __device__ __forceinline__ void mma_colshare(Tile<T> (&acc)[N], int n, int kb, int ke, const LA &la, const LB &lb) {
   static_assert(C0 + NG <= N, "tile group outside the accumulator array");
   if (kb >= ke || C0 >= n) return;
   const int lane = threadIdx.x & 63, i = lane & 15, kk = lane >> 4;
-  cplx<T> b0 = lb(kb + kk, i), a0[NG];
+  cplx<T> a[2][NG], b[2];
+  b[0] = lb(kb + kk, i);
 #pragma unroll
-  for (int c = 0; c < NG; ++c) a0[c] = C0 + c < n ? la(C0 + c, i, kb + kk) : cz<T>();
-  for (int k0 = kb; k0 < ke; k0 += 4) {
-    const cplx<T> b1 = lb(k0 + 4 + kk, i);
-    cplx<T> a1[NG];
+  for (int c = 0; c < NG; ++c)
+    if (C0 + c < n) a[0][c] = la(C0 + c, i, kb + kk);
+  int k0 = kb;
+  while (true) {
 #pragma unroll
-    for (int c = 0; c < NG; ++c) a1[c] = C0 + c < n ? la(C0 + c, i, k0 + 4 + kk) : cz<T>();
+    for (int h = 0; h < 2; ++h) {
+      b[h ^ 1] = lb(k0 + 4 + kk, i);
 #pragma unroll
-    for (int c = 0; c < NG; ++c)
-      if (C0 + c < n) mma4(acc[C0 + c], a0[c], b0);
-    b0 = b1;
+      for (int c = 0; c < NG; ++c)
+        if (C0 + c < n) a[h ^ 1][c] = la(C0 + c, i, k0 + 4 + kk);
+      const bool kin = k0 + kk < ke;
 #pragma unroll
-    for (int c = 0; c < NG; ++c) a0[c] = a1[c];
+      for (int c = 0; c < NG; ++c)
+        if (C0 + c < n) mma4<CB>(acc[C0 + c], amask(a[h][c], kin), b[h]);
+      k0 += 4;
+      if (k0 >= ke) return;
+    }
   }
 }
 
 // acc[C0+c] += A_c * B_c with independent operands.  la(c, i, k), lb(c, k, j).
-template <int C0, int NG, typename T, int N, typename LA, typename LB>
+template <int C0, int NG, bool CB, typename T, int N, typename LA, typename LB>
 __device__ __forceinline__ void mma_multi(Tile<T> (&acc)[N], int n, int kb, int ke, const LA &la, const LB &lb) {
   static_assert(C0 + NG <= N, "tile group outside the accumulator array");
   if (kb >= ke || C0 >= n) return;
   const int lane = threadIdx.x & 63, i = lane & 15, kk = lane >> 4;
-  cplx<T> a0[NG], b0[NG];
+  cplx<T> a[2][NG], b[2][NG];
 #pragma unroll
-  for (int c = 0; c < NG; ++c) {
-    a0[c] = C0 + c < n ? la(C0 + c, i, kb + kk) : cz<T>();
-    b0[c] = C0 + c < n ? lb(C0 + c, kb + kk, i) : cz<T>();
-  }
-  for (int k0 = kb; k0 < ke; k0 += 4) {
-    cplx<T> a1[NG], b1[NG];
-#pragma unroll
-    for (int c = 0; c < NG; ++c) {
-      a1[c] = C0 + c < n ? la(C0 + c, i, k0 + 4 + kk) : cz<T>();
-      b1[c] = C0 + c < n ? lb(C0 + c, k0 + 4 + kk, i) : cz<T>();
+  for (int c = 0; c < NG; ++c)
+    if (C0 + c < n) {
+      a[0][c] = la(C0 + c, i, kb + kk);
+      b[0][c] = lb(C0 + c, kb + kk, i);
     }
+  int k0 = kb;
+  while (true) {
 #pragma unroll
-    for (int c = 0; c < NG; ++c)
-      if (C0 + c < n) mma4(acc[C0 + c], a0[c], b0[c]);
+    for (int h = 0; h < 2; ++h) {
 #pragma unroll
-    for (int c = 0; c < NG; ++c) {
-      a0[c] = a1[c];
-      b0[c] = b1[c];
+      for (int c = 0; c < NG; ++c)
+        if (C0 + c < n) {
+          a[h ^ 1][c] = la(C0 + c, i, k0 + 4 + kk);
+          b[h ^ 1][c] = lb(C0 + c, k0 + 4 + kk, i);
+        }
+      const bool kin = k0 + kk < ke;
+#pragma unroll
+      for (int c = 0; c < NG; ++c)
+        if (C0 + c < n) mma4<CB>(acc[C0 + c], amask(a[h][c], kin), b[h][c]);
+      k0 += 4;
+      if (k0 >= ke) return;
     }
   }
 }
 
 // All persistent tiles of a wave, in groups of Group<T>::n.
-template <typename T, typename LA, typename LB>
-__device__ __forceinline__ void mma_rowshare_all(Acc<T> &acc, int n, int kb, int ke, const LA &la, const LB &lb) {
+template <bool CB, typename T, typename LA, typename LB>
+__device__ __forceinline__ void mma_rowshare_all(Acc<T> &acc, int n, int kb, int ke, bool arow, const LA &la,
+                                                 const LB &lb) {
   constexpr int NG = Group<T>::n, MT = Cfg<T>::MT;
-  mma_rowshare<0, NG < MT ? NG : MT>(acc, n, kb, ke, la, lb);
-  if (NG < MT) mma_rowshare<NG < MT ? NG : 0, NG < MT ? MT - NG : 1>(acc, n, kb, ke, la, lb);
+  mma_rowshare<0, NG < MT ? NG : MT, CB>(acc, n, kb, ke, arow, la, lb);
+  if (NG < MT) mma_rowshare<NG < MT ? NG : 0, NG < MT ? MT - NG : 1, CB>(acc, n, kb, ke, arow, la, lb);
 }
-template <typename T, typename LA, typename LB>
+template <bool CB, typename T, typename LA, typename LB>
 __device__ __forceinline__ void mma_multi_all(Acc<T> &acc, int n, int kb, int ke, const LA &la, const LB &lb) {
   constexpr int NG = Group<T>::n, MT = Cfg<T>::MT;
-  mma_multi<0, NG < MT ? NG : MT>(acc, n, kb, ke, la, lb);
-  if (NG < MT) mma_multi<NG < MT ? NG : 0, NG < MT ? MT - NG : 1>(acc, n, kb, ke, la, lb);
+  mma_multi<0, NG < MT ? NG : MT, CB>(acc, n, kb, ke, la, lb);
+  if (NG < MT) mma_multi<NG < MT ? NG : 0, NG < MT ? MT - NG : 1, CB>(acc, n, kb, ke, la, lb);
 }
 
 // Split of K into S parts for part p (boundaries multiples of 4).
@@ -204,7 +241,7 @@ __device__ __forceinline__ int choose_split(int nt) {
 // T); callers sync before reusing it.
 template <typename T, int N>
 __device__ __forceinline__ void split_reduce(Tile<T> (&acc)[N], int nmine, int S, T *red) {
-  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, kp = wave % S;
+  const int tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63, kp = wave % S;
   __syncthreads();
   if (kp != 0) {
     T *slot = red + wave * (N * 8 * 64);
@@ -269,7 +306,7 @@ __device__ __forceinline__ void stage_rows(cplx<T> *dst, int ld, const cplx<T> *
 
 // C (M x N) = A (M x K) B (K x N) into LDS as independent jobs dealt to the
 // waves round-robin, job j to wave (first + j) % NW; returns the job count so
-// that a following product can continue the deal.  Row-share jobs are
+// that a following product can continue the deal.  Loaders clamp (see above).  Row-share jobs are
 // (row tile, group of NG column tiles) sharing the A fragment of each k-step;
 // column-share jobs are (group of NG row tiles, column tile) sharing B.
 // la(row, k), lb(k, col), st(row, col, value) (st only for row < M, col < N).
@@ -277,15 +314,15 @@ template <typename T, int NG, typename LA, typename LB, typename ST>
 __device__ __forceinline__ int gemm_jobs_rs_g(int M, int N, int K, int first, const LA &la, const LB &lb,
                                               const ST &st) {
   constexpr int NW = Cfg<T>::NW;
-  const int wave = threadIdx.x >> 6;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int ntm = (M + 15) / 16, ntn = (N + 15) / 16, ngr = (ntn + NG - 1) / NG, njobs = ntm * ngr;
   for (int j = (wave - first % NW + NW) % NW; j < njobs; j += NW) {
     const int mt = j / ngr, g = (j - mt * ngr) * NG, n = min(NG, ntn - g), m0 = mt * 16;
     Tiles<T> acc;
 #pragma unroll
     for (int c = 0; c < NG; ++c) tile_zero(acc[c]);
-    mma_rowshare<0, NG>(acc, n, 0, K, [&](int i, int k) { return la(m0 + i, k); },
-                        [&](int c, int k, int jj) { return lb(k, (g + c) * 16 + jj); });
+    mma_rowshare<0, NG, false>(acc, n, 0, K, true, [&](int i, int k) { return la(m0 + i, k); },
+                               [&](int c, int k, int jj) { return lb(k, (g + c) * 16 + jj); });
 #pragma unroll
     for (int c = 0; c < NG; ++c)
       if (c < n)
@@ -301,15 +338,15 @@ template <typename T, int NG, typename LA, typename LB, typename ST>
 __device__ __forceinline__ int gemm_jobs_cs_g(int M, int N, int K, int first, const LA &la, const LB &lb,
                                               const ST &st) {
   constexpr int NW = Cfg<T>::NW;
-  const int wave = threadIdx.x >> 6;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int ntm = (M + 15) / 16, ntn = (N + 15) / 16, ngr = (ntm + NG - 1) / NG, njobs = ngr * ntn;
   for (int j = (wave - first % NW + NW) % NW; j < njobs; j += NW) {
     const int gt = j / ntn, nt = j - gt * ntn, g = gt * NG, n = min(NG, ntm - g), n0 = nt * 16;
     Tiles<T> acc;
 #pragma unroll
     for (int c = 0; c < NG; ++c) tile_zero(acc[c]);
-    mma_colshare<0, NG>(acc, n, 0, K, [&](int c, int i, int k) { return la((g + c) * 16 + i, k); },
-                        [&](int k, int jj) { return lb(k, n0 + jj); });
+    mma_colshare<0, NG, false>(acc, n, 0, K, [&](int c, int i, int k) { return la((g + c) * 16 + i, k); },
+                               [&](int k, int jj) { return lb(k, n0 + jj); });
 #pragma unroll
     for (int c = 0; c < NG; ++c)
       if (c < n)
@@ -360,7 +397,7 @@ __global__ __launch_bounds__(Cfg<T>::NT) void p0_mfma_kernel(const P0Item *__res
   const int t = (int)(w % nt_half);
   const P0Item it = items[w / nt_half];
   constexpr int NW = Cfg<T>::NW, NT = Cfg<T>::NT, MT = Cfg<T>::MT;
-  const int tid = threadIdx.x, wave = tid >> 6;
+  const int tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cplx<T> *lds = reinterpret_cast<cplx<T> *>(smem_raw);
   const int q = it.q, nband = it.nband, nct = (q + 15) / 16;
@@ -391,11 +428,11 @@ __global__ __launch_bounds__(Cfg<T>::NT) void p0_mfma_kernel(const P0Item *__res
         // E[(r,a)][n] = sum_m V[r][a][m] G2[m][n]
         {
           auto la = [&](int row, int m) {
-            if (row >= nr * k1 || m >= k2) return cz<T>();
+            row = min(row, nr * k1 - 1);
             const int r = row / k1, a = row - r * k1;
-            return bufV[r * ldk + a * k2 + m];
+            return bufV[r * ldk + a * k2 + min(m, k2 - 1)];
           };
-          auto lb = [&](int m, int n) { return (m < k2 && n < k2) ? G2[m * k2 + n] : cz<T>(); };
+          auto lb = [&](int m, int n) { return G2[min(m, k2 - 1) * k2 + min(n, k2 - 1)]; };
           auto st = [&](int row, int n, cplx<T> v) {
             const int r = row / k1, a = row - r * k1;
             bufE[r * ldk + a * k2 + n] = v;
@@ -406,11 +443,11 @@ __global__ __launch_bounds__(Cfg<T>::NT) void p0_mfma_kernel(const P0Item *__res
         __syncthreads();
         // C[r][p][n] = sum_a G1[p][a] E[r][a][n]  (into bufV)
         gemm_jobs_rs<T>(
-            k1, nr * k2, k1, 0, [&](int p, int a) { return (p < k1 && a < k1) ? G1[p * k1 + a] : cz<T>(); },
+            k1, nr * k2, k1, 0, [&](int p, int a) { return G1[min(p, k1 - 1) * k1 + min(a, k1 - 1)]; },
             [&](int a, int col) {
-              if (a >= k1 || col >= nr * k2) return cz<T>();
+              col = min(col, nr * k2 - 1);
               const int r = col / k2, n = col - r * k2;
-              return bufE[r * ldk + a * k2 + n];
+              return bufE[r * ldk + min(a, k1 - 1) * k2 + n];
             },
             [&](int p, int col, cplx<T> v) {
               const int r = col / k2, n = col - r * k2;
@@ -418,18 +455,18 @@ __global__ __launch_bounds__(Cfg<T>::NT) void p0_mfma_kernel(const P0Item *__res
             });
         __syncthreads();
         // acc[r0 + r][P] += sum_x C[r][x] conj(V[P][x]) over this wave's part of x
-        mma_rowshare_all(
-            acc, nmine, kb, ke,
-            [&](int i, int x) {
-              const int rr = i - r0;
-              return (rr >= 0 && rr < nr && x < ke) ? bufV[rr * ldk + x] : cz<T>();
-            },
-            [&](int c, int x, int j) {
-              const int P = (tg + TG * c) * 16 + j;
-              if (x >= ke || P >= q) return cz<T>();
-              const cplx<T> v = Vb[(int64_t)P * K + x];
-              return cplx<T>{v.re, -v.im};
-            });
+        {
+          // rows of the band outside this chunk contribute nothing
+          const int rr = (threadIdx.x & 15) - r0;
+          const bool arow = rr >= 0 && rr < nr;
+          const int rc = min(max(rr, 0), nr - 1);
+          mma_rowshare_all<true>(
+              acc, nmine, kb, ke, arow, [&](int i, int x) { return bufV[rc * ldk + min(x, K - 1)]; },
+              [&](int c, int x, int j) {
+                const int P = min((tg + TG * c) * 16 + j, q - 1);
+                return Vb[(int64_t)P * K + min(x, K - 1)];
+              });
+        }
       }
     }
   }
@@ -453,13 +490,16 @@ __global__ __launch_bounds__(Cfg<T>::NT) void p0_mfma_kernel(const P0Item *__res
 
 // ---------------------------------------------------------------------------
 // Sigma.  Slab = rows [I0, I0+Ib) x columns [J0, J0+Jb) of ao block ik1 of iBZ
-// point k, time t, spin s.  For every V block (q, iq, ik2), in chunks of qsub
-// aux rows Q (E and Z stored with the (Q, n) reduction index contiguous):
-//   Z_Q = sum_P conj(P_{QP}(t)) V_P,  E_Q = V_Q G^{k-q}(t),
-//   Sigma[i][j] += sum_{Q,n} E_Q[i][n] conj(Z_Q[j][n])
+// point k, spin s, for kSigTimes consecutive times t0 + tb.  For every V block
+// (q, iq, ik2), in chunks of qsub aux rows Q (E and Z stored with the (Q, n)
+// reduction index contiguous):
+//   Z_Q(t) = sum_P conj(P_{QP}(t)) V_P,  E_Q(t) = V_Q G^{k-q}(t),
+//   Sigma(t)[i][j] += sum_{Q,n} E_Q(t)[i][n] conj(Z_Q(t)[j][n])
 // the contraction of cugw_qkpt_block.cu:381-407 (Y1 = V G, Y2 = Y1 P,
 // Sigma += Y2 V^H) with the aux contraction moved onto V so that the largest
-// GEMM (q x q x k1 k2) reads V straight from memory.
+// GEMM (q x q x k1 k2) reads V straight from memory.  The times of a slab
+// share every V fragment: their P rows are stacked as rows of one Z product
+// and their G^{k-q} blocks as columns of one E product.
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ __launch_bounds__(Cfg<T>::NT) void sigma_mfma_kernel(const SigItem *__restrict__ items,
@@ -467,20 +507,22 @@ __global__ __launch_bounds__(Cfg<T>::NT) void sigma_mfma_kernel(const SigItem *_
                                                        int ns, const cplx<T> *__restrict__ V,
                                                        const cplx<T> *__restrict__ G, const cplx<T> *__restrict__ Pf,
                                                        cplx<double> *__restrict__ Sig, double pref) {
+  constexpr int TB = kSigTimes;
   const int64_t w = xcd_logical(blockIdx.x, gridDim.x);
   if (w >= n_work) return;
-  const int per = nts * ns;
+  const int per = (nts / TB) * ns;
   const SigItem it = items[w / per];
-  const int ts = (int)(w % per), t = ts / ns, s = ts - (ts / ns) * ns;
-  constexpr int NW = Cfg<T>::NW, NT = Cfg<T>::NT, MT = Cfg<T>::MT;
-  const int tid = threadIdx.x, wave = tid >> 6;
+  const int ts = (int)(w % per), t0 = (ts / ns) * TB, s = ts - (ts / ns) * ns;
+  constexpr int NW = Cfg<T>::NW, MT = Cfg<T>::MT;
+  const int tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cplx<T> *lds = reinterpret_cast<cplx<T> *>(smem_raw);
   const int k1 = it.k1, I0 = it.I0, Ib = it.Ib, J0 = it.J0, Jb = it.Jb;
-  const int nti = (Ib + 15) / 16, ntj = (Jb + 15) / 16, ntile = nti * ntj;
-  // output tiles: S waves share a tile over S parts of the (Q, n) reduction
-  const int S = choose_split<NW, MT>(ntile), TG = NW / S, kp = wave % S, tg = wave / S;
-  const int nmine = ntile > tg ? (ntile - tg + TG - 1) / TG : 0;  // tiles tg, tg + TG, ...
+  const int nti = (Ib + 15) / 16, ntj = (Jb + 15) / 16, ntile = nti * ntj, nall = TB * ntile;
+  // output tiles u = tb * ntile + tt: S waves share a tile over S parts of the
+  // (Q, n) reduction
+  const int S = choose_split<NW, MT>(nall), TG = NW / S, kp = wave % S, tg = wave / S;
+  const int nmine = nall > tg ? (nall - tg + TG - 1) / TG : 0;  // tiles tg, tg + TG, ...
   Acc<T> acc;
 #pragma unroll
   for (int c = 0; c < MT; ++c) tile_zero(acc[c]);
@@ -489,73 +531,83 @@ __global__ __launch_bounds__(Cfg<T>::NT) void sigma_mfma_kernel(const SigItem *_
     const SigBlk B = blks[b];
     const int q = B.q, k2 = B.k2, K = k1 * k2, qsub = B.qsub, ldz = B.ldz, lde = B.lde;
     const cplx<T> *Vb = V + B.v_off;
-    const cplx<T> *G2 = G + B.g2_base + (int64_t)t * B.g2_tstride + (int64_t)s * B.g2_sstride;
-    const cplx<T> *Pb = Pf + B.p_base + (int64_t)t * B.p_tstride;
+    const cplx<T> *G2 = G + B.g2_base + (int64_t)t0 * B.g2_tstride + (int64_t)s * B.g2_sstride;
+    const cplx<T> *Pb = Pf + B.p_base + (int64_t)t0 * B.p_tstride;
+    const int g2ts = B.g2_tstride, pts = B.p_tstride;
     // y = r*k2 + n indexes (aux row of the chunk, orbital n of k - q)
-    cplx<T> *bufZ = lds;                 // [Jb][ldz]  Z[r][j][n] at j*ldz + y
-    cplx<T> *bufE = bufZ + Jb * ldz;     // [Ib][lde]  E[r][i][n] at i*lde + y
-    cplx<T> *bufP = bufE + Ib * lde;     // [qsub][q]  conj(P) rows
+    cplx<T> *bufZ = lds;                      // [TB][Jb][ldz]  Z[r][j][n] at j*ldz + y
+    cplx<T> *bufE = bufZ + TB * Jb * ldz;     // [TB][Ib][lde]  E[r][i][n] at i*lde + y
+    cplx<T> *bufP = bufE + TB * Ib * lde;     // [TB][qsub][q]  conj(P) rows
     const int zc = Jb * k2;
     const cplx<T> *Vj = Vb + J0 * k2;
     for (int Q0 = 0; Q0 < q; Q0 += qsub) {
       const int nr = min(qsub, q - Q0);
       __syncthreads();
-      stage_rows<T, 4>(bufP, q, Pb + (int64_t)Q0 * q, nr * q, q, [](cplx<T> p) { return cplx<T>{p.re, -p.im}; });
+#pragma unroll
+      for (int tb = 0; tb < TB; ++tb)
+        stage_rows<T, 4>(bufP + tb * qsub * q, q, Pb + (int64_t)tb * pts + (int64_t)Q0 * q, nr * q, q,
+                         [](cplx<T> p) { return cplx<T>{p.re, -p.im}; });
       __syncthreads();
-      // Z[r][x] = sum_P conj(P[Q0+r][P]) V[P][J0*k2 + x],  x < Jb*k2
+      // Z[(tb,r)][x] = sum_P conj(P_tb[Q0+r][P]) V[P][J0*k2 + x],  x < Jb*k2
       int nj;
       {
-        auto la = [&](int r, int P) { return (r < nr && P < q) ? bufP[r * q + P] : cz<T>(); };
-        auto lb = [&](int P, int x) { return (P < q && x < zc) ? Vj[(int64_t)P * K + x] : cz<T>(); };
-        auto st = [&](int r, int x, cplx<T> v) {
-          const int jj = x / k2, n = x - jj * k2;
-          bufZ[jj * ldz + r * k2 + n] = v;
+        auto la = [&](int row, int P) {
+          row = min(row, TB * nr - 1);
+          const int tb = row / nr, r = row - tb * nr;
+          return bufP[(tb * qsub + r) * q + min(P, q - 1)];
+        };
+        auto lb = [&](int P, int x) { return Vj[(int64_t)min(P, q - 1) * K + min(x, zc - 1)]; };
+        auto st = [&](int row, int x, cplx<T> v) {
+          const int tb = row / nr, r = row - tb * nr, jj = x / k2, n = x - jj * k2;
+          bufZ[(tb * Jb + jj) * ldz + r * k2 + n] = v;
         };
         // several row tiles: share each V fragment among them
-        if (nr > 16) nj = gemm_jobs_cs<T>(nr, zc, q, 0, la, lb, st);
-        else nj = gemm_jobs_rs<T>(nr, zc, q, 0, la, lb, st);
+        if (TB * nr > 16) nj = gemm_jobs_cs<T>(TB * nr, zc, q, 0, la, lb, st);
+        else nj = gemm_jobs_rs<T>(TB * nr, zc, q, 0, la, lb, st);
       }
-      // E[(r,i)][n] = sum_m V[Q0+r][I0+i][m] G2[m][n]  (jobs dealt on after Z's)
+      // E[(r,i)][(tb,n)] = sum_m V[Q0+r][I0+i][m] G2_tb[m][n]  (jobs dealt on after Z's)
       {
         auto la = [&](int row, int m) {
-          if (row >= nr * Ib || m >= k2) return cz<T>();
+          row = min(row, nr * Ib - 1);
           const int r = row / Ib, ii = row - r * Ib;
-          return Vb[(int64_t)(Q0 + r) * K + (I0 + ii) * k2 + m];
+          return Vb[(int64_t)(Q0 + r) * K + (I0 + ii) * k2 + min(m, k2 - 1)];
         };
-        auto lb = [&](int m, int n) { return (m < k2 && n < k2) ? G2[m * k2 + n] : cz<T>(); };
-        auto st = [&](int row, int n, cplx<T> v) {
-          const int r = row / Ib, ii = row - r * Ib;
-          bufE[ii * lde + r * k2 + n] = v;
+        auto lb = [&](int m, int col) {
+          col = min(col, TB * k2 - 1);
+          const int tb = col / k2, n = col - tb * k2;
+          return G2[(int64_t)tb * g2ts + min(m, k2 - 1) * k2 + n];
         };
-        if (k2 > 16) gemm_jobs_rs<T>(nr * Ib, k2, k2, nj, la, lb, st);
-        else gemm_jobs_cs<T>(nr * Ib, k2, k2, nj, la, lb, st);
+        auto st = [&](int row, int col, cplx<T> v) {
+          const int r = row / Ib, ii = row - r * Ib, tb = col / k2, n = col - tb * k2;
+          bufE[(tb * Ib + ii) * lde + r * k2 + n] = v;
+        };
+        if (TB * k2 > 16) gemm_jobs_rs<T>(nr * Ib, TB * k2, k2, nj, la, lb, st);
+        else gemm_jobs_cs<T>(nr * Ib, TB * k2, k2, nj, la, lb, st);
       }
       __syncthreads();
-      // acc[i][j] += sum_y E[i][y] conj(Z[j][y]) over this wave's part of y
+      // acc[i][j] += sum_y E_tb[i][y] conj(Z_tb[j][y]) over this wave's part of y
       const int Y = nr * k2;
       int yb, ye;
       k_part(Y, S, kp, yb, ye);
-      mma_multi_all(
+      mma_multi_all<true>(
           acc, nmine, yb, ye,
           [&](int c, int i, int y) {
-            const int i0 = ((tg + TG * c) / ntj) * 16;
-            return (i0 + i < Ib && y < ye) ? bufE[(i0 + i) * lde + y] : cz<T>();
+            const int u = tg + TG * c, tb = u / ntile, i0 = ((u - tb * ntile) / ntj) * 16;
+            return bufE[(tb * Ib + min(i0 + i, Ib - 1)) * lde + min(y, Y - 1)];
           },
           [&](int c, int y, int j) {
-            const int j0 = ((tg + TG * c) % ntj) * 16;
-            if (j0 + j >= Jb || y >= ye) return cz<T>();
-            const cplx<T> z = bufZ[(j0 + j) * ldz + y];
-            return cplx<T>{z.re, -z.im};
+            const int u = tg + TG * c, tb = u / ntile, j0 = ((u - tb * ntile) % ntj) * 16;
+            return bufZ[(tb * Jb + min(j0 + j, Jb - 1)) * ldz + min(y, Y - 1)];
           });
     }
   }
-  cplx<double> *out = Sig + it.out_base + (int64_t)t * it.out_tstride + (int64_t)s * it.out_sstride;
   if (S > 1) split_reduce(acc, nmine, S, reinterpret_cast<T *>(lds));
   if (kp == 0) {
 #pragma unroll
     for (int c = 0; c < MT; ++c) {
-      const int tt = tg + TG * c;
+      const int u = tg + TG * c, tb = u / ntile, tt = u - tb * ntile;
       if (c < nmine) {
+        cplx<double> *out = Sig + it.out_base + (int64_t)(t0 + tb) * it.out_tstride + (int64_t)s * it.out_sstride;
         const int i0 = (tt / ntj) * 16, j0 = (tt % ntj) * 16;
         tile_store(acc[c], [&](int i, int j, cplx<T> v) {
           if (i0 + i < Ib && j0 + j < Jb)
@@ -583,7 +635,7 @@ template <typename T>
 hipError_t launch_sigma(const SigItem *items, const SigBlk *blks, int n_items, int nts, int ns, const cplx<T> *V,
                         const cplx<T> *G, const cplx<T> *Pf, cplx<double> *Sig, double pref, size_t lds,
                         hipStream_t st) {
-  const int64_t n_work = (int64_t)n_items * nts * ns;
+  const int64_t n_work = (int64_t)n_items * (nts / kSigTimes) * ns;
   if (n_work == 0) return hipSuccess;
   (void)hipFuncSetAttribute((const void *)sigma_mfma_kernel<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)lds);

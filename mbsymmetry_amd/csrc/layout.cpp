@@ -323,15 +323,17 @@ bool build_worklists(const Layout &L, int eb, WorkLists &W, std::string &err) {
           B.p_base = L.qaux_off[q] * nts + L.aux_offsets[q][t[0]];
           B.p_tstride = (int32_t)L.qaux_size[q];
           B.pad = 0;
-          // Z ([band][qsub*k2]), E ([band][qsub*k2]) and conj(P) rows ([qsub][q])
-          // of one chunk share the LDS budget (+2 elements of row padding)
-          int per_row = B.q + 2 * band * B.k2;
-          int cap = std::min(kSigRows, (budget - 2 * band) / per_row);
+          // per time of the slab: Z ([band][qsub*k2]), E ([band][qsub*k2]) and
+          // conj(P) rows ([qsub][q]) of one chunk share the LDS budget (+1
+          // element of row padding each)
+          const int TB = kSigTimes;
+          int per_row = TB * (B.q + 2 * band * B.k2);
+          int cap = std::min(kSigRows, (budget - 2 * TB * band) / per_row);
           if (cap < 1) { err = fmt("Sigma block too large for the LDS plan (q, k2)", B.q, B.k2); return false; }
           B.qsub = balanced(B.q, cap);
           B.ldz = odd_ld(B.qsub * B.k2);
           B.lde = odd_ld(B.qsub * B.k2);
-          sig_lds = std::max(sig_lds, (size_t)(band * (B.ldz + B.lde) + B.qsub * B.q) * eb);
+          sig_lds = std::max(sig_lds, (size_t)TB * (band * (B.ldz + B.lde) + B.qsub * B.q) * eb);
           W.sig_blks.push_back(B);
         }
       }

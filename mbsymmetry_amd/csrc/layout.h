@@ -56,8 +56,11 @@ constexpr int kContractTiles = 2;
 constexpr int kP0Band = 16;
 // Largest Sigma slab edge (4 x 4 tiles of 16).
 constexpr int kSigBand = 64;
-// Most aux rows Q of one Sigma LDS chunk (four row tiles of the Z product).
+// Most aux rows Q of one Sigma LDS chunk.
 constexpr int kSigRows = 64;
+// Consecutive times of one Sigma slab (they share every V fragment); nts is
+// even, so 2 divides it.
+constexpr int kSigTimes = 2;
 
 // One V-block contraction that feeds a P0 output slab (cugw_qkpt_block.cu:219-261).
 struct P0Blk {
