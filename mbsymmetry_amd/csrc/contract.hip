@@ -93,6 +93,11 @@ __device__ __forceinline__ void mma4(Tile<T> &acc, const cplx<T> &a, const cplx<
 // operand is zeroed where k >= ke or the lane's A row is masked (arow false)
 // at the moment it is used.
 
+// Keeps the scheduler from sinking the next step's loads below this step's
+// MFMAs (it does so under register pressure, which serialises every k-step on
+// the memory latency).
+__device__ __forceinline__ void prefetch_fence() { __builtin_amdgcn_sched_barrier(0); }
+
 template <typename T>
 __device__ __forceinline__ cplx<T> amask(const cplx<T> &a, bool ok) {
   return cplx<T>{ok ? a.re : T(0), ok ? a.im : T(0)};
@@ -120,6 +125,7 @@ __device__ __forceinline__ void mma_rowshare(Tile<T> (&acc)[N], int n, int kb, i
 #pragma unroll
       for (int c = 0; c < NG; ++c)
         if (C0 + c < n) b[h ^ 1][c] = lb(C0 + c, k0 + 4 + kk, i);
+      prefetch_fence();
       const cplx<T> am = amask(a[h], arow && k0 + kk < ke);
 #pragma unroll
       for (int c = 0; c < NG; ++c)
@@ -149,6 +155,7 @@ __device__ __forceinline__ void mma_colshare(Tile<T> (&acc)[N], int n, int kb, i
 #pragma unroll
       for (int c = 0; c < NG; ++c)
         if (C0 + c < n) a[h ^ 1][c] = la(C0 + c, i, k0 + 4 + kk);
+      prefetch_fence();
       const bool kin = k0 + kk < ke;
 #pragma unroll
       for (int c = 0; c < NG; ++c)
@@ -182,6 +189,7 @@ __device__ __forceinline__ void mma_multi(Tile<T> (&acc)[N], int n, int kb, int 
           a[h ^ 1][c] = la(C0 + c, i, k0 + 4 + kk);
           b[h ^ 1][c] = lb(C0 + c, k0 + 4 + kk, i);
         }
+      prefetch_fence();
       const bool kin = k0 + kk < ke;
 #pragma unroll
       for (int c = 0; c < NG; ++c)
