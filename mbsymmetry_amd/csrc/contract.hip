@@ -499,8 +499,8 @@ __global__ __launch_bounds__(Cfg<T>::NT) void p0_mfma_kernel(const P0Item *__res
 // ---------------------------------------------------------------------------
 // Sigma.  Slab = rows [I0, I0+Ib) x columns [J0, J0+Jb) of ao block ik1 of iBZ
 // point k, spin s, for kSigTimes consecutive times t0 + tb.  For every V block
-// (q, iq, ik2), in chunks of qsub aux rows Q (E and Z stored with the (Q, n)
-// reduction index contiguous):
+// (q, iq, ik2), in chunks of qsub aux rows Q by nsub orbitals n (E and Z stored
+// with the (Q, n) reduction index contiguous):
 //   Z_Q(t) = sum_P conj(P_{QP}(t)) V_P,  E_Q(t) = V_Q G^{k-q}(t),
 //   Sigma(t)[i][j] += sum_{Q,n} E_Q(t)[i][n] conj(Z_Q(t)[j][n])
 // the contraction of cugw_qkpt_block.cu:381-407 (Y1 = V G, Y2 = Y1 P,
@@ -537,16 +537,15 @@ __global__ __launch_bounds__(Cfg<T>::NT) void sigma_mfma_kernel(const SigItem *_
 
   for (int b = it.blk_begin; b < it.blk_end; ++b) {
     const SigBlk B = blks[b];
-    const int q = B.q, k2 = B.k2, K = k1 * k2, qsub = B.qsub, ldz = B.ldz, lde = B.lde;
+    const int q = B.q, k2 = B.k2, K = k1 * k2, qsub = B.qsub, nsub = B.nsub, ldz = B.ldz, lde = B.lde;
     const cplx<T> *Vb = V + B.v_off;
     const cplx<T> *G2 = G + B.g2_base + (int64_t)t0 * B.g2_tstride + (int64_t)s * B.g2_sstride;
     const cplx<T> *Pb = Pf + B.p_base + (int64_t)t0 * B.p_tstride;
     const int g2ts = B.g2_tstride, pts = B.p_tstride;
-    // y = r*k2 + n indexes (aux row of the chunk, orbital n of k - q)
+    // y = r*nn + l indexes (aux row of the chunk, orbital n0 + l of k - q)
     cplx<T> *bufZ = lds;                      // [TB][Jb][ldz]  Z[r][j][n] at j*ldz + y
     cplx<T> *bufE = bufZ + TB * Jb * ldz;     // [TB][Ib][lde]  E[r][i][n] at i*lde + y
     cplx<T> *bufP = bufE + TB * Ib * lde;     // [TB][qsub][q]  conj(P) rows
-    const int zc = Jb * k2;
     const cplx<T> *Vj = Vb + J0 * k2;
     for (int Q0 = 0; Q0 < q; Q0 += qsub) {
       const int nr = min(qsub, q - Q0);
@@ -555,58 +554,65 @@ __global__ __launch_bounds__(Cfg<T>::NT) void sigma_mfma_kernel(const SigItem *_
       for (int tb = 0; tb < TB; ++tb)
         stage_rows<T, 4>(bufP + tb * qsub * q, q, Pb + (int64_t)tb * pts + (int64_t)Q0 * q, nr * q, q,
                          [](cplx<T> p) { return cplx<T>{p.re, -p.im}; });
-      __syncthreads();
-      // Z[(tb,r)][x] = sum_P conj(P_tb[Q0+r][P]) V[P][J0*k2 + x],  x < Jb*k2
-      int nj;
-      {
-        auto la = [&](int row, int P) {
-          row = min(row, TB * nr - 1);
-          const int tb = row / nr, r = row - tb * nr;
-          return bufP[(tb * qsub + r) * q + min(P, q - 1)];
-        };
-        auto lb = [&](int P, int x) { return Vj[(int64_t)min(P, q - 1) * K + min(x, zc - 1)]; };
-        auto st = [&](int row, int x, cplx<T> v) {
-          const int tb = row / nr, r = row - tb * nr, jj = x / k2, n = x - jj * k2;
-          bufZ[(tb * Jb + jj) * ldz + r * k2 + n] = v;
-        };
-        // several row tiles: share each V fragment among them
-        if (TB * nr > 16) nj = gemm_jobs_cs<T>(TB * nr, zc, q, 0, la, lb, st);
-        else nj = gemm_jobs_rs<T>(TB * nr, zc, q, 0, la, lb, st);
+      for (int n0 = 0; n0 < k2; n0 += nsub) {
+        const int nn = min(nsub, k2 - n0), zc = Jb * nn;
+        __syncthreads();
+        // Z[(tb,r)][(j,l)] = sum_P conj(P_tb[Q0+r][P]) V[P][J0+j][n0+l]
+        int nj;
+        {
+          auto la = [&](int row, int P) {
+            row = min(row, TB * nr - 1);
+            const int tb = row / nr, r = row - tb * nr;
+            return bufP[(tb * qsub + r) * q + min(P, q - 1)];
+          };
+          auto lb = [&](int P, int x) {
+            x = min(x, zc - 1);
+            const int jj = x / nn, l = x - jj * nn;
+            return Vj[(int64_t)min(P, q - 1) * K + jj * k2 + n0 + l];
+          };
+          auto st = [&](int row, int x, cplx<T> v) {
+            const int tb = row / nr, r = row - tb * nr, jj = x / nn, l = x - jj * nn;
+            bufZ[(tb * Jb + jj) * ldz + r * nn + l] = v;
+          };
+          // several row tiles: share each V fragment among them
+          if (TB * nr > 16) nj = gemm_jobs_cs<T>(TB * nr, zc, q, 0, la, lb, st);
+          else nj = gemm_jobs_rs<T>(TB * nr, zc, q, 0, la, lb, st);
+        }
+        // E[(r,i)][(tb,l)] = sum_m V[Q0+r][I0+i][m] G2_tb[m][n0+l]  (jobs dealt on after Z's)
+        {
+          auto la = [&](int row, int m) {
+            row = min(row, nr * Ib - 1);
+            const int r = row / Ib, ii = row - r * Ib;
+            return Vb[(int64_t)(Q0 + r) * K + (I0 + ii) * k2 + min(m, k2 - 1)];
+          };
+          auto lb = [&](int m, int col) {
+            col = min(col, TB * nn - 1);
+            const int tb = col / nn, l = col - tb * nn;
+            return G2[(int64_t)tb * g2ts + min(m, k2 - 1) * k2 + n0 + l];
+          };
+          auto st = [&](int row, int col, cplx<T> v) {
+            const int r = row / Ib, ii = row - r * Ib, tb = col / nn, l = col - tb * nn;
+            bufE[(tb * Ib + ii) * lde + r * nn + l] = v;
+          };
+          if (TB * nn > 16) gemm_jobs_rs<T>(nr * Ib, TB * nn, k2, nj, la, lb, st);
+          else gemm_jobs_cs<T>(nr * Ib, TB * nn, k2, nj, la, lb, st);
+        }
+        __syncthreads();
+        // acc[i][j] += sum_y E_tb[i][y] conj(Z_tb[j][y]) over this wave's part of y
+        const int Y = nr * nn;
+        int yb, ye;
+        k_part(Y, S, kp, yb, ye);
+        mma_multi_all<true>(
+            acc, nmine, yb, ye,
+            [&](int c, int i, int y) {
+              const int u = tg + TG * c, tb = u / ntile, i0 = ((u - tb * ntile) / ntj) * 16;
+              return bufE[(tb * Ib + min(i0 + i, Ib - 1)) * lde + min(y, Y - 1)];
+            },
+            [&](int c, int y, int j) {
+              const int u = tg + TG * c, tb = u / ntile, j0 = ((u - tb * ntile) % ntj) * 16;
+              return bufZ[(tb * Jb + min(j0 + j, Jb - 1)) * ldz + min(y, Y - 1)];
+            });
       }
-      // E[(r,i)][(tb,n)] = sum_m V[Q0+r][I0+i][m] G2_tb[m][n]  (jobs dealt on after Z's)
-      {
-        auto la = [&](int row, int m) {
-          row = min(row, nr * Ib - 1);
-          const int r = row / Ib, ii = row - r * Ib;
-          return Vb[(int64_t)(Q0 + r) * K + (I0 + ii) * k2 + min(m, k2 - 1)];
-        };
-        auto lb = [&](int m, int col) {
-          col = min(col, TB * k2 - 1);
-          const int tb = col / k2, n = col - tb * k2;
-          return G2[(int64_t)tb * g2ts + min(m, k2 - 1) * k2 + n];
-        };
-        auto st = [&](int row, int col, cplx<T> v) {
-          const int r = row / Ib, ii = row - r * Ib, tb = col / k2, n = col - tb * k2;
-          bufE[(tb * Ib + ii) * lde + r * k2 + n] = v;
-        };
-        if (TB * k2 > 16) gemm_jobs_rs<T>(nr * Ib, TB * k2, k2, nj, la, lb, st);
-        else gemm_jobs_cs<T>(nr * Ib, TB * k2, k2, nj, la, lb, st);
-      }
-      __syncthreads();
-      // acc[i][j] += sum_y E_tb[i][y] conj(Z_tb[j][y]) over this wave's part of y
-      const int Y = nr * k2;
-      int yb, ye;
-      k_part(Y, S, kp, yb, ye);
-      mma_multi_all<true>(
-          acc, nmine, yb, ye,
-          [&](int c, int i, int y) {
-            const int u = tg + TG * c, tb = u / ntile, i0 = ((u - tb * ntile) / ntj) * 16;
-            return bufE[(tb * Ib + min(i0 + i, Ib - 1)) * lde + min(y, Y - 1)];
-          },
-          [&](int c, int y, int j) {
-            const int u = tg + TG * c, tb = u / ntile, j0 = ((u - tb * ntile) % ntj) * 16;
-            return bufZ[(tb * Jb + min(j0 + j, Jb - 1)) * ldz + min(y, Y - 1)];
-          });
     }
   }
   if (S > 1) split_reduce(acc, nmine, S, reinterpret_cast<T *>(lds));

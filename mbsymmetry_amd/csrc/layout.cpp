@@ -7,6 +7,7 @@
 #include "layout.h"
 
 #include <algorithm>
+#include <map>
 #include <sstream>
 
 namespace symgw {
@@ -246,6 +247,58 @@ int balanced(int n, int cap) {
 }
 }  // namespace
 
+int64_t pad_to(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
+
+// Chunking of one Sigma block contraction: qsub aux rows Q by nsub orbitals n
+// of k - q per LDS chunk.  Per time of the slab a chunk holds Z
+// ([band][qsub*nsub]), E ([band][qsub*nsub]) and conj(P) rows ([qsub][q]).
+// Among the shapes that fit, pick the one with the fewest executed MFMA
+// FLOPs (16x16x4 tiles, padding included) plus a fixed cost per chunk for
+// its barriers and staging.
+bool sig_chunks(int q, int k1, int k2, int band, int budget, std::map<std::array<int, 4>, std::array<int, 2>> &memo,
+                int &qsub, int &nsub) {
+  const std::array<int, 4> key{q, k1, k2, band};
+  auto f = memo.find(key);
+  if (f != memo.end()) {
+    qsub = f->second[0];
+    nsub = f->second[1];
+    return true;
+  }
+  const int TB = kSigTimes;
+  const double chunk_cost = 2.5e5;  // ~2000 cycles of a CU's FP64 MFMA rate
+  double best = -1;
+  for (int qs = 1; qs <= std::min(q, kSigRows); ++qs) {
+    const int rem = budget - TB * qs * q - 2 * TB * band;
+    if (rem <= 0) break;
+    const int ns_max = std::min(k2, rem / (2 * TB * band * qs));
+    if (ns_max < 1) continue;
+    const int qb = balanced(q, qs);
+    for (int ns = ns_max; ns >= 1 && ns >= ns_max / 2; --ns) {
+      const int nb = balanced(k2, ns);
+      double cost = 0;
+      for (int Q0 = 0; Q0 < q; Q0 += qb) {
+        const int nr = std::min(qb, q - Q0);
+        cost += chunk_cost;
+        for (int n0 = 0; n0 < k2; n0 += nb) {
+          const int nn = std::min(nb, k2 - n0);
+          cost += 8.0 * pad_to(TB * nr, 16) * pad_to((int64_t)band * nn, 16) * pad_to(q, 4);     // Z
+          cost += 8.0 * pad_to((int64_t)nr * band, 16) * pad_to(TB * nn, 16) * pad_to(k2, 4);  // E
+          cost += TB * 8.0 * pad_to(band, 16) * pad_to(band, 16) * pad_to(nr * nn, 4);          // final
+          cost += chunk_cost;
+        }
+      }
+      if (best < 0 || cost < best) {
+        best = cost;
+        qsub = qb;
+        nsub = nb;
+      }
+    }
+  }
+  if (best < 0) return false;
+  memo[key] = {qsub, nsub};
+  return true;
+}
+
 bool build_worklists(const Layout &L, int eb, WorkLists &W, std::string &err) {
   W = WorkLists();
   const int nk = L.nk, nts = L.nts, ns = L.ns;
@@ -302,6 +355,7 @@ bool build_worklists(const Layout &L, int eb, WorkLists &W, std::string &err) {
 
   // ---- Sigma: items per (k_ir, ik1, row band, column band); blocks in (q, V-block) order ----
   size_t sig_lds = 0;
+  std::map<std::array<int, 4>, std::array<int, 2>> sig_plans;
   for (int ikr = 0; ikr < L.ink; ++ikr) {
     int k = L.irre_list[ikr];
     for (size_t i1 = 0; i1 < L.ao_sizes[k].size(); ++i1) {
@@ -322,17 +376,13 @@ bool build_worklists(const Layout &L, int eb, WorkLists &W, std::string &err) {
           B.g2_tstride = (int32_t)(ns * L.kao_size[k2i]); B.g2_sstride = (int32_t)L.kao_size[k2i];
           B.p_base = L.qaux_off[q] * nts + L.aux_offsets[q][t[0]];
           B.p_tstride = (int32_t)L.qaux_size[q];
-          B.pad = 0;
-          // per time of the slab: Z ([band][qsub*k2]), E ([band][qsub*k2]) and
-          // conj(P) rows ([qsub][q]) of one chunk share the LDS budget (+1
-          // element of row padding each)
           const int TB = kSigTimes;
-          int per_row = TB * (B.q + 2 * band * B.k2);
-          int cap = std::min(kSigRows, (budget - 2 * TB * band) / per_row);
-          if (cap < 1) { err = fmt("Sigma block too large for the LDS plan (q, k2)", B.q, B.k2); return false; }
-          B.qsub = balanced(B.q, cap);
-          B.ldz = odd_ld(B.qsub * B.k2);
-          B.lde = odd_ld(B.qsub * B.k2);
+          if (!sig_chunks(B.q, k1, B.k2, band, budget, sig_plans, B.qsub, B.nsub)) {
+            err = fmt("Sigma block too large for the LDS plan (q, k2)", B.q, B.k2);
+            return false;
+          }
+          B.ldz = odd_ld(B.qsub * B.nsub);
+          B.lde = odd_ld(B.qsub * B.nsub);
           sig_lds = std::max(sig_lds, (size_t)TB * (band * (B.ldz + B.lde) + B.qsub * B.q) * eb);
           W.sig_blks.push_back(B);
         }

@@ -95,7 +95,7 @@ struct SigBlk {
   int32_t q, k2;
   int32_t qsub;        // Q rows per LDS chunk
   int32_t ldz, lde;    // LDS row strides of the Z and E chunks
-  int32_t pad;
+  int32_t nsub;        // orbitals n of k - q per LDS chunk
 };
 // Rows [I0, I0 + Ib) x columns [J0, J0 + Jb) of ao block ik1 of iBZ point
 // k_ir, summed over all q.
