@@ -392,8 +392,9 @@ __device__ __forceinline__ int gemm_jobs_cs(int M, int N, int K, int first, cons
 // q, time t < nts/2.  For every V block (k, ik1, ik2) and spin:
 //   E_Q = V_Q G^{k+q}(t),  C_Q = G^k(beta - t) E_Q,  P0[Q][P] += sum_x C_Q[x] conj(V_P[x])
 // i.e. the three GEMMs of cugw_qkpt_block.cu:231-260 (X1 = G V, X2 = conj(V) G^T,
-// P0^T += X2 X1) regrouped around one intermediate per Q row, processed in row
-// chunks of qsub rows sized for the LDS.
+// P0^T += X2 X1) regrouped around one intermediate per Q row.  All rows of the
+// slab go through each product together, in chunks of nsub orbitals n of k+q
+// sized for the LDS (E and C of a chunk stay in LDS; V is read from memory).
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ __launch_bounds__(Cfg<T>::NT) void p0_mfma_kernel(const P0Item *__restrict__ items, const P0Blk *__restrict__ blks,
@@ -418,61 +419,57 @@ __global__ __launch_bounds__(Cfg<T>::NT) void p0_mfma_kernel(const P0Item *__res
 
   for (int b = it.blk_begin; b < it.blk_end; ++b) {
     const P0Blk B = blks[b];
-    const int k1 = B.k1, k2 = B.k2, K = k1 * k2, ldk = B.ldk, qsub = B.qsub;
+    const int k1 = B.k1, k2 = B.k2, K = k1 * k2, nsub = B.nsub, lde = B.lde;
     const cplx<T> *Vb = V + B.v_off;
-    cplx<T> *bufV = lds;               // V rows, then C
-    cplx<T> *bufE = lds + qsub * ldk;  // E = V G2
-    int kb, ke;
-    k_part(K, S, kp, kb, ke);
+    const cplx<T> *Vr = Vb + (int64_t)it.Q0 * K;   // V rows of the band
+    cplx<T> *bufE = lds;                 // [nband][lde]  E[r][a][l] at a*nn + l
+    cplx<T> *bufC = lds + nband * lde;   // [nband][lde]  C[r][p][l] at p*nn + l
     for (int s = 0; s < ns; ++s) {
       const cplx<T> *G1 = G + B.g1_base + (int64_t)(nts - 1 - t) * B.g1_tstride + (int64_t)s * B.g1_sstride;
       const cplx<T> *G2 = G + B.g2_base + (int64_t)t * B.g2_tstride + (int64_t)s * B.g2_sstride;
-      for (int r0 = 0; r0 < nband; r0 += qsub) {
-        const int nr = min(qsub, nband - r0);
-        const cplx<T> *Vr = Vb + (int64_t)(it.Q0 + r0) * K;
+      for (int n0 = 0; n0 < k2; n0 += nsub) {
+        const int nn = min(nsub, k2 - n0), X = k1 * nn;
         __syncthreads();
-        stage_rows<T, 8>(bufV, ldk, Vr, nr * K, K, [](cplx<T> v) { return v; });
-        __syncthreads();
-        // E[(r,a)][n] = sum_m V[r][a][m] G2[m][n]
+        // E[(r,a)][l] = sum_m V[Q0+r][a][m] G2[m][n0+l]
         {
           auto la = [&](int row, int m) {
-            row = min(row, nr * k1 - 1);
-            const int r = row / k1, a = row - r * k1;
-            return bufV[r * ldk + a * k2 + min(m, k2 - 1)];
+            row = min(row, nband * k1 - 1);
+            return Vr[(int64_t)row * k2 + min(m, k2 - 1)];
           };
-          auto lb = [&](int m, int n) { return G2[min(m, k2 - 1) * k2 + min(n, k2 - 1)]; };
-          auto st = [&](int row, int n, cplx<T> v) {
+          auto lb = [&](int m, int l) { return G2[min(m, k2 - 1) * k2 + n0 + min(l, nn - 1)]; };
+          auto st = [&](int row, int l, cplx<T> v) {
             const int r = row / k1, a = row - r * k1;
-            bufE[r * ldk + a * k2 + n] = v;
+            bufE[r * lde + a * nn + l] = v;
           };
-          if (k2 > 16) gemm_jobs_rs<T>(nr * k1, k2, k2, 0, la, lb, st);
-          else gemm_jobs_cs<T>(nr * k1, k2, k2, 0, la, lb, st);
+          if (nn > 16) gemm_jobs_rs<T>(nband * k1, nn, k2, 0, la, lb, st);
+          else gemm_jobs_cs<T>(nband * k1, nn, k2, 0, la, lb, st);
         }
         __syncthreads();
-        // C[r][p][n] = sum_a G1[p][a] E[r][a][n]  (into bufV)
+        // C[r][p][l] = sum_a G1[p][a] E[r][a][l]
         gemm_jobs_rs<T>(
-            k1, nr * k2, k1, 0, [&](int p, int a) { return G1[min(p, k1 - 1) * k1 + min(a, k1 - 1)]; },
+            k1, nband * nn, k1, 0, [&](int p, int a) { return G1[min(p, k1 - 1) * k1 + min(a, k1 - 1)]; },
             [&](int a, int col) {
-              col = min(col, nr * k2 - 1);
-              const int r = col / k2, n = col - r * k2;
-              return bufE[r * ldk + min(a, k1 - 1) * k2 + n];
+              col = min(col, nband * nn - 1);
+              const int r = col / nn, l = col - r * nn;
+              return bufE[r * lde + min(a, k1 - 1) * nn + l];
             },
             [&](int p, int col, cplx<T> v) {
-              const int r = col / k2, n = col - r * k2;
-              bufV[r * ldk + p * k2 + n] = v;
+              const int r = col / nn, l = col - r * nn;
+              bufC[r * lde + p * nn + l] = v;
             });
         __syncthreads();
-        // acc[r0 + r][P] += sum_x C[r][x] conj(V[P][x]) over this wave's part of x
+        // acc[r][P] += sum_{(p,l)} C[r][p][l] conj(V[P][p][n0+l]) over this wave's part of (p, l)
         {
-          // rows of the band outside this chunk contribute nothing
-          const int rr = (threadIdx.x & 15) - r0;
-          const bool arow = rr >= 0 && rr < nr;
-          const int rc = min(max(rr, 0), nr - 1);
+          int kb, ke;
+          k_part(X, S, kp, kb, ke);
+          const int rc = min(threadIdx.x & 15, nband - 1);   // rows >= nband are never stored
           mma_rowshare_all<true>(
-              acc, nmine, kb, ke, arow, [&](int i, int x) { return bufV[rc * ldk + min(x, K - 1)]; },
+              acc, nmine, kb, ke, true, [&](int i, int x) { return bufC[rc * lde + min(x, X - 1)]; },
               [&](int c, int x, int j) {
+                x = min(x, X - 1);
+                const int p = x / nn, l = x - p * nn;
                 const int P = min((tg + TG * c) * 16 + j, q - 1);
-                return Vb[(int64_t)P * K + min(x, K - 1)];
+                return Vb[(int64_t)P * K + p * k2 + n0 + l];
               });
         }
       }

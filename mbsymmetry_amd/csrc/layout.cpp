@@ -305,7 +305,7 @@ bool build_worklists(const Layout &L, int eb, WorkLists &W, std::string &err) {
   const int budget = contract_lds(eb) / eb;  // complex elements of LDS per workgroup
   flop_model(L, W.flop_p0, W.flop_sig, W.n_p0_contractions, W.n_sig_contractions);
 
-  // ---- P0: items per (q_ir, iq, 16-row band); blocks in (k, V-block) order ----
+  // ---- P0: items per (q_ir, iq, band of <= 16 rows); blocks in (k, V-block) order ----
   size_t p0_lds = 0;
   for (int iqr = 0; iqr < L.ink; ++iqr) {
     int q = L.irre_list[iqr];
@@ -327,20 +327,23 @@ bool build_worklists(const Layout &L, int eb, WorkLists &W, std::string &err) {
           B.g2_base = L.kao_off[k2] * ns * nts + L.ao_offsets[k2][t[2]];
           B.g1_tstride = (int32_t)(ns * L.kao_size[k]); B.g1_sstride = (int32_t)L.kao_size[k];
           B.g2_tstride = (int32_t)(ns * L.kao_size[k2]); B.g2_sstride = (int32_t)L.kao_size[k2];
-          B.ldk = odd_ld(B.k1 * B.k2);
-          // V rows and E rows of one chunk share the LDS budget
-          int cap = std::min(kP0Band, budget / (2 * B.ldk));
+          // E and C of one chunk ([band][k1*nsub] each, +1 element of row
+          // padding) share the LDS budget
+          const int band = balanced(qs, kP0Band);
+          const int cap = std::min(B.k2, (budget / (2 * band) - 1) / B.k1);
           if (cap < 1) { err = fmt("P0 block too large for the LDS plan (k1, k2)", B.k1, B.k2); return false; }
-          B.qsub = balanced(std::min(kP0Band, qs), cap);
-          p0_lds = std::max(p0_lds, (size_t)(2 * B.qsub * B.ldk) * eb);
+          B.nsub = balanced(B.k2, cap);
+          B.lde = odd_ld(B.k1 * B.nsub);
+          p0_lds = std::max(p0_lds, (size_t)(2 * band * B.lde) * eb);
           W.p0_blks.push_back(B);
         }
       }
       int b1 = (int)W.p0_blks.size();
-      for (int Q0 = 0; Q0 < qs; Q0 += kP0Band) {
+      const int band = balanced(qs, kP0Band);
+      for (int Q0 = 0; Q0 < qs; Q0 += band) {
         P0Item it;
         it.blk_begin = b0; it.blk_end = b1;
-        it.q = qs; it.Q0 = Q0; it.nband = std::min(kP0Band, qs - Q0); it.pad = 0;
+        it.q = qs; it.Q0 = Q0; it.nband = std::min(band, qs - Q0); it.pad = 0;
         it.out_base = (int64_t)nts * L.qaux_off_irre[iqr] + L.aux_offsets[q][iq] + (int64_t)Q0 * qs;
         it.out_tstride = (int32_t)L.qaux_size_irre[iqr];
         it.pad2 = 0;

@@ -52,7 +52,8 @@ constexpr int contract_lds(int elem_bytes) { return elem_bytes > 0 ? 160 * 1024 
 constexpr int contract_waves(int elem_bytes) { return elem_bytes > 0 ? 12 : 0; }
 // Accumulator tiles each contraction wave keeps across a slab.
 constexpr int kContractTiles = 2;
-// Output rows of one P0 slab (one MFMA tile row).
+// Most output rows of one P0 slab (one MFMA tile row); the rows of an aux
+// block are split into balanced bands.
 constexpr int kP0Band = 16;
 // Largest Sigma slab edge (4 x 4 tiles of 16).
 constexpr int kSigBand = 64;
@@ -72,8 +73,8 @@ struct P0Blk {
   int32_t g2_tstride;
   int32_t g2_sstride;
   int32_t k1, k2;
-  int32_t qsub;        // Q rows per LDS chunk
-  int32_t ldk;         // LDS row stride (k1*k2, odd)
+  int32_t nsub;        // orbitals n of k + q per LDS chunk
+  int32_t lde;         // LDS row stride of E and C (k1*nsub, odd)
 };
 // Rows [Q0, Q0 + nband) of aux block iq of iBZ point q_ir, summed over all k.
 struct P0Item {
