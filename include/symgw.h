@@ -209,8 +209,9 @@ int symgw_dyson(symgw_ctx *ctx, double mu, double sigma_scale);
  * the device G (sc_loop_gf_block.cpp:18-20 density matrix, traced). */
 int symgw_density(symgw_ctx *ctx, const double *kweights, double *nel);
 /* Chemical-potential search (absent from the reference, whose mu is fixed):
- * bisection on mu in [mu_lo, mu_hi] of N(mu) = nel_target through repeated
- * device Dyson solves; leaves G of the final mu on the device. */
+ * bisection of N(mu) = nel_target through repeated device Dyson solves,
+ * starting from [mu_lo, mu_hi] and widening it until it brackets the target
+ * (SYMGW_ERR_ARG if it never does); leaves G of the final mu on the device. */
 int symgw_mu_search(symgw_ctx *ctx, double nel_target, const double *kweights,
                     double sigma_scale, double mu_lo, double mu_hi, double tol,
                     double *mu_out, double *nel_out);

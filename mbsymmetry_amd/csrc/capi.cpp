@@ -883,8 +883,21 @@ int symgw_density(symgw_ctx *ctx, const double *kweights, double *nel) {
 int symgw_mu_search(symgw_ctx *ctx, double nel_target, const double *kweights, double sigma_scale, double mu_lo,
                     double mu_hi, double tol, double *mu_out, double *nel_out) {
   if (!ctx || !kweights || !mu_out || mu_hi <= mu_lo || tol <= 0) return SYMGW_ERR_ARG;
-  double lo = mu_lo, hi = mu_hi, mu = 0.5 * (lo + hi), n = 0;
+  double lo = mu_lo, hi = mu_hi, mu = 0.5 * (lo + hi), n = 0, nlo = 0, nhi = 0;
   int r;
+  auto count = [&](double m, double &nn) {
+    int rr = symgw_dyson(ctx, m, sigma_scale);
+    return rr ? rr : symgw_density(ctx, kweights, &nn);
+  };
+  // widen [lo, hi] (doubling its width on the short side) until it brackets the target
+  if ((r = count(lo, nlo)) || (r = count(hi, nhi))) return r;
+  for (int ex = 0; ex < 8 && (nlo > nel_target || nhi < nel_target); ++ex) {
+    const double w = hi - lo;
+    if (nlo > nel_target && (r = count(lo -= w, nlo))) return r;
+    if (nhi < nel_target && (r = count(hi += w, nhi))) return r;
+  }
+  if (nlo > nel_target || nhi < nel_target)
+    return fail(ctx, SYMGW_ERR_ARG, "mu_search: the electron count is not bracketed");
   for (int it = 0; it < 200; ++it) {
     mu = 0.5 * (lo + hi);
     if ((r = symgw_dyson(ctx, mu, sigma_scale))) return r;
