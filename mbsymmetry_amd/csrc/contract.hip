@@ -50,7 +50,7 @@ struct Tile {
 // Tiles a wave advances together, sharing one operand fragment per k-step:
 // 2 for FP64 and 4 for FP32 (two waves per SIMD within the register file).
 template <typename T>
-struct Group { static constexpr int n = sizeof(T) == 8 ? 2 : 4; };
+struct Group { static constexpr int n = sizeof(T) == 8 ? 1 : 4; };
 template <typename T>
 using Tiles = Tile<T>[4];
 template <typename T>
@@ -103,6 +103,10 @@ __device__ __forceinline__ cplx<T> amask(const cplx<T> &a, bool ok) {
   return cplx<T>{ok ? a.re : T(0), ok ? a.im : T(0)};
 }
 
+// Operand k-steps kept in flight ahead of the MFMAs (register sets used in a
+// ring of kAhead + 1).
+constexpr int kAhead = 2;
+
 // acc[C0+c] += A (16 x K) * B_c (K x 16) for c < NG with C0 + c < n, over k in
 // [kb, ke): one A fragment per k-step shared by the tiles.  la(i, k), lb(c, k, j)
 // with c the absolute tile index.  n must be uniform across the wave.
@@ -110,21 +114,26 @@ template <int C0, int NG, bool CB, typename T, int N, typename LA, typename LB>
 __device__ __forceinline__ void mma_rowshare(Tile<T> (&acc)[N], int n, int kb, int ke, bool arow, const LA &la,
                                              const LB &lb) {
   static_assert(C0 + NG <= N, "tile group outside the accumulator array");
+  constexpr int R = kAhead + 1;
   if (kb >= ke || C0 >= n) return;
   const int lane = threadIdx.x & 63, i = lane & 15, kk = lane >> 4;
-  cplx<T> a[2], b[2][NG];
-  a[0] = la(i, kb + kk);
+  cplx<T> a[R], b[R][NG];
 #pragma unroll
-  for (int c = 0; c < NG; ++c)
-    if (C0 + c < n) b[0][c] = lb(C0 + c, kb + kk, i);
+  for (int d = 0; d < kAhead; ++d) {
+    a[d] = la(i, kb + 4 * d + kk);
+#pragma unroll
+    for (int c = 0; c < NG; ++c)
+      if (C0 + c < n) b[d][c] = lb(C0 + c, kb + 4 * d + kk, i);
+  }
   int k0 = kb;
   while (true) {
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      a[h ^ 1] = la(i, k0 + 4 + kk);
+    for (int h = 0; h < R; ++h) {
+      const int hl = (h + kAhead) % R;
+      a[hl] = la(i, k0 + 4 * kAhead + kk);
 #pragma unroll
       for (int c = 0; c < NG; ++c)
-        if (C0 + c < n) b[h ^ 1][c] = lb(C0 + c, k0 + 4 + kk, i);
+        if (C0 + c < n) b[hl][c] = lb(C0 + c, k0 + 4 * kAhead + kk, i);
       prefetch_fence();
       const cplx<T> am = amask(a[h], arow && k0 + kk < ke);
 #pragma unroll
@@ -140,21 +149,26 @@ __device__ __forceinline__ void mma_rowshare(Tile<T> (&acc)[N], int n, int kb, i
 template <int C0, int NG, bool CB, typename T, int N, typename LA, typename LB>
 __device__ __forceinline__ void mma_colshare(Tile<T> (&acc)[N], int n, int kb, int ke, const LA &la, const LB &lb) {
   static_assert(C0 + NG <= N, "tile group outside the accumulator array");
+  constexpr int R = kAhead + 1;
   if (kb >= ke || C0 >= n) return;
   const int lane = threadIdx.x & 63, i = lane & 15, kk = lane >> 4;
-  cplx<T> a[2][NG], b[2];
-  b[0] = lb(kb + kk, i);
+  cplx<T> a[R][NG], b[R];
 #pragma unroll
-  for (int c = 0; c < NG; ++c)
-    if (C0 + c < n) a[0][c] = la(C0 + c, i, kb + kk);
+  for (int d = 0; d < kAhead; ++d) {
+    b[d] = lb(kb + 4 * d + kk, i);
+#pragma unroll
+    for (int c = 0; c < NG; ++c)
+      if (C0 + c < n) a[d][c] = la(C0 + c, i, kb + 4 * d + kk);
+  }
   int k0 = kb;
   while (true) {
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      b[h ^ 1] = lb(k0 + 4 + kk, i);
+    for (int h = 0; h < R; ++h) {
+      const int hl = (h + kAhead) % R;
+      b[hl] = lb(k0 + 4 * kAhead + kk, i);
 #pragma unroll
       for (int c = 0; c < NG; ++c)
-        if (C0 + c < n) a[h ^ 1][c] = la(C0 + c, i, k0 + 4 + kk);
+        if (C0 + c < n) a[hl][c] = la(C0 + c, i, k0 + 4 * kAhead + kk);
       prefetch_fence();
       const bool kin = k0 + kk < ke;
 #pragma unroll
@@ -170,24 +184,28 @@ __device__ __forceinline__ void mma_colshare(Tile<T> (&acc)[N], int n, int kb, i
 template <int C0, int NG, bool CB, typename T, int N, typename LA, typename LB>
 __device__ __forceinline__ void mma_multi(Tile<T> (&acc)[N], int n, int kb, int ke, const LA &la, const LB &lb) {
   static_assert(C0 + NG <= N, "tile group outside the accumulator array");
+  constexpr int R = kAhead + 1;
   if (kb >= ke || C0 >= n) return;
   const int lane = threadIdx.x & 63, i = lane & 15, kk = lane >> 4;
-  cplx<T> a[2][NG], b[2][NG];
+  cplx<T> a[R][NG], b[R][NG];
 #pragma unroll
-  for (int c = 0; c < NG; ++c)
-    if (C0 + c < n) {
-      a[0][c] = la(C0 + c, i, kb + kk);
-      b[0][c] = lb(C0 + c, kb + kk, i);
-    }
+  for (int d = 0; d < kAhead; ++d)
+#pragma unroll
+    for (int c = 0; c < NG; ++c)
+      if (C0 + c < n) {
+        a[d][c] = la(C0 + c, i, kb + 4 * d + kk);
+        b[d][c] = lb(C0 + c, kb + 4 * d + kk, i);
+      }
   int k0 = kb;
   while (true) {
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
+    for (int h = 0; h < R; ++h) {
+      const int hl = (h + kAhead) % R;
 #pragma unroll
       for (int c = 0; c < NG; ++c)
         if (C0 + c < n) {
-          a[h ^ 1][c] = la(C0 + c, i, k0 + 4 + kk);
-          b[h ^ 1][c] = lb(C0 + c, k0 + 4 + kk, i);
+          a[hl][c] = la(C0 + c, i, k0 + 4 * kAhead + kk);
+          b[hl][c] = lb(C0 + c, k0 + 4 * kAhead + kk, i);
         }
       prefetch_fence();
       const bool kin = k0 + kk < ke;
