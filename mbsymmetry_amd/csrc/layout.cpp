@@ -312,6 +312,10 @@ bool build_worklists(const Layout &L, int eb, WorkLists &W, std::string &err) {
     for (size_t iq = 0; iq < L.aux_sizes[q].size(); ++iq) {
       int qs = L.aux_sizes[q][iq];
       if (qs > 256) { err = fmt("aux block larger than 256", qs); return false; }
+      if ((qs + 15) / 16 > contract_waves(eb) * kContractTiles) {
+        err = fmt("P0 slab has more output tiles than the workgroup holds", qs);
+        return false;
+      }
       int b0 = (int)W.p0_blks.size();
       for (int k = 0; k < nk; ++k) {
         int k2 = L.kpq[(size_t)k * nk + q];
@@ -391,6 +395,13 @@ bool build_worklists(const Layout &L, int eb, WorkLists &W, std::string &err) {
         }
       }
       int b1 = (int)W.sig_blks.size();
+      {
+        const int nt = (band + 15) / 16;
+        if (kSigTimes * nt * nt > contract_waves(eb) * kContractTiles) {
+          err = fmt("Sigma slab has more output tiles than the workgroup holds", band);
+          return false;
+        }
+      }
       for (int I0 = 0; I0 < k1; I0 += band)
         for (int J0 = 0; J0 < k1; J0 += band) {
           SigItem it;

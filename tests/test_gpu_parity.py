@@ -268,3 +268,37 @@ def test_self_consistent_loop_holds_the_electron_count():
             hi = mid
     assert abs(res.history[0]["mu"] - 0.5 * (lo + hi)) < 1e-7
     ctx.close()
+
+
+def test_large_blocks_parity():
+    """Blocks that exercise every chunking path of the contraction kernels:
+    an ao block of 66 (> 64: two output bands in Sigma, orbital chunks in P0)
+    and an aux block of 100 (row bands in P0, aux-row chunks in Sigma), next
+    to small blocks.  P0, W, Sigma and the new G against the oracle."""
+    import os
+    from mbsymmetry_amd.transforms import chebyshev
+    sysm = S.synthetic("big_blocks", 2, 4, [66, 10], [100, 24], 1, chebyshev(4, 6.0), 31, v_scale=0.1)
+    _say("large blocks", "ao [66, 10] aux [100, 24]")
+    nth = min(16, len(os.sched_getaffinity(0)))
+    orc, ctx, G0 = _setup(sysm, mu=0.0)
+    V = sysm.host_V()
+    Gf = orc.rotate_G(G0)
+    P0 = orc.P0(V, Gf, nthreads=nth)
+    ctx.solve_P0()
+    got = ctx.copy_P0_to_host(sysm.zeros_q_t_QP()) / sysm.nk
+    h = sysm.nts // 2
+    qoff = sysm.qaux_off_irre()
+    for iqr, q in enumerate(sysm.irre_list):
+        qs = sysm.qslice(int(q))
+        base = sysm.nts * qoff[iqr]
+        a, b = got[base:base + h * qs], P0[base:base + h * qs]
+        assert np.abs(a - b).max() < 1e-10 * max(1.0, np.abs(b).max()), iqr
+    Pref = orc.W(P0)
+    ctx.solve_W()
+    assert _rel(ctx.copy_P0_to_host(sysm.zeros_q_t_QP()), Pref) < 1e-9
+    ctx.upload_P(Pref)
+    ctx.solve_Sigma()
+    Sg = ctx.copy_Sigma_to_host(sysm.zeros_ts_kij()) / sysm.nk
+    Sref = orc.Sigma(V, Gf, orc.rotate_P(Pref), nthreads=nth)
+    assert _rel(Sg, Sref) < 1e-10
+    ctx.close()
