@@ -178,6 +178,14 @@ def test_fp32_mode_error_is_bounded(systems):
         mask[sysm.nts * qoff[iqr]:sysm.nts * qoff[iqr] + h * qs] = True
     err = np.abs(got[mask] - ref[mask]).max() / np.abs(ref[mask]).max()
     assert err < 1e-5, err
+    # Sigma from the oracle's P in FP32 arithmetic
+    Pref = orc.W(ref)
+    ctx.upload_P(Pref)
+    ctx.solve_Sigma()
+    Sg = ctx.copy_Sigma_to_host(sysm.zeros_ts_kij()) / sysm.nk
+    Sref = orc.Sigma(sysm.host_V(), orc.rotate_G(G0), orc.rotate_P(Pref))
+    err = np.abs(Sg - Sref).max() / np.abs(Sref).max()
+    assert err < 1e-5, err
     ctx.close()
 
 
