@@ -407,108 +407,155 @@ __device__ __forceinline__ int gemm_jobs_cs(int M, int N, int K, int first, cons
 
 // ---------------------------------------------------------------------------
 // P0.  Slab = rows [Q0, Q0 + nband) (nband <= 16) of aux block iq of iBZ point
-// q, time t < nts/2.  For every V block (k, ik1, ik2) and spin:
+// q, for kP0Times consecutive times t < nts/2.  For every V block (k, ik1, ik2)
+// and spin:
 //   E_Q = V_Q G^{k+q}(t),  C_Q = G^k(beta - t) E_Q,  P0[Q][P] += sum_x C_Q[x] conj(V_P[x])
 // i.e. the three GEMMs of cugw_qkpt_block.cu:231-260 (X1 = G V, X2 = conj(V) G^T,
 // P0^T += X2 X1) regrouped around one intermediate per Q row.  All rows of the
 // slab go through each product together, in chunks of nsub orbitals n of k+q
 // sized for the LDS (E and C of a chunk stay in LDS; V is read from memory).
+// When every wave can hold one column tile (q <= 16 * waves) the times share
+// every V fragment: their G^{k+q} blocks are stacked as columns of one E
+// product and their C rows as the row tiles of the rank-K update; otherwise
+// the times are processed one after the other.
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ __launch_bounds__(Cfg<T>::NT) void p0_mfma_kernel(const P0Item *__restrict__ items, const P0Blk *__restrict__ blks,
                                                     int64_t n_work, int nt_half, int nts, int ns,
                                                     const cplx<T> *__restrict__ V, const cplx<T> *__restrict__ G,
                                                     cplx<double> *__restrict__ P0, double pref) {
+  constexpr int TB = kP0Times;
   const int64_t w = xcd_logical(blockIdx.x, gridDim.x);
   if (w >= n_work) return;
-  const int t = (int)(w % nt_half);
-  const P0Item it = items[w / nt_half];
+  const int np = (nt_half + TB - 1) / TB;
+  const int t0 = (int)(w % np) * TB, ntimes = min(TB, nt_half - t0);
+  const P0Item it = items[w / np];
   constexpr int NW = Cfg<T>::NW, NT = Cfg<T>::NT, MT = Cfg<T>::MT;
+  static_assert(TB <= MT, "the stacked times need one accumulator tile each");
   const int tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cplx<T> *lds = reinterpret_cast<cplx<T> *>(smem_raw);
   const int q = it.q, nband = it.nband, nct = (q + 15) / 16;
-  // the nct column tiles of the slab: S waves share a tile over S parts of x
-  const int S = choose_split<NW, MT>(nct), TG = NW / S, kp = wave % S, tg = wave / S;
-  const int nmine = nct > tg ? (nct - tg + TG - 1) / TG : 0;  // tiles tg, tg + TG, ...
-  Acc<T> acc;
-#pragma unroll
-  for (int c = 0; c < MT; ++c) tile_zero(acc[c]);
+  const bool stack = nct <= NW;
+  const int npass = stack ? 1 : ntimes, tbn = stack ? ntimes : 1;
+  // column tiles: S waves share a tile over S parts of x.  Stacked: one column
+  // tile per wave (tile tg) with one accumulator per time; otherwise up to MT
+  // column tiles per wave (tg, tg + TG, ...) for one time.
+  const int S = stack ? choose_split<NW, 1>(nct) : choose_split<NW, MT>(nct);
+  const int TG = NW / S, kp = wave % S, tg = wave / S;
+  const int nmine = stack ? (tg < nct ? tbn : 0) : (nct > tg ? (nct - tg + TG - 1) / TG : 0);
 
-  for (int b = it.blk_begin; b < it.blk_end; ++b) {
-    const P0Blk B = blks[b];
-    const int k1 = B.k1, k2 = B.k2, K = k1 * k2, nsub = B.nsub, lde = B.lde;
-    const cplx<T> *Vb = V + B.v_off;
-    const cplx<T> *Vr = Vb + (int64_t)it.Q0 * K;   // V rows of the band
-    cplx<T> *bufE = lds;                 // [nband][lde]  E[r][a][l] at a*nn + l
-    cplx<T> *bufC = lds + nband * lde;   // [nband][lde]  C[r][p][l] at p*nn + l
-    for (int s = 0; s < ns; ++s) {
-      const cplx<T> *G1 = G + B.g1_base + (int64_t)(nts - 1 - t) * B.g1_tstride + (int64_t)s * B.g1_sstride;
-      const cplx<T> *G2 = G + B.g2_base + (int64_t)t * B.g2_tstride + (int64_t)s * B.g2_sstride;
-      for (int n0 = 0; n0 < k2; n0 += nsub) {
-        const int nn = min(nsub, k2 - n0), X = k1 * nn;
-        __syncthreads();
-        // E[(r,a)][l] = sum_m V[Q0+r][a][m] G2[m][n0+l]
-        {
-          auto la = [&](int row, int m) {
-            row = min(row, nband * k1 - 1);
-            return Vr[(int64_t)row * k2 + min(m, k2 - 1)];
-          };
-          auto lb = [&](int m, int l) { return G2[min(m, k2 - 1) * k2 + n0 + min(l, nn - 1)]; };
-          auto st = [&](int row, int l, cplx<T> v) {
-            const int r = row / k1, a = row - r * k1;
-            bufE[r * lde + a * nn + l] = v;
-          };
-          if (nn > 16) gemm_jobs_rs<T>(nband * k1, nn, k2, 0, la, lb, st);
-          else gemm_jobs_cs<T>(nband * k1, nn, k2, 0, la, lb, st);
-        }
-        __syncthreads();
-        // C[r][p][l] = sum_a G1[p][a] E[r][a][l]
-        gemm_jobs_rs<T>(
-            k1, nband * nn, k1, 0, [&](int p, int a) { return G1[min(p, k1 - 1) * k1 + min(a, k1 - 1)]; },
-            [&](int a, int col) {
-              col = min(col, nband * nn - 1);
-              const int r = col / nn, l = col - r * nn;
-              return bufE[r * lde + min(a, k1 - 1) * nn + l];
-            },
-            [&](int p, int col, cplx<T> v) {
-              const int r = col / nn, l = col - r * nn;
-              bufC[r * lde + p * nn + l] = v;
-            });
-        __syncthreads();
-        // acc[r][P] += sum_{(p,l)} C[r][p][l] conj(V[P][p][n0+l]) over this wave's part of (p, l)
-        {
-          int kb, ke;
-          k_part(X, S, kp, kb, ke);
-          const int rc = min(threadIdx.x & 15, nband - 1);   // rows >= nband are never stored
-          mma_rowshare_all<true>(
-              acc, nmine, kb, ke, true, [&](int i, int x) { return bufC[rc * lde + min(x, X - 1)]; },
-              [&](int c, int x, int j) {
-                x = min(x, X - 1);
-                const int p = x / nn, l = x - p * nn;
-                const int P = min((tg + TG * c) * 16 + j, q - 1);
-                return Vb[(int64_t)P * K + p * k2 + n0 + l];
-              });
+  for (int pass = 0; pass < npass; ++pass) {
+    const int ta = t0 + pass;  // first time of this pass
+    Acc<T> acc;
+#pragma unroll
+    for (int c = 0; c < MT; ++c) tile_zero(acc[c]);
+    for (int b = it.blk_begin; b < it.blk_end; ++b) {
+      const P0Blk B = blks[b];
+      const int k1 = B.k1, k2 = B.k2, K = k1 * k2, nsub = B.nsub, lde = B.lde;
+      const cplx<T> *Vb = V + B.v_off;
+      const cplx<T> *Vr = Vb + (int64_t)it.Q0 * K;        // V rows of the band
+      cplx<T> *bufE = lds;                                // [tbn][nband][lde]  E[r][a][l] at a*nn + l
+      cplx<T> *bufC = lds + TB * nband * lde;             // [tbn][nband][lde]  C[r][p][l] at p*nn + l
+      const int g1ts = B.g1_tstride, g2ts = B.g2_tstride;
+      for (int s = 0; s < ns; ++s) {
+        // time ta + tb: G1 at nts - 1 - (ta + tb), G2 at ta + tb
+        const cplx<T> *G1 = G + B.g1_base + (int64_t)(nts - 1 - ta) * g1ts + (int64_t)s * B.g1_sstride;
+        const cplx<T> *G2 = G + B.g2_base + (int64_t)ta * g2ts + (int64_t)s * B.g2_sstride;
+        for (int n0 = 0; n0 < k2; n0 += nsub) {
+          const int nn = min(nsub, k2 - n0), X = k1 * nn;
+          __syncthreads();
+          // E_tb[(r,a)][l] = sum_m V[Q0+r][a][m] G2_tb[m][n0+l], columns (tb, l)
+          {
+            auto la = [&](int row, int m) {
+              row = min(row, nband * k1 - 1);
+              return Vr[(int64_t)row * k2 + min(m, k2 - 1)];
+            };
+            auto lb = [&](int m, int col) {
+              col = min(col, tbn * nn - 1);
+              const int tb = col / nn, l = col - tb * nn;
+              return G2[(int64_t)tb * g2ts + min(m, k2 - 1) * k2 + n0 + l];
+            };
+            auto st = [&](int row, int col, cplx<T> v) {
+              const int r = row / k1, a = row - r * k1, tb = col / nn, l = col - tb * nn;
+              bufE[(tb * nband + r) * lde + a * nn + l] = v;
+            };
+            if (tbn * nn > 16) gemm_jobs_rs<T>(nband * k1, tbn * nn, k2, 0, la, lb, st);
+            else gemm_jobs_cs<T>(nband * k1, tbn * nn, k2, 0, la, lb, st);
+          }
+          __syncthreads();
+          // C_tb[r][p][l] = sum_a G1_tb[p][a] E_tb[r][a][l]  (the times' jobs dealt in turn)
+          {
+            int first = 0;
+            for (int tb = 0; tb < tbn; ++tb) {
+              const cplx<T> *G1t = G1 - (int64_t)tb * g1ts;
+              const cplx<T> *Et = bufE + tb * nband * lde;
+              cplx<T> *Ct = bufC + tb * nband * lde;
+              first += gemm_jobs_rs<T>(
+                  k1, nband * nn, k1, first,
+                  [&](int p, int a) { return G1t[min(p, k1 - 1) * k1 + min(a, k1 - 1)]; },
+                  [&](int a, int col) {
+                    col = min(col, nband * nn - 1);
+                    const int r = col / nn, l = col - r * nn;
+                    return Et[r * lde + min(a, k1 - 1) * nn + l];
+                  },
+                  [&](int p, int col, cplx<T> v) {
+                    const int r = col / nn, l = col - r * nn;
+                    Ct[r * lde + p * nn + l] = v;
+                  });
+            }
+          }
+          __syncthreads();
+          // acc[r][P] += sum_{(p,l)} C[r][p][l] conj(V[P][p][n0+l]) over this wave's part of (p, l)
+          {
+            int kb, ke;
+            k_part(X, S, kp, kb, ke);
+            const int rc = min(threadIdx.x & 15, nband - 1);   // rows >= nband are never stored
+            if (stack) {
+              // one V fragment per k-step for the tbn time tiles of column tile tg
+              if (nmine > 0)
+                mma_colshare<0, TB, true>(
+                    acc, tbn, kb, ke, [&](int c, int i, int x) { return bufC[(c * nband + rc) * lde + min(x, X - 1)]; },
+                    [&](int x, int j) {
+                      x = min(x, X - 1);
+                      const int p = x / nn, l = x - p * nn;
+                      return Vb[(int64_t)min(tg * 16 + j, q - 1) * K + p * k2 + n0 + l];
+                    });
+            } else {
+              mma_rowshare_all<true>(
+                  acc, nmine, kb, ke, true, [&](int i, int x) { return bufC[rc * lde + min(x, X - 1)]; },
+                  [&](int c, int x, int j) {
+                    x = min(x, X - 1);
+                    const int p = x / nn, l = x - p * nn;
+                    const int P = min((tg + TG * c) * 16 + j, q - 1);
+                    return Vb[(int64_t)P * K + p * k2 + n0 + l];
+                  });
+            }
+          }
         }
       }
     }
-  }
-  cplx<double> *out = P0 + it.out_base + (int64_t)t * it.out_tstride;
-  if (S > 1) split_reduce(acc, nmine, S, reinterpret_cast<T *>(lds));
-  if (kp == 0) {
+    if (S > 1) split_reduce(acc, nmine, S, reinterpret_cast<T *>(lds));
+    if (kp == 0) {
 #pragma unroll
-    for (int c = 0; c < MT; ++c) {
-      const int jt = tg + TG * c;
-      if (c < nmine)
-        tile_store(acc[c], [&](int i, int j, cplx<T> v) {
-          const int P = jt * 16 + j;
-          if (i < nband && P < q) out[i * q + P] = cplx<double>{pref * (double)v.re, pref * (double)v.im};
-        });
+      for (int c = 0; c < MT; ++c) {
+        if (c < nmine) {
+          const int t = stack ? ta + c : ta, jt = stack ? tg : tg + TG * c;
+          cplx<double> *out = P0 + it.out_base + (int64_t)t * it.out_tstride;
+          tile_store(acc[c], [&](int i, int j, cplx<T> v) {
+            const int P = jt * 16 + j;
+            if (i < nband && P < q) out[i * q + P] = cplx<double>{pref * (double)v.re, pref * (double)v.im};
+          });
+        }
+      }
     }
+    // rows t >= nts/2 of the reference's device P0 stay zero (cugw_routines_block.cu:79)
+    for (int tb = 0; tb < tbn; ++tb) {
+      cplx<double> *mir = P0 + it.out_base + (int64_t)(nts - 1 - (ta + tb)) * it.out_tstride;
+      for (int e = tid; e < nband * q; e += NT) mir[e] = cplx<double>{0.0, 0.0};
+    }
+    __syncthreads();
   }
-  // rows t >= nts/2 of the reference's device P0 stay zero (cugw_routines_block.cu:79)
-  cplx<double> *mir = P0 + it.out_base + (int64_t)(nts - 1 - t) * it.out_tstride;
-  for (int e = tid; e < nband * q; e += NT) mir[e] = cplx<double>{0.0, 0.0};
 }
 
 // ---------------------------------------------------------------------------
@@ -651,7 +698,7 @@ template <typename T>
 hipError_t launch_p0(const P0Item *items, const P0Blk *blks, int n_items, int nts, int ns, const cplx<T> *V,
                      const cplx<T> *G, cplx<double> *P0, double pref, size_t lds, hipStream_t st) {
   const int nt_half = nts / 2;
-  const int64_t n_work = (int64_t)n_items * nt_half;
+  const int64_t n_work = (int64_t)n_items * ((nt_half + kP0Times - 1) / kP0Times);
   if (n_work == 0) return hipSuccess;
   (void)hipFuncSetAttribute((const void *)p0_mfma_kernel<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   hipLaunchKernelGGL((p0_mfma_kernel<T>), dim3((unsigned)xcd_grid(n_work)), dim3(Cfg<T>::NT), lds, st, items, blks,

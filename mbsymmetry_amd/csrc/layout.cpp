@@ -331,14 +331,14 @@ bool build_worklists(const Layout &L, int eb, WorkLists &W, std::string &err) {
           B.g2_base = L.kao_off[k2] * ns * nts + L.ao_offsets[k2][t[2]];
           B.g1_tstride = (int32_t)(ns * L.kao_size[k]); B.g1_sstride = (int32_t)L.kao_size[k];
           B.g2_tstride = (int32_t)(ns * L.kao_size[k2]); B.g2_sstride = (int32_t)L.kao_size[k2];
-          // E and C of one chunk ([band][k1*nsub] each, +1 element of row
-          // padding) share the LDS budget
+          // E and C of one chunk ([times][band][k1*nsub] each, +1 element of
+          // row padding) share the LDS budget
           const int band = balanced(qs, kP0Band);
-          const int cap = std::min(B.k2, (budget / (2 * band) - 1) / B.k1);
+          const int cap = std::min(B.k2, (budget / (2 * kP0Times * band) - 1) / B.k1);
           if (cap < 1) { err = fmt("P0 block too large for the LDS plan (k1, k2)", B.k1, B.k2); return false; }
           B.nsub = balanced(B.k2, cap);
           B.lde = odd_ld(B.k1 * B.nsub);
-          p0_lds = std::max(p0_lds, (size_t)(2 * band * B.lde) * eb);
+          p0_lds = std::max(p0_lds, (size_t)(2 * kP0Times * band * B.lde) * eb);
           W.p0_blks.push_back(B);
         }
       }

@@ -278,15 +278,17 @@ def test_self_consistent_loop_holds_the_electron_count():
     ctx.close()
 
 
-def test_large_blocks_parity():
+@pytest.mark.parametrize("ao,aux", [([66, 10], [100, 24]), ([10, 6], [200, 20])])
+def test_large_blocks_parity(ao, aux):
     """Blocks that exercise every chunking path of the contraction kernels:
-    an ao block of 66 (> 64: two output bands in Sigma, orbital chunks in P0)
-    and an aux block of 100 (row bands in P0, aux-row chunks in Sigma), next
-    to small blocks.  P0, W, Sigma and the new G against the oracle."""
+    an ao block of 66 (> 48: two output bands in Sigma, orbital chunks in P0)
+    and an aux block of 100 (row bands in P0, aux-row chunks in Sigma); an aux
+    block of 200 (more P0 column tiles than waves: the two times of a slab
+    run one after the other).  P0, W and Sigma against the oracle."""
     import os
     from mbsymmetry_amd.transforms import chebyshev
-    sysm = S.synthetic("big_blocks", 2, 4, [66, 10], [100, 24], 1, chebyshev(4, 6.0), 31, v_scale=0.1)
-    _say("large blocks", "ao [66, 10] aux [100, 24]")
+    sysm = S.synthetic("big_blocks", 2, 4, ao, aux, 1, chebyshev(4, 6.0), 31, v_scale=0.1)
+    _say("large blocks", "ao", ao, "aux", aux)
     nth = min(16, len(os.sched_getaffinity(0)))
     orc, ctx, G0 = _setup(sysm, mu=0.0)
     V = sysm.host_V()
