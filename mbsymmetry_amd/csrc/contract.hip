@@ -15,6 +15,9 @@
 
 namespace symgw {
 
+// LDS of one rotation workgroup (the W = X O^H column chunk).
+constexpr int kRotLds = 64 * 1024;
+
 namespace {
 
 // Waves per workgroup: 12 (three per SIMD, at most 168 VGPRs each) on the
@@ -336,7 +339,7 @@ __device__ __forceinline__ void stage_rows(cplx<T> *dst, int ld, const cplx<T> *
 // (row tile, group of NG column tiles) sharing the A fragment of each k-step;
 // column-share jobs are (group of NG row tiles, column tile) sharing B.
 // la(row, k), lb(k, col), st(row, col, value) (st only for row < M, col < N).
-template <typename T, int NG, typename LA, typename LB, typename ST>
+template <typename T, int NG, bool CB, typename LA, typename LB, typename ST>
 __device__ __forceinline__ int gemm_jobs_rs_g(int M, int N, int K, int first, const LA &la, const LB &lb,
                                               const ST &st) {
   constexpr int NW = Cfg<T>::NW;
@@ -347,7 +350,7 @@ __device__ __forceinline__ int gemm_jobs_rs_g(int M, int N, int K, int first, co
     Tiles<T> acc;
 #pragma unroll
     for (int c = 0; c < NG; ++c) tile_zero(acc[c]);
-    mma_rowshare<0, NG, false>(acc, n, 0, K, true, [&](int i, int k) { return la(m0 + i, k); },
+    mma_rowshare<0, NG, CB>(acc, n, 0, K, true, [&](int i, int k) { return la(m0 + i, k); },
                                [&](int c, int k, int jj) { return lb(k, (g + c) * 16 + jj); });
 #pragma unroll
     for (int c = 0; c < NG; ++c)
@@ -360,7 +363,7 @@ __device__ __forceinline__ int gemm_jobs_rs_g(int M, int N, int K, int first, co
   return njobs;
 }
 
-template <typename T, int NG, typename LA, typename LB, typename ST>
+template <typename T, int NG, bool CB, typename LA, typename LB, typename ST>
 __device__ __forceinline__ int gemm_jobs_cs_g(int M, int N, int K, int first, const LA &la, const LB &lb,
                                               const ST &st) {
   constexpr int NW = Cfg<T>::NW;
@@ -371,7 +374,7 @@ __device__ __forceinline__ int gemm_jobs_cs_g(int M, int N, int K, int first, co
     Tiles<T> acc;
 #pragma unroll
     for (int c = 0; c < NG; ++c) tile_zero(acc[c]);
-    mma_colshare<0, NG, false>(acc, n, 0, K, [&](int c, int i, int k) { return la((g + c) * 16 + i, k); },
+    mma_colshare<0, NG, CB>(acc, n, 0, K, [&](int c, int i, int k) { return la((g + c) * 16 + i, k); },
                                [&](int k, int jj) { return lb(k, n0 + jj); });
 #pragma unroll
     for (int c = 0; c < NG; ++c)
@@ -386,21 +389,21 @@ __device__ __forceinline__ int gemm_jobs_cs_g(int M, int N, int K, int first, co
 
 // Grouped jobs when there are enough of them to keep every wave busy, else
 // single-tile jobs (less operand sharing, shorter critical path).
-template <typename T, typename LA, typename LB, typename ST>
+template <typename T, bool CB = false, typename LA, typename LB, typename ST>
 __device__ __forceinline__ int gemm_jobs_rs(int M, int N, int K, int first, const LA &la, const LB &lb,
                                             const ST &st) {
   constexpr int NG = Group<T>::n;
   const int ntm = (M + 15) / 16, ntn = (N + 15) / 16;
-  if (ntm * ((ntn + NG - 1) / NG) < Cfg<T>::NW) return gemm_jobs_rs_g<T, 1>(M, N, K, first, la, lb, st);
-  return gemm_jobs_rs_g<T, NG>(M, N, K, first, la, lb, st);
+  if (ntm * ((ntn + NG - 1) / NG) < Cfg<T>::NW) return gemm_jobs_rs_g<T, 1, CB>(M, N, K, first, la, lb, st);
+  return gemm_jobs_rs_g<T, NG, CB>(M, N, K, first, la, lb, st);
 }
-template <typename T, typename LA, typename LB, typename ST>
+template <typename T, bool CB = false, typename LA, typename LB, typename ST>
 __device__ __forceinline__ int gemm_jobs_cs(int M, int N, int K, int first, const LA &la, const LB &lb,
                                             const ST &st) {
   constexpr int NG = Group<T>::n;
   const int ntm = (M + 15) / 16, ntn = (N + 15) / 16;
-  if (((ntm + NG - 1) / NG) * ntn < Cfg<T>::NW) return gemm_jobs_cs_g<T, 1>(M, N, K, first, la, lb, st);
-  return gemm_jobs_cs_g<T, NG>(M, N, K, first, la, lb, st);
+  if (((ntm + NG - 1) / NG) * ntn < Cfg<T>::NW) return gemm_jobs_cs_g<T, 1, CB>(M, N, K, first, la, lb, st);
+  return gemm_jobs_cs_g<T, NG, CB>(M, N, K, first, la, lb, st);
 }
 
 }  // namespace
@@ -693,6 +696,79 @@ __global__ __launch_bounds__(Cfg<T>::NT) void sigma_mfma_kernel(const SigItem *_
     }
   }
 }
+
+// ---------------------------------------------------------------------------
+// Rotation of one block of one (t, s) slice from the iBZ representative to a
+// full-BZ point (cugw_routines_block.cu:134-186, cugw_qkpt_block.cu:288-340):
+//   Y = O X O^H (O is m x n), or a copy, conjugated for time-reversal partners.
+// On the matrix cores: W = X O^H into LDS (in column chunks of O's rows), then
+// Y = O W straight to memory.  src: iBZ array; the (t,s) slice of the
+// representative of target k starts at src_slice[k] + ts * src_ts_stride[k].
+// dst: k_ts_ij over the full BZ.
+// ---------------------------------------------------------------------------
+template <typename TO>
+__global__ __launch_bounds__(Cfg<double>::NT) void rotate_mfma_kernel(
+    const RotBlk *__restrict__ rb, int n_rb, int nts_ns, const cplx<double> *__restrict__ src,
+    const int64_t *__restrict__ src_ts_stride, const int64_t *__restrict__ src_slice,
+    const cplx<double> *__restrict__ rep, cplx<TO> *__restrict__ dst, const int64_t *__restrict__ dst_slice,
+    const int64_t *__restrict__ dst_slice_size) {
+  constexpr int NT = Cfg<double>::NT;
+  const int64_t w = xcd_logical(blockIdx.x, gridDim.x);
+  if (w >= (int64_t)n_rb * nts_ns) return;
+  const int r = (int)(w / nts_ns), ts = (int)(w % nts_ns);
+  const RotBlk R = rb[r];
+  const int tid = threadIdx.x;
+  const cplx<double> *X = src + (int64_t)ts * src_ts_stride[R.target] + src_slice[R.target] + R.src_off;
+  cplx<TO> *Y = dst + dst_slice[R.target] * nts_ns + (int64_t)ts * dst_slice_size[R.target] + R.dst_off;
+  const int m = R.m, n = R.n;
+  const double sgn = R.conj ? -1.0 : 1.0;
+  if (R.rep_off < 0) {
+    for (int i = tid; i < m * m; i += NT) {
+      const cplx<double> v = X[i];
+      Y[i] = cplx<TO>{(TO)v.re, (TO)(sgn * v.im)};
+    }
+    return;
+  }
+  const cplx<double> *O = rep + R.rep_off;  // m x n
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cplx<double> *Wb = reinterpret_cast<cplx<double> *>(smem_raw);  // [n][jc]
+  const int jc_max = max(16, min(m, (kRotLds / 16) / n));
+  for (int j0 = 0; j0 < m; j0 += jc_max) {
+    const int jc = min(jc_max, m - j0);
+    __syncthreads();
+    // W[a][jj] = sum_b X[a][b] conj(O[j0+jj][b])
+    gemm_jobs_rs<double, true>(
+        n, jc, n, 0, [&](int a, int b) { return X[min(a, n - 1) * n + min(b, n - 1)]; },
+        [&](int b, int jj) { return O[(j0 + min(jj, jc - 1)) * n + min(b, n - 1)]; },
+        [&](int a, int jj, cplx<double> v) { Wb[a * jc + jj] = v; });
+    __syncthreads();
+    // Y[i][j0+jj] = sum_a O[i][a] W[a][jj]
+    gemm_jobs_rs<double>(
+        m, jc, n, 0, [&](int i, int a) { return O[min(i, m - 1) * n + min(a, n - 1)]; },
+        [&](int a, int jj) { return Wb[min(a, n - 1) * jc + min(jj, jc - 1)]; },
+        [&](int i, int jj, cplx<double> v) { Y[i * m + j0 + jj] = cplx<TO>{(TO)v.re, (TO)(sgn * v.im)}; });
+  }
+}
+
+template <typename TO>
+hipError_t launch_rotate(const RotBlk *rb, int n_rb, int nts_ns, const cplx<double> *src, const int64_t *src_ts_stride,
+                         const int64_t *src_slice, const cplx<double> *rep, cplx<TO> *dst, const int64_t *dst_slice,
+                         const int64_t *dst_slice_size, hipStream_t st) {
+  const int64_t n_work = (int64_t)n_rb * nts_ns;
+  if (n_work == 0) return hipSuccess;
+  (void)hipFuncSetAttribute((const void *)rotate_mfma_kernel<TO>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            kRotLds);
+  hipLaunchKernelGGL((rotate_mfma_kernel<TO>), dim3((unsigned)xcd_grid(n_work)), dim3(Cfg<double>::NT), kRotLds, st,
+                     rb, n_rb, nts_ns, src, src_ts_stride, src_slice, rep, dst, dst_slice, dst_slice_size);
+  return hipGetLastError();
+}
+
+template hipError_t launch_rotate<double>(const RotBlk *, int, int, const cplx<double> *, const int64_t *,
+                                          const int64_t *, const cplx<double> *, cplx<double> *, const int64_t *,
+                                          const int64_t *, hipStream_t);
+template hipError_t launch_rotate<float>(const RotBlk *, int, int, const cplx<double> *, const int64_t *,
+                                         const int64_t *, const cplx<double> *, cplx<float> *, const int64_t *,
+                                         const int64_t *, hipStream_t);
 
 template <typename T>
 hipError_t launch_p0(const P0Item *items, const P0Blk *blks, int n_items, int nts, int ns, const cplx<T> *V,

@@ -15,63 +15,6 @@ template <typename T>
 __device__ inline cplx<T> czero() { return cplx<T>{T(0), T(0)}; }
 
 // ---------------------------------------------------------------------------
-// iBZ -> full-BZ rotation of one block for one (t, s) slice:
-//   Y = O X O^H (or a copy), conjugated for time-reversal partners.
-// src: iBZ array; the (t,s) slice of the representative of target k starts at
-// src_slice[k] + ts * src_ts_stride[k].  dst: k_ts_ij over the full BZ.
-// ---------------------------------------------------------------------------
-template <typename TO>
-__global__ __launch_bounds__(NT) void rotate_kernel(const RotBlk *__restrict__ rb, int n_rb, int nts_ns,
-                                                   const cplx<double> *__restrict__ src,
-                                                   const int64_t *__restrict__ src_ts_stride,
-                                                   const int64_t *__restrict__ src_slice,
-                                                   const cplx<double> *__restrict__ rep, cplx<TO> *__restrict__ dst,
-                                                   const int64_t *__restrict__ dst_slice,
-                                                   const int64_t *__restrict__ dst_slice_size) {
-  const int64_t w = blockIdx.x;
-  const int r = (int)(w / nts_ns);
-  const int ts = (int)(w % nts_ns);
-  if (r >= n_rb) return;
-  const RotBlk R = rb[r];
-  const int tid = threadIdx.x;
-  const cplx<double> *X = src + (int64_t)ts * src_ts_stride[R.target] + src_slice[R.target] + R.src_off;
-  cplx<TO> *Y = dst + dst_slice[R.target] * nts_ns + (int64_t)ts * dst_slice_size[R.target] + R.dst_off;
-  const int m = R.m, n = R.n;
-  if (R.rep_off < 0) {
-    for (int i = tid; i < m * m; i += NT) {
-      cplx<double> v = X[i];
-      if (R.conj) v.im = -v.im;
-      Y[i] = cplx<TO>{(TO)v.re, (TO)v.im};
-    }
-    return;
-  }
-  const cplx<double> *O = rep + R.rep_off;  // m x n
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  cplx<double> *tmp = reinterpret_cast<cplx<double> *>(smem_raw);  // [n][jc]
-  const int jc_max = max(1, 4096 / n);
-  for (int j0 = 0; j0 < m; j0 += jc_max) {
-    const int jc = min(jc_max, m - j0);
-    __syncthreads();
-    // tmp[a][jj] = sum_b X[a][b] conj(O[j0+jj][b])
-    for (int i = tid; i < n * jc; i += NT) {
-      const int a = i / jc, jj = i - a * jc;
-      cplx<double> acc{0.0, 0.0};
-      for (int bb = 0; bb < n; ++bb) cfmac(acc, X[a * n + bb], O[(j0 + jj) * n + bb]);
-      tmp[i] = acc;
-    }
-    __syncthreads();
-    // Y[i][j0+jj] = sum_a O[i][a] tmp[a][jj]
-    for (int i = tid; i < m * jc; i += NT) {
-      const int ii = i / jc, jj = i - ii * jc;
-      cplx<double> acc{0.0, 0.0};
-      for (int a = 0; a < n; ++a) cfma(acc, O[ii * n + a], tmp[a * jc + jj]);
-      if (R.conj) acc.im = -acc.im;
-      Y[ii * m + j0 + jj] = cplx<TO>{(TO)acc.re, (TO)acc.im};
-    }
-  }
-}
-
-// ---------------------------------------------------------------------------
 // Batched transform GEMM (FP64): out_b[r][c] = alpha * sum_j T[r][j] in_b[row(j)][c]
 // for r < M, j < KT, c < ncol_b.  Columns are split into nsplit groups of
 // xlen_b: column c -> (c / xlen) * split_b + c % xlen, with per-batch split
@@ -350,17 +293,6 @@ __global__ __launch_bounds__(NT) void reorder_kernel(const cplx<double> *__restr
 }
 
 // ---- launchers ---------------------------------------------------------------
-template <typename TO>
-hipError_t launch_rotate(const RotBlk *rb, int n_rb, int nts_ns, const cplx<double> *src, const int64_t *src_ts_stride,
-                         const int64_t *src_slice, const cplx<double> *rep, cplx<TO> *dst, const int64_t *dst_slice,
-                         const int64_t *dst_slice_size, hipStream_t st) {
-  if (n_rb == 0) return hipSuccess;
-  hipLaunchKernelGGL((rotate_kernel<TO>), dim3((unsigned)((int64_t)n_rb * nts_ns)), dim3(NT),
-                     4096 * sizeof(cplx<double>), st, rb, n_rb, nts_ns, src, src_ts_stride, src_slice, rep, dst,
-                     dst_slice, dst_slice_size);
-  return hipGetLastError();
-}
-
 hipError_t launch_tgemm(const TBatch *tb, int nbatch, int max_cols, int M, int KT, int nsplit,
                         const cplx<double> *Tm, const cplx<double> *in, cplx<double> *out, int mode, int row_shift,
                         int nts, double alpha, int64_t in_split_stride, int64_t out_split_stride, hipStream_t st) {
@@ -419,12 +351,6 @@ hipError_t launch_reorder(const cplx<double> *in, cplx<double> *out, const int64
   return hipGetLastError();
 }
 
-template hipError_t launch_rotate<double>(const RotBlk *, int, int, const cplx<double> *, const int64_t *, const int64_t *,
-                                          const cplx<double> *, cplx<double> *, const int64_t *, const int64_t *,
-                                          hipStream_t);
-template hipError_t launch_rotate<float>(const RotBlk *, int, int, const cplx<double> *, const int64_t *, const int64_t *,
-                                         const cplx<double> *, cplx<float> *, const int64_t *, const int64_t *,
-                                         hipStream_t);
 template hipError_t launch_genv<double>(const GenBlk *, int, uint64_t, double, const double *, int, cplx<double> *,
                                         hipStream_t);
 template hipError_t launch_genv<float>(const GenBlk *, int, uint64_t, double, const double *, int, cplx<float> *,
