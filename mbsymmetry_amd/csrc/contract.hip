@@ -583,15 +583,16 @@ __global__ __launch_bounds__(Cfg<T>::NT) void sigma_mfma_kernel(const SigItem *_
   constexpr int TB = kSigTimes;
   const int64_t w = xcd_logical(blockIdx.x, gridDim.x);
   if (w >= n_work) return;
-  const int per = (nts / TB) * ns;
+  const int np = (nts + TB - 1) / TB, per = np * ns;
   const SigItem it = items[w / per];
   const int ts = (int)(w % per), t0 = (ts / ns) * TB, s = ts - (ts / ns) * ns;
+  const int tbn = min(TB, nts - t0);  // times of this slab
   constexpr int NW = Cfg<T>::NW, MT = Cfg<T>::MT;
   const int tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cplx<T> *lds = reinterpret_cast<cplx<T> *>(smem_raw);
   const int k1 = it.k1, I0 = it.I0, Ib = it.Ib, J0 = it.J0, Jb = it.Jb;
-  const int nti = (Ib + 15) / 16, ntj = (Jb + 15) / 16, ntile = nti * ntj, nall = TB * ntile;
+  const int nti = (Ib + 15) / 16, ntj = (Jb + 15) / 16, ntile = nti * ntj, nall = tbn * ntile;
   // output tiles u = tb * ntile + tt: S waves share a tile over S parts of the
   // (Q, n) reduction
   const int S = choose_split<NW, MT>(nall), TG = NW / S, kp = wave % S, tg = wave / S;
@@ -616,7 +617,7 @@ __global__ __launch_bounds__(Cfg<T>::NT) void sigma_mfma_kernel(const SigItem *_
       const int nr = min(qsub, q - Q0);
       __syncthreads();
 #pragma unroll
-      for (int tb = 0; tb < TB; ++tb)
+      for (int tb = 0; tb < tbn; ++tb)
         stage_rows<T, 4>(bufP + tb * qsub * q, q, Pb + (int64_t)tb * pts + (int64_t)Q0 * q, nr * q, q,
                          [](cplx<T> p) { return cplx<T>{p.re, -p.im}; });
       for (int n0 = 0; n0 < k2; n0 += nsub) {
@@ -626,7 +627,7 @@ __global__ __launch_bounds__(Cfg<T>::NT) void sigma_mfma_kernel(const SigItem *_
         int nj;
         {
           auto la = [&](int row, int P) {
-            row = min(row, TB * nr - 1);
+            row = min(row, tbn * nr - 1);
             const int tb = row / nr, r = row - tb * nr;
             return bufP[(tb * qsub + r) * q + min(P, q - 1)];
           };
@@ -640,8 +641,8 @@ __global__ __launch_bounds__(Cfg<T>::NT) void sigma_mfma_kernel(const SigItem *_
             bufZ[(tb * Jb + jj) * ldz + r * nn + l] = v;
           };
           // several row tiles: share each V fragment among them
-          if (TB * nr > 16) nj = gemm_jobs_cs<T>(TB * nr, zc, q, 0, la, lb, st);
-          else nj = gemm_jobs_rs<T>(TB * nr, zc, q, 0, la, lb, st);
+          if (tbn * nr > 16) nj = gemm_jobs_cs<T>(tbn * nr, zc, q, 0, la, lb, st);
+          else nj = gemm_jobs_rs<T>(tbn * nr, zc, q, 0, la, lb, st);
         }
         // E[(r,i)][(tb,l)] = sum_m V[Q0+r][I0+i][m] G2_tb[m][n0+l]  (jobs dealt on after Z's)
         {
@@ -651,7 +652,7 @@ __global__ __launch_bounds__(Cfg<T>::NT) void sigma_mfma_kernel(const SigItem *_
             return Vb[(int64_t)(Q0 + r) * K + (I0 + ii) * k2 + min(m, k2 - 1)];
           };
           auto lb = [&](int m, int col) {
-            col = min(col, TB * nn - 1);
+            col = min(col, tbn * nn - 1);
             const int tb = col / nn, l = col - tb * nn;
             return G2[(int64_t)tb * g2ts + min(m, k2 - 1) * k2 + n0 + l];
           };
@@ -659,8 +660,8 @@ __global__ __launch_bounds__(Cfg<T>::NT) void sigma_mfma_kernel(const SigItem *_
             const int r = row / Ib, ii = row - r * Ib, tb = col / nn, l = col - tb * nn;
             bufE[(tb * Ib + ii) * lde + r * nn + l] = v;
           };
-          if (TB * nn > 16) gemm_jobs_rs<T>(nr * Ib, TB * nn, k2, nj, la, lb, st);
-          else gemm_jobs_cs<T>(nr * Ib, TB * nn, k2, nj, la, lb, st);
+          if (tbn * nn > 16) gemm_jobs_rs<T>(nr * Ib, tbn * nn, k2, nj, la, lb, st);
+          else gemm_jobs_cs<T>(nr * Ib, tbn * nn, k2, nj, la, lb, st);
         }
         __syncthreads();
         // acc[i][j] += sum_y E_tb[i][y] conj(Z_tb[j][y]) over this wave's part of y
@@ -787,7 +788,7 @@ template <typename T>
 hipError_t launch_sigma(const SigItem *items, const SigBlk *blks, int n_items, int nts, int ns, const cplx<T> *V,
                         const cplx<T> *G, const cplx<T> *Pf, cplx<double> *Sig, double pref, size_t lds,
                         hipStream_t st) {
-  const int64_t n_work = (int64_t)n_items * (nts / kSigTimes) * ns;
+  const int64_t n_work = (int64_t)n_items * ((nts + kSigTimes - 1) / kSigTimes) * ns;
   if (n_work == 0) return hipSuccess;
   (void)hipFuncSetAttribute((const void *)sigma_mfma_kernel<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)lds);

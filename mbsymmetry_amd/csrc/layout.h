@@ -62,8 +62,8 @@ constexpr int kP0Times = 2;
 constexpr int kSigBand = 48;
 // Most aux rows Q of one Sigma LDS chunk.
 constexpr int kSigRows = 64;
-// Consecutive times of one Sigma slab (they share every V fragment); nts is
-// even, so 2 divides it.
+// Consecutive times of one Sigma slab (they share every V fragment); the last
+// slab takes the remaining nts % kSigTimes times.
 constexpr int kSigTimes = 2;
 
 // One V-block contraction that feeds a P0 output slab (cugw_qkpt_block.cu:219-261).
