@@ -46,6 +46,17 @@ struct Mfma<float> {
   __device__ static int row(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
 
+// x / d for a divisor fixed over a product: one multiply-high by
+// m = ceil(2^32 / d) instead of the ~25-instruction integer division.  Exact
+// for 0 <= x with x * d < 2^32 (x * m / 2^32 = x / d + x * e / 2^32 with
+// 0 <= e < 1, which stays below the next multiple of 1/d); every index here is
+// below 2^17 and every divisor below 2^13.
+struct FDiv {
+  uint32_t m, d;
+  __device__ explicit FDiv(int dd) : m(dd > 1 ? 0xFFFFFFFFu / (uint32_t)dd + 1u : 0u), d((uint32_t)dd) {}
+  __device__ int div(int x) const { return d == 1 ? x : (int)__umulhi((uint32_t)x, m); }
+};
+
 template <typename T>
 struct Tile {
   typename Mfma<T>::v4 re, im;
@@ -315,6 +326,7 @@ __device__ __forceinline__ void stage_rows(cplx<T> *dst, int ld, const cplx<T> *
                                            const F &f) {
   constexpr int NT = Cfg<T>::NT;
   const int tid = threadIdx.x;
+  const FDiv dlen(len);
   for (int base = 0; base < n; base += U * NT) {
     cplx<T> v[U];
 #pragma unroll
@@ -326,7 +338,7 @@ __device__ __forceinline__ void stage_rows(cplx<T> *dst, int ld, const cplx<T> *
     for (int u = 0; u < U; ++u) {
       const int e = base + u * NT + tid;
       if (e < n) {
-        const int r = e / len, x = e - r * len;
+        const int r = dlen.div(e), x = e - r * len;
         dst[r * ld + x] = f(v[u]);
       }
     }
@@ -345,8 +357,11 @@ __device__ __forceinline__ int gemm_jobs_rs_g(int M, int N, int K, int first, co
   constexpr int NW = Cfg<T>::NW;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int ntm = (M + 15) / 16, ntn = (N + 15) / 16, ngr = (ntn + NG - 1) / NG, njobs = ntm * ngr;
-  for (int j = (wave - first % NW + NW) % NW; j < njobs; j += NW) {
-    const int mt = j / ngr, g = (j - mt * ngr) * NG, n = min(NG, ntn - g), m0 = mt * 16;
+  // job j = mt * ngr + gi, advanced by NW per round without dividing
+  int j = (wave - first % NW + NW) % NW, mt = j / ngr, gi = j - mt * ngr;
+  const int dmt = NW / ngr, dgi = NW - dmt * ngr;
+  for (; j < njobs; j += NW, gi += dgi, mt += dmt + (gi >= ngr ? 1 : 0), gi -= gi >= ngr ? ngr : 0) {
+    const int g = gi * NG, n = min(NG, ntn - g), m0 = mt * 16;
     Tiles<T> acc;
 #pragma unroll
     for (int c = 0; c < NG; ++c) tile_zero(acc[c]);
@@ -369,8 +384,11 @@ __device__ __forceinline__ int gemm_jobs_cs_g(int M, int N, int K, int first, co
   constexpr int NW = Cfg<T>::NW;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int ntm = (M + 15) / 16, ntn = (N + 15) / 16, ngr = (ntm + NG - 1) / NG, njobs = ngr * ntn;
-  for (int j = (wave - first % NW + NW) % NW; j < njobs; j += NW) {
-    const int gt = j / ntn, nt = j - gt * ntn, g = gt * NG, n = min(NG, ntm - g), n0 = nt * 16;
+  // job j = gt * ntn + nt, advanced by NW per round without dividing
+  int j = (wave - first % NW + NW) % NW, gt = j / ntn, nt = j - gt * ntn;
+  const int dgt = NW / ntn, dnt = NW - dgt * ntn;
+  for (; j < njobs; j += NW, nt += dnt, gt += dgt + (nt >= ntn ? 1 : 0), nt -= nt >= ntn ? ntn : 0) {
+    const int g = gt * NG, n = min(NG, ntm - g), n0 = nt * 16;
     Tiles<T> acc;
 #pragma unroll
     for (int c = 0; c < NG; ++c) tile_zero(acc[c]);
@@ -456,6 +474,7 @@ __global__ __launch_bounds__(Cfg<T>::NT) void p0_mfma_kernel(const P0Item *__res
     for (int b = it.blk_begin; b < it.blk_end; ++b) {
       const P0Blk B = blks[b];
       const int k1 = B.k1, k2 = B.k2, K = k1 * k2, nsub = B.nsub, lde = B.lde;
+      const FDiv dk1(k1);
       const cplx<T> *Vb = V + B.v_off;
       const cplx<T> *Vr = Vb + (int64_t)it.Q0 * K;        // V rows of the band
       cplx<T> *bufE = lds;                                // [tbn][nband][lde]  E[r][a][l] at a*nn + l
@@ -467,6 +486,7 @@ __global__ __launch_bounds__(Cfg<T>::NT) void p0_mfma_kernel(const P0Item *__res
         const cplx<T> *G2 = G + B.g2_base + (int64_t)ta * g2ts + (int64_t)s * B.g2_sstride;
         for (int n0 = 0; n0 < k2; n0 += nsub) {
           const int nn = min(nsub, k2 - n0), X = k1 * nn;
+          const FDiv dnn(nn);
           __syncthreads();
           // E_tb[(r,a)][l] = sum_m V[Q0+r][a][m] G2_tb[m][n0+l], columns (tb, l)
           {
@@ -476,11 +496,11 @@ __global__ __launch_bounds__(Cfg<T>::NT) void p0_mfma_kernel(const P0Item *__res
             };
             auto lb = [&](int m, int col) {
               col = min(col, tbn * nn - 1);
-              const int tb = col / nn, l = col - tb * nn;
+              const int tb = dnn.div(col), l = col - tb * nn;
               return G2[(int64_t)tb * g2ts + min(m, k2 - 1) * k2 + n0 + l];
             };
             auto st = [&](int row, int col, cplx<T> v) {
-              const int r = row / k1, a = row - r * k1, tb = col / nn, l = col - tb * nn;
+              const int r = dk1.div(row), a = row - r * k1, tb = dnn.div(col), l = col - tb * nn;
               bufE[(tb * nband + r) * lde + a * nn + l] = v;
             };
             if (tbn * nn > 16) gemm_jobs_rs<T>(nband * k1, tbn * nn, k2, 0, la, lb, st);
@@ -499,11 +519,11 @@ __global__ __launch_bounds__(Cfg<T>::NT) void p0_mfma_kernel(const P0Item *__res
                   [&](int p, int a) { return G1t[min(p, k1 - 1) * k1 + min(a, k1 - 1)]; },
                   [&](int a, int col) {
                     col = min(col, nband * nn - 1);
-                    const int r = col / nn, l = col - r * nn;
+                    const int r = dnn.div(col), l = col - r * nn;
                     return Et[r * lde + min(a, k1 - 1) * nn + l];
                   },
                   [&](int p, int col, cplx<T> v) {
-                    const int r = col / nn, l = col - r * nn;
+                    const int r = dnn.div(col), l = col - r * nn;
                     Ct[r * lde + p * nn + l] = v;
                   });
             }
@@ -521,7 +541,7 @@ __global__ __launch_bounds__(Cfg<T>::NT) void p0_mfma_kernel(const P0Item *__res
                     acc, tbn, kb, ke, [&](int c, int i, int x) { return bufC[(c * nband + rc) * lde + min(x, X - 1)]; },
                     [&](int x, int j) {
                       x = min(x, X - 1);
-                      const int p = x / nn, l = x - p * nn;
+                      const int p = dnn.div(x), l = x - p * nn;
                       return Vb[(int64_t)min(tg * 16 + j, q - 1) * K + p * k2 + n0 + l];
                     });
             } else {
@@ -529,7 +549,7 @@ __global__ __launch_bounds__(Cfg<T>::NT) void p0_mfma_kernel(const P0Item *__res
                   acc, nmine, kb, ke, true, [&](int i, int x) { return bufC[rc * lde + min(x, X - 1)]; },
                   [&](int c, int x, int j) {
                     x = min(x, X - 1);
-                    const int p = x / nn, l = x - p * nn;
+                    const int p = dnn.div(x), l = x - p * nn;
                     const int P = min((tg + TG * c) * 16 + j, q - 1);
                     return Vb[(int64_t)P * K + p * k2 + n0 + l];
                   });
@@ -613,8 +633,10 @@ __global__ __launch_bounds__(Cfg<T>::NT) void sigma_mfma_kernel(const SigItem *_
     cplx<T> *bufE = bufZ + TB * Jb * ldz;     // [TB][Ib][lde]  E[r][i][n] at i*lde + y
     cplx<T> *bufP = bufE + TB * Ib * lde;     // [TB][qsub][q]  conj(P) rows
     const cplx<T> *Vj = Vb + J0 * k2;
+    const FDiv dIb(Ib);
     for (int Q0 = 0; Q0 < q; Q0 += qsub) {
       const int nr = min(qsub, q - Q0);
+      const FDiv dnr(nr);
       __syncthreads();
 #pragma unroll
       for (int tb = 0; tb < tbn; ++tb)
@@ -622,22 +644,23 @@ __global__ __launch_bounds__(Cfg<T>::NT) void sigma_mfma_kernel(const SigItem *_
                          [](cplx<T> p) { return cplx<T>{p.re, -p.im}; });
       for (int n0 = 0; n0 < k2; n0 += nsub) {
         const int nn = min(nsub, k2 - n0), zc = Jb * nn;
+        const FDiv dnn(nn);
         __syncthreads();
         // Z[(tb,r)][(j,l)] = sum_P conj(P_tb[Q0+r][P]) V[P][J0+j][n0+l]
         int nj;
         {
           auto la = [&](int row, int P) {
             row = min(row, tbn * nr - 1);
-            const int tb = row / nr, r = row - tb * nr;
+            const int tb = dnr.div(row), r = row - tb * nr;
             return bufP[(tb * qsub + r) * q + min(P, q - 1)];
           };
           auto lb = [&](int P, int x) {
             x = min(x, zc - 1);
-            const int jj = x / nn, l = x - jj * nn;
+            const int jj = dnn.div(x), l = x - jj * nn;
             return Vj[(int64_t)min(P, q - 1) * K + jj * k2 + n0 + l];
           };
           auto st = [&](int row, int x, cplx<T> v) {
-            const int tb = row / nr, r = row - tb * nr, jj = x / nn, l = x - jj * nn;
+            const int tb = dnr.div(row), r = row - tb * nr, jj = dnn.div(x), l = x - jj * nn;
             bufZ[(tb * Jb + jj) * ldz + r * nn + l] = v;
           };
           // several row tiles: share each V fragment among them
@@ -648,16 +671,16 @@ __global__ __launch_bounds__(Cfg<T>::NT) void sigma_mfma_kernel(const SigItem *_
         {
           auto la = [&](int row, int m) {
             row = min(row, nr * Ib - 1);
-            const int r = row / Ib, ii = row - r * Ib;
+            const int r = dIb.div(row), ii = row - r * Ib;
             return Vb[(int64_t)(Q0 + r) * K + (I0 + ii) * k2 + min(m, k2 - 1)];
           };
           auto lb = [&](int m, int col) {
             col = min(col, tbn * nn - 1);
-            const int tb = col / nn, l = col - tb * nn;
+            const int tb = dnn.div(col), l = col - tb * nn;
             return G2[(int64_t)tb * g2ts + min(m, k2 - 1) * k2 + n0 + l];
           };
           auto st = [&](int row, int col, cplx<T> v) {
-            const int r = row / Ib, ii = row - r * Ib, tb = col / nn, l = col - tb * nn;
+            const int r = dIb.div(row), ii = row - r * Ib, tb = dnn.div(col), l = col - tb * nn;
             bufE[(tb * Ib + ii) * lde + r * nn + l] = v;
           };
           if (tbn * nn > 16) gemm_jobs_rs<T>(nr * Ib, tbn * nn, k2, nj, la, lb, st);
