@@ -53,8 +53,8 @@ struct Mfma<float> {
 // below 2^17 and every divisor below 2^13.
 struct FDiv {
   uint32_t m, d;
-  __device__ explicit FDiv(int dd) : m(dd > 1 ? 0xFFFFFFFFu / (uint32_t)dd + 1u : 0u), d((uint32_t)dd) {}
-  __device__ int div(int x) const { return d == 1 ? x : (int)__umulhi((uint32_t)x, m); }
+  __device__ __forceinline__ explicit FDiv(int dd) : m(dd > 1 ? 0xFFFFFFFFu / (uint32_t)dd + 1u : 0u), d((uint32_t)dd) {}
+  __device__ __forceinline__ int div(int x) const { return d == 1 ? x : (int)__umulhi((uint32_t)x, m); }
 };
 
 template <typename T>
@@ -632,6 +632,7 @@ __global__ __launch_bounds__(Cfg<T>::NT) void sigma_mfma_kernel(const SigItem *_
     cplx<T> *bufZ = lds;                      // [TB][Jb][ldz]  Z[r][j][n] at j*ldz + y
     cplx<T> *bufE = bufZ + TB * Jb * ldz;     // [TB][Ib][lde]  E[r][i][n] at i*lde + y
     cplx<T> *bufP = bufE + TB * Ib * lde;     // [TB][qsub][q]  conj(P) rows
+    cplx<T> *bufG = bufP + TB * qsub * q;     // [k2][tbn * nn]  G2_tb[m][n0 + l] at m * tbn * nn + tb * nn + l
     const cplx<T> *Vj = Vb + J0 * k2;
     const FDiv dIb(Ib);
     for (int Q0 = 0; Q0 < q; Q0 += qsub) {
@@ -643,8 +644,17 @@ __global__ __launch_bounds__(Cfg<T>::NT) void sigma_mfma_kernel(const SigItem *_
         stage_rows<T, 4>(bufP + tb * qsub * q, q, Pb + (int64_t)tb * pts + (int64_t)Q0 * q, nr * q, q,
                          [](cplx<T> p) { return cplx<T>{p.re, -p.im}; });
       for (int n0 = 0; n0 < k2; n0 += nsub) {
-        const int nn = min(nsub, k2 - n0), zc = Jb * nn;
+        const int nn = min(nsub, k2 - n0), zc = Jb * nn, NC = tbn * nn;
         const FDiv dnn(nn);
+        // G chunk of the E product: every wave has passed the barrier after
+        // the previous chunk's E, the only reader of bufG
+        {
+          const FDiv dNC(NC);
+          for (int e = tid; e < k2 * NC; e += Cfg<T>::NT) {
+            const int m = dNC.div(e), c = e - m * NC, tb = dnn.div(c), l = c - tb * nn;
+            bufG[e] = G2[(int64_t)tb * g2ts + m * k2 + n0 + l];
+          }
+        }
         __syncthreads();
         // Z[(tb,r)][(j,l)] = sum_P conj(P_tb[Q0+r][P]) V[P][J0+j][n0+l]
         int nj;
@@ -674,11 +684,7 @@ __global__ __launch_bounds__(Cfg<T>::NT) void sigma_mfma_kernel(const SigItem *_
             const int r = dIb.div(row), ii = row - r * Ib;
             return Vb[(int64_t)(Q0 + r) * K + (I0 + ii) * k2 + min(m, k2 - 1)];
           };
-          auto lb = [&](int m, int col) {
-            col = min(col, tbn * nn - 1);
-            const int tb = dnn.div(col), l = col - tb * nn;
-            return G2[(int64_t)tb * g2ts + min(m, k2 - 1) * k2 + n0 + l];
-          };
+          auto lb = [&](int m, int col) { return bufG[min(m, k2 - 1) * NC + min(col, NC - 1)]; };
           auto st = [&](int row, int col, cplx<T> v) {
             const int r = dIb.div(row), ii = row - r * Ib, tb = dnn.div(col), l = col - tb * nn;
             bufE[(tb * Ib + ii) * lde + r * nn + l] = v;

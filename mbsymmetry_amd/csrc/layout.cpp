@@ -270,7 +270,8 @@ bool sig_chunks(int q, int k1, int k2, int band, int budget, std::map<std::array
   for (int qs = 1; qs <= std::min(q, kSigRows); ++qs) {
     const int rem = budget - TB * qs * q - 2 * TB * band;
     if (rem <= 0) break;
-    const int ns_max = std::min(k2, rem / (2 * TB * band * qs));
+    // per orbital of the chunk: Z and E columns of every slab row, and a G column of every orbital m
+    const int ns_max = std::min(k2, rem / (2 * TB * band * qs + TB * k2));
     if (ns_max < 1) continue;
     const int qb = balanced(q, qs);
     for (int ns = ns_max; ns >= 1 && ns >= ns_max / 2; --ns) {
@@ -390,7 +391,7 @@ bool build_worklists(const Layout &L, int eb, WorkLists &W, std::string &err) {
           }
           B.ldz = odd_ld(B.qsub * B.nsub);
           B.lde = odd_ld(B.qsub * B.nsub);
-          sig_lds = std::max(sig_lds, (size_t)TB * (band * (B.ldz + B.lde) + B.qsub * B.q) * eb);
+          sig_lds = std::max(sig_lds, (size_t)TB * (band * (B.ldz + B.lde) + B.qsub * B.q + B.k2 * B.nsub) * eb);
           W.sig_blks.push_back(B);
         }
       }
