@@ -13,6 +13,12 @@ with the reference's damping convention (1: no damping, 0: full damping,
 sc_loop_block_t.h:70-72).  With `nel` given, mu is re-found after every
 iteration by bisection over device Dyson solves so that the electron count
 -(2/ns) sum_k w_k Tr[G_k(beta) S_k] / nk stays at nel.
+
+`save_results` writes the outcome to one .npz archive: the run parameters
+under the names the reference stores in its results file
+(SymmetryMBPT/src/common/params_t.cpp:27-48, default "sim.h5"), the final G
+and Sigma (ts_kij layout) and the per-iteration history.  HDF5 is not used
+because no HDF5 library is available on the device hosts.
 """
 from __future__ import annotations
 
@@ -29,13 +35,16 @@ class SCResult:
     iterations: int
     converged: bool
     history: List[dict] = field(default_factory=list)   # per iteration: dG, mu, nel
+    Sigma: Optional[np.ndarray] = None  # Sigma of the last iteration (/nk), ts_kij layout, if kept
 
 
 def self_consistent(ctx, system, G0: np.ndarray, mu: float, max_iter: int = 20, tol: float = 1e-8,
                     damp: float = 1.0, nel: Optional[float] = None, mu_bracket=(-2.0, 2.0),
-                    use_graph: bool = True) -> SCResult:
+                    use_graph: bool = True, keep_sigma: bool = False) -> SCResult:
     """Iterate GW on `ctx` (V, rotations, transforms and Fock already set)
-    from G0 until max|G_{n+1} - G_n| < tol or max_iter iterations."""
+    from G0 until max|G_{n+1} - G_n| < tol or max_iter iterations.  With
+    keep_sigma the last iteration's Sigma is copied back, divided by nk as the
+    reference's caller does (cugw_solver_block.cpp:302-304)."""
     if not 0.0 < damp <= 1.0:
         raise ValueError("damp must be in (0, 1]")
     shape = (system.nts, system.ns, system.flat_ao)
@@ -60,4 +69,40 @@ def self_consistent(ctx, system, G0: np.ndarray, mu: float, max_iter: int = 20, 
         if dG < tol:
             converged = True
             break
-    return SCResult(G=G, mu=mu, iterations=it, converged=converged, history=hist)
+    Sigma = None
+    if keep_sigma:
+        Sigma = ctx.copy_Sigma_to_host(system.zeros_ts_kij()) / system.nk
+    return SCResult(G=G, mu=mu, iterations=it, converged=converged, history=hist, Sigma=Sigma)
+
+
+def save_results(path: str, res: SCResult, system, params: Optional[dict] = None) -> None:
+    """Write `res` to `path` (.npz).  Keys: "params/<name>" for the system
+    sizes and run options (nel_cell, nao, nk, ink, NQ, ns, nts, beta, mu, tol,
+    damping, ... plus anything in `params`), "G_tau", "Selfenergy" (when kept),
+    "iterations", "converged" and "history/<field>" arrays."""
+    out = {
+        "params/nao": np.int64(sum(system.ao_sizes[int(system.irre_list[0])])),
+        "params/NQ": np.int64(sum(system.aux_sizes[int(system.irre_list[0])])),
+        "params/nk": np.int64(system.nk),
+        "params/ink": np.int64(len(system.irre_list)),
+        "params/ns": np.int64(system.ns),
+        "params/nts": np.int64(system.nts),
+        "params/beta": np.float64(system.transforms.beta),
+        "params/mu": np.float64(res.mu),
+        "G_tau": res.G,
+        "iterations": np.int64(res.iterations),
+        "converged": np.bool_(res.converged),
+    }
+    for k, v in (params or {}).items():
+        out["params/" + k] = np.asarray(v)
+    if res.Sigma is not None:
+        out["Selfenergy"] = res.Sigma
+    for fld in ("iteration", "max_abs_dG", "mu", "nel"):
+        out["history/" + fld] = np.array([np.nan if h[fld] is None else h[fld] for h in res.history], dtype=np.float64)
+    np.savez(path, **out)
+
+
+def load_results(path: str) -> dict:
+    """Read an archive written by save_results (no pickled objects)."""
+    with np.load(path, allow_pickle=False) as z:
+        return {k: z[k] for k in z.files}

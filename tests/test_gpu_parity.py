@@ -236,14 +236,15 @@ def test_self_consistent_loop_matches_oracle(name, systems):
     _say("sc loop", name)
     mu, damp = 0.02, 0.6
     orc, ctx, G0 = _setup(sysm, mu=mu)
-    res = self_consistent(ctx, sysm, G0, mu, max_iter=3, tol=0.0, damp=damp)
+    res = self_consistent(ctx, sysm, G0, mu, max_iter=3, tol=0.0, damp=damp, keep_sigma=True)
     G = G0.copy()
     for _ in range(3):
-        _, _, _, Gn = orc.gw_iteration(G, mu)
+        _, _, Sig, Gn = orc.gw_iteration(G, mu)
         G = damp * Gn + (1 - damp) * G
     assert res.iterations == 3 and not res.converged
     # one iteration agrees to 1e-10 (test_gw_iteration_parity_and_graph); three compound it
     assert _rel(res.G, G) < 1e-9, name
+    assert _rel(res.Sigma, Sig) < 1e-9, name
     assert [h["iteration"] for h in res.history] == [1, 2, 3]
     ctx.close()
 

@@ -116,3 +116,31 @@ def test_footprint_model_matches_allocation():
         f_out = settled_free()
         used = f_out - f_in   # what destroying the context gives back
         assert abs(used - model) <= 0.1 * model + (32 << 20), (prec, used, model)
+
+
+def test_results_archive_round_trip(tmp_path):
+    """save_results writes the parameters under the reference's results-file
+    names, G, Sigma and the history; load_results reads them back without
+    unpickling anything."""
+    from mbsymmetry_amd import system as S
+    from mbsymmetry_amd.scloop import SCResult, save_results, load_results
+    sysm = S.tiny()
+    rng = np.random.default_rng(3)
+    shape = (sysm.nts, sysm.ns, sysm.flat_ao)
+    G = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    Sig = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    hist = [{"iteration": 1, "max_abs_dG": 0.5, "mu": 0.1, "nel": None},
+            {"iteration": 2, "max_abs_dG": 1e-9, "mu": 0.2, "nel": None}]
+    res = SCResult(G=G, mu=0.2, iterations=2, converged=True, history=hist, Sigma=Sig)
+    path = str(tmp_path / "sim.npz")
+    save_results(path, res, sysm, params={"nel_cell": 4.0, "tol": 1e-9, "damping": 0.7})
+    z = load_results(path)
+    assert np.array_equal(z["G_tau"], G) and np.array_equal(z["Selfenergy"], Sig)
+    assert z["params/mu"] == 0.2 and z["params/nk"] == sysm.nk and z["params/nts"] == sysm.nts
+    assert z["params/ns"] == sysm.ns and z["params/beta"] == sysm.transforms.beta
+    assert z["params/nao"] == sum(sysm.ao_sizes[int(sysm.irre_list[0])])
+    assert z["params/NQ"] == sum(sysm.aux_sizes[int(sysm.irre_list[0])])
+    assert z["params/nel_cell"] == 4.0 and z["params/damping"] == 0.7
+    assert bool(z["converged"]) and z["iterations"] == 2
+    assert np.array_equal(z["history/max_abs_dG"], [0.5, 1e-9])
+    assert np.isnan(z["history/nel"]).all()
