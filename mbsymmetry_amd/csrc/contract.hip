@@ -631,23 +631,18 @@ __global__ __launch_bounds__(Cfg<T>::NT) void sigma_mfma_kernel(const SigItem *_
     // y = r*nn + l indexes (aux row of the chunk, orbital n0 + l of k - q)
     cplx<T> *bufZ = lds;                      // [TB][Jb][ldz]  Z[r][j][n] at j*ldz + y
     cplx<T> *bufE = bufZ + TB * Jb * ldz;     // [TB][Ib][lde]  E[r][i][n] at i*lde + y
-    cplx<T> *bufP = bufE + TB * Ib * lde;     // [TB][qsub][q]  conj(P) rows
-    cplx<T> *bufG = bufP + TB * qsub * q;     // [k2][tbn * nn]  G2_tb[m][n0 + l] at m * tbn * nn + tb * nn + l
+    cplx<T> *bufG = bufE + TB * Ib * lde;     // [k2][tbn * nn]  G2_tb[m][n0 + l] at m * tbn * nn + tb * nn + l
     const cplx<T> *Vj = Vb + J0 * k2;
     const FDiv dIb(Ib);
     for (int Q0 = 0; Q0 < q; Q0 += qsub) {
       const int nr = min(qsub, q - Q0);
       const FDiv dnr(nr);
-      __syncthreads();
-#pragma unroll
-      for (int tb = 0; tb < tbn; ++tb)
-        stage_rows<T, 4>(bufP + tb * qsub * q, q, Pb + (int64_t)tb * pts + (int64_t)Q0 * q, nr * q, q,
-                         [](cplx<T> p) { return cplx<T>{p.re, -p.im}; });
       for (int n0 = 0; n0 < k2; n0 += nsub) {
         const int nn = min(nsub, k2 - n0), zc = Jb * nn, NC = tbn * nn;
         const FDiv dnn(nn);
         // G chunk of the E product: every wave has passed the barrier after
-        // the previous chunk's E, the only reader of bufG
+        // the previous chunk's E, the only reader of bufG; the barrier below
+        // also ends the previous chunk's final product, the reader of bufZ and bufE
         {
           const FDiv dNC(NC);
           for (int e = tid; e < k2 * NC; e += Cfg<T>::NT) {
@@ -656,13 +651,15 @@ __global__ __launch_bounds__(Cfg<T>::NT) void sigma_mfma_kernel(const SigItem *_
           }
         }
         __syncthreads();
-        // Z[(tb,r)][(j,l)] = sum_P conj(P_tb[Q0+r][P]) V[P][J0+j][n0+l]
+        // Z[(tb,r)][(j,l)] = sum_P conj(P_tb[Q0+r][P]) V[P][J0+j][n0+l], formed as the
+        // conjugate of sum_P P_tb[Q0+r][P] conj(V[P][J0+j][n0+l]) with P read
+        // straight from memory (rows of the chunk, contiguous in P)
         int nj;
         {
           auto la = [&](int row, int P) {
             row = min(row, tbn * nr - 1);
             const int tb = dnr.div(row), r = row - tb * nr;
-            return bufP[(tb * qsub + r) * q + min(P, q - 1)];
+            return Pb[(int64_t)tb * pts + (int64_t)(Q0 + r) * q + min(P, q - 1)];
           };
           auto lb = [&](int P, int x) {
             x = min(x, zc - 1);
@@ -671,11 +668,11 @@ __global__ __launch_bounds__(Cfg<T>::NT) void sigma_mfma_kernel(const SigItem *_
           };
           auto st = [&](int row, int x, cplx<T> v) {
             const int tb = dnr.div(row), r = row - tb * nr, jj = dnn.div(x), l = x - jj * nn;
-            bufZ[(tb * Jb + jj) * ldz + r * nn + l] = v;
+            bufZ[(tb * Jb + jj) * ldz + r * nn + l] = cplx<T>{v.re, -v.im};
           };
           // several row tiles: share each V fragment among them
-          if (tbn * nr > 16) nj = gemm_jobs_cs<T>(tbn * nr, zc, q, 0, la, lb, st);
-          else nj = gemm_jobs_rs<T>(tbn * nr, zc, q, 0, la, lb, st);
+          if (tbn * nr > 16) nj = gemm_jobs_cs<T, true>(tbn * nr, zc, q, 0, la, lb, st);
+          else nj = gemm_jobs_rs<T, true>(tbn * nr, zc, q, 0, la, lb, st);
         }
         // E[(r,i)][(tb,l)] = sum_m V[Q0+r][I0+i][m] G2_tb[m][n0+l]  (jobs dealt on after Z's)
         {

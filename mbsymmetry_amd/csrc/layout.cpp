@@ -268,7 +268,7 @@ bool sig_chunks(int q, int k1, int k2, int band, int budget, std::map<std::array
   const double chunk_cost = 2.5e5;  // ~2000 cycles of a CU's FP64 MFMA rate
   double best = -1;
   for (int qs = 1; qs <= std::min(q, kSigRows); ++qs) {
-    const int rem = budget - TB * qs * q - 2 * TB * band;
+    const int rem = budget - 2 * TB * band;
     if (rem <= 0) break;
     // per orbital of the chunk: Z and E columns of every slab row, and a G column of every orbital m
     const int ns_max = std::min(k2, rem / (2 * TB * band * qs + TB * k2));
@@ -391,7 +391,7 @@ bool build_worklists(const Layout &L, int eb, WorkLists &W, std::string &err) {
           }
           B.ldz = odd_ld(B.qsub * B.nsub);
           B.lde = odd_ld(B.qsub * B.nsub);
-          sig_lds = std::max(sig_lds, (size_t)TB * (band * (B.ldz + B.lde) + B.qsub * B.q + B.k2 * B.nsub) * eb);
+          sig_lds = std::max(sig_lds, (size_t)TB * (band * (B.ldz + B.lde) + B.k2 * B.nsub) * eb);
           W.sig_blks.push_back(B);
         }
       }
