@@ -24,7 +24,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAK = {"fp64": 78.6, "fp32": 157.3}  # MI355X dense matrix TFLOP/s (datasheet)
+PEAK = {"fp64": 78.6, "fp32": 157.3, "hbm_gbps": 8000.0}  # MI355X datasheet: dense matrix TFLOP/s, HBM3E GB/s
 
 
 def log(*a):
@@ -200,6 +200,8 @@ def main():
 
     sysm = make_system(args.config)
     log("system", workload_name(sysm))
+    peaks = _lib.measure_peaks(0)  # the box's own peaks, beside the datasheet values
+    log("measured peaks", peaks)
     # G0 = Dyson with Sigma = 0 on the device (sc_loop init_g)
     ctx = setup_ctx(sysm, _lib.PREC_FP64, args.mu)
     ctx.upload_G(sysm.zeros_ts_kij())
@@ -248,7 +250,14 @@ def main():
         "phase_ms": {"P0_kernel": p0_ms, "W+P_rotation": float(plain["phase_ms"][1]), "Sigma_kernel": sig_ms,
                      "Dyson+G_rotation": float(plain["phase_ms"][3])},
         "roofline": {"bound": "mfma", "kernel": dom_kernel, "achieved": achieved, "peak": PEAK["fp64"],
-                     "unit": "TFLOP/s", "frac": achieved / PEAK["fp64"], "traffic": traffic},
+                     "unit": "TFLOP/s", "frac": achieved / PEAK["fp64"], "traffic": traffic,
+                     "measured_peak": peaks["fp64_mfma_tflops"],
+                     "frac_of_measured_peak": achieved / peaks["fp64_mfma_tflops"]},
+        "peaks": {"datasheet": {"fp64_mfma_tflops": PEAK["fp64"], "fp32_mfma_tflops": PEAK["fp32"],
+                                "hbm_gbps": PEAK["hbm_gbps"]},
+                  "measured": peaks},
+        "kernel_tflops": {"P0": plain["flop_p0"] / (p0_ms * 1e-3) / 1e12,
+                          "Sigma": plain["flop_sig"] / (sig_ms * 1e-3) / 1e12},
     }
     if mixed is not None:
         dsig = float(np.abs(mixed["sigma"] - plain["sigma"]).max())
@@ -256,7 +265,16 @@ def main():
         out["mixed_fp32"] = {"iter_per_s": 1.0 / mixed["sec_per_iter"], "speedup_vs_plain":
                              plain["sec_per_iter"] / mixed["sec_per_iter"], "max_abs_dSigma_vs_plain": dsig,
                              "max_abs_dG_vs_plain": dg, "meets_1e-8": dsig <= 1e-8,
+                             "max_abs_Sigma_plain": float(np.abs(plain["sigma"]).max()),
+                             "max_abs_G_plain": float(np.abs(plain["G1"]).max()),
                              "phase_ms": list(map(float, mixed["phase_ms"][:4]))}
+        # the FP32 path against the FP32 matrix-core roofline
+        mp0, msig = float(mixed["phase_ms"][0]), float(mixed["phase_ms"][2])
+        mt = {"P0": mixed["flop_p0"] / (mp0 * 1e-3) / 1e12, "Sigma": mixed["flop_sig"] / (msig * 1e-3) / 1e12}
+        out["mixed_fp32"]["kernel_tflops"] = mt
+        out["mixed_fp32"]["p0_sigma_tflops"] = flops_iter / (1e-3 * (mp0 + msig)) / 1e12
+        out["mixed_fp32"]["frac_fp32_peak"] = {k: v / PEAK["fp32"] for k, v in mt.items()}
+        out["mixed_fp32"]["frac_measured_fp32_peak"] = {k: v / peaks["fp32_mfma_tflops"] for k, v in mt.items()}
     if not args.no_serving:
         out["serving"] = serving_run(args.serving_requests, args.serving_rate, not args.no_mixed)
     if not args.no_cpu:

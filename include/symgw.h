@@ -121,6 +121,15 @@ int symgw_layout_footprint(const symgw_layout_desc *desc, int precision, int64_t
  * makes before sizing its streams. */
 int symgw_device_memory(int device, int64_t *free_bytes, int64_t *total_bytes);
 
+/* Measured device peaks, for the roofline next to the datasheet values (no
+ * counterpart in the reference): out[0] FP64 and out[1] FP32 matrix-core
+ * TFLOP/s of the contraction kernels' complex 16x16x4 step (four real MFMAs,
+ * operands in registers, 12 waves per workgroup, 4 workgroups per CU), out[2]
+ * HBM GB/s of a 1 GiB device-to-device copy (bytes read + bytes written;
+ * a copy reaches about two thirds of the 8 TB/s datasheet figure).
+ * Allocates ~2 GiB for the duration of the call. */
+int symgw_measure_peaks(int device, double *out);
+
 /* Offset of the flat V slice of ordered pair (k1, k2) in complex elements, or
  * -1 when the pair is not stored (symmetry_utils.h:114-123). */
 int64_t symgw_kpair_offset(const symgw_ctx *ctx, int k1, int k2);

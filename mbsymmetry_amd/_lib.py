@@ -46,6 +46,7 @@ SIGNATURES = {
     "symgw_kpair_offset": (C.c_int64, [C.c_void_p, C.c_int, C.c_int]),
     "symgw_layout_footprint": (C.c_int, [C.POINTER(LayoutDesc), C.c_int, C.POINTER(C.c_int64)]),
     "symgw_device_memory": (C.c_int, [C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "symgw_measure_peaks": (C.c_int, [C.c_int, C.POINTER(C.c_double)]),
     "symgw_create": (C.c_int, [C.POINTER(LayoutDesc), C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
     "symgw_destroy": (C.c_int, [C.c_void_p]),
     "symgw_last_error": (C.c_char_p, [C.c_void_p]),
@@ -258,6 +259,17 @@ def device_memory(device=0):
     if rc != SYMGW_OK:
         raise SymGWError(f"symgw_device_memory failed: {lib.symgw_create_error().decode()}")
     return int(fr.value), int(tot.value)
+
+
+def measure_peaks(device=0):
+    """Measured peaks of `device`: FP64 and FP32 matrix-core TFLOP/s of the
+    contraction kernels' complex MFMA step, and HBM copy GB/s."""
+    lib = load()
+    out = (C.c_double * 3)()
+    rc = lib.symgw_measure_peaks(device, out)
+    if rc != SYMGW_OK:
+        raise SymGWError(f"symgw_measure_peaks failed: {lib.symgw_create_error().decode()}")
+    return {"fp64_mfma_tflops": out[0], "fp32_mfma_tflops": out[1], "hbm_copy_gbps": out[2]}
 
 
 def generate_V_pair_host(system, seed, scale, decay, k1, k2, n_complex):

@@ -10,7 +10,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libsymgw.so")
-SOURCES = ["kernels.hip", "contract.hip", "capi.cpp", "layout.cpp"]
+SOURCES = ["kernels.hip", "contract.hip", "peaks.hip", "capi.cpp", "layout.cpp"]
 
 
 def hipcc() -> str:

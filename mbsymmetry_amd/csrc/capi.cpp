@@ -311,6 +311,22 @@ void drop_graphs(symgw_ctx *c) {
 
 }  // namespace
 
+namespace {
+template <typename T>
+int time_mfma_peak(hipStream_t st, hipEvent_t e0, hipEvent_t e1, void *out, int n_wg, int iters, double *tflops) {
+  if (launch_mfma_peak<T>(out, n_wg, 16, st) != hipSuccess) return SYMGW_ERR_HIP;  // warm-up
+  if (hipEventRecord(e0, st) != hipSuccess || launch_mfma_peak<T>(out, n_wg, iters, st) != hipSuccess ||
+      hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess)
+    return SYMGW_ERR_HIP;
+  float ms = 0;
+  if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess || ms <= 0) return SYMGW_ERR_HIP;
+  // waves x iterations x 2 tiles x 4 MFMAs x 2*16*16*4 flops
+  const double flops = (double)n_wg * (mfma_peak_threads() / 64) * iters * 2 * 4 * 2048.0;
+  *tflops = flops / (ms * 1e-3) / 1e12;
+  return SYMGW_OK;
+}
+}  // namespace
+
 extern "C" {
 
 const char *symgw_create_error(void) { return g_create_error.c_str(); }
@@ -383,6 +399,83 @@ int symgw_device_memory(int device, int64_t *free_bytes, int64_t *total_bytes) {
   *free_bytes = (int64_t)fr;
   *total_bytes = (int64_t)tot;
   return SYMGW_OK;
+}
+
+int symgw_measure_peaks(int device, double *out) {
+  if (!out) return SYMGW_ERR_ARG;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+    (void)hipGetLastError();
+    g_create_error = "no HIP device";
+    return SYMGW_ERR_NODEVICE;
+  }
+  if (device < 0 || device >= ndev) {
+    g_create_error = "device index out of range";
+    return SYMGW_ERR_ARG;
+  }
+  hipDeviceProp_t prop;
+  if (hipSetDevice(device) != hipSuccess || hipGetDeviceProperties(&prop, device) != hipSuccess) {
+    g_create_error = "device query failed";
+    return SYMGW_ERR_HIP;
+  }
+  const int n_wg = 4 * prop.multiProcessorCount;
+  size_t fr = 0, tot = 0;
+  if (hipMemGetInfo(&fr, &tot) != hipSuccess) {
+    g_create_error = "hipMemGetInfo failed";
+    return SYMGW_ERR_HIP;
+  }
+  // two copy buffers of up to 1 GiB each, at most a quarter of the free memory
+  const int64_t cbytes = std::min<int64_t>(int64_t(1) << 30, (int64_t)(fr / 4)) & ~int64_t(4095);
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  void *res = nullptr, *src = nullptr, *dst = nullptr;
+  int r = SYMGW_OK;
+  std::string what;
+  if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+      hipEventCreate(&e1) != hipSuccess) {
+    r = SYMGW_ERR_HIP;
+    what = "stream/event creation failed";
+  }
+  if (!r && (dev_malloc(&res, (size_t)n_wg * mfma_peak_threads() * sizeof(double)) != hipSuccess ||
+             dev_malloc(&src, (size_t)cbytes) != hipSuccess || dev_malloc(&dst, (size_t)cbytes) != hipSuccess)) {
+    r = SYMGW_ERR_NOMEM;
+    what = "peak benchmark buffers";
+  }
+  // best of three runs of each (the first ones also bring the clocks up)
+  out[0] = out[1] = out[2] = 0.0;
+  for (int rep = 0; rep < 3 && !r; ++rep) {
+    double t64 = 0, t32 = 0;
+    if ((r = time_mfma_peak<double>(st, e0, e1, res, n_wg, 4096, &t64)) != SYMGW_OK) what = "FP64 MFMA peak";
+    if (!r && (r = time_mfma_peak<float>(st, e0, e1, res, n_wg, 8192, &t32)) != SYMGW_OK) what = "FP32 MFMA peak";
+    out[0] = std::max(out[0], t64);
+    out[1] = std::max(out[1], t32);
+  }
+  if (!r && hipMemsetAsync(src, 0, (size_t)cbytes, st) != hipSuccess) r = SYMGW_ERR_HIP;
+  for (int rep = 0; rep < 3 && !r; ++rep) {
+    const int reps = 20, cwg = 64 * prop.multiProcessorCount;
+    float ms = 0;
+    if (launch_hbm_copy(src, dst, cbytes, cwg, st) != hipSuccess || hipEventRecord(e0, st) != hipSuccess)
+      r = SYMGW_ERR_HIP;
+    for (int i = 0; !r && i < reps; ++i)
+      if (launch_hbm_copy(src, dst, cbytes, cwg, st) != hipSuccess) r = SYMGW_ERR_HIP;
+    if (!r && (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+               hipEventElapsedTime(&ms, e0, e1) != hipSuccess || ms <= 0))
+      r = SYMGW_ERR_HIP;
+    if (r) what = "HBM copy";
+    else out[2] = std::max(out[2], 2.0 * (double)cbytes * reps / (ms * 1e-3) / 1e9);  // bytes read + written
+  }
+  if (st) (void)hipStreamSynchronize(st);
+  if (res) dev_free(res);
+  if (src) dev_free(src);
+  if (dst) dev_free(dst);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (st) (void)hipStreamDestroy(st);
+  if (r) {
+    (void)hipGetLastError();
+    g_create_error = what + " failed";
+  }
+  return r;
 }
 
 int symgw_create(const symgw_layout_desc *desc, int device, int precision, symgw_ctx **out) {

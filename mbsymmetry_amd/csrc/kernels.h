@@ -35,6 +35,11 @@ template <typename T>
 hipError_t launch_sigma(const SigItem *items, const SigBlk *blks, int n_items, int nts, int ns, const cplx<T> *V,
                         const cplx<T> *G, const cplx<T> *Pf, cplx<double> *Sig, double pref, size_t lds,
                         hipStream_t st);
+// peaks.hip: device peak microbenchmarks
+template <typename T>
+hipError_t launch_mfma_peak(void *out, int n_wg, int iters, hipStream_t st);
+int mfma_peak_threads();
+hipError_t launch_hbm_copy(const void *src, void *dst, int64_t bytes, int n_wg, hipStream_t st);
 template <typename TO>
 hipError_t launch_rotate(const RotBlk *rb, int n_rb, int nts_ns, const cplx<double> *src, const int64_t *src_ts_stride,
                          const int64_t *src_slice, const cplx<double> *rep, cplx<TO> *dst, const int64_t *dst_slice,

@@ -313,3 +313,14 @@ def test_large_blocks_parity(ao, aux):
     Sref = orc.Sigma(V, Gf, orc.rotate_P(Pref), nthreads=nth)
     assert _rel(Sg, Sref) < 1e-10
     ctx.close()
+
+
+def test_measured_peaks_are_sane():
+    """The peak microbenchmarks run and land near the MI355X datasheet: FP64
+    matrix 78.6 TFLOP/s, FP32 157.3, HBM 8 TB/s (a copy reaches about two
+    thirds of the datasheet)."""
+    _say("measure peaks")
+    pk = _lib.measure_peaks(0)
+    assert 0.6 * 78.6 < pk["fp64_mfma_tflops"] < 1.02 * 78.6, pk
+    assert 0.6 * 157.3 < pk["fp32_mfma_tflops"] < 1.02 * 157.3, pk
+    assert 3000 < pk["hbm_copy_gbps"] < 8200, pk
