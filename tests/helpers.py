@@ -121,3 +121,16 @@ def noninteracting_G_ibz(sysm, mu=0.0):
                 G[:, s, koffi[ikr] + o:koffi[ikr] + o + b * b] = Gt.reshape(len(tau), b * b)
                 eig[(s, ikr, o)] = (e, U)
     return G, eig
+
+
+def sigma_hermiticity_error(sysm, Sig):
+    """max over (t, s, iBZ k, ao block) of |Sigma_b - Sigma_b^H|, relative to
+    max |Sigma|; Sig in ts_kij layout."""
+    Sig = np.asarray(Sig).reshape(sysm.nts, sysm.ns, sysm.flat_ao)
+    err, offs = 0.0, sysm.kao_off_irre()
+    for ik, k in enumerate(sysm.irre_list):
+        k = int(k)
+        for o, b in zip(sysm.ao_block_offsets(k), sysm.ao_sizes[k]):
+            blk = Sig[:, :, offs[ik] + o:offs[ik] + o + b * b].reshape(sysm.nts, sysm.ns, b, b)
+            err = max(err, float(np.abs(blk - blk.conj().swapaxes(-1, -2)).max()))
+    return err / max(1e-300, float(np.abs(Sig).max()))

@@ -8,6 +8,7 @@ import pytest
 
 from mbsymmetry_amd import _lib
 from mbsymmetry_amd import system as S
+from tests import helpers as H
 from oracle.oracle import Oracle
 
 pytestmark = pytest.mark.gpu
@@ -324,3 +325,38 @@ def test_measured_peaks_are_sane():
     assert 0.6 * 78.6 < pk["fp64_mfma_tflops"] < 1.02 * 78.6, pk
     assert 0.6 * 157.3 < pk["fp32_mfma_tflops"] < 1.02 * 157.3, pk
     assert 3000 < pk["hbm_copy_gbps"] < 8200, pk
+
+
+@pytest.mark.parametrize("name,tol", [("tr_square", 1e-12), ("config1", 1e-9)])
+def test_device_sigma_blocks_hermitian(name, tol, systems):
+    """The device Sigma of a physical iteration (G from Dyson, P from the W
+    solve) is Hermitian per ao block, to the Hermiticity of its G0 (config 1's
+    Chebyshev transforms hold G0 to ~1e-11)."""
+    sysm = systems[name]
+    _say("sigma hermitian", name)
+    _, ctx, G0 = _setup(sysm, mu=0.0)
+    ctx.gw_iteration(0.0, use_graph=True)
+    ctx.sync()
+    Sig = ctx.copy_Sigma_to_host(sysm.zeros_ts_kij())
+    ctx.close()
+    assert H.sigma_hermiticity_error(sysm, Sig) < tol, name
+
+
+def test_density_is_monotonic_in_mu():
+    """N(mu) of the device Dyson solve rises monotonically with mu (the
+    mu-search's bracketing relies on it), with the Sigma of one GW iteration
+    of a weakly interacting config-1 system held fixed."""
+    from mbsymmetry_amd.transforms import chebyshev
+    sysm = S.synthetic("config1-weak", 2, 4, [8, 6, 6, 4, 2], [40, 32, 32, 24], 1, chebyshev(60, 50.0),
+                       S.SEED_BASE + 1, v_scale=0.05, all_pairs=True)
+    _say("N(mu) monotonic", "config1-weak")
+    _, ctx, _ = _setup(sysm, mu=0.0)
+    ctx.gw_iteration(0.0, use_graph=True)
+    ctx.sync()
+    ns = []
+    for mu in np.linspace(-1.5, 1.5, 13):
+        ctx.dyson(float(mu), 1.0 / sysm.nk)
+        ns.append(ctx.density(sysm.kweights))
+    ctx.close()
+    assert all(b > a for a, b in zip(ns, ns[1:])), ns
+    assert 0.0 <= ns[0] < ns[-1] <= 2.0 * sum(sysm.ao_sizes[int(sysm.irre_list[0])]) + 1e-9

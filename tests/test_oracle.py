@@ -359,3 +359,17 @@ def test_oracle_matches_golden(name, request):
         tol = 1e-11 * ref["norm"]
         assert abs(got["norm"] - ref["norm"]) < tol, (name, k)
         assert np.abs(np.array(got["proj"]) - np.array(ref["proj"])).max() < tol * 10, (name, k)
+
+
+@pytest.mark.parametrize("name", ["tiny_rot", "tiny_tr"])
+def test_sigma_blocks_hermitian(name, request):
+    """With G(t) from a Dyson solve and P(t) from the W solve (both Hermitian
+    per block), Sigma(t) = -sum_{QP} V_Q G P_QP V_P^H is Hermitian per ao block
+    (the hermiticity pin of the Sigma contraction, gw_block.cpp:86-158)."""
+    sysm = request.getfixturevalue(name)
+    orc = Oracle(sysm)
+    G0 = orc.dyson(sysm.zeros_ts_kij(), 0.0)
+    assert H.sigma_hermiticity_error(sysm, G0) < 1e-13
+    _, _, Sig, _ = orc.gw_iteration(G0, 0.0)
+    assert np.abs(Sig).max() > 0
+    assert H.sigma_hermiticity_error(sysm, Sig) < 1e-12
