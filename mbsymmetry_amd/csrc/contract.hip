@@ -233,8 +233,8 @@ __device__ __forceinline__ void mma_multi(Tile<T> (&acc)[N], int n, int kb, int 
 }
 
 // All persistent tiles of a wave, in groups of Group<T>::n.
-template <bool CB, typename T, typename LA, typename LB>
-__device__ __forceinline__ void mma_rowshare_all(Acc<T> &acc, int n, int kb, int ke, bool arow, const LA &la,
+template <bool CB, typename T, int NA, typename LA, typename LB>
+__device__ __forceinline__ void mma_rowshare_all(Tile<T> (&acc)[NA], int n, int kb, int ke, bool arow, const LA &la,
                                                  const LB &lb) {
   constexpr int NG = Group<T>::n, MT = Cfg<T>::MT;
   mma_rowshare<0, NG < MT ? NG : MT, CB>(acc, n, kb, ke, arow, la, lb);
@@ -428,7 +428,7 @@ __device__ __forceinline__ int gemm_jobs_cs(int M, int N, int K, int first, cons
 
 // ---------------------------------------------------------------------------
 // P0.  Slab = rows [Q0, Q0 + nband) (nband <= 16) of aux block iq of iBZ point
-// q, for kP0Times consecutive times t < nts/2.  For every V block (k, ik1, ik2)
+// q, for p0_times consecutive times t < nts/2.  For every V block (k, ik1, ik2)
 // and spin:
 //   E_Q = V_Q G^{k+q}(t),  C_Q = G^k(beta - t) E_Q,  P0[Q][P] += sum_x C_Q[x] conj(V_P[x])
 // i.e. the three GEMMs of cugw_qkpt_block.cu:231-260 (X1 = G V, X2 = conj(V) G^T,
@@ -445,14 +445,15 @@ __global__ __launch_bounds__(Cfg<T>::NT) void p0_mfma_kernel(const P0Item *__res
                                                     int64_t n_work, int nt_half, int nts, int ns,
                                                     const cplx<T> *__restrict__ V, const cplx<T> *__restrict__ G,
                                                     cplx<double> *__restrict__ P0, double pref) {
-  constexpr int TB = kP0Times;
+  constexpr int TB = p0_times((int)sizeof(cplx<T>));
   const int64_t w = xcd_logical(blockIdx.x, gridDim.x);
   if (w >= n_work) return;
   const int np = (nt_half + TB - 1) / TB;
   const int t0 = (int)(w % np) * TB, ntimes = min(TB, nt_half - t0);
   const P0Item it = items[w / np];
   constexpr int NW = Cfg<T>::NW, NT = Cfg<T>::NT, MT = Cfg<T>::MT;
-  static_assert(TB <= MT, "the stacked times need one accumulator tile each");
+  // accumulators: one per stacked time, at least the MT column tiles of the unstacked passes
+  constexpr int MTP = TB > MT ? TB : MT;
   const int tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cplx<T> *lds = reinterpret_cast<cplx<T> *>(smem_raw);
@@ -468,9 +469,9 @@ __global__ __launch_bounds__(Cfg<T>::NT) void p0_mfma_kernel(const P0Item *__res
 
   for (int pass = 0; pass < npass; ++pass) {
     const int ta = t0 + pass;  // first time of this pass
-    Acc<T> acc;
+    Tile<T> acc[MTP];
 #pragma unroll
-    for (int c = 0; c < MT; ++c) tile_zero(acc[c]);
+    for (int c = 0; c < MTP; ++c) tile_zero(acc[c]);
     for (int b = it.blk_begin; b < it.blk_end; ++b) {
       const P0Blk B = blks[b];
       const int k1 = B.k1, k2 = B.k2, K = k1 * k2, nsub = B.nsub, lde = B.lde;
@@ -561,7 +562,7 @@ __global__ __launch_bounds__(Cfg<T>::NT) void p0_mfma_kernel(const P0Item *__res
     if (S > 1) split_reduce(acc, nmine, S, reinterpret_cast<T *>(lds));
     if (kp == 0) {
 #pragma unroll
-      for (int c = 0; c < MT; ++c) {
+      for (int c = 0; c < MTP; ++c) {
         if (c < nmine) {
           const int t = stack ? ta + c : ta, jt = stack ? tg : tg + TG * c;
           cplx<double> *out = P0 + it.out_base + (int64_t)t * it.out_tstride;
@@ -801,7 +802,8 @@ template <typename T>
 hipError_t launch_p0(const P0Item *items, const P0Blk *blks, int n_items, int nts, int ns, const cplx<T> *V,
                      const cplx<T> *G, cplx<double> *P0, double pref, size_t lds, hipStream_t st) {
   const int nt_half = nts / 2;
-  const int64_t n_work = (int64_t)n_items * ((nt_half + kP0Times - 1) / kP0Times);
+  constexpr int TB = p0_times((int)sizeof(cplx<T>));
+  const int64_t n_work = (int64_t)n_items * ((nt_half + TB - 1) / TB);
   if (n_work == 0) return hipSuccess;
   (void)hipFuncSetAttribute((const void *)p0_mfma_kernel<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   hipLaunchKernelGGL((p0_mfma_kernel<T>), dim3((unsigned)xcd_grid(n_work)), dim3(Cfg<T>::NT), lds, st, items, blks,

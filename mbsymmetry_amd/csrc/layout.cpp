@@ -335,11 +335,11 @@ bool build_worklists(const Layout &L, int eb, WorkLists &W, std::string &err) {
           // E and C of one chunk ([times][band][k1*nsub] each, +1 element of
           // row padding) share the LDS budget
           const int band = balanced(qs, kP0Band);
-          const int cap = std::min(B.k2, (budget / (2 * kP0Times * band) - 1) / B.k1);
+          const int cap = std::min(B.k2, (budget / (2 * p0_times(eb) * band) - 1) / B.k1);
           if (cap < 1) { err = fmt("P0 block too large for the LDS plan (k1, k2)", B.k1, B.k2); return false; }
           B.nsub = balanced(B.k2, cap);
           B.lde = odd_ld(B.k1 * B.nsub);
-          p0_lds = std::max(p0_lds, (size_t)(2 * kP0Times * band * B.lde) * eb);
+          p0_lds = std::max(p0_lds, (size_t)(2 * p0_times(eb) * band * B.lde) * eb);
           W.p0_blks.push_back(B);
         }
       }
@@ -359,7 +359,9 @@ bool build_worklists(const Layout &L, int eb, WorkLists &W, std::string &err) {
   // the split-K epilogue passes partial tiles through LDS: one slot per wave of
   // kContractTiles tiles x 64 lanes x 8 reals
   const size_t red_bytes = (size_t)contract_waves(eb) * kContractTiles * 64 * 8 * (eb / 2);
-  W.p0_lds = std::max(p0_lds, red_bytes);
+  // P0 keeps one accumulator per stacked time (at least kContractTiles)
+  const size_t red_p0 = (size_t)contract_waves(eb) * std::max(kContractTiles, p0_times(eb)) * 64 * 8 * (eb / 2);
+  W.p0_lds = std::max(p0_lds, red_p0);
 
   // ---- Sigma: items per (k_ir, ik1, row band, column band); blocks in (q, V-block) order ----
   size_t sig_lds = 0;

@@ -55,8 +55,9 @@ constexpr int kContractTiles = 2;
 // Most output rows of one P0 slab (one MFMA tile row); the rows of an aux
 // block are split into balanced bands.
 constexpr int kP0Band = 16;
-// Consecutive times of one P0 slab (they share every V fragment).
-constexpr int kP0Times = 2;
+// Consecutive times of one P0 slab (they share every V fragment): four with
+// complex64 operands, two with complex128 (four spill registers there).
+constexpr int p0_times(int elem_bytes) { return elem_bytes == 8 ? 4 : 2; }
 // Largest Sigma slab edge: 3 x 3 tiles of 16 per time, so the kSigTimes
 // slabs of a workgroup (18 tiles) fit its 12 waves x 2 accumulator tiles.
 constexpr int kSigBand = 48;
