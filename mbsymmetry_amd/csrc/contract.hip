@@ -27,7 +27,7 @@ struct Cfg {
   static constexpr int NW = contract_waves(2 * (int)sizeof(T));
   static constexpr int NT = 64 * NW;
   // accumulator tiles a wave keeps across the whole slab
-  static constexpr int MT = kContractTiles;
+  static constexpr int MT = contract_tiles((int)sizeof(cplx<T>));
 };
 
 template <typename T>
@@ -584,7 +584,7 @@ __global__ __launch_bounds__(Cfg<T>::NT) void p0_mfma_kernel(const P0Item *__res
 
 // ---------------------------------------------------------------------------
 // Sigma.  Slab = rows [I0, I0+Ib) x columns [J0, J0+Jb) of ao block ik1 of iBZ
-// point k, spin s, for kSigTimes consecutive times t0 + tb.  For every V block
+// point k, spin s, for sig_times consecutive times t0 + tb.  For every V block
 // (q, iq, ik2), in chunks of qsub aux rows Q by nsub orbitals n (E and Z stored
 // with the (Q, n) reduction index contiguous):
 //   Z_Q(t) = sum_P conj(P_{QP}(t)) V_P,  E_Q(t) = V_Q G^{k-q}(t),
@@ -601,7 +601,7 @@ __global__ __launch_bounds__(Cfg<T>::NT) void sigma_mfma_kernel(const SigItem *_
                                                        int ns, const cplx<T> *__restrict__ V,
                                                        const cplx<T> *__restrict__ G, const cplx<T> *__restrict__ Pf,
                                                        cplx<double> *__restrict__ Sig, double pref) {
-  constexpr int TB = kSigTimes;
+  constexpr int TB = sig_times((int)sizeof(cplx<T>));
   const int64_t w = xcd_logical(blockIdx.x, gridDim.x);
   if (w >= n_work) return;
   const int np = (nts + TB - 1) / TB, per = np * ns;
@@ -622,6 +622,7 @@ __global__ __launch_bounds__(Cfg<T>::NT) void sigma_mfma_kernel(const SigItem *_
 #pragma unroll
   for (int c = 0; c < MT; ++c) tile_zero(acc[c]);
 
+  int zend_prev = 0;  // end of the previous block's Z and E buffers (elements)
   for (int b = it.blk_begin; b < it.blk_end; ++b) {
     const SigBlk B = blks[b];
     const int q = B.q, k2 = B.k2, K = k1 * k2, qsub = B.qsub, nsub = B.nsub, ldz = B.ldz, lde = B.lde;
@@ -633,6 +634,12 @@ __global__ __launch_bounds__(Cfg<T>::NT) void sigma_mfma_kernel(const SigItem *_
     cplx<T> *bufZ = lds;                      // [TB][Jb][ldz]  Z[r][j][n] at j*ldz + y
     cplx<T> *bufE = bufZ + TB * Jb * ldz;     // [TB][Ib][lde]  E[r][i][n] at i*lde + y
     cplx<T> *bufG = bufE + TB * Ib * lde;     // [k2][tbn * nn]  G2_tb[m][n0 + l] at m * tbn * nn + tb * nn + l
+    // The G chunk is staged before the chunk's first barrier, while slower
+    // waves may still run the previous block's final product on its Z and E
+    // buffers: where this block's G buffer starts below their end, wait.
+    const int zend = TB * Jb * ldz + TB * Ib * lde;
+    if (zend < zend_prev) __syncthreads();
+    zend_prev = zend;
     const cplx<T> *Vj = Vb + J0 * k2;
     const FDiv dIb(Ib);
     for (int Q0 = 0; Q0 < q; Q0 += qsub) {
@@ -816,7 +823,8 @@ template <typename T>
 hipError_t launch_sigma(const SigItem *items, const SigBlk *blks, int n_items, int nts, int ns, const cplx<T> *V,
                         const cplx<T> *G, const cplx<T> *Pf, cplx<double> *Sig, double pref, size_t lds,
                         hipStream_t st) {
-  const int64_t n_work = (int64_t)n_items * ((nts + kSigTimes - 1) / kSigTimes) * ns;
+  constexpr int TB = sig_times((int)sizeof(cplx<T>));
+  const int64_t n_work = (int64_t)n_items * ((nts + TB - 1) / TB) * ns;
   if (n_work == 0) return hipSuccess;
   (void)hipFuncSetAttribute((const void *)sigma_mfma_kernel<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)lds);

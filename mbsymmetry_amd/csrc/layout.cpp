@@ -255,8 +255,8 @@ int64_t pad_to(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
 // Among the shapes that fit, pick the one with the fewest executed MFMA
 // FLOPs (16x16x4 tiles, padding included) plus a fixed cost per chunk for
 // its barriers and staging.
-bool sig_chunks(int q, int k1, int k2, int band, int budget, std::map<std::array<int, 4>, std::array<int, 2>> &memo,
-                int &qsub, int &nsub) {
+bool sig_chunks(int q, int k1, int k2, int band, int budget, int TB,
+                std::map<std::array<int, 4>, std::array<int, 2>> &memo, int &qsub, int &nsub) {
   const std::array<int, 4> key{q, k1, k2, band};
   auto f = memo.find(key);
   if (f != memo.end()) {
@@ -264,7 +264,6 @@ bool sig_chunks(int q, int k1, int k2, int band, int budget, std::map<std::array
     nsub = f->second[1];
     return true;
   }
-  const int TB = kSigTimes;
   const double chunk_cost = 2.5e5;  // ~2000 cycles of a CU's FP64 MFMA rate
   double best = -1;
   for (int qs = 1; qs <= std::min(q, kSigRows); ++qs) {
@@ -313,7 +312,7 @@ bool build_worklists(const Layout &L, int eb, WorkLists &W, std::string &err) {
     for (size_t iq = 0; iq < L.aux_sizes[q].size(); ++iq) {
       int qs = L.aux_sizes[q][iq];
       if (qs > 256) { err = fmt("aux block larger than 256", qs); return false; }
-      if ((qs + 15) / 16 > contract_waves(eb) * kContractTiles) {
+      if ((qs + 15) / 16 > contract_waves(eb) * contract_tiles(eb)) {
         err = fmt("P0 slab has more output tiles than the workgroup holds", qs);
         return false;
       }
@@ -357,10 +356,10 @@ bool build_worklists(const Layout &L, int eb, WorkLists &W, std::string &err) {
     }
   }
   // the split-K epilogue passes partial tiles through LDS: one slot per wave of
-  // kContractTiles tiles x 64 lanes x 8 reals
-  const size_t red_bytes = (size_t)contract_waves(eb) * kContractTiles * 64 * 8 * (eb / 2);
-  // P0 keeps one accumulator per stacked time (at least kContractTiles)
-  const size_t red_p0 = (size_t)contract_waves(eb) * std::max(kContractTiles, p0_times(eb)) * 64 * 8 * (eb / 2);
+  // contract_tiles tiles x 64 lanes x 8 reals
+  const size_t red_bytes = (size_t)contract_waves(eb) * contract_tiles(eb) * 64 * 8 * (eb / 2);
+  // P0 keeps one accumulator per stacked time (at least contract_tiles)
+  const size_t red_p0 = (size_t)contract_waves(eb) * std::max(contract_tiles(eb), p0_times(eb)) * 64 * 8 * (eb / 2);
   W.p0_lds = std::max(p0_lds, red_p0);
 
   // ---- Sigma: items per (k_ir, ik1, row band, column band); blocks in (q, V-block) order ----
@@ -386,8 +385,8 @@ bool build_worklists(const Layout &L, int eb, WorkLists &W, std::string &err) {
           B.g2_tstride = (int32_t)(ns * L.kao_size[k2i]); B.g2_sstride = (int32_t)L.kao_size[k2i];
           B.p_base = L.qaux_off[q] * nts + L.aux_offsets[q][t[0]];
           B.p_tstride = (int32_t)L.qaux_size[q];
-          const int TB = kSigTimes;
-          if (!sig_chunks(B.q, k1, B.k2, band, budget, sig_plans, B.qsub, B.nsub)) {
+          const int TB = sig_times(eb);
+          if (!sig_chunks(B.q, k1, B.k2, band, budget, TB, sig_plans, B.qsub, B.nsub)) {
             err = fmt("Sigma block too large for the LDS plan (q, k2)", B.q, B.k2);
             return false;
           }
@@ -398,9 +397,15 @@ bool build_worklists(const Layout &L, int eb, WorkLists &W, std::string &err) {
         }
       }
       int b1 = (int)W.sig_blks.size();
+      // blocks in ascending LDS row stride, so that the Z and E buffers never
+      // shrink from one block to the next and the kernel's block-boundary
+      // wait (the next G chunk must not land on buffers still being read) is
+      // never needed; the order is the same for every slab of the item
+      std::stable_sort(W.sig_blks.begin() + b0, W.sig_blks.begin() + b1,
+                       [](const SigBlk &x, const SigBlk &y) { return x.ldz + x.lde < y.ldz + y.lde; });
       {
         const int nt = (band + 15) / 16;
-        if (kSigTimes * nt * nt > contract_waves(eb) * kContractTiles) {
+        if (sig_times(eb) * nt * nt > contract_waves(eb) * contract_tiles(eb)) {
           err = fmt("Sigma slab has more output tiles than the workgroup holds", band);
           return false;
         }

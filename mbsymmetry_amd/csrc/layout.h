@@ -50,22 +50,26 @@ bool build_layout(const symgw_layout_desc *d, Layout &L, std::string &err);
 // 16-row MFMA tiles.  Same shape for both element sizes.
 constexpr int contract_lds(int elem_bytes) { return elem_bytes > 0 ? 160 * 1024 : 0; }
 constexpr int contract_waves(int elem_bytes) { return elem_bytes > 0 ? 12 : 0; }
-// Accumulator tiles each contraction wave keeps across a slab.
-constexpr int kContractTiles = 2;
+// Accumulator tiles each contraction wave keeps across a slab: three with
+// complex64 operands, two with complex128 (the register file of three waves
+// per SIMD holds no more).
+constexpr int contract_tiles(int elem_bytes) { return elem_bytes == 8 ? 3 : 2; }
 // Most output rows of one P0 slab (one MFMA tile row); the rows of an aux
 // block are split into balanced bands.
 constexpr int kP0Band = 16;
 // Consecutive times of one P0 slab (they share every V fragment): four with
 // complex64 operands, two with complex128 (four spill registers there).
 constexpr int p0_times(int elem_bytes) { return elem_bytes == 8 ? 4 : 2; }
-// Largest Sigma slab edge: 3 x 3 tiles of 16 per time, so the kSigTimes
-// slabs of a workgroup (18 tiles) fit its 12 waves x 2 accumulator tiles.
+// Largest Sigma slab edge: 3 x 3 tiles of 16 per time, so the sig_times
+// slabs of a workgroup (18 tiles for complex128, 36 for complex64) fit its
+// 12 waves x contract_tiles accumulator tiles.
 constexpr int kSigBand = 48;
 // Most aux rows Q of one Sigma LDS chunk.
 constexpr int kSigRows = 64;
-// Consecutive times of one Sigma slab (they share every V fragment); the last
-// slab takes the remaining nts % kSigTimes times.
-constexpr int kSigTimes = 2;
+// Consecutive times of one Sigma slab (they share every V fragment): four with
+// complex64 operands, two with complex128; the last slab takes the remaining
+// nts % sig_times times.
+constexpr int sig_times(int elem_bytes) { return elem_bytes == 8 ? 4 : 2; }
 
 // One V-block contraction that feeds a P0 output slab (cugw_qkpt_block.cu:219-261).
 struct P0Blk {
