@@ -5,9 +5,10 @@
 // no atomics or locks (the reference serialises per-q / per-k AXPYs behind
 // spin locks, cugw_qkpt_block.cu:269-285 and :414-436).  All complex products
 // are 16x16 tiles built from four real MFMAs (v_mfma_f64_16x16x4f64 for the
-// plain path, v_mfma_f32_16x16x4f32 for the FP32 path); operands are staged in
-// LDS, intermediates never leave the workgroup.  A wave works on up to four
-// tiles at a time that share one operand fragment per k-step.
+// plain path, v_mfma_f32_16x16x4f32 for the FP32 path); operands stream from
+// memory or from LDS, intermediates never leave the workgroup's LDS.  A wave
+// works on up to four tiles at a time that share one operand fragment per
+// k-step.
 #include <hip/hip_runtime.h>
 
 #include "common.h"
@@ -61,8 +62,8 @@ template <typename T>
 struct Tile {
   typename Mfma<T>::v4 re, im;
 };
-// Tiles a wave advances together, sharing one operand fragment per k-step:
-// 2 for FP64 and 4 for FP32 (two waves per SIMD within the register file).
+// Tiles of one job, sharing one operand fragment per k-step: 1 for FP64 and 4
+// for FP32 (more spill registers at three waves per SIMD in FP64).
 template <typename T>
 struct Group { static constexpr int n = sizeof(T) == 8 ? 1 : 4; };
 template <typename T>
@@ -98,9 +99,9 @@ __device__ __forceinline__ void mma4(Tile<T> &acc, const cplx<T> &a, const cplx<
   }
 }
 
-// The k-loops below keep one k-step of operands in flight: the fragments of
-// step k+1 are loaded while the MFMAs of step k run, in two register sets
-// used alternately (no copies), and every load is unconditional.  Loaders
+// The k-loops below keep kAhead k-steps of operands in flight: the fragments
+// of step k+kAhead are loaded while the MFMAs of step k run, in a ring of
+// kAhead + 1 register sets (no copies), and every load is unconditional.  Loaders
 // therefore clamp their indices and return some element of their operand for
 // any row, column or k; garbage in rows or columns outside the product only
 // reaches accumulator rows or columns that are never stored, and the A
