@@ -316,6 +316,18 @@ bool build_worklists(const Layout &L, int eb, WorkLists &W, std::string &err) {
         err = fmt("P0 slab has more output tiles than the workgroup holds", qs);
         return false;
       }
+      // rows per slab: up to kP0Band, fewer where the widest ao block k1 of
+      // the slab's V blocks would not leave LDS for one orbital per chunk
+      // (E and C rows of k1 + 1 elements for every time and row)
+      int maxk1 = 1;
+      for (int k = 0; k < nk; ++k) {
+        const size_t p = (size_t)k * nk + L.kpq[(size_t)k * nk + q];
+        for (auto &t : L.V_irreps[p])
+          if (t[0] == (int)iq) maxk1 = std::max(maxk1, L.ao_sizes[k][t[1]]);
+      }
+      const int band_cap = std::min(kP0Band, budget / (2 * p0_times(eb) * (maxk1 + 1)));
+      if (band_cap < 1) { err = fmt("P0 ao block too large for the LDS plan (k1)", maxk1); return false; }
+      const int band = balanced(qs, band_cap);
       int b0 = (int)W.p0_blks.size();
       for (int k = 0; k < nk; ++k) {
         int k2 = L.kpq[(size_t)k * nk + q];
@@ -333,7 +345,6 @@ bool build_worklists(const Layout &L, int eb, WorkLists &W, std::string &err) {
           B.g2_tstride = (int32_t)(ns * L.kao_size[k2]); B.g2_sstride = (int32_t)L.kao_size[k2];
           // E and C of one chunk ([times][band][k1*nsub] each, +1 element of
           // row padding) share the LDS budget
-          const int band = balanced(qs, kP0Band);
           const int cap = std::min(B.k2, (budget / (2 * p0_times(eb) * band) - 1) / B.k1);
           if (cap < 1) { err = fmt("P0 block too large for the LDS plan (k1, k2)", B.k1, B.k2); return false; }
           B.nsub = balanced(B.k2, cap);
@@ -343,7 +354,6 @@ bool build_worklists(const Layout &L, int eb, WorkLists &W, std::string &err) {
         }
       }
       int b1 = (int)W.p0_blks.size();
-      const int band = balanced(qs, kP0Band);
       for (int Q0 = 0; Q0 < qs; Q0 += band) {
         P0Item it;
         it.blk_begin = b0; it.blk_end = b1;

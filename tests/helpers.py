@@ -134,3 +134,19 @@ def sigma_hermiticity_error(sysm, Sig):
             blk = Sig[:, :, offs[ik] + o:offs[ik] + o + b * b].reshape(sysm.nts, sysm.ns, b, b)
             err = max(err, float(np.abs(blk - blk.conj().swapaxes(-1, -2)).max()))
     return err / max(1e-300, float(np.abs(Sig).max()))
+
+
+def p0_hermiticity_error(sysm, P0):
+    """max over (iBZ q, t < nts/2, aux block) of |P0_b - P0_b^H|, relative to
+    max |P0|; P0 in q_t_QP layout."""
+    P0 = np.asarray(P0)
+    err, qoff = 0.0, sysm.qaux_off_irre()
+    for iq, q in enumerate(sysm.irre_list):
+        q = int(q)
+        qs = sysm.qslice(q)
+        for t in range(sysm.nts // 2):
+            base = sysm.nts * qoff[iq] + t * qs
+            for o, b in zip(sysm.aux_block_offsets(q), sysm.aux_sizes[q]):
+                blk = P0[base + o:base + o + b * b].reshape(b, b)
+                err = max(err, float(np.abs(blk - blk.conj().T).max()))
+    return err / max(1e-300, float(np.abs(P0).max()))
