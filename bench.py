@@ -140,6 +140,31 @@ def cpu_baseline(sysm, G0, budget_s=15.0):
     return dict(rate=fl / dt, seconds=dt, cores=nq, nkk=nkk, nq=nq, ntau=ntau, flops=fl)
 
 
+def uneven_run():
+    """Irrep blocks of every size from 1 to 256 in one system (the uneven-batching
+    stress case at a size that fits one test): P0 and Sigma kernel rates from
+    HIP events, after one untimed pass, in both precisions."""
+    from mbsymmetry_amd import _lib, system as S
+    from mbsymmetry_amd.transforms import chebyshev
+    ao, aux = [1, 3, 9, 27, 81, 256], [1, 4, 16, 64, 256]
+    sysm = S.synthetic("uneven-1-256", 2, 4, ao, aux, 1, chebyshev(20, 20.0), S.SEED_BASE + 3)
+    out = {"workload": f"2x2x2 k-mesh (4 iBZ points), ao blocks {ao}, aux blocks {aux}, nts {sysm.nts}"}
+    for prec, name in ((_lib.PREC_FP64, "fp64"), (_lib.PREC_FP32, "fp32")):
+        ctx = setup_ctx(sysm, prec, 0.0)
+        ctx.upload_G(sysm.zeros_ts_kij())
+        ctx.dyson(0.0, 0.0)
+        for timed in (False, True):
+            ctx.set_timing(timed)
+            ctx.solve_P0(); ctx.solve_W(); ctx.solve_Sigma()
+            ctx.sync()
+        lt, inf = ctx.last_times(), ctx.info()
+        ctx.close()
+        out[name] = {"P0_ms": float(lt[0]), "Sigma_ms": float(lt[2]), "P0_tflops": float(inf.flop_P0 / lt[0] / 1e9),
+                     "Sigma_tflops": float(inf.flop_Sigma / lt[2] / 1e9)}
+        log("uneven", name, out[name])
+    return out
+
+
 def serving_run(n_requests, rate, with_fp32):
     """The serving loop on a seeded request mix: Poisson arrivals, HBM-footprint
     admission, continuous batching over per-system contexts; latency from
@@ -177,6 +202,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mu", type=float, default=0.0)
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-uneven", action="store_true", help="skip the blocks-1-to-256 stress case")
     ap.add_argument("--serving-requests", type=int, default=32)
     ap.add_argument("--serving-rate", type=float, default=50.0, help="Poisson arrival rate (requests/s)")
     ap.add_argument("--copy-calib", action="store_true",
@@ -275,6 +301,8 @@ def main():
         out["mixed_fp32"]["p0_sigma_tflops"] = flops_iter / (1e-3 * (mp0 + msig)) / 1e12
         out["mixed_fp32"]["frac_fp32_peak"] = {k: v / PEAK["fp32"] for k, v in mt.items()}
         out["mixed_fp32"]["frac_measured_fp32_peak"] = {k: v / peaks["fp32_mfma_tflops"] for k, v in mt.items()}
+    if not args.no_uneven:
+        out["uneven_blocks"] = uneven_run()
     if not args.no_serving:
         out["serving"] = serving_run(args.serving_requests, args.serving_rate, not args.no_mixed)
     if not args.no_cpu:
