@@ -75,6 +75,29 @@ def self_consistent(ctx, system, G0: np.ndarray, mu: float, max_iter: int = 20, 
     return SCResult(G=G, mu=mu, iterations=it, converged=converged, history=hist, Sigma=Sigma)
 
 
+def self_consistent_cleanup(ctx_sp, ctx_dp, system, G0: np.ndarray, mu: float, max_iter: int = 20,
+                            tol: float = 1e-8, sp_tol: float = 1e-6, damp: float = 1.0,
+                            nel: Optional[float] = None, mu_bracket=(-2.0, 2.0), use_graph: bool = True,
+                            keep_sigma: bool = False) -> SCResult:
+    """Single precision with a cleanup in double precision (the reference's
+    single_prec = 2, SymmetryMBPT/src/common/params_t.cpp:76, switched per
+    solver by solver_block_t::single_prec_run, cugw_solver_block.h:70-77):
+    iterate on `ctx_sp` (complex64 contractions) until max|dG| < sp_tol, which
+    should sit above the FP32 noise floor (~1e-7 relative), then continue from
+    that G on `ctx_dp` (complex128) until max|dG| < tol.  Both contexts hold
+    the same system.  History entries carry the precision of their iteration."""
+    r1 = self_consistent(ctx_sp, system, G0, mu, max_iter, sp_tol, damp, nel, mu_bracket, use_graph)
+    for h in r1.history:
+        h["precision"] = "fp32"
+    r2 = self_consistent(ctx_dp, system, r1.G, r1.mu, max_iter - r1.iterations, tol, damp, nel, mu_bracket,
+                         use_graph, keep_sigma)
+    for h in r2.history:
+        h["precision"] = "fp64"
+        h["iteration"] += r1.iterations
+    return SCResult(G=r2.G, mu=r2.mu, iterations=r1.iterations + r2.iterations, converged=r2.converged,
+                    history=r1.history + r2.history, Sigma=r2.Sigma)
+
+
 def save_results(path: str, res: SCResult, system, params: Optional[dict] = None) -> None:
     """Write `res` to `path` (.npz).  Keys: "params/<name>" for the system
     sizes and run options (nel_cell, nao, nk, ink, NQ, ns, nts, beta, mu, tol,

@@ -404,3 +404,31 @@ def test_uneven_blocks_1_to_256():
     assert H.sigma_hermiticity_error(sysm, Sig) < 1e-10
     assert _rel(out[_lib.PREC_FP32][0], P0) < 1e-4
     assert _rel(out[_lib.PREC_FP32][1], Sig) < 1e-4
+
+
+def test_single_precision_with_double_cleanup():
+    """The reference's single_prec = 2 mode: iterations on the FP32 path until
+    the update reaches 1e-6, then FP64 iterations to 1e-11 converge to the same
+    fixed point as the FP64-only loop (to 1e-9), with fewer FP64 iterations."""
+    from mbsymmetry_amd.scloop import self_consistent, self_consistent_cleanup
+    from mbsymmetry_amd.transforms import chebyshev
+    sysm = S.synthetic("config1-weak", 2, 4, [8, 6, 6, 4, 2], [40, 32, 32, 24], 1, chebyshev(60, 50.0),
+                       S.SEED_BASE + 1, v_scale=0.05, all_pairs=True)
+    _say("sp + dp cleanup", "config1-weak")
+    ctxs = {}
+    for prec in (_lib.PREC_FP64, _lib.PREC_FP32):
+        c = _lib.Context(sysm, 0, prec)
+        c.generate_V(sysm.v_seed, sysm.v_scale, sysm.decay_table())
+        c.set_reps(sysm.orep, sysm.auxrep); c.set_transforms(sysm.transforms); c.set_fock(sysm.F, sysm.S)
+        ctxs[prec] = c
+    c64, c32 = ctxs[_lib.PREC_FP64], ctxs[_lib.PREC_FP32]
+    c64.upload_G(sysm.zeros_ts_kij()); c64.dyson(0.0, 0.0)
+    G0 = c64.download_G((sysm.nts, sysm.ns, sysm.flat_ao))
+    ref = self_consistent(c64, sysm, G0, 0.0, max_iter=20, tol=1e-11)
+    mix = self_consistent_cleanup(c32, c64, sysm, G0, 0.0, max_iter=20, tol=1e-11, sp_tol=1e-6)
+    c64.close(); c32.close()
+    assert ref.converged and mix.converged
+    n64 = sum(1 for h in mix.history if h["precision"] == "fp64")
+    n32 = sum(1 for h in mix.history if h["precision"] == "fp32")
+    assert n32 >= 1 and n64 < ref.iterations, (n32, n64, ref.iterations)
+    assert np.abs(mix.G - ref.G).max() < 1e-9
