@@ -40,9 +40,12 @@ class SCResult:
 
 def self_consistent(ctx, system, G0: np.ndarray, mu: float, max_iter: int = 20, tol: float = 1e-8,
                     damp: float = 1.0, nel: Optional[float] = None, mu_bracket=(-2.0, 2.0),
-                    use_graph: bool = True, keep_sigma: bool = False) -> SCResult:
+                    use_graph: bool = True, keep_sigma: bool = False,
+                    stall_ratio: Optional[float] = None) -> SCResult:
     """Iterate GW on `ctx` (V, rotations, transforms and Fock already set)
-    from G0 until max|G_{n+1} - G_n| < tol or max_iter iterations.  With
+    from G0 until max|G_{n+1} - G_n| < tol or max_iter iterations, or, with
+    stall_ratio, until an update shrinks by less than that factor (the loop
+    has reached its precision's noise floor; converged stays False).  With
     keep_sigma the last iteration's Sigma is copied back, divided by nk as the
     reference's caller does (cugw_solver_block.cpp:302-304)."""
     if not 0.0 < damp <= 1.0:
@@ -69,6 +72,8 @@ def self_consistent(ctx, system, G0: np.ndarray, mu: float, max_iter: int = 20, 
         if dG < tol:
             converged = True
             break
+        if stall_ratio is not None and len(hist) > 1 and dG > stall_ratio * hist[-2]["max_abs_dG"]:
+            break
     Sigma = None
     if keep_sigma:
         Sigma = ctx.copy_Sigma_to_host(system.zeros_ts_kij()) / system.nk
@@ -82,11 +87,14 @@ def self_consistent_cleanup(ctx_sp, ctx_dp, system, G0: np.ndarray, mu: float, m
     """Single precision with a cleanup in double precision (the reference's
     single_prec = 2, SymmetryMBPT/src/common/params_t.cpp:76, switched per
     solver by solver_block_t::single_prec_run, cugw_solver_block.h:70-77):
-    iterate on `ctx_sp` (complex64 contractions) until max|dG| < sp_tol, which
-    should sit above the FP32 noise floor (~1e-7 relative), then continue from
-    that G on `ctx_dp` (complex128) until max|dG| < tol.  Both contexts hold
-    the same system.  History entries carry the precision of their iteration."""
-    r1 = self_consistent(ctx_sp, system, G0, mu, max_iter, sp_tol, damp, nel, mu_bracket, use_graph)
+    iterate on `ctx_sp` (complex64 contractions) until max|dG| < sp_tol or
+    the updates stop shrinking by half (the FP32 noise floor), then continue
+    from that G on `ctx_dp` (complex128) until max|dG| < tol.  Both contexts
+    hold the same system.  History entries carry the precision of their
+    iteration.  It saves time only where the loop needs many iterations: the
+    FP64 phase still takes the iterations from the FP32 floor down to tol."""
+    r1 = self_consistent(ctx_sp, system, G0, mu, max_iter, sp_tol, damp, nel, mu_bracket, use_graph,
+                         stall_ratio=0.5)
     for h in r1.history:
         h["precision"] = "fp32"
     r2 = self_consistent(ctx_dp, system, r1.G, r1.mu, max_iter - r1.iterations, tol, damp, nel, mu_bracket,
