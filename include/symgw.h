@@ -141,8 +141,8 @@ int symgw_create(const symgw_layout_desc *desc, int device, int precision, symgw
 int symgw_destroy(symgw_ctx *ctx);
 const char *symgw_last_error(const symgw_ctx *ctx);
 /* Message of the last failure on the calling thread of symgw_create,
- * symgw_layout_query, symgw_layout_footprint or symgw_device_memory (no
- * context exists then). */
+ * symgw_layout_query, symgw_layout_footprint, symgw_device_memory or
+ * symgw_measure_peaks (no context exists then). */
 const char *symgw_create_error(void);
 int symgw_layout_info_ctx(const symgw_ctx *ctx, symgw_layout_info *info);
 
