@@ -165,19 +165,23 @@ def uneven_run():
     return out
 
 
-def serving_run(n_requests, rate, with_fp32):
+def _ints(text):
+    return [int(x) for x in text.split(",") if x]
+
+
+def serving_run(n_requests, rate, with_fp32, naos=(26, 52, 104), meshes=(2, 4)):
     """The serving loop on a seeded request mix: Poisson arrivals, HBM-footprint
     admission, continuous batching over per-system contexts; latency from
     arrival to the final G on the host."""
     from mbsymmetry_amd import _lib, serving as SV
     from mbsymmetry_amd import system as S
-    mix = S.serving_mix(n_requests)
+    mix = S.serving_mix(n_requests, meshes=tuple(meshes), naos=tuple(naos))
     arr = SV.poisson_arrivals(len(mix), rate, seed=S.SEED_BASE + 5)
     reqs = [SV.Request(i, s, float(arr[i]), iterations=it) for i, (s, it) in enumerate(mix)]
     flops = sum(it * (_lib.layout_query(s).flop_P0 + _lib.layout_query(s).flop_Sigma) for s, it in mix)
     out = {"requests": len(reqs), "arrival_rate_per_s": rate,
-           "mix": "synthetic systems on 2^3 / 4^3 meshes, nao in {26, 52, 104} (naux = 4 nao), nts 32, "
-                  "2 GW iterations each",
+           "mix": f"synthetic systems on {' / '.join(f'{m}^3' for m in meshes)} meshes, nao in "
+                  f"{{{', '.join(map(str, naos))}}} (naux = 4 nao), nts 32, 2 GW iterations each",
            "P0_Sigma_TFLOP": flops / 1e12}
     for name, prec in (("fp64", _lib.PREC_FP64), ("fp32", _lib.PREC_FP32)):
         if name == "fp32" and not with_fp32:
@@ -203,6 +207,9 @@ def main():
     ap.add_argument("--mu", type=float, default=0.0)
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-uneven", action="store_true", help="skip the blocks-1-to-256 stress case")
+    ap.add_argument("--serving-naos", default="26,52,104", help="nao values of the serving mix")
+    ap.add_argument("--serving-meshes", default="2,4", help="k-mesh edges of the serving mix")
+    ap.add_argument("--serving-only", action="store_true", help="run only the serving mix and print its JSON")
     ap.add_argument("--serving-requests", type=int, default=32)
     ap.add_argument("--serving-rate", type=float, default=50.0, help="Poisson arrival rate (requests/s)")
     ap.add_argument("--copy-calib", action="store_true",
@@ -224,6 +231,11 @@ def main():
         sys.exit(2)
     from mbsymmetry_amd import _lib
 
+    if args.serving_only:
+        print(json.dumps({"serving": serving_run(args.serving_requests, args.serving_rate, not args.no_mixed,
+                                                 _ints(args.serving_naos), _ints(args.serving_meshes))}),
+              flush=True)
+        return
     sysm = make_system(args.config)
     log("system", workload_name(sysm))
     peaks = _lib.measure_peaks(0)  # the box's own peaks, beside the datasheet values
@@ -304,7 +316,8 @@ def main():
     if not args.no_uneven:
         out["uneven_blocks"] = uneven_run()
     if not args.no_serving:
-        out["serving"] = serving_run(args.serving_requests, args.serving_rate, not args.no_mixed)
+        out["serving"] = serving_run(args.serving_requests, args.serving_rate, not args.no_mixed,
+                                     _ints(args.serving_naos), _ints(args.serving_meshes))
     if not args.no_cpu:
         cb = cpu_baseline(sysm, G0)
         out["cpu_baseline"] = {"value": cb["rate"] / flops_iter, "unit": "GW iterations/s", "cores": cb["cores"],
