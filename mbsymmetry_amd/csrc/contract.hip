@@ -585,7 +585,7 @@ __global__ __launch_bounds__(Cfg<T>::NT) void p0_mfma_kernel(const P0Item *__res
 
 // ---------------------------------------------------------------------------
 // Sigma.  Slab = rows [I0, I0+Ib) x columns [J0, J0+Jb) of ao block ik1 of iBZ
-// point k, spin s, for sig_times consecutive times t0 + tb.  For every V block
+// point k, spin s, for TB consecutive times t0 + tb.  For every V block
 // (q, iq, ik2), in chunks of qsub aux rows Q by nsub orbitals n (E and Z stored
 // with the (Q, n) reduction index contiguous):
 //   Z_Q(t) = sum_P conj(P_{QP}(t)) V_P,  E_Q(t) = V_Q G^{k-q}(t),
@@ -596,13 +596,12 @@ __global__ __launch_bounds__(Cfg<T>::NT) void p0_mfma_kernel(const P0Item *__res
 // share every V fragment: their P rows are stacked as rows of one Z product
 // and their G^{k-q} blocks as columns of one E product.
 // ---------------------------------------------------------------------------
-template <typename T>
+template <typename T, int TB>
 __global__ __launch_bounds__(Cfg<T>::NT) void sigma_mfma_kernel(const SigItem *__restrict__ items,
                                                        const SigBlk *__restrict__ blks, int64_t n_work, int nts,
                                                        int ns, const cplx<T> *__restrict__ V,
                                                        const cplx<T> *__restrict__ G, const cplx<T> *__restrict__ Pf,
                                                        cplx<double> *__restrict__ Sig, double pref) {
-  constexpr int TB = sig_times((int)sizeof(cplx<T>));
   const int64_t w = xcd_logical(blockIdx.x, gridDim.x);
   if (w >= n_work) return;
   const int np = (nts + TB - 1) / TB, per = np * ns;
@@ -820,18 +819,26 @@ hipError_t launch_p0(const P0Item *items, const P0Blk *blks, int n_items, int nt
   return hipGetLastError();
 }
 
+template <typename T, int TB>
+hipError_t launch_sigma_tb(const SigItem *items, const SigBlk *blks, int n_items, int nts, int ns, const cplx<T> *V,
+                           const cplx<T> *G, const cplx<T> *Pf, cplx<double> *Sig, double pref, size_t lds,
+                           hipStream_t st) {
+  const int64_t n_work = (int64_t)n_items * ((nts + TB - 1) / TB) * ns;
+  if (n_work == 0) return hipSuccess;
+  (void)hipFuncSetAttribute((const void *)sigma_mfma_kernel<T, TB>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)lds);
+  hipLaunchKernelGGL((sigma_mfma_kernel<T, TB>), dim3((unsigned)xcd_grid(n_work)), dim3(Cfg<T>::NT), lds, st, items,
+                     blks, n_work, nts, ns, V, G, Pf, Sig, pref);
+  return hipGetLastError();
+}
+
 template <typename T>
 hipError_t launch_sigma(const SigItem *items, const SigBlk *blks, int n_items, int nts, int ns, const cplx<T> *V,
                         const cplx<T> *G, const cplx<T> *Pf, cplx<double> *Sig, double pref, size_t lds,
-                        hipStream_t st) {
-  constexpr int TB = sig_times((int)sizeof(cplx<T>));
-  const int64_t n_work = (int64_t)n_items * ((nts + TB - 1) / TB) * ns;
-  if (n_work == 0) return hipSuccess;
-  (void)hipFuncSetAttribute((const void *)sigma_mfma_kernel<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)lds);
-  hipLaunchKernelGGL((sigma_mfma_kernel<T>), dim3((unsigned)xcd_grid(n_work)), dim3(Cfg<T>::NT), lds, st, items, blks,
-                     n_work, nts, ns, V, G, Pf, Sig, pref);
-  return hipGetLastError();
+                        hipStream_t st, int times) {
+  if (times == 4) return launch_sigma_tb<T, 4>(items, blks, n_items, nts, ns, V, G, Pf, Sig, pref, lds, st);
+  if (times == 2) return launch_sigma_tb<T, 2>(items, blks, n_items, nts, ns, V, G, Pf, Sig, pref, lds, st);
+  return hipErrorInvalidValue;
 }
 
 template hipError_t launch_p0<double>(const P0Item *, const P0Blk *, int, int, int, const cplx<double> *,
@@ -840,9 +847,9 @@ template hipError_t launch_p0<float>(const P0Item *, const P0Blk *, int, int, in
                                      const cplx<float> *, cplx<double> *, double, size_t, hipStream_t);
 template hipError_t launch_sigma<double>(const SigItem *, const SigBlk *, int, int, int, const cplx<double> *,
                                          const cplx<double> *, const cplx<double> *, cplx<double> *, double, size_t,
-                                         hipStream_t);
+                                         hipStream_t, int);
 template hipError_t launch_sigma<float>(const SigItem *, const SigBlk *, int, int, int, const cplx<float> *,
                                         const cplx<float> *, const cplx<float> *, cplx<double> *, double, size_t,
-                                        hipStream_t);
+                                        hipStream_t, int);
 
 }  // namespace symgw

@@ -34,7 +34,7 @@ hipError_t launch_p0(const P0Item *items, const P0Blk *blks, int n_items, int nt
 template <typename T>
 hipError_t launch_sigma(const SigItem *items, const SigBlk *blks, int n_items, int nts, int ns, const cplx<T> *V,
                         const cplx<T> *G, const cplx<T> *Pf, cplx<double> *Sig, double pref, size_t lds,
-                        hipStream_t st);
+                        hipStream_t st, int times);
 // peaks.hip: device peak microbenchmarks
 template <typename T>
 hipError_t launch_mfma_peak(void *out, int n_wg, int iters, hipStream_t st);

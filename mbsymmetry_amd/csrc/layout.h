@@ -67,9 +67,10 @@ constexpr int kSigBand = 48;
 // Most aux rows Q of one Sigma LDS chunk.
 constexpr int kSigRows = 64;
 // Consecutive times of one Sigma slab (they share every V fragment): four with
-// complex64 operands, two with complex128; the last slab takes the remaining
-// nts % sig_times times.
-constexpr int sig_times(int elem_bytes) { return elem_bytes == 8 ? 4 : 2; }
+// complex64 operands; with complex128 four for ao blocks of at most 32
+// orbitals (4 times x 2 x 2 tiles fit 12 waves x 2 accumulators) and two for
+// wider ones (48-wide bands).  The last slab takes the remaining times.
+constexpr int sig_times(int elem_bytes, int k1) { return elem_bytes == 8 || k1 <= 32 ? 4 : 2; }
 
 // One V-block contraction that feeds a P0 output slab (cugw_qkpt_block.cu:219-261).
 struct P0Blk {
@@ -140,7 +141,8 @@ struct WorkLists {
   std::vector<P0Blk> p0_blks;
   std::vector<P0Item> p0_items;
   std::vector<SigBlk> sig_blks;
-  std::vector<SigItem> sig_items;
+  std::vector<SigItem> sig_items;   // slabs of two times
+  std::vector<SigItem> sig_items4;  // slabs of four times
   std::vector<RotBlk> g_rot;      // targets over the full BZ (ao)
   std::vector<RotBlk> p_rot;      // targets over the full BZ (aux)
   std::vector<SolveBlk> w_blocks; // aux blocks of every iBZ q
@@ -149,7 +151,7 @@ struct WorkLists {
   int64_t d_scr_group = 0;        // scratch elements per (w, s) (n^2 per block)
   int64_t n_p0_contractions = 0, n_sig_contractions = 0;
   double flop_p0 = 0, flop_sig = 0;
-  size_t p0_lds = 0, sig_lds = 0; // dynamic LDS bytes of the contraction kernels
+  size_t p0_lds = 0, sig_lds = 0, sig_lds4 = 0; // dynamic LDS bytes of the contraction kernels
 };
 
 // elem_bytes: bytes of one complex element in the contraction precision.
