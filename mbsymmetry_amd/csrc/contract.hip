@@ -838,6 +838,7 @@ hipError_t launch_sigma(const SigItem *items, const SigBlk *blks, int n_items, i
                         hipStream_t st, int times) {
   if (times == 4) return launch_sigma_tb<T, 4>(items, blks, n_items, nts, ns, V, G, Pf, Sig, pref, lds, st);
   if (times == 2) return launch_sigma_tb<T, 2>(items, blks, n_items, nts, ns, V, G, Pf, Sig, pref, lds, st);
+  if (times == 1) return launch_sigma_tb<T, 1>(items, blks, n_items, nts, ns, V, G, Pf, Sig, pref, lds, st);
   return hipErrorInvalidValue;
 }
 

@@ -60,17 +60,36 @@ constexpr int kP0Band = 16;
 // Consecutive times of one P0 slab (they share every V fragment): four with
 // complex64 operands, two with complex128 (four spill registers there).
 constexpr int p0_times(int elem_bytes) { return elem_bytes == 8 ? 4 : 2; }
-// Largest Sigma slab edge: 3 x 3 tiles of 16 per time, so the sig_times
-// slabs of a workgroup (18 tiles for complex128, 36 for complex64) fit its
-// 12 waves x contract_tiles accumulator tiles.
+// Widest ao block that one Sigma slab covers whole (3 x 3 tiles of 16 per time).
 constexpr int kSigBand = 48;
 // Most aux rows Q of one Sigma LDS chunk.
 constexpr int kSigRows = 64;
-// Consecutive times of one Sigma slab (they share every V fragment): four with
-// complex64 operands; with complex128 four for ao blocks of at most 32
-// orbitals (4 times x 2 x 2 tiles fit 12 waves x 2 accumulators) and two for
-// wider ones (48-wide bands).  The last slab takes the remaining times.
-constexpr int sig_times(int elem_bytes, int k1) { return elem_bytes == 8 || k1 <= 32 ? 4 : 2; }
+// Shape of the Sigma slabs of an ao block of k1 orbitals: `times` consecutive
+// times per slab (they share every V fragment; the last slab takes the
+// remaining times) and square `band` x `band` output bands, with
+// times * ceil(band / 16)^2 within the workgroup's 12 waves x contract_tiles
+// accumulator tiles (24 for complex128, 36 for complex64).
+//  - k1 <= 32: the whole block, four times.
+//  - k1 <= 48: the whole block, four times (complex64) or two (complex128).
+//  - wider blocks are split into bands of whole tiles, and every pair of bands
+//    recomputes its Z (once per row band) and E (once per column band), so
+//    fewer, wider bands save work while fewer times per slab cost V loads:
+//    one time of the widest bands (6 tiles for complex64, 4 for complex128)
+//    where that cuts the band count to 2/3 or less of two times of the
+//    widest bands that fit (4 and 3 tiles), else the latter (the faster of
+//    four, two and one times on MI355X at every k1 of 56..256 measured).
+struct SigShape {
+  int times, band;
+};
+inline SigShape sig_shape(int elem_bytes, int k1) {
+  if (k1 <= 32) return {4, k1};
+  if (k1 <= kSigBand) return {elem_bytes == 8 ? 4 : 2, k1};
+  const int tiles = (k1 + 15) / 16, cap2 = elem_bytes == 8 ? 4 : 3, cap1 = elem_bytes == 8 ? 6 : 4;
+  auto bands = [&](int cap) { return (tiles + cap - 1) / cap; };
+  auto even = [&](int cap) { const int nb = bands(cap); return 16 * ((tiles + nb - 1) / nb); };
+  if (3 * bands(cap1) <= 2 * bands(cap2)) return {1, even(cap1)};
+  return {2, even(cap2)};
+}
 
 // One V-block contraction that feeds a P0 output slab (cugw_qkpt_block.cu:219-261).
 struct P0Blk {
@@ -143,6 +162,7 @@ struct WorkLists {
   std::vector<SigBlk> sig_blks;
   std::vector<SigItem> sig_items;   // slabs of two times
   std::vector<SigItem> sig_items4;  // slabs of four times
+  std::vector<SigItem> sig_items1;  // slabs of one time (wide blocks)
   std::vector<RotBlk> g_rot;      // targets over the full BZ (ao)
   std::vector<RotBlk> p_rot;      // targets over the full BZ (aux)
   std::vector<SolveBlk> w_blocks; // aux blocks of every iBZ q
@@ -151,7 +171,7 @@ struct WorkLists {
   int64_t d_scr_group = 0;        // scratch elements per (w, s) (n^2 per block)
   int64_t n_p0_contractions = 0, n_sig_contractions = 0;
   double flop_p0 = 0, flop_sig = 0;
-  size_t p0_lds = 0, sig_lds = 0, sig_lds4 = 0; // dynamic LDS bytes of the contraction kernels
+  size_t p0_lds = 0, sig_lds = 0, sig_lds4 = 0, sig_lds1 = 0; // dynamic LDS bytes of the contraction kernels
 };
 
 // elem_bytes: bytes of one complex element in the contraction precision.
