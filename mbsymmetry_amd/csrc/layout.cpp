@@ -299,6 +299,58 @@ bool sig_chunks(int q, int k1, int k2, int band, int budget, int TB,
   return true;
 }
 
+// (k1, k2) of the V blocks that feed the P0 slabs of aux block iq of q.
+static std::vector<std::array<int, 2>> p0_dims(const Layout &L, int q, int iq) {
+  std::vector<std::array<int, 2>> d;
+  const int nk = L.nk;
+  for (int k = 0; k < nk; ++k) {
+    const int k2 = L.kpq[(size_t)k * nk + q];
+    const size_t p = (size_t)k * nk + k2;
+    for (auto &t : L.V_irreps[p])
+      if (t[0] == iq) d.push_back({L.ao_sizes[k][t[1]], L.ao_sizes[k2][t[2]]});
+  }
+  return d;
+}
+
+// Rows per P0 slab of an aux block of qs rows.  A chunk of nn orbitals holds E
+// and C ([times][band][k1 * nn] each), so narrower bands leave room for wider
+// chunks: the E product (columns: times x nn) fills its 16-wide MFMA tiles
+// while the final product pads the band to 16 rows.  Among the bands up to
+// band_cap, pick the one with the fewest executed MFMA FLOPs (16x16x4 tiles,
+// padding included) plus a fixed cost per chunk, summed over the bands of
+// the block.
+static int p0_band(int qs, int band_cap, int budget, int eb, const std::vector<std::array<int, 2>> &dims) {
+  const int TB = p0_times(eb);
+  const bool stack = (qs + 15) / 16 <= contract_waves(eb);  // the times share one pass
+  const int tb = stack ? TB : 1, passes = stack ? 1 : TB;
+  const double chunk_cost = 2.5e5;
+  int best_band = balanced(qs, band_cap);
+  double best = -1;
+  for (int cap = band_cap; cap >= 1; --cap) {
+    const int band = balanced(qs, cap);
+    const int nbands = (qs + band - 1) / band;
+    double cost = 0;
+    bool ok = true;
+    for (auto &d : dims) {
+      const int k1 = d[0], k2 = d[1];
+      const int nn_cap = std::min(k2, (budget / (2 * TB * band) - 1) / k1);
+      if (nn_cap < 1) { ok = false; break; }
+      const int nn = balanced(k2, nn_cap), chunks = (k2 + nn - 1) / nn;
+      const double e = 8.0 * pad_to((int64_t)band * k1, 16) * pad_to(tb * nn, 16) * pad_to(k2, 4);
+      const double c = tb * 8.0 * pad_to(k1, 16) * pad_to((int64_t)band * nn, 16) * pad_to(k1, 4);
+      const double f = tb * 8.0 * 16 * pad_to(qs, 16) * pad_to((int64_t)k1 * nn, 4);
+      cost += (double)passes * chunks * (e + c + f + chunk_cost);
+    }
+    if (!ok) continue;
+    cost *= nbands;
+    if (best < 0 || cost < best * 0.999) {
+      best = cost;
+      best_band = band;
+    }
+  }
+  return best_band;
+}
+
 bool build_worklists(const Layout &L, int eb, WorkLists &W, std::string &err) {
   W = WorkLists();
   const int nk = L.nk, nts = L.nts, ns = L.ns;
@@ -327,7 +379,7 @@ bool build_worklists(const Layout &L, int eb, WorkLists &W, std::string &err) {
       }
       const int band_cap = std::min(kP0Band, budget / (2 * p0_times(eb) * (maxk1 + 1)));
       if (band_cap < 1) { err = fmt("P0 ao block too large for the LDS plan (k1)", maxk1); return false; }
-      const int band = balanced(qs, band_cap);
+      const int band = p0_band(qs, band_cap, budget, eb, p0_dims(L, q, (int)iq));
       int b0 = (int)W.p0_blks.size();
       for (int k = 0; k < nk; ++k) {
         int k2 = L.kpq[(size_t)k * nk + q];
