@@ -4,12 +4,15 @@
 // a fixed order ("owner computes"), so results are bitwise reproducible with
 // no atomics or locks (the reference serialises per-q / per-k AXPYs behind
 // spin locks, cugw_qkpt_block.cu:269-285 and :414-436).  All complex products
-// are 16x16 tiles built from four real MFMAs (v_mfma_f64_16x16x4f64 for the
-// plain path, v_mfma_f32_16x16x4f32 for the FP32 path); operands stream from
+// are 16x16 tiles of real MFMAs (v_mfma_f64_16x16x4f64 for the plain path,
+// v_mfma_f32_16x16x4f32 for the FP32 path): four per complex k-step, or three
+// (3M) in the FP64 products whose tiles live for one job; operands stream from
 // memory or from LDS, intermediates never leave the workgroup's LDS.  A wave
 // works on up to four tiles at a time that share one operand fragment per
 // k-step.
 #include <hip/hip_runtime.h>
+
+#include <type_traits>
 
 #include "common.h"
 #include "kernels.h"
@@ -60,14 +63,26 @@ struct FDiv {
 
 template <typename T>
 struct Tile {
+  using elem = T;
   typename Mfma<T>::v4 re, im;
 };
+// A complex tile as the three real products of the 3M (Gauss) scheme:
+// P1 = sum Ar Br, P2 = sum Ai Bi, P3 = sum (Ar + Ai)(Br + Bi) (Br - Bi when
+// B is conjugated); A B = (P1 - P2) + i (P3 - P1 - P2), A conj(B) =
+// (P1 + P2) + i (P3 - P1 + P2).  Three MFMAs per complex k-step instead of
+// four, for 50% more accumulator registers: used by the FP64 products whose
+// tiles live only for one job (the Z and E products of Sigma, E and C of P0).
+template <typename T>
+struct Tile3 {
+  using elem = T;
+  typename Mfma<T>::v4 p1, p2, p3;
+};
+template <typename T>
+using JobTile = typename std::conditional<sizeof(T) == 8, Tile3<T>, Tile<T>>::type;
 // Tiles of one job, sharing one operand fragment per k-step: 1 for FP64 and 4
 // for FP32 (more spill registers at three waves per SIMD in FP64).
 template <typename T>
 struct Group { static constexpr int n = sizeof(T) == 8 ? 1 : 4; };
-template <typename T>
-using Tiles = Tile<T>[4];
 template <typename T>
 using Acc = Tile<T>[Cfg<T>::MT];
 
@@ -75,6 +90,12 @@ template <typename T>
 __device__ __forceinline__ void tile_zero(Tile<T> &t) {
   t.re = typename Mfma<T>::v4{T(0), T(0), T(0), T(0)};
   t.im = t.re;
+}
+template <typename T>
+__device__ __forceinline__ void tile_zero(Tile3<T> &t) {
+  t.p1 = typename Mfma<T>::v4{T(0), T(0), T(0), T(0)};
+  t.p2 = t.p1;
+  t.p3 = t.p1;
 }
 
 template <typename T>
@@ -97,6 +118,12 @@ __device__ __forceinline__ void mma4(Tile<T> &acc, const cplx<T> &a, const cplx<
     acc.re = Mfma<T>::op(-a.im, b.im, acc.re);
     acc.im = Mfma<T>::op(a.im, b.re, acc.im);
   }
+}
+template <bool CB, typename T>
+__device__ __forceinline__ void mma4(Tile3<T> &acc, const cplx<T> &a, const cplx<T> &b) {
+  acc.p1 = Mfma<T>::op(a.re, b.re, acc.p1);
+  acc.p2 = Mfma<T>::op(a.im, b.im, acc.p2);
+  acc.p3 = Mfma<T>::op(a.re + a.im, CB ? b.re - b.im : b.re + b.im, acc.p3);
 }
 
 // The k-loops below keep kAhead k-steps of operands in flight: the fragments
@@ -125,9 +152,10 @@ constexpr int kAhead = 2;
 // acc[C0+c] += A (16 x K) * B_c (K x 16) for c < NG with C0 + c < n, over k in
 // [kb, ke): one A fragment per k-step shared by the tiles.  la(i, k), lb(c, k, j)
 // with c the absolute tile index.  n must be uniform across the wave.
-template <int C0, int NG, bool CB, typename T, int N, typename LA, typename LB>
-__device__ __forceinline__ void mma_rowshare(Tile<T> (&acc)[N], int n, int kb, int ke, bool arow, const LA &la,
+template <int C0, int NG, bool CB, typename TT, int N, typename LA, typename LB>
+__device__ __forceinline__ void mma_rowshare(TT (&acc)[N], int n, int kb, int ke, bool arow, const LA &la,
                                              const LB &lb) {
+  using T = typename TT::elem;
   static_assert(C0 + NG <= N, "tile group outside the accumulator array");
   constexpr int R = kAhead + 1;
   if (kb >= ke || C0 >= n) return;
@@ -161,8 +189,9 @@ __device__ __forceinline__ void mma_rowshare(Tile<T> (&acc)[N], int n, int kb, i
 }
 
 // acc[C0+c] += A_c (16 x K) * B (K x 16): one B fragment shared.  la(c, i, k), lb(k, j).
-template <int C0, int NG, bool CB, typename T, int N, typename LA, typename LB>
-__device__ __forceinline__ void mma_colshare(Tile<T> (&acc)[N], int n, int kb, int ke, const LA &la, const LB &lb) {
+template <int C0, int NG, bool CB, typename TT, int N, typename LA, typename LB>
+__device__ __forceinline__ void mma_colshare(TT (&acc)[N], int n, int kb, int ke, const LA &la, const LB &lb) {
+  using T = typename TT::elem;
   static_assert(C0 + NG <= N, "tile group outside the accumulator array");
   constexpr int R = kAhead + 1;
   if (kb >= ke || C0 >= n) return;
@@ -318,6 +347,20 @@ __device__ __forceinline__ void tile_store(const Tile<T> &acc, const F &f) {
 #pragma unroll
   for (int r = 0; r < 4; ++r) f(Mfma<T>::row(lane, r), lane & 15, cplx<T>{acc.re[r], acc.im[r]});
 }
+// The same for a job tile (CB: the product conjugated B).
+template <bool CB, typename T, typename F>
+__device__ __forceinline__ void job_store(const Tile<T> &acc, const F &f) {
+  tile_store(acc, f);
+}
+template <bool CB, typename T, typename F>
+__device__ __forceinline__ void job_store(const Tile3<T> &acc, const F &f) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const T p1 = acc.p1[r], p2 = acc.p2[r], p3 = acc.p3[r];
+    f(Mfma<T>::row(lane, r), lane & 15, CB ? cplx<T>{p1 + p2, p3 - p1 + p2} : cplx<T>{p1 - p2, p3 - p1 - p2});
+  }
+}
 
 // Copy n contiguous elements src[0, n) to dst[ (e / len) * ld + e % len ]
 // (rows of length len, row stride ld) through f: all U loads of a thread are
@@ -363,7 +406,7 @@ __device__ __forceinline__ int gemm_jobs_rs_g(int M, int N, int K, int first, co
   const int dmt = NW / ngr, dgi = NW - dmt * ngr;
   for (; j < njobs; j += NW, gi += dgi, mt += dmt + (gi >= ngr ? 1 : 0), gi -= gi >= ngr ? ngr : 0) {
     const int g = gi * NG, n = min(NG, ntn - g), m0 = mt * 16;
-    Tiles<T> acc;
+    JobTile<T> acc[NG];
 #pragma unroll
     for (int c = 0; c < NG; ++c) tile_zero(acc[c]);
     mma_rowshare<0, NG, CB>(acc, n, 0, K, true, [&](int i, int k) { return la(m0 + i, k); },
@@ -371,7 +414,7 @@ __device__ __forceinline__ int gemm_jobs_rs_g(int M, int N, int K, int first, co
 #pragma unroll
     for (int c = 0; c < NG; ++c)
       if (c < n)
-        tile_store(acc[c], [&](int i, int jj, cplx<T> v) {
+        job_store<CB>(acc[c], [&](int i, int jj, cplx<T> v) {
           const int row = m0 + i, col = (g + c) * 16 + jj;
           if (row < M && col < N) st(row, col, v);
         });
@@ -390,7 +433,7 @@ __device__ __forceinline__ int gemm_jobs_cs_g(int M, int N, int K, int first, co
   const int dgt = NW / ntn, dnt = NW - dgt * ntn;
   for (; j < njobs; j += NW, nt += dnt, gt += dgt + (nt >= ntn ? 1 : 0), nt -= nt >= ntn ? ntn : 0) {
     const int g = gt * NG, n = min(NG, ntm - g), n0 = nt * 16;
-    Tiles<T> acc;
+    JobTile<T> acc[NG];
 #pragma unroll
     for (int c = 0; c < NG; ++c) tile_zero(acc[c]);
     mma_colshare<0, NG, CB>(acc, n, 0, K, [&](int c, int i, int k) { return la((g + c) * 16 + i, k); },
@@ -398,7 +441,7 @@ __device__ __forceinline__ int gemm_jobs_cs_g(int M, int N, int K, int first, co
 #pragma unroll
     for (int c = 0; c < NG; ++c)
       if (c < n)
-        tile_store(acc[c], [&](int i, int jj, cplx<T> v) {
+        job_store<CB>(acc[c], [&](int i, int jj, cplx<T> v) {
           const int row = (g + c) * 16 + i, col = n0 + jj;
           if (row < M && col < N) st(row, col, v);
         });
