@@ -6,13 +6,11 @@
 // spin locks, cugw_qkpt_block.cu:269-285 and :414-436).  All complex products
 // are 16x16 tiles of real MFMAs (v_mfma_f64_16x16x4f64 for the plain path,
 // v_mfma_f32_16x16x4f32 for the FP32 path): four per complex k-step, or three
-// (3M) in the FP64 products whose tiles live for one job; operands stream from
+// (3M) in the products whose tiles live for one job; operands stream from
 // memory or from LDS, intermediates never leave the workgroup's LDS.  A wave
-// works on up to four tiles at a time that share one operand fragment per
-// k-step.
+// works on up to four persistent tiles, or on one or two job tiles, that share
+// one operand fragment per k-step.
 #include <hip/hip_runtime.h>
-
-#include <type_traits>
 
 #include "common.h"
 #include "kernels.h"
@@ -70,19 +68,18 @@ struct Tile {
 // P1 = sum Ar Br, P2 = sum Ai Bi, P3 = sum (Ar + Ai)(Br + Bi) (Br - Bi when
 // B is conjugated); A B = (P1 - P2) + i (P3 - P1 - P2), A conj(B) =
 // (P1 + P2) + i (P3 - P1 + P2).  Three MFMAs per complex k-step instead of
-// four, for 50% more accumulator registers: used by the FP64 products whose
-// tiles live only for one job (the Z and E products of Sigma, E and C of P0).
+// four, for 50% more accumulator registers: used by the products whose tiles
+// live only for one job (the Z and E products of Sigma, E and C of P0).
 template <typename T>
 struct Tile3 {
   using elem = T;
   typename Mfma<T>::v4 p1, p2, p3;
 };
+// Tiles of one job (and of one pass over the persistent tiles) sharing one
+// operand fragment per k-step: 1 for FP64 and 2 for FP32, the most that three
+// waves per SIMD hold without spilling in the Sigma kernel.
 template <typename T>
-using JobTile = typename std::conditional<sizeof(T) == 8, Tile3<T>, Tile<T>>::type;
-// Tiles of one job, sharing one operand fragment per k-step: 1 for FP64 and 4
-// for FP32 (more spill registers at three waves per SIMD in FP64).
-template <typename T>
-struct Group { static constexpr int n = sizeof(T) == 8 ? 1 : 4; };
+struct Group { static constexpr int n = sizeof(T) == 8 ? 1 : 2; };
 template <typename T>
 using Acc = Tile<T>[Cfg<T>::MT];
 
@@ -347,11 +344,7 @@ __device__ __forceinline__ void tile_store(const Tile<T> &acc, const F &f) {
 #pragma unroll
   for (int r = 0; r < 4; ++r) f(Mfma<T>::row(lane, r), lane & 15, cplx<T>{acc.re[r], acc.im[r]});
 }
-// The same for a job tile (CB: the product conjugated B).
-template <bool CB, typename T, typename F>
-__device__ __forceinline__ void job_store(const Tile<T> &acc, const F &f) {
-  tile_store(acc, f);
-}
+// The same for a 3M tile (CB: the product conjugated B).
 template <bool CB, typename T, typename F>
 __device__ __forceinline__ void job_store(const Tile3<T> &acc, const F &f) {
   const int lane = threadIdx.x & 63;
@@ -406,7 +399,7 @@ __device__ __forceinline__ int gemm_jobs_rs_g(int M, int N, int K, int first, co
   const int dmt = NW / ngr, dgi = NW - dmt * ngr;
   for (; j < njobs; j += NW, gi += dgi, mt += dmt + (gi >= ngr ? 1 : 0), gi -= gi >= ngr ? ngr : 0) {
     const int g = gi * NG, n = min(NG, ntn - g), m0 = mt * 16;
-    JobTile<T> acc[NG];
+    Tile3<T> acc[NG];
 #pragma unroll
     for (int c = 0; c < NG; ++c) tile_zero(acc[c]);
     mma_rowshare<0, NG, CB>(acc, n, 0, K, true, [&](int i, int k) { return la(m0 + i, k); },
@@ -433,7 +426,7 @@ __device__ __forceinline__ int gemm_jobs_cs_g(int M, int N, int K, int first, co
   const int dgt = NW / ntn, dnt = NW - dgt * ntn;
   for (; j < njobs; j += NW, nt += dnt, gt += dgt + (nt >= ntn ? 1 : 0), nt -= nt >= ntn ? ntn : 0) {
     const int g = gt * NG, n = min(NG, ntm - g), n0 = nt * 16;
-    JobTile<T> acc[NG];
+    Tile3<T> acc[NG];
 #pragma unroll
     for (int c = 0; c < NG; ++c) tile_zero(acc[c]);
     mma_colshare<0, NG, CB>(acc, n, 0, K, [&](int c, int i, int k) { return la((g + c) * 16 + i, k); },
