@@ -145,21 +145,25 @@ __device__ __forceinline__ cplx<T> amask(const cplx<T> &a, bool ok) {
 // Operand k-steps kept in flight ahead of the MFMAs (register sets used in a
 // ring of kAhead + 1).
 constexpr int kAhead = 2;
+// The same for the job products (3M tiles), measured per kernel: one step in
+// FP64 (no spills; Sigma +2.3%) and in P0 (FP32 +3%), two in the FP32 Sigma.
+template <typename T>
+constexpr int kAheadJob = sizeof(T) == 8 ? 1 : 2;
 
 // acc[C0+c] += A (16 x K) * B_c (K x 16) for c < NG with C0 + c < n, over k in
 // [kb, ke): one A fragment per k-step shared by the tiles.  la(i, k), lb(c, k, j)
 // with c the absolute tile index.  n must be uniform across the wave.
-template <int C0, int NG, bool CB, typename TT, int N, typename LA, typename LB>
+template <int C0, int NG, bool CB, int AH = kAhead, typename TT, int N, typename LA, typename LB>
 __device__ __forceinline__ void mma_rowshare(TT (&acc)[N], int n, int kb, int ke, bool arow, const LA &la,
                                              const LB &lb) {
   using T = typename TT::elem;
   static_assert(C0 + NG <= N, "tile group outside the accumulator array");
-  constexpr int R = kAhead + 1;
+  constexpr int R = AH + 1;
   if (kb >= ke || C0 >= n) return;
   const int lane = threadIdx.x & 63, i = lane & 15, kk = lane >> 4;
   cplx<T> a[R], b[R][NG];
 #pragma unroll
-  for (int d = 0; d < kAhead; ++d) {
+  for (int d = 0; d < AH; ++d) {
     a[d] = la(i, kb + 4 * d + kk);
 #pragma unroll
     for (int c = 0; c < NG; ++c)
@@ -169,11 +173,11 @@ __device__ __forceinline__ void mma_rowshare(TT (&acc)[N], int n, int kb, int ke
   while (true) {
 #pragma unroll
     for (int h = 0; h < R; ++h) {
-      const int hl = (h + kAhead) % R;
-      a[hl] = la(i, k0 + 4 * kAhead + kk);
+      const int hl = (h + AH) % R;
+      a[hl] = la(i, k0 + 4 * AH + kk);
 #pragma unroll
       for (int c = 0; c < NG; ++c)
-        if (C0 + c < n) b[hl][c] = lb(C0 + c, k0 + 4 * kAhead + kk, i);
+        if (C0 + c < n) b[hl][c] = lb(C0 + c, k0 + 4 * AH + kk, i);
       prefetch_fence();
       const cplx<T> am = amask(a[h], arow && k0 + kk < ke);
 #pragma unroll
@@ -186,16 +190,16 @@ __device__ __forceinline__ void mma_rowshare(TT (&acc)[N], int n, int kb, int ke
 }
 
 // acc[C0+c] += A_c (16 x K) * B (K x 16): one B fragment shared.  la(c, i, k), lb(k, j).
-template <int C0, int NG, bool CB, typename TT, int N, typename LA, typename LB>
+template <int C0, int NG, bool CB, int AH = kAhead, typename TT, int N, typename LA, typename LB>
 __device__ __forceinline__ void mma_colshare(TT (&acc)[N], int n, int kb, int ke, const LA &la, const LB &lb) {
   using T = typename TT::elem;
   static_assert(C0 + NG <= N, "tile group outside the accumulator array");
-  constexpr int R = kAhead + 1;
+  constexpr int R = AH + 1;
   if (kb >= ke || C0 >= n) return;
   const int lane = threadIdx.x & 63, i = lane & 15, kk = lane >> 4;
   cplx<T> a[R][NG], b[R];
 #pragma unroll
-  for (int d = 0; d < kAhead; ++d) {
+  for (int d = 0; d < AH; ++d) {
     b[d] = lb(kb + 4 * d + kk, i);
 #pragma unroll
     for (int c = 0; c < NG; ++c)
@@ -205,11 +209,11 @@ __device__ __forceinline__ void mma_colshare(TT (&acc)[N], int n, int kb, int ke
   while (true) {
 #pragma unroll
     for (int h = 0; h < R; ++h) {
-      const int hl = (h + kAhead) % R;
-      b[hl] = lb(k0 + 4 * kAhead + kk, i);
+      const int hl = (h + AH) % R;
+      b[hl] = lb(k0 + 4 * AH + kk, i);
 #pragma unroll
       for (int c = 0; c < NG; ++c)
-        if (C0 + c < n) a[hl][c] = la(C0 + c, i, k0 + 4 * kAhead + kk);
+        if (C0 + c < n) a[hl][c] = la(C0 + c, i, k0 + 4 * AH + kk);
       prefetch_fence();
       const bool kin = k0 + kk < ke;
 #pragma unroll
@@ -388,7 +392,7 @@ __device__ __forceinline__ void stage_rows(cplx<T> *dst, int ld, const cplx<T> *
 // (row tile, group of NG column tiles) sharing the A fragment of each k-step;
 // column-share jobs are (group of NG row tiles, column tile) sharing B.
 // la(row, k), lb(k, col), st(row, col, value) (st only for row < M, col < N).
-template <typename T, int NG, bool CB, typename LA, typename LB, typename ST>
+template <typename T, int NG, bool CB, int AH, typename LA, typename LB, typename ST>
 __device__ __forceinline__ int gemm_jobs_rs_g(int M, int N, int K, int first, const LA &la, const LB &lb,
                                               const ST &st) {
   constexpr int NW = Cfg<T>::NW;
@@ -402,7 +406,7 @@ __device__ __forceinline__ int gemm_jobs_rs_g(int M, int N, int K, int first, co
     Tile3<T> acc[NG];
 #pragma unroll
     for (int c = 0; c < NG; ++c) tile_zero(acc[c]);
-    mma_rowshare<0, NG, CB>(acc, n, 0, K, true, [&](int i, int k) { return la(m0 + i, k); },
+    mma_rowshare<0, NG, CB, AH>(acc, n, 0, K, true, [&](int i, int k) { return la(m0 + i, k); },
                                [&](int c, int k, int jj) { return lb(k, (g + c) * 16 + jj); });
 #pragma unroll
     for (int c = 0; c < NG; ++c)
@@ -415,7 +419,7 @@ __device__ __forceinline__ int gemm_jobs_rs_g(int M, int N, int K, int first, co
   return njobs;
 }
 
-template <typename T, int NG, bool CB, typename LA, typename LB, typename ST>
+template <typename T, int NG, bool CB, int AH, typename LA, typename LB, typename ST>
 __device__ __forceinline__ int gemm_jobs_cs_g(int M, int N, int K, int first, const LA &la, const LB &lb,
                                               const ST &st) {
   constexpr int NW = Cfg<T>::NW;
@@ -429,7 +433,7 @@ __device__ __forceinline__ int gemm_jobs_cs_g(int M, int N, int K, int first, co
     Tile3<T> acc[NG];
 #pragma unroll
     for (int c = 0; c < NG; ++c) tile_zero(acc[c]);
-    mma_colshare<0, NG, CB>(acc, n, 0, K, [&](int c, int i, int k) { return la((g + c) * 16 + i, k); },
+    mma_colshare<0, NG, CB, AH>(acc, n, 0, K, [&](int c, int i, int k) { return la((g + c) * 16 + i, k); },
                                [&](int k, int jj) { return lb(k, n0 + jj); });
 #pragma unroll
     for (int c = 0; c < NG; ++c)
@@ -444,21 +448,21 @@ __device__ __forceinline__ int gemm_jobs_cs_g(int M, int N, int K, int first, co
 
 // Grouped jobs when there are enough of them to keep every wave busy, else
 // single-tile jobs (less operand sharing, shorter critical path).
-template <typename T, bool CB = false, typename LA, typename LB, typename ST>
+template <typename T, bool CB = false, int AH = kAheadJob<T>, typename LA, typename LB, typename ST>
 __device__ __forceinline__ int gemm_jobs_rs(int M, int N, int K, int first, const LA &la, const LB &lb,
                                             const ST &st) {
   constexpr int NG = Group<T>::n;
   const int ntm = (M + 15) / 16, ntn = (N + 15) / 16;
-  if (ntm * ((ntn + NG - 1) / NG) < Cfg<T>::NW) return gemm_jobs_rs_g<T, 1, CB>(M, N, K, first, la, lb, st);
-  return gemm_jobs_rs_g<T, NG, CB>(M, N, K, first, la, lb, st);
+  if (ntm * ((ntn + NG - 1) / NG) < Cfg<T>::NW) return gemm_jobs_rs_g<T, 1, CB, AH>(M, N, K, first, la, lb, st);
+  return gemm_jobs_rs_g<T, NG, CB, AH>(M, N, K, first, la, lb, st);
 }
-template <typename T, bool CB = false, typename LA, typename LB, typename ST>
+template <typename T, bool CB = false, int AH = kAheadJob<T>, typename LA, typename LB, typename ST>
 __device__ __forceinline__ int gemm_jobs_cs(int M, int N, int K, int first, const LA &la, const LB &lb,
                                             const ST &st) {
   constexpr int NG = Group<T>::n;
   const int ntm = (M + 15) / 16, ntn = (N + 15) / 16;
-  if (((ntm + NG - 1) / NG) * ntn < Cfg<T>::NW) return gemm_jobs_cs_g<T, 1, CB>(M, N, K, first, la, lb, st);
-  return gemm_jobs_cs_g<T, NG, CB>(M, N, K, first, la, lb, st);
+  if (((ntm + NG - 1) / NG) * ntn < Cfg<T>::NW) return gemm_jobs_cs_g<T, 1, CB, AH>(M, N, K, first, la, lb, st);
+  return gemm_jobs_cs_g<T, NG, CB, AH>(M, N, K, first, la, lb, st);
 }
 
 }  // namespace
@@ -541,8 +545,8 @@ __global__ __launch_bounds__(Cfg<T>::NT) void p0_mfma_kernel(const P0Item *__res
               const int r = dk1.div(row), a = row - r * k1, tb = dnn.div(col), l = col - tb * nn;
               bufE[(tb * nband + r) * lde + a * nn + l] = v;
             };
-            if (tbn * nn > 16) gemm_jobs_rs<T>(nband * k1, tbn * nn, k2, 0, la, lb, st);
-            else gemm_jobs_cs<T>(nband * k1, tbn * nn, k2, 0, la, lb, st);
+            if (tbn * nn > 16) gemm_jobs_rs<T, false, 1>(nband * k1, tbn * nn, k2, 0, la, lb, st);
+            else gemm_jobs_cs<T, false, 1>(nband * k1, tbn * nn, k2, 0, la, lb, st);
           }
           __syncthreads();
           // C_tb[r][p][l] = sum_a G1_tb[p][a] E_tb[r][a][l]  (the times' jobs dealt in turn)
@@ -552,7 +556,7 @@ __global__ __launch_bounds__(Cfg<T>::NT) void p0_mfma_kernel(const P0Item *__res
               const cplx<T> *G1t = G1 - (int64_t)tb * g1ts;
               const cplx<T> *Et = bufE + tb * nband * lde;
               cplx<T> *Ct = bufC + tb * nband * lde;
-              first += gemm_jobs_rs<T>(
+              first += gemm_jobs_rs<T, false, 1>(
                   k1, nband * nn, k1, first,
                   [&](int p, int a) { return G1t[min(p, k1 - 1) * k1 + min(a, k1 - 1)]; },
                   [&](int a, int col) {
