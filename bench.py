@@ -261,7 +261,13 @@ def main():
     dom_flop = plain["flop_sig"] if dom == "Sigma" else plain["flop_p0"]
     achieved = dom_flop / (dom_ms * 1e-3) / 1e12
     flops_iter = plain["flop_p0"] + plain["flop_sig"]
-    dom_kernel = ("sigma" if dom == "Sigma" else "p0") + "_mfma_kernel<double>"
+    if dom == "Sigma":
+        # Sigma runs one launch per slab depth (4 times per slab for ao blocks up
+        # to 32 orbitals, 2 or 1 for wider ones); the phase time covers them all
+        widest = max(b for s in sysm.ao_sizes for b in s)
+        dom_kernel = "sigma_mfma_kernel<double, 4>" if widest <= 32 else "sigma_mfma_kernel<double, 1|2|4>"
+    else:
+        dom_kernel = "p0_mfma_kernel<double>"
     traffic = None
     tpath = os.path.join(ROOT, "profiles", f"traffic_{dom.lower()}_{args.config}_fp64.json")
     if os.path.exists(tpath):  # HBM bytes per launch from the FETCH_SIZE / WRITE_SIZE runs
