@@ -22,8 +22,8 @@ constexpr int kRotLds = 64 * 1024;
 
 namespace {
 
-// Waves per workgroup: 12 (three per SIMD, at most 168 VGPRs each) on the
-// full 160 KiB LDS of a CU, in both precisions.
+// Waves per workgroup (16 for FP64, 12 for FP32) on the full 160 KiB LDS of
+// a CU.
 template <typename T>
 struct Cfg {
   static constexpr int NW = contract_waves(2 * (int)sizeof(T));

@@ -45,14 +45,15 @@ struct Layout {
 bool build_layout(const symgw_layout_desc *d, Layout &L, std::string &err);
 
 // ---- device work lists ------------------------------------------------------
-// Contraction workgroup shape: 12 waves (three per SIMD) on the whole
-// 160 KiB LDS of a CU, so large blocks still get aux-row chunks that fill the
-// 16-row MFMA tiles.  Same shape for both element sizes.
+// Contraction workgroup shape: one workgroup per CU on the whole 160 KiB LDS,
+// so large blocks still get aux-row chunks that fill the 16-row MFMA tiles;
+// 16 waves (four per SIMD, at most 128 VGPRs each) with complex128 operands,
+// 12 (three per SIMD, 168 VGPRs) with complex64, whose kernels need the
+// registers.
 constexpr int contract_lds(int elem_bytes) { return elem_bytes > 0 ? 160 * 1024 : 0; }
-constexpr int contract_waves(int elem_bytes) { return elem_bytes > 0 ? 12 : 0; }
+constexpr int contract_waves(int elem_bytes) { return elem_bytes == 16 ? 16 : elem_bytes > 0 ? 12 : 0; }
 // Accumulator tiles each contraction wave keeps across a slab: three with
-// complex64 operands, two with complex128 (the register file of three waves
-// per SIMD holds no more).
+// complex64 operands, two with complex128 (the register file holds no more).
 constexpr int contract_tiles(int elem_bytes) { return elem_bytes == 8 ? 3 : 2; }
 // Most output rows of one P0 slab (one MFMA tile row); the rows of an aux
 // block are split into balanced bands.
@@ -67,8 +68,8 @@ constexpr int kSigRows = 64;
 // Shape of the Sigma slabs of an ao block of k1 orbitals: `times` consecutive
 // times per slab (they share every V fragment; the last slab takes the
 // remaining times) and square `band` x `band` output bands, with
-// times * ceil(band / 16)^2 within the workgroup's 12 waves x contract_tiles
-// accumulator tiles (24 for complex128, 36 for complex64).
+// times * ceil(band / 16)^2 within the workgroup's contract_waves x
+// contract_tiles accumulator tiles (32 for complex128, 36 for complex64).
 //  - k1 <= 32: the whole block, four times.
 //  - k1 <= 48: the whole block, four times (complex64) or two (complex128).
 //  - wider blocks are split into bands of whole tiles, and every pair of bands
