@@ -75,17 +75,17 @@ constexpr int kSigRows = 64;
 //  - wider blocks are split into bands of whole tiles, and every pair of bands
 //    recomputes its Z (once per row band) and E (once per column band), so
 //    fewer, wider bands save work while fewer times per slab cost V loads:
-//    one time of the widest bands (6 tiles for complex64, 4 for complex128)
+//    one time of the widest bands (6 tiles for complex64, 5 for complex128)
 //    where that cuts the band count to 2/3 or less of two times of the
-//    widest bands that fit (4 and 3 tiles), else the latter (the faster of
-//    four, two and one times on MI355X at every k1 of 56..256 measured).
+//    widest bands that fit (4 tiles), else the latter (the faster of four,
+//    two and one times on MI355X at every k1 of 56..256 measured).
 struct SigShape {
   int times, band;
 };
 inline SigShape sig_shape(int elem_bytes, int k1) {
   if (k1 <= 32) return {4, k1};
   if (k1 <= kSigBand) return {elem_bytes == 8 ? 4 : 2, k1};
-  const int tiles = (k1 + 15) / 16, cap2 = elem_bytes == 8 ? 4 : 3, cap1 = elem_bytes == 8 ? 6 : 4;
+  const int tiles = (k1 + 15) / 16, cap2 = 4, cap1 = elem_bytes == 8 ? 6 : 5;
   auto bands = [&](int cap) { return (tiles + cap - 1) / cap; };
   auto even = [&](int cap) { const int nb = bands(cap); return 16 * ((tiles + nb - 1) / nb); };
   if (3 * bands(cap1) <= 2 * bands(cap2)) return {1, even(cap1)};
