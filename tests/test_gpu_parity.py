@@ -280,10 +280,12 @@ def test_self_consistent_loop_holds_the_electron_count():
     ctx.close()
 
 
-@pytest.mark.parametrize("ao,aux", [([66, 10], [100, 24]), ([10, 6], [200, 20])])
+@pytest.mark.parametrize("ao,aux", [([66, 10], [100, 24]), ([10, 6], [200, 20]), ([100, 10], [64, 24])])
 def test_large_blocks_parity(ao, aux):
     """Blocks that exercise every chunking path of the contraction kernels:
-    an ao block of 66 (> 48: two output bands in Sigma, orbital chunks in P0)
+    an ao block of 66 (> 48: the wide-block Sigma slabs, one time of one
+    80-wide band in FP64; orbital chunks in P0), an ao block of 100 (two times
+    of 64- and 36-wide bands: four band pairs per block)
     and an aux block of 100 (row bands in P0, aux-row chunks in Sigma); an aux
     block of 200 (more P0 column tiles than waves: the two times of a slab
     run one after the other).  P0, W and Sigma against the oracle."""
