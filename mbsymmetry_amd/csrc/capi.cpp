@@ -61,9 +61,7 @@ struct symgw_ctx {
   // device tables
   P0Item *d_p0_items = nullptr;
   P0Blk *d_p0_blks = nullptr;
-  SigItem *d_sig_items = nullptr;   // slabs of two times
-  SigItem *d_sig_items4 = nullptr;  // slabs of four times
-  SigItem *d_sig_items1 = nullptr;  // slabs of one time (wide blocks)
+  SigItem *d_sig_items[kNumSigDepths] = {};  // slabs of kSigDepths[d] times
   SigBlk *d_sig_blks = nullptr;
   RotBlk *d_g_rot = nullptr, *d_p_rot = nullptr;
   SolveBlk *d_w_blocks = nullptr, *d_d_blocks = nullptr;
@@ -205,17 +203,12 @@ int run_p0(symgw_ctx *c) {
 template <typename T>
 int run_sigma(symgw_ctx *c) {
   const Layout &L = c->L;
-  hipError_t e = launch_sigma<T>(c->d_sig_items, c->d_sig_blks, (int)c->W.sig_items.size(), L.nts, L.ns,
-                                 static_cast<const cplx<T> *>(c->V), static_cast<const cplx<T> *>(c->G_full),
-                                 static_cast<const cplx<T> *>(c->P_full), c->Sig, -1.0, c->W.sig_lds, c->st, 2);
-  if (e == hipSuccess)
-    e = launch_sigma<T>(c->d_sig_items4, c->d_sig_blks, (int)c->W.sig_items4.size(), L.nts, L.ns,
+  hipError_t e = hipSuccess;
+  for (int d = 0; d < kNumSigDepths && e == hipSuccess; ++d)
+    e = launch_sigma<T>(c->d_sig_items[d], c->d_sig_blks, (int)c->W.sig_items[d].size(), L.nts, L.ns,
                         static_cast<const cplx<T> *>(c->V), static_cast<const cplx<T> *>(c->G_full),
-                        static_cast<const cplx<T> *>(c->P_full), c->Sig, -1.0, c->W.sig_lds4, c->st, 4);
-  if (e == hipSuccess)
-    e = launch_sigma<T>(c->d_sig_items1, c->d_sig_blks, (int)c->W.sig_items1.size(), L.nts, L.ns,
-                        static_cast<const cplx<T> *>(c->V), static_cast<const cplx<T> *>(c->G_full),
-                        static_cast<const cplx<T> *>(c->P_full), c->Sig, -1.0, c->W.sig_lds1, c->st, 1);
+                        static_cast<const cplx<T> *>(c->P_full), c->Sig, -1.0, c->W.sig_lds[d], c->st,
+                        kSigDepths[d]);
   if (e != hipSuccess) return fail(c, SYMGW_ERR_HIP, std::string("Sigma kernel: ") + hipGetErrorString(e));
   return SYMGW_OK;
 }
@@ -519,8 +512,7 @@ int symgw_create(const symgw_layout_desc *desc, int device, int precision, symgw
     return SYMGW_ERR_NODEVICE;
   }
   c->device = device;
-  if (c->W.p0_lds > 160 * 1024 || c->W.sig_lds > 160 * 1024 || c->W.sig_lds4 > 160 * 1024 ||
-      c->W.sig_lds1 > 160 * 1024) {
+  if (c->W.p0_lds > 160 * 1024 || *std::max_element(c->W.sig_lds, c->W.sig_lds + kNumSigDepths) > 160 * 1024) {
     g_create_error = "LDS plan exceeds 160 KiB";
     return SYMGW_ERR_ARG;
   }
@@ -569,9 +561,7 @@ int symgw_create(const symgw_layout_desc *desc, int device, int precision, symgw
   // work lists
   r = r ? r : upload(cp, &c->d_p0_items, c->W.p0_items);
   r = r ? r : upload(cp, &c->d_p0_blks, c->W.p0_blks);
-  r = r ? r : upload(cp, &c->d_sig_items, c->W.sig_items);
-  r = r ? r : upload(cp, &c->d_sig_items4, c->W.sig_items4);
-  r = r ? r : upload(cp, &c->d_sig_items1, c->W.sig_items1);
+  for (int d = 0; d < kNumSigDepths; ++d) r = r ? r : upload(cp, &c->d_sig_items[d], c->W.sig_items[d]);
   r = r ? r : upload(cp, &c->d_sig_blks, c->W.sig_blks);
   r = r ? r : upload(cp, &c->d_g_rot, c->W.g_rot);
   r = r ? r : upload(cp, &c->d_p_rot, c->W.p_rot);

@@ -425,7 +425,7 @@ bool build_worklists(const Layout &L, int eb, WorkLists &W, std::string &err) {
   W.p0_lds = std::max(p0_lds, red_p0);
 
   // ---- Sigma: items per (k_ir, ik1, row band, column band); blocks in (q, V-block) order ----
-  size_t sig_lds = 0, sig_lds4 = 0, sig_lds1 = 0;
+  size_t sig_lds[kNumSigDepths] = {};
   std::map<std::array<int, 4>, std::array<int, 2>> sig_plans;
   for (int ikr = 0; ikr < L.ink; ++ikr) {
     int k = L.irre_list[ikr];
@@ -454,7 +454,7 @@ bool build_worklists(const Layout &L, int eb, WorkLists &W, std::string &err) {
           }
           B.ldz = odd_ld(B.qsub * B.nsub);
           B.lde = odd_ld(B.qsub * B.nsub);
-          size_t &lds_t = TB == 4 ? sig_lds4 : TB == 1 ? sig_lds1 : sig_lds;
+          size_t &lds_t = sig_lds[sig_depth_index(TB)];
           lds_t = std::max(lds_t, (size_t)TB * (band * (B.ldz + B.lde) + B.k2 * B.nsub) * eb);
           W.sig_blks.push_back(B);
         }
@@ -482,13 +482,11 @@ bool build_worklists(const Layout &L, int eb, WorkLists &W, std::string &err) {
           it.out_base = L.kao_off_irre[ikr] * ns * nts + L.ao_offsets[k][i1] + (int64_t)I0 * k1;
           it.out_tstride = (int32_t)(ns * L.kao_size_irre[ikr]);
           it.out_sstride = (int32_t)L.kao_size_irre[ikr];
-          (TB == 4 ? W.sig_items4 : TB == 1 ? W.sig_items1 : W.sig_items).push_back(it);
+          W.sig_items[sig_depth_index(TB)].push_back(it);
         }
     }
   }
-  W.sig_lds = W.sig_items.empty() ? 0 : std::max(sig_lds, red_bytes);
-  W.sig_lds4 = W.sig_items4.empty() ? 0 : std::max(sig_lds4, red_bytes);
-  W.sig_lds1 = W.sig_items1.empty() ? 0 : std::max(sig_lds1, red_bytes);
+  for (int d = 0; d < kNumSigDepths; ++d) W.sig_lds[d] = W.sig_items[d].empty() ? 0 : std::max(sig_lds[d], red_bytes);
 
   // ---- rotations of G (ao) and P (aux) to the full BZ ----
   for (int which = 0; which < 2; ++which) {
