@@ -287,13 +287,14 @@ __device__ __forceinline__ void k_part(int K, int S, int p, int &kb, int &ke) {
 
 // Work split of nt accumulator tiles over the NW waves: S waves share each
 // tile over S parts of the reduction (split-K), so nt tiles are spread over
-// NW / S tile groups with at most MT tiles per wave.  S (a divisor of NW)
-// minimises the per-wave work ceil(nt / (NW / S)) / S; ties keep the smaller S.
+// floor(NW / S) tile groups with at most MT tiles per wave; the NW mod S
+// waves past the last group hold no tiles (callers give them none).  S
+// minimises the per-wave work ceil(nt / floor(NW / S)) / S; ties keep the
+// smaller S.
 template <int NW, int MT>
 __device__ __forceinline__ int choose_split(int nt) {
   int best = 1, num = 1 << 30, den = 1;
   for (int s = 1; s <= NW; ++s) {
-    if (NW % s) continue;
     const int g = NW / s, tpw = (nt + g - 1) / g;
     if (tpw > MT) continue;
     if (tpw * den < num * s) {
@@ -506,7 +507,7 @@ __global__ __launch_bounds__(Cfg<T>::NT) void p0_mfma_kernel(const P0Item *__res
   // column tiles per wave (tg, tg + TG, ...) for one time.
   const int S = stack ? choose_split<NW, 1>(nct) : choose_split<NW, MT>(nct);
   const int TG = NW / S, kp = wave % S, tg = wave / S;
-  const int nmine = stack ? (tg < nct ? tbn : 0) : (nct > tg ? (nct - tg + TG - 1) / TG : 0);
+  const int nmine = stack ? (tg < nct ? tbn : 0) : (tg < TG && nct > tg ? (nct - tg + TG - 1) / TG : 0);
 
   for (int pass = 0; pass < npass; ++pass) {
     const int ta = t0 + pass;  // first time of this pass
@@ -657,7 +658,7 @@ __global__ __launch_bounds__(Cfg<T>::NT) void sigma_mfma_kernel(const SigItem *_
   // output tiles u = tb * ntile + tt: S waves share a tile over S parts of the
   // (Q, n) reduction
   const int S = choose_split<NW, MT>(nall), TG = NW / S, kp = wave % S, tg = wave / S;
-  const int nmine = nall > tg ? (nall - tg + TG - 1) / TG : 0;  // tiles tg, tg + TG, ...
+  const int nmine = tg < TG && nall > tg ? (nall - tg + TG - 1) / TG : 0;  // tiles tg, tg + TG, ...
   Acc<T> acc;
 #pragma unroll
   for (int c = 0; c < MT; ++c) tile_zero(acc[c]);
