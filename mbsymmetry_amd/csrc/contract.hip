@@ -482,7 +482,7 @@ __device__ __forceinline__ int gemm_jobs_cs(int M, int N, int K, int first, cons
 // product and their C rows as the row tiles of the rank-K update; otherwise
 // the times are processed one after the other.
 // ---------------------------------------------------------------------------
-template <typename T>
+template <typename T, bool STACK>
 __global__ __launch_bounds__(Cfg<T>::NT) void p0_mfma_kernel(const P0Item *__restrict__ items, const P0Blk *__restrict__ blks,
                                                     int64_t n_work, int nt_half, int nts, int ns,
                                                     const cplx<T> *__restrict__ V, const cplx<T> *__restrict__ G,
@@ -500,7 +500,7 @@ __global__ __launch_bounds__(Cfg<T>::NT) void p0_mfma_kernel(const P0Item *__res
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cplx<T> *lds = reinterpret_cast<cplx<T> *>(smem_raw);
   const int q = it.q, nband = it.nband, nct = (q + 15) / 16;
-  const bool stack = nct <= NW;
+  constexpr bool stack = STACK;  // nct <= NW for every item of a stacked list
   const int npass = stack ? 1 : ntimes, tbn = stack ? ntimes : 1;
   // column tiles: S waves share a tile over S parts of x.  Stacked: one column
   // tile per wave (tile tg) with one accumulator per time; otherwise up to MT
@@ -846,18 +846,25 @@ template hipError_t launch_rotate<float>(const RotBlk *, int, int, const cplx<do
                                          const int64_t *, const cplx<double> *, cplx<float> *, const int64_t *,
                                          const int64_t *, hipStream_t);
 
-template <typename T>
-hipError_t launch_p0(const P0Item *items, const P0Blk *blks, int n_items, int nts, int ns, const cplx<T> *V,
-                     const cplx<T> *G, cplx<double> *P0, double pref, size_t lds, hipStream_t st) {
+template <typename T, bool STACK>
+hipError_t launch_p0_k(const P0Item *items, const P0Blk *blks, int n_items, int nts, int ns, const cplx<T> *V,
+                       const cplx<T> *G, cplx<double> *P0, double pref, size_t lds, hipStream_t st) {
   const int nt_half = nts / 2;
   constexpr int TB = p0_times((int)sizeof(cplx<T>));
   const int64_t n_work = (int64_t)n_items * ((nt_half + TB - 1) / TB);
   if (n_work == 0) return hipSuccess;
-  (void)hipFuncSetAttribute((const void *)p0_mfma_kernel<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  hipLaunchKernelGGL((p0_mfma_kernel<T>), dim3((unsigned)xcd_grid(n_work)), dim3(Cfg<T>::NT), lds, st, items, blks,
-                     n_work,
-                     nt_half, nts, ns, V, G, P0, pref);
+  (void)hipFuncSetAttribute((const void *)p0_mfma_kernel<T, STACK>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)lds);
+  hipLaunchKernelGGL((p0_mfma_kernel<T, STACK>), dim3((unsigned)xcd_grid(n_work)), dim3(Cfg<T>::NT), lds, st, items,
+                     blks, n_work, nt_half, nts, ns, V, G, P0, pref);
   return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_p0(const P0Item *items, const P0Blk *blks, int n_items, int nts, int ns, const cplx<T> *V,
+                     const cplx<T> *G, cplx<double> *P0, double pref, size_t lds, hipStream_t st, bool stacked) {
+  return stacked ? launch_p0_k<T, true>(items, blks, n_items, nts, ns, V, G, P0, pref, lds, st)
+                 : launch_p0_k<T, false>(items, blks, n_items, nts, ns, V, G, P0, pref, lds, st);
 }
 
 template <typename T, int TB>
@@ -884,9 +891,9 @@ hipError_t launch_sigma(const SigItem *items, const SigBlk *blks, int n_items, i
 }
 
 template hipError_t launch_p0<double>(const P0Item *, const P0Blk *, int, int, int, const cplx<double> *,
-                                      const cplx<double> *, cplx<double> *, double, size_t, hipStream_t);
+                                      const cplx<double> *, cplx<double> *, double, size_t, hipStream_t, bool);
 template hipError_t launch_p0<float>(const P0Item *, const P0Blk *, int, int, int, const cplx<float> *,
-                                     const cplx<float> *, cplx<double> *, double, size_t, hipStream_t);
+                                     const cplx<float> *, cplx<double> *, double, size_t, hipStream_t, bool);
 template hipError_t launch_sigma<double>(const SigItem *, const SigBlk *, int, int, int, const cplx<double> *,
                                          const cplx<double> *, const cplx<double> *, cplx<double> *, double, size_t,
                                          hipStream_t, int);

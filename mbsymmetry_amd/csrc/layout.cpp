@@ -321,7 +321,7 @@ static std::vector<std::array<int, 2>> p0_dims(const Layout &L, int q, int iq) {
 // the block.
 static int p0_band(int qs, int band_cap, int budget, int eb, const std::vector<std::array<int, 2>> &dims) {
   const int TB = p0_times(eb);
-  const bool stack = (qs + 15) / 16 <= contract_waves(eb);  // the times share one pass
+  const bool stack = p0_stacked(eb, qs);  // the times share one pass
   const int tb = stack ? TB : 1, passes = stack ? 1 : TB;
   const double chunk_cost = 2.5e5;
   int best_band = balanced(qs, band_cap);
@@ -413,7 +413,7 @@ bool build_worklists(const Layout &L, int eb, WorkLists &W, std::string &err) {
         it.out_base = (int64_t)nts * L.qaux_off_irre[iqr] + L.aux_offsets[q][iq] + (int64_t)Q0 * qs;
         it.out_tstride = (int32_t)L.qaux_size_irre[iqr];
         it.pad2 = 0;
-        W.p0_items.push_back(it);
+        W.p0_items[p0_stacked(eb, qs) ? 1 : 0].push_back(it);
       }
     }
   }

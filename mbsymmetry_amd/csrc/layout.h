@@ -61,6 +61,11 @@ constexpr int kP0Band = 16;
 // Consecutive times of one P0 slab (they share every V fragment): four with
 // complex64 operands, two with complex128 (four spill registers there).
 constexpr int p0_times(int elem_bytes) { return elem_bytes == 8 ? 4 : 2; }
+// P0 slabs of aux blocks whose column tiles fit one per wave run the times of
+// a slab in one pass (stacked); larger aux blocks one time after the other.
+// Each kind has its own kernel instantiation, item list and launch, so that
+// the registers of the rarely used unstacked path do not burden the other.
+constexpr bool p0_stacked(int elem_bytes, int qs) { return (qs + 15) / 16 <= contract_waves(elem_bytes); }
 // Widest ao block that one Sigma slab covers whole (3 x 3 tiles of 16 per time).
 constexpr int kSigBand = 48;
 // Most aux rows Q of one Sigma LDS chunk.
@@ -165,7 +170,7 @@ struct SolveBlk {
 
 struct WorkLists {
   std::vector<P0Blk> p0_blks;
-  std::vector<P0Item> p0_items;
+  std::vector<P0Item> p0_items[2];  // [1]: stacked slabs (p0_stacked), [0]: the others
   std::vector<SigBlk> sig_blks;
   std::vector<SigItem> sig_items[kNumSigDepths];  // slabs of kSigDepths[d] times
   std::vector<RotBlk> g_rot;      // targets over the full BZ (ao)
