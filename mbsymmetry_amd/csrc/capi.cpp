@@ -59,7 +59,7 @@ struct symgw_ctx {
   double *kw = nullptr, *nel_dev = nullptr;
 
   // device tables
-  P0Item *d_p0_items[2] = {nullptr, nullptr};  // unstacked, stacked slabs
+  P0Item *d_p0_items = nullptr;
   P0Blk *d_p0_blks = nullptr;
   SigItem *d_sig_items[kNumSigDepths] = {};  // slabs of kSigDepths[d] times
   SigBlk *d_sig_blks = nullptr;
@@ -193,11 +193,9 @@ template <typename T>
 int run_p0(symgw_ctx *c) {
   const Layout &L = c->L;
   const double pref = L.ns == 1 ? -2.0 : -1.0;  // cugw_qkpt_block.cu:210
-  hipError_t e = hipSuccess;
-  for (int k = 0; k < 2 && e == hipSuccess; ++k)
-    e = launch_p0<T>(c->d_p0_items[k], c->d_p0_blks, (int)c->W.p0_items[k].size(), L.nts, L.ns,
-                     static_cast<const cplx<T> *>(c->V), static_cast<const cplx<T> *>(c->G_full), c->P, pref,
-                     c->W.p0_lds, c->st, k == 1);
+  hipError_t e = launch_p0<T>(c->d_p0_items, c->d_p0_blks, (int)c->W.p0_items.size(), L.nts, L.ns,
+                              static_cast<const cplx<T> *>(c->V), static_cast<const cplx<T> *>(c->G_full), c->P, pref,
+                              c->W.p0_lds, c->st, c->W.p0_mode);
   if (e != hipSuccess) return fail(c, SYMGW_ERR_HIP, std::string("P0 kernel: ") + hipGetErrorString(e));
   return SYMGW_OK;
 }
@@ -561,7 +559,7 @@ int symgw_create(const symgw_layout_desc *desc, int device, int precision, symgw
     if (!r && hipMemsetAsync(b.p, 0, b.bytes, c->st) != hipSuccess) r = fail(cp, SYMGW_ERR_HIP, "initial memset failed");
   if (!r && hipStreamSynchronize(c->st) != hipSuccess) r = fail(cp, SYMGW_ERR_HIP, "initial memset failed");
   // work lists
-  for (int k = 0; k < 2; ++k) r = r ? r : upload(cp, &c->d_p0_items[k], c->W.p0_items[k]);
+  r = r ? r : upload(cp, &c->d_p0_items, c->W.p0_items);
   r = r ? r : upload(cp, &c->d_p0_blks, c->W.p0_blks);
   for (int d = 0; d < kNumSigDepths; ++d) r = r ? r : upload(cp, &c->d_sig_items[d], c->W.sig_items[d]);
   r = r ? r : upload(cp, &c->d_sig_blks, c->W.sig_blks);

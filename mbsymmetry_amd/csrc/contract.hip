@@ -149,6 +149,10 @@ constexpr int kAhead = 2;
 // FP64 (no spills; Sigma +2.3%) and in P0 (FP32 +3%), two in the FP32 Sigma.
 template <typename T>
 constexpr int kAheadJob = sizeof(T) == 8 ? 1 : 2;
+// Sigma's Z and E products with one job layout each (column- and row-share)
+// instead of a per-chunk choice: FP64 (Sigma +1.2%, 5 -> 1 spilled registers).
+template <typename T>
+constexpr bool kSigOneLayout = sizeof(T) == 8;
 
 // acc[C0+c] += A (16 x K) * B_c (K x 16) for c < NG with C0 + c < n, over k in
 // [kb, ke): one A fragment per k-step shared by the tiles.  la(i, k), lb(c, k, j)
@@ -482,7 +486,7 @@ __device__ __forceinline__ int gemm_jobs_cs(int M, int N, int K, int first, cons
 // product and their C rows as the row tiles of the rank-K update; otherwise
 // the times are processed one after the other.
 // ---------------------------------------------------------------------------
-template <typename T, bool STACK>
+template <typename T, int MODE>
 __global__ __launch_bounds__(Cfg<T>::NT) void p0_mfma_kernel(const P0Item *__restrict__ items, const P0Blk *__restrict__ blks,
                                                     int64_t n_work, int nt_half, int nts, int ns,
                                                     const cplx<T> *__restrict__ V, const cplx<T> *__restrict__ G,
@@ -500,7 +504,8 @@ __global__ __launch_bounds__(Cfg<T>::NT) void p0_mfma_kernel(const P0Item *__res
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cplx<T> *lds = reinterpret_cast<cplx<T> *>(smem_raw);
   const int q = it.q, nband = it.nband, nct = (q + 15) / 16;
-  constexpr bool stack = STACK;  // nct <= NW for every item of a stacked list
+  // MODE 1: every slab stacked, 0: none, 2: per slab
+  const bool stack = MODE == 2 ? nct <= NW : MODE == 1;
   const int npass = stack ? 1 : ntimes, tbn = stack ? ntimes : 1;
   // column tiles: S waves share a tile over S parts of x.  Stacked: one column
   // tile per wave (tile tg) with one accumulator per time; otherwise up to MT
@@ -719,8 +724,9 @@ __global__ __launch_bounds__(Cfg<T>::NT) void sigma_mfma_kernel(const SigItem *_
             const int tb = dnr.div(row), r = row - tb * nr, jj = dnn.div(x), l = x - jj * nn;
             bufZ[(tb * Jb + jj) * ldz + r * nn + l] = cplx<T>{v.re, -v.im};
           };
-          // several row tiles: share each V fragment among them
-          if (tbn * nr > 16) nj = gemm_jobs_cs<T, true>(tbn * nr, zc, q, 0, la, lb, st);
+          // several row tiles: share each V fragment among them (always in
+          // FP64, where one job layout per product saves 4 spilled registers)
+          if (kSigOneLayout<T> || tbn * nr > 16) nj = gemm_jobs_cs<T, true>(tbn * nr, zc, q, 0, la, lb, st);
           else nj = gemm_jobs_rs<T, true>(tbn * nr, zc, q, 0, la, lb, st);
         }
         // E[(r,i)][(tb,l)] = sum_m V[Q0+r][I0+i][m] G2_tb[m][n0+l]  (jobs dealt on after Z's)
@@ -735,7 +741,7 @@ __global__ __launch_bounds__(Cfg<T>::NT) void sigma_mfma_kernel(const SigItem *_
             const int r = dIb.div(row), ii = row - r * Ib, tb = dnn.div(col), l = col - tb * nn;
             bufE[(tb * Ib + ii) * lde + r * nn + l] = v;
           };
-          if (tbn * nn > 16) gemm_jobs_rs<T>(nr * Ib, tbn * nn, k2, nj, la, lb, st);
+          if (kSigOneLayout<T> || tbn * nn > 16) gemm_jobs_rs<T>(nr * Ib, tbn * nn, k2, nj, la, lb, st);
           else gemm_jobs_cs<T>(nr * Ib, tbn * nn, k2, nj, la, lb, st);
         }
         __syncthreads();
@@ -846,25 +852,26 @@ template hipError_t launch_rotate<float>(const RotBlk *, int, int, const cplx<do
                                          const int64_t *, const cplx<double> *, cplx<float> *, const int64_t *,
                                          const int64_t *, hipStream_t);
 
-template <typename T, bool STACK>
+template <typename T, int MODE>
 hipError_t launch_p0_k(const P0Item *items, const P0Blk *blks, int n_items, int nts, int ns, const cplx<T> *V,
                        const cplx<T> *G, cplx<double> *P0, double pref, size_t lds, hipStream_t st) {
   const int nt_half = nts / 2;
   constexpr int TB = p0_times((int)sizeof(cplx<T>));
   const int64_t n_work = (int64_t)n_items * ((nt_half + TB - 1) / TB);
   if (n_work == 0) return hipSuccess;
-  (void)hipFuncSetAttribute((const void *)p0_mfma_kernel<T, STACK>, hipFuncAttributeMaxDynamicSharedMemorySize,
+  (void)hipFuncSetAttribute((const void *)p0_mfma_kernel<T, MODE>, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)lds);
-  hipLaunchKernelGGL((p0_mfma_kernel<T, STACK>), dim3((unsigned)xcd_grid(n_work)), dim3(Cfg<T>::NT), lds, st, items,
+  hipLaunchKernelGGL((p0_mfma_kernel<T, MODE>), dim3((unsigned)xcd_grid(n_work)), dim3(Cfg<T>::NT), lds, st, items,
                      blks, n_work, nt_half, nts, ns, V, G, P0, pref);
   return hipGetLastError();
 }
 
 template <typename T>
 hipError_t launch_p0(const P0Item *items, const P0Blk *blks, int n_items, int nts, int ns, const cplx<T> *V,
-                     const cplx<T> *G, cplx<double> *P0, double pref, size_t lds, hipStream_t st, bool stacked) {
-  return stacked ? launch_p0_k<T, true>(items, blks, n_items, nts, ns, V, G, P0, pref, lds, st)
-                 : launch_p0_k<T, false>(items, blks, n_items, nts, ns, V, G, P0, pref, lds, st);
+                     const cplx<T> *G, cplx<double> *P0, double pref, size_t lds, hipStream_t st, int mode) {
+  if (mode == 1) return launch_p0_k<T, 1>(items, blks, n_items, nts, ns, V, G, P0, pref, lds, st);
+  if (mode == 0) return launch_p0_k<T, 0>(items, blks, n_items, nts, ns, V, G, P0, pref, lds, st);
+  return launch_p0_k<T, 2>(items, blks, n_items, nts, ns, V, G, P0, pref, lds, st);
 }
 
 template <typename T, int TB>
@@ -891,9 +898,9 @@ hipError_t launch_sigma(const SigItem *items, const SigBlk *blks, int n_items, i
 }
 
 template hipError_t launch_p0<double>(const P0Item *, const P0Blk *, int, int, int, const cplx<double> *,
-                                      const cplx<double> *, cplx<double> *, double, size_t, hipStream_t, bool);
+                                      const cplx<double> *, cplx<double> *, double, size_t, hipStream_t, int);
 template hipError_t launch_p0<float>(const P0Item *, const P0Blk *, int, int, int, const cplx<float> *,
-                                     const cplx<float> *, cplx<double> *, double, size_t, hipStream_t, bool);
+                                     const cplx<float> *, cplx<double> *, double, size_t, hipStream_t, int);
 template hipError_t launch_sigma<double>(const SigItem *, const SigBlk *, int, int, int, const cplx<double> *,
                                          const cplx<double> *, const cplx<double> *, cplx<double> *, double, size_t,
                                          hipStream_t, int);

@@ -359,6 +359,7 @@ bool build_worklists(const Layout &L, int eb, WorkLists &W, std::string &err) {
 
   // ---- P0: items per (q_ir, iq, band of <= 16 rows); blocks in (k, V-block) order ----
   size_t p0_lds = 0;
+  int64_t n_stacked = 0, n_unstacked = 0;
   for (int iqr = 0; iqr < L.ink; ++iqr) {
     int q = L.irre_list[iqr];
     for (size_t iq = 0; iq < L.aux_sizes[q].size(); ++iq) {
@@ -413,7 +414,8 @@ bool build_worklists(const Layout &L, int eb, WorkLists &W, std::string &err) {
         it.out_base = (int64_t)nts * L.qaux_off_irre[iqr] + L.aux_offsets[q][iq] + (int64_t)Q0 * qs;
         it.out_tstride = (int32_t)L.qaux_size_irre[iqr];
         it.pad2 = 0;
-        W.p0_items[p0_stacked(eb, qs) ? 1 : 0].push_back(it);
+        W.p0_items.push_back(it);
+        (p0_stacked(eb, qs) ? n_stacked : n_unstacked) += 1;
       }
     }
   }
@@ -423,6 +425,7 @@ bool build_worklists(const Layout &L, int eb, WorkLists &W, std::string &err) {
   // P0 keeps one accumulator per stacked time (at least contract_tiles)
   const size_t red_p0 = (size_t)contract_waves(eb) * std::max(contract_tiles(eb), p0_times(eb)) * 64 * 8 * (eb / 2);
   W.p0_lds = std::max(p0_lds, red_p0);
+  W.p0_mode = n_unstacked == 0 ? 1 : n_stacked == 0 ? 0 : 2;
 
   // ---- Sigma: items per (k_ir, ik1, row band, column band); blocks in (q, V-block) order ----
   size_t sig_lds[kNumSigDepths] = {};

@@ -63,8 +63,10 @@ constexpr int kP0Band = 16;
 constexpr int p0_times(int elem_bytes) { return elem_bytes == 8 ? 4 : 2; }
 // P0 slabs of aux blocks whose column tiles fit one per wave run the times of
 // a slab in one pass (stacked); larger aux blocks one time after the other.
-// Each kind has its own kernel instantiation, item list and launch, so that
-// the registers of the rarely used unstacked path do not burden the other.
+// A system with one kind runs the kernel instantiation for that kind alone,
+// so that the registers of the other path do not burden it; a system with
+// both runs one launch of the instantiation that chooses per slab (two
+// launches in sequence would leave the GPU to the tail of the first).
 constexpr bool p0_stacked(int elem_bytes, int qs) { return (qs + 15) / 16 <= contract_waves(elem_bytes); }
 // Widest ao block that one Sigma slab covers whole (3 x 3 tiles of 16 per time).
 constexpr int kSigBand = 48;
@@ -170,7 +172,8 @@ struct SolveBlk {
 
 struct WorkLists {
   std::vector<P0Blk> p0_blks;
-  std::vector<P0Item> p0_items[2];  // [1]: stacked slabs (p0_stacked), [0]: the others
+  std::vector<P0Item> p0_items;
+  int p0_mode = 0;  // 1: every slab stacked, 0: none, 2: both kinds (chosen per slab)
   std::vector<SigBlk> sig_blks;
   std::vector<SigItem> sig_items[kNumSigDepths];  // slabs of kSigDepths[d] times
   std::vector<RotBlk> g_rot;      // targets over the full BZ (ao)
