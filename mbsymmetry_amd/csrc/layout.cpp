@@ -490,6 +490,9 @@ bool build_worklists(const Layout &L, int eb, WorkLists &W, std::string &err) {
     }
   }
   for (int d = 0; d < kNumSigDepths; ++d) W.sig_lds[d] = W.sig_items[d].empty() ? 0 : std::max(sig_lds[d], red_bytes);
+  for (int d = 0; d < kNumSigDepths; ++d)
+    if (W.sig_lds[d] > (size_t)contract_lds(eb)) { err = fmt("Sigma LDS plan exceeds the CU's LDS (bytes)", (int)W.sig_lds[d]); return false; }
+  if (W.p0_lds > (size_t)contract_lds(eb)) { err = fmt("P0 LDS plan exceeds the CU's LDS (bytes)", (int)W.p0_lds); return false; }
 
   // ---- rotations of G (ao) and P (aux) to the full BZ ----
   for (int which = 0; which < 2; ++which) {

@@ -83,3 +83,21 @@ def test_create_without_device_fails_cleanly():
     sysm = S.tiny()
     with pytest.raises(_lib.SymGWError):
         _lib.Context(sysm)
+
+
+def test_planner_fits_every_block_width():
+    """The host planner finds work lists within the LDS and accumulator
+    capacity of a workgroup for ao blocks of every width up to 256 and aux
+    blocks up to 256, in both precisions: the widths sit on the edges of the
+    planner's choices (16-wide tiles, whole-block Sigma slabs up to 32 / 48,
+    64- / 80- / 96-wide bands, P0 slabs stacked up to 192 or 256 aux rows).
+    symgw_layout_footprint runs the same planner as symgw_create and reports
+    its errors."""
+    from mbsymmetry_amd.transforms import chebyshev
+    widths = [1, 15, 16, 17, 31, 32, 33, 47, 48, 49, 63, 64, 65, 79, 80, 81, 96, 97, 128, 129, 160, 161, 200,
+              255, 256]
+    for k1 in widths:
+        for qs in (1, 17, 191, 192, 193, 256):
+            sysm = S.synthetic("widths", 2, 4, [k1, 5], [qs, 7], 1, chebyshev(4, 6.0), 3)
+            for prec in (_lib.PREC_FP64, _lib.PREC_FP32):
+                assert _lib.layout_footprint(sysm, prec) > 0, (k1, qs, prec)
