@@ -10,7 +10,7 @@
 namespace symgw {
 namespace {
 
-constexpr int kPeakThreads = 768;  // the contraction kernels' workgroup: 12 waves
+constexpr int kPeakThreads = 768;  // 12 waves, three per SIMD (the FP32 contraction workgroup)
 
 template <typename T>
 struct PeakOp;
