@@ -474,7 +474,7 @@ __device__ __forceinline__ int gemm_jobs_cs(int M, int N, int K, int first, cons
 
 // ---------------------------------------------------------------------------
 // P0.  Slab = rows [Q0, Q0 + nband) (nband <= 16) of aux block iq of iBZ point
-// q, for p0_times consecutive times t < nts/2.  For every V block (k, ik1, ik2)
+// q, for p0_times(MODE) consecutive times t < nts/2.  For every V block (k, ik1, ik2)
 // and spin:
 //   E_Q = V_Q G^{k+q}(t),  C_Q = G^k(beta - t) E_Q,  P0[Q][P] += sum_x C_Q[x] conj(V_P[x])
 // i.e. the three GEMMs of cugw_qkpt_block.cu:231-260 (X1 = G V, X2 = conj(V) G^T,
@@ -491,7 +491,7 @@ __global__ __launch_bounds__(Cfg<T>::NT) void p0_mfma_kernel(const P0Item *__res
                                                     int64_t n_work, int nt_half, int nts, int ns,
                                                     const cplx<T> *__restrict__ V, const cplx<T> *__restrict__ G,
                                                     cplx<double> *__restrict__ P0, double pref) {
-  constexpr int TB = p0_times((int)sizeof(cplx<T>));
+  constexpr int TB = p0_times((int)sizeof(cplx<T>), MODE);
   const int64_t w = xcd_logical(blockIdx.x, gridDim.x);
   if (w >= n_work) return;
   const int np = (nt_half + TB - 1) / TB;
@@ -856,7 +856,7 @@ template <typename T, int MODE>
 hipError_t launch_p0_k(const P0Item *items, const P0Blk *blks, int n_items, int nts, int ns, const cplx<T> *V,
                        const cplx<T> *G, cplx<double> *P0, double pref, size_t lds, hipStream_t st) {
   const int nt_half = nts / 2;
-  constexpr int TB = p0_times((int)sizeof(cplx<T>));
+  constexpr int TB = p0_times((int)sizeof(cplx<T>), MODE);
   const int64_t n_work = (int64_t)n_items * ((nt_half + TB - 1) / TB);
   if (n_work == 0) return hipSuccess;
   (void)hipFuncSetAttribute((const void *)p0_mfma_kernel<T, MODE>, hipFuncAttributeMaxDynamicSharedMemorySize,

@@ -319,8 +319,7 @@ static std::vector<std::array<int, 2>> p0_dims(const Layout &L, int q, int iq) {
 // band_cap, pick the one with the fewest executed MFMA FLOPs (16x16x4 tiles,
 // padding included) plus a fixed cost per chunk, summed over the bands of
 // the block.
-static int p0_band(int qs, int band_cap, int budget, int eb, const std::vector<std::array<int, 2>> &dims) {
-  const int TB = p0_times(eb);
+static int p0_band(int qs, int band_cap, int budget, int eb, int TB, const std::vector<std::array<int, 2>> &dims) {
   const bool stack = p0_stacked(eb, qs);  // the times share one pass
   const int tb = stack ? TB : 1, passes = stack ? 1 : TB;
   const double chunk_cost = 2.5e5;
@@ -359,7 +358,12 @@ bool build_worklists(const Layout &L, int eb, WorkLists &W, std::string &err) {
 
   // ---- P0: items per (q_ir, iq, band of <= 16 rows); blocks in (k, V-block) order ----
   size_t p0_lds = 0;
+  // kernel mode first: it fixes the times per slab, which the chunking needs
   int64_t n_stacked = 0, n_unstacked = 0;
+  for (int iqr = 0; iqr < L.ink; ++iqr)
+    for (int qs : L.aux_sizes[L.irre_list[iqr]]) (p0_stacked(eb, qs) ? n_stacked : n_unstacked) += 1;
+  W.p0_mode = n_unstacked == 0 ? 1 : n_stacked == 0 ? 0 : 2;
+  const int p0_tb = p0_times(eb, W.p0_mode);
   for (int iqr = 0; iqr < L.ink; ++iqr) {
     int q = L.irre_list[iqr];
     for (size_t iq = 0; iq < L.aux_sizes[q].size(); ++iq) {
@@ -378,9 +382,9 @@ bool build_worklists(const Layout &L, int eb, WorkLists &W, std::string &err) {
         for (auto &t : L.V_irreps[p])
           if (t[0] == (int)iq) maxk1 = std::max(maxk1, L.ao_sizes[k][t[1]]);
       }
-      const int band_cap = std::min(kP0Band, budget / (2 * p0_times(eb) * (maxk1 + 1)));
+      const int band_cap = std::min(kP0Band, budget / (2 * p0_tb * (maxk1 + 1)));
       if (band_cap < 1) { err = fmt("P0 ao block too large for the LDS plan (k1)", maxk1); return false; }
-      const int band = p0_band(qs, band_cap, budget, eb, p0_dims(L, q, (int)iq));
+      const int band = p0_band(qs, band_cap, budget, eb, p0_tb, p0_dims(L, q, (int)iq));
       int b0 = (int)W.p0_blks.size();
       for (int k = 0; k < nk; ++k) {
         int k2 = L.kpq[(size_t)k * nk + q];
@@ -398,11 +402,11 @@ bool build_worklists(const Layout &L, int eb, WorkLists &W, std::string &err) {
           B.g2_tstride = (int32_t)(ns * L.kao_size[k2]); B.g2_sstride = (int32_t)L.kao_size[k2];
           // E and C of one chunk ([times][band][k1*nsub] each, +1 element of
           // row padding) share the LDS budget
-          const int cap = std::min(B.k2, (budget / (2 * p0_times(eb) * band) - 1) / B.k1);
+          const int cap = std::min(B.k2, (budget / (2 * p0_tb * band) - 1) / B.k1);
           if (cap < 1) { err = fmt("P0 block too large for the LDS plan (k1, k2)", B.k1, B.k2); return false; }
           B.nsub = balanced(B.k2, cap);
           B.lde = odd_ld(B.k1 * B.nsub);
-          p0_lds = std::max(p0_lds, (size_t)(2 * p0_times(eb) * band * B.lde) * eb);
+          p0_lds = std::max(p0_lds, (size_t)(2 * p0_tb * band * B.lde) * eb);
           W.p0_blks.push_back(B);
         }
       }
@@ -415,7 +419,6 @@ bool build_worklists(const Layout &L, int eb, WorkLists &W, std::string &err) {
         it.out_tstride = (int32_t)L.qaux_size_irre[iqr];
         it.pad2 = 0;
         W.p0_items.push_back(it);
-        (p0_stacked(eb, qs) ? n_stacked : n_unstacked) += 1;
       }
     }
   }
@@ -423,9 +426,8 @@ bool build_worklists(const Layout &L, int eb, WorkLists &W, std::string &err) {
   // contract_tiles tiles x 64 lanes x 8 reals
   const size_t red_bytes = (size_t)contract_waves(eb) * contract_tiles(eb) * 64 * 8 * (eb / 2);
   // P0 keeps one accumulator per stacked time (at least contract_tiles)
-  const size_t red_p0 = (size_t)contract_waves(eb) * std::max(contract_tiles(eb), p0_times(eb)) * 64 * 8 * (eb / 2);
+  const size_t red_p0 = (size_t)contract_waves(eb) * std::max(contract_tiles(eb), p0_tb) * 64 * 8 * (eb / 2);
   W.p0_lds = std::max(p0_lds, red_p0);
-  W.p0_mode = n_unstacked == 0 ? 1 : n_stacked == 0 ? 0 : 2;
 
   // ---- Sigma: items per (k_ir, ik1, row band, column band); blocks in (q, V-block) order ----
   size_t sig_lds[kNumSigDepths] = {};

@@ -58,9 +58,12 @@ constexpr int contract_tiles(int elem_bytes) { return elem_bytes == 8 ? 3 : 2; }
 // Most output rows of one P0 slab (one MFMA tile row); the rows of an aux
 // block are split into balanced bands.
 constexpr int kP0Band = 16;
-// Consecutive times of one P0 slab (they share every V fragment): four with
-// complex64 operands, two with complex128 (four spill registers there).
-constexpr int p0_times(int elem_bytes) { return elem_bytes == 8 ? 4 : 2; }
+// Consecutive times of one P0 slab (they share every V fragment), per kernel
+// mode (p0_mode below): with complex64 operands six in the stacked-only
+// kernel (the split-K buffer of 12 waves x 6 accumulator tiles then takes 144
+// of the 160 KiB of LDS; 3.8% faster than four) and four in the others (six
+// spill there); two with complex128 (four spill registers there).
+constexpr int p0_times(int elem_bytes, int mode) { return elem_bytes == 8 ? (mode == 1 ? 6 : 4) : 2; }
 // P0 slabs of aux blocks whose column tiles fit one per wave run the times of
 // a slab in one pass (stacked); larger aux blocks one time after the other.
 // A system with one kind runs the kernel instantiation for that kind alone,
