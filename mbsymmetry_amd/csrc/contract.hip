@@ -95,9 +95,6 @@ __device__ __forceinline__ void tile_zero(Tile3<T> &t) {
   t.p3 = t.p1;
 }
 
-template <typename T>
-__device__ __forceinline__ cplx<T> cz() { return cplx<T>{T(0), T(0)}; }
-
 // acc += a * b (CB: acc += a * conj(b)) for one 16x16x4 complex step, four
 // real MFMAs.  Lane l holds A[l%16][k0+l/16] and B[k0+l/16][l%16] (operand
 // layout of the 16x16x4 MFMAs).  The conjugate of B is folded into the signs
@@ -361,33 +358,6 @@ __device__ __forceinline__ void job_store(const Tile3<T> &acc, const F &f) {
   for (int r = 0; r < 4; ++r) {
     const T p1 = acc.p1[r], p2 = acc.p2[r], p3 = acc.p3[r];
     f(Mfma<T>::row(lane, r), lane & 15, CB ? cplx<T>{p1 + p2, p3 - p1 + p2} : cplx<T>{p1 - p2, p3 - p1 - p2});
-  }
-}
-
-// Copy n contiguous elements src[0, n) to dst[ (e / len) * ld + e % len ]
-// (rows of length len, row stride ld) through f: all U loads of a thread are
-// issued before its stores, so U * NT loads are in flight per round.
-template <typename T, int U, typename F>
-__device__ __forceinline__ void stage_rows(cplx<T> *dst, int ld, const cplx<T> *__restrict__ src, int n, int len,
-                                           const F &f) {
-  constexpr int NT = Cfg<T>::NT;
-  const int tid = threadIdx.x;
-  const FDiv dlen(len);
-  for (int base = 0; base < n; base += U * NT) {
-    cplx<T> v[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int e = base + u * NT + tid;
-      v[u] = e < n ? src[e] : cz<T>();
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int e = base + u * NT + tid;
-      if (e < n) {
-        const int r = dlen.div(e), x = e - r * len;
-        dst[r * ld + x] = f(v[u]);
-      }
-    }
   }
 }
 
