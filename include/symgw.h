@@ -113,7 +113,10 @@ int symgw_layout_query(const symgw_layout_desc *desc, symgw_layout_info *info);
 /* Device bytes a context of this layout allocates in the given precision
  * (SYMGW_PREC_*): the footprint model behind admission control.  Counterpart
  * of gw_qkpt_block::size (SymmetryMBPT/include/cuda/cugw_qkpt_block.h:45-53),
- * for the whole resident system instead of one stream's scratch. */
+ * for the whole resident system instead of one stream's scratch.  Runs the
+ * same work planner as symgw_create, so a layout the kernels cannot take
+ * (a block above 256, a plan beyond the CU's LDS) fails here with the same
+ * message (symgw_create_error). */
 int symgw_layout_footprint(const symgw_layout_desc *desc, int precision, int64_t *bytes);
 
 /* Free and total memory of a device (hipMemGetInfo); the check
