@@ -133,6 +133,34 @@ int symgw_device_memory(int device, int64_t *free_bytes, int64_t *total_bytes);
  * Allocates ~2 GiB for the duration of the call. */
 int symgw_measure_peaks(int device, double *out);
 
+/* Intermediate-representation basis of the logistic kernel (no counterpart in
+ * the reference, which reads precomputed grids: transformer_t.cpp:48-82 reads
+ * xsample, wsample, uxl, uwl, ux1l of ir_grid/<Lambda>_<nl>.h5).  Computes the
+ * first nl singular functions u_l(x), x in [-1, 1], of
+ *   statistics 1 (fermionic): K(x, y) = e^{-Lambda y (x+1)/2} / (1 + e^{-Lambda y})
+ *   statistics 0 (bosonic):   K(x, y) = y e^{-Lambda y (x+1)/2} / (1 - e^{-Lambda y})
+ * (binary128 SVD of a Gauss-Legendre discretization with deg points per
+ * segment; deg = 24 resolves the functions to ~1e-12), normalized with
+ * u_l(1) > 0, as per-segment Legendre series: *nseg_out segments with
+ * edges[0 .. nseg], coef[l][seg][k] (coef holds nl x max_seg x deg doubles).
+ * Also returns the singular values s[nl], the tau sampling points
+ * xsample[nl] (midpoints between the zeros of u_{nl-1} with -1 and 1 added),
+ * the Matsubara sampling indices wsample[nl] (the discrete extrema of
+ * u^_{nl-1}, mirrored; fermionic nl even, bosonic nl odd) and
+ * uwl[2 * (i * nl + l)] = u^_l(i w_{wsample[i]}) (re, im), where
+ * u^_l(i w_n) = 2^{-1/2} int_{-1}^{1} e^{i pi (n + zeta/2)(x + 1)} u_l(x) dx.
+ * Returns 0; 1 when nl exceeds the numerical rank; 2 on bad arguments (or
+ * max_seg too small); 3 / 4 when the Matsubara / tau sampling points cannot
+ * be placed (nl beyond what the discretization resolves).  Host only, uses
+ * the calling process's OpenMP threads; Lambda = 1e4, nl = 104 takes ~15 s
+ * on 8 cores. */
+int symgw_ir_basis(int statistics, double lambda, int nl, int max_seg, int deg, int *nseg_out, double *edges,
+                   double *coef, double *s, double *xsample, int64_t *wsample, double *uwl);
+/* u^_l(i w_n) of such a basis at nfreq caller-given indices n[i]:
+ * out[2 * (i * nl + l)] (re, im).  Returns 0, or 2 on bad arguments. */
+int symgw_ir_uhat(int statistics, int nl, int nseg, int deg, int max_seg, const double *edges, const double *coef,
+                  int nfreq, const int64_t *n, double *out);
+
 /* Offset of the flat V slice of ordered pair (k1, k2) in complex elements, or
  * -1 when the pair is not stored (symmetry_utils.h:114-123). */
 int64_t symgw_kpair_offset(const symgw_ctx *ctx, int k1, int k2);

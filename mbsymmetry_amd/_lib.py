@@ -76,6 +76,10 @@ SIGNATURES = {
     "symgw_sync": (C.c_int, [C.c_void_p]),
     "symgw_last_times": (C.c_int, [C.c_void_p, _dp]),
     "symgw_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
+    "symgw_ir_basis": (C.c_int, [C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int), _dp, _dp, _dp,
+                                 _dp, C.POINTER(C.c_int64), _dp]),
+    "symgw_ir_uhat": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _dp, _dp, C.c_int,
+                                C.POINTER(C.c_int64), _dp]),
 }
 
 _lib = None
@@ -97,8 +101,28 @@ def load():
 
 
 def dptr(a: np.ndarray):
-    assert a.dtype in (np.float64, np.complex128) and a.flags["C_CONTIGUOUS"]
+    if not isinstance(a, np.ndarray) or a.dtype not in (np.float64, np.complex128) or not a.flags["C_CONTIGUOUS"]:
+        raise TypeError("expected a C-contiguous float64 or complex128 numpy array")
     return a.ctypes.data_as(_dp)
+
+
+def checked(a, dtype, n, what):
+    """`a` as a C-contiguous array of `dtype` holding exactly `n` elements
+    (the length the library reads or writes for it); ValueError otherwise."""
+    a = np.ascontiguousarray(a, dtype=dtype)
+    if a.size != n:
+        raise ValueError(f"{what}: {a.size} elements given, the layout needs {n}")
+    return a
+
+
+def checked_out(a, n, what):
+    """An output buffer the library writes in place: must already be a
+    C-contiguous complex128 array of exactly `n` elements."""
+    if not isinstance(a, np.ndarray) or a.dtype != np.complex128 or not a.flags["C_CONTIGUOUS"]:
+        raise TypeError(f"{what}: expected a C-contiguous complex128 numpy array")
+    if a.size != n:
+        raise ValueError(f"{what}: {a.size} elements given, the layout needs {n}")
+    return a
 
 
 class SymGWError(RuntimeError):
@@ -118,6 +142,15 @@ class Context:
             raise SymGWError(f"symgw_create failed ({rc}): {self.lib.symgw_create_error().decode()}")
         self.h = h
         self.precision = precision
+        inf = self.info()
+        d = self.desc
+        # element counts of every host array the entry points read or write
+        self.n_ts_kij = d.nts * d.ns * inf.flat_ao_size
+        self.n_q_t_QP = d.nts * inf.flat_aux_size
+        self.n_fock = d.ns * inf.flat_ao_size
+        self.n_V = inf.flat_V_size
+        self.n_orep, self.n_auxrep = inf.flat_orep_size, inf.flat_auxrep_size
+        self.nts, self.nw, self.nw_b, self.ink = d.nts, d.nw, d.nw_b, d.ink
 
     def _chk(self, rc, what):
         if rc != SYMGW_OK:
@@ -140,7 +173,7 @@ class Context:
         return inf
 
     def set_V(self, V: np.ndarray):
-        V = np.ascontiguousarray(V, dtype=np.complex128)
+        V = checked(V, np.complex128, self.n_V, "set_V")
         self._chk(self.lib.symgw_set_V(self.h, dptr(V), V.size), "set_V")
 
     def generate_V(self, seed: int, scale: float, decay: np.ndarray):
@@ -148,17 +181,17 @@ class Context:
         self._chk(self.lib.symgw_generate_V(self.h, seed, scale, dptr(decay), decay.size), "generate_V")
 
     def set_reps(self, orep: np.ndarray, auxrep: np.ndarray):
-        o = np.ascontiguousarray(orep, dtype=np.complex128)
-        a = np.ascontiguousarray(auxrep, dtype=np.complex128)
+        o = checked(orep, np.complex128, self.n_orep, "set_reps orep")
+        a = checked(auxrep, np.complex128, self.n_auxrep, "set_reps auxrep")
         self._chk(self.lib.symgw_set_reps(self.h, dptr(o) if o.size else None, dptr(a) if a.size else None),
                   "set_reps")
 
     def upload_G(self, G):
-        G = np.ascontiguousarray(G, dtype=np.complex128)
+        G = checked(G, np.complex128, self.n_ts_kij, "upload_G")
         self._chk(self.lib.symgw_upload_G(self.h, dptr(G)), "upload_G")
 
     def download_G(self, shape):
-        out = np.zeros(shape, dtype=np.complex128)
+        out = checked_out(np.zeros(shape, dtype=np.complex128), self.n_ts_kij, "download_G")
         self._chk(self.lib.symgw_download_G(self.h, dptr(out)), "download_G")
         return out
 
@@ -166,16 +199,20 @@ class Context:
         self._chk(self.lib.symgw_solve_P0(self.h), "solve_P0")
 
     def copy_P0_to_host(self, out: np.ndarray):
-        assert out.dtype == np.complex128 and out.flags["C_CONTIGUOUS"]
+        checked_out(out, self.n_q_t_QP, "copy_P0_to_host")
         self._chk(self.lib.symgw_copy_P0_to_host(self.h, dptr(out)), "copy_P0_to_host")
         return out
 
     def upload_P(self, P):
-        P = np.ascontiguousarray(P, dtype=np.complex128)
+        P = checked(P, np.complex128, self.n_q_t_QP, "upload_P")
         self._chk(self.lib.symgw_upload_P(self.h, dptr(P)), "upload_P")
 
     def set_transforms(self, tr):
-        arrs = [np.ascontiguousarray(a, dtype=np.complex128) for a in (tr.Tnt, tr.Ttn, tr.Tnt_BF, tr.Ttn_FB, tr.iwn)]
+        ni = self.nts - 2
+        want = ((tr.Tnt, self.nw * ni, "Tnt"), (tr.Ttn, self.nts * self.nw, "Ttn"),
+                (tr.Tnt_BF, self.nw_b * ni, "Tnt_BF"), (tr.Ttn_FB, self.nts * self.nw_b, "Ttn_FB"),
+                (tr.iwn, self.nw, "iwn"))
+        arrs = [checked(a, np.complex128, n, "set_transforms " + name) for a, n, name in want]
         self._keep = arrs
         self._chk(self.lib.symgw_set_transforms(self.h, *[dptr(a) for a in arrs]), "set_transforms")
 
@@ -186,30 +223,30 @@ class Context:
         self._chk(self.lib.symgw_solve_Sigma(self.h), "solve_Sigma")
 
     def copy_Sigma_to_host(self, out: np.ndarray):
-        assert out.dtype == np.complex128 and out.flags["C_CONTIGUOUS"]
+        checked_out(out, self.n_ts_kij, "copy_Sigma_to_host")
         self._chk(self.lib.symgw_copy_Sigma_to_host(self.h, dptr(out)), "copy_Sigma_to_host")
         return out
 
     def upload_Sigma(self, S):
-        S = np.ascontiguousarray(S, dtype=np.complex128)
+        S = checked(S, np.complex128, self.n_ts_kij, "upload_Sigma")
         self._chk(self.lib.symgw_upload_Sigma(self.h, dptr(S)), "upload_Sigma")
 
     def set_fock(self, F, S):
-        F = np.ascontiguousarray(F, dtype=np.complex128)
-        S = np.ascontiguousarray(S, dtype=np.complex128)
+        F = checked(F, np.complex128, self.n_fock, "set_fock F")
+        S = checked(S, np.complex128, self.n_fock, "set_fock S")
         self._chk(self.lib.symgw_set_fock(self.h, dptr(F), dptr(S)), "set_fock")
 
     def dyson(self, mu: float, sigma_scale: float):
         self._chk(self.lib.symgw_dyson(self.h, mu, sigma_scale), "dyson")
 
     def density(self, kweights):
-        kw = np.ascontiguousarray(kweights, dtype=np.float64)
+        kw = checked(kweights, np.float64, self.ink, "density kweights")
         v = C.c_double()
         self._chk(self.lib.symgw_density(self.h, dptr(kw), C.byref(v)), "density")
         return v.value
 
     def mu_search(self, nel, kweights, sigma_scale, lo, hi, tol):
-        kw = np.ascontiguousarray(kweights, dtype=np.float64)
+        kw = checked(kweights, np.float64, self.ink, "mu_search kweights")
         mu, n = C.c_double(), C.c_double()
         self._chk(self.lib.symgw_mu_search(self.h, nel, dptr(kw), sigma_scale, lo, hi, tol, C.byref(mu), C.byref(n)),
                   "mu_search")
@@ -274,6 +311,12 @@ def measure_peaks(device=0):
 
 def generate_V_pair_host(system, seed, scale, decay, k1, k2, n_complex):
     lib = load()
+    p = k1 * system.nk + k2
+    offs = system.kpair_offsets()
+    if p not in offs:
+        raise ValueError(f"generate_V_pair_host: pair ({k1}, {k2}) is not stored")
+    if n_complex != offs[p][1]:
+        raise ValueError(f"generate_V_pair_host: {n_complex} elements requested, the pair holds {offs[p][1]}")
     d = system.desc()
     out = np.zeros(n_complex, dtype=np.complex128)
     decay = np.ascontiguousarray(decay, dtype=np.float64)
@@ -281,3 +324,38 @@ def generate_V_pair_host(system, seed, scale, decay, k1, k2, n_complex):
     if rc != SYMGW_OK:
         raise SymGWError("symgw_generate_V_pair_host failed")
     return out
+
+
+def ir_basis(statistics: int, lam: float, nl: int, deg: int = 24, max_seg: int = 256):
+    """IR basis of the logistic kernel (symgw_ir_basis): dict with the
+    segment edges, per-segment Legendre coefficients coef[l][seg][k], the
+    singular values and the tau / Matsubara sampling points with u^ there."""
+    lib = load()
+    nseg = C.c_int(0)
+    edges = np.zeros(max_seg + 1)
+    coef = np.zeros((nl, max_seg, deg))
+    s = np.zeros(nl)
+    xs = np.zeros(nl)
+    ws = np.zeros(nl, dtype=np.int64)
+    uwl = np.zeros((nl, nl, 2))
+    rc = lib.symgw_ir_basis(statistics, lam, nl, max_seg, deg, C.byref(nseg), dptr(edges), dptr(coef), dptr(s),
+                            dptr(xs), ws.ctypes.data_as(C.POINTER(C.c_int64)), dptr(uwl))
+    if rc != SYMGW_OK:
+        raise SymGWError(f"symgw_ir_basis failed ({rc}) for statistics {statistics}, Lambda {lam}, nl {nl}")
+    ns = nseg.value
+    return {"statistics": statistics, "lambda": lam, "nl": nl, "deg": deg, "max_seg": max_seg, "nseg": ns,
+            "edges": edges, "coef": coef, "s": s, "xsample": xs, "wsample": ws,
+            "uwl": uwl[..., 0] + 1j * uwl[..., 1]}
+
+
+def ir_uhat(basis, n):
+    """u^_l(i w_n) of a basis from ir_basis at the Matsubara indices n: (len(n), nl)."""
+    lib = load()
+    n = np.ascontiguousarray(n, dtype=np.int64)
+    out = np.zeros((n.size, basis["nl"], 2))
+    rc = lib.symgw_ir_uhat(basis["statistics"], basis["nl"], basis["nseg"], basis["deg"], basis["max_seg"],
+                           dptr(basis["edges"]), dptr(basis["coef"]), n.size, n.ctypes.data_as(C.POINTER(C.c_int64)),
+                           dptr(out))
+    if rc != SYMGW_OK:
+        raise SymGWError(f"symgw_ir_uhat failed ({rc})")
+    return out[..., 0] + 1j * out[..., 1]

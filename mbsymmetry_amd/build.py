@@ -11,6 +11,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libsymgw.so")
 SOURCES = ["kernels.hip", "contract.hip", "peaks.hip", "capi.cpp", "layout.cpp"]
+# host-only C (binary128 arithmetic and OpenMP, built with gcc)
+C_SOURCES = ["irbasis.c"]
+LINK_LIBS = ["-lquadmath", "-lgomp"]
 
 
 def hipcc() -> str:
@@ -45,6 +48,13 @@ def build(verbose: bool = False) -> str:
             print(" ".join(cmd), flush=True)
         procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))
         objs.append(obj)
+    for src in C_SOURCES:
+        obj = os.path.join(bdir, os.path.splitext(src)[0] + ".o")
+        cmd = ["gcc", "-O2", "-fPIC", "-fopenmp", "-std=c11", "-Wall", "-c", os.path.join(CSRC, src), "-o", obj]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))
+        objs.append(obj)
     for src, p in procs:
         out, _ = p.communicate()
         text = out.decode(errors="replace")
@@ -55,7 +65,7 @@ def build(verbose: bool = False) -> str:
         if verbose and lines:
             print("\n".join(lines))
     tmp = LIB + ".tmp"
-    cmd = [cc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs
+    cmd = [cc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs + LINK_LIBS
     r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stdout.decode(errors="replace"))

@@ -91,6 +91,32 @@ int fail(symgw_ctx *c, int code, const std::string &msg) {
   return code;
 }
 
+// Runs the body of an extern "C" entry point so that no C++ exception crosses
+// the ABI: a failed host allocation becomes SYMGW_ERR_NOMEM, anything else
+// SYMGW_ERR_ARG, with the message on the context (or, without one, in the
+// thread's create-error string).
+template <typename F>
+int guarded(symgw_ctx *c, F &&body) noexcept {
+  auto report = [c](int code, const char *what, const char *detail) noexcept {
+    try {
+      std::string msg = std::string(what) + detail;
+      if (c) c->err = msg;
+      else g_create_error = msg;
+    } catch (...) {
+    }
+    return code;
+  };
+  try {
+    return body();
+  } catch (const std::bad_alloc &) {
+    return report(SYMGW_ERR_NOMEM, "host memory allocation failed", "");
+  } catch (const std::exception &e) {
+    return report(SYMGW_ERR_ARG, "internal error: ", e.what());
+  } catch (...) {
+    return report(SYMGW_ERR_ARG, "internal error", "");
+  }
+}
+
 #define HIPCHK(ctx, call)                                                              \
   do {                                                                                 \
     hipError_t e_ = (call);                                                            \
@@ -157,7 +183,6 @@ int64_t footprint(const Layout &L, const WorkLists &W, int eb) {
   b += (int64_t)L.nts * L.ns * L.flat_ao_full * eb;      // G_full
   b += (int64_t)L.nts * L.flat_aux * 16;                 // P
   b += (int64_t)L.nts * L.flat_aux_full * eb;            // P_full
-  if (eb != 16) b += (int64_t)L.nts * (L.flat_aux_full + L.ns * L.flat_ao_full) * 16;
   b += (int64_t)L.nw_b * L.flat_aux * 16 + (int64_t)L.nw * L.ns * L.flat_ao * 16;
   b += (int64_t)L.nw_b * W.w_scr_group * 16 + (int64_t)L.nw * L.ns * W.d_scr_group * 16;
   b += (int64_t)(W.p0_blks.size() * sizeof(P0Blk) + W.sig_blks.size() * sizeof(SigBlk));
@@ -337,338 +362,352 @@ const char *symgw_create_error(void) { return g_create_error.c_str(); }
 const char *symgw_last_error(const symgw_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
 int symgw_layout_query(const symgw_layout_desc *desc, symgw_layout_info *info) {
-  if (!info) return SYMGW_ERR_ARG;
-  Layout L;
-  std::string err;
-  if (!build_layout(desc, L, err)) {
-    g_create_error = err;
-    return SYMGW_ERR_ARG;
-  }
-  WorkLists W;
-  if (!build_worklists(L, 16, W, err)) {
-    g_create_error = err;
-    return SYMGW_ERR_ARG;
-  }
-  info->flat_ao_size = L.flat_ao;
-  info->flat_ao_size_full = L.flat_ao_full;
-  info->flat_aux_size = L.flat_aux;
-  info->flat_aux_size_full = L.flat_aux_full;
-  info->flat_V_size = L.flat_V;
-  info->flat_orep_size = L.flat_orep;
-  info->flat_auxrep_size = L.flat_auxrep;
-  info->n_P0_blocks = W.n_p0_contractions;
-  info->n_Sigma_blocks = W.n_sig_contractions;
-  info->flop_P0 = W.flop_p0;
-  info->flop_Sigma = W.flop_sig;
-  info->device_bytes = footprint(L, W, 16);
-  return SYMGW_OK;
+  return guarded(nullptr, [&]() -> int {
+    if (!info) return SYMGW_ERR_ARG;
+    Layout L;
+    std::string err;
+    if (!build_layout(desc, L, err)) {
+      g_create_error = err;
+      return SYMGW_ERR_ARG;
+    }
+    WorkLists W;
+    if (!build_worklists(L, 16, W, err)) {
+      g_create_error = err;
+      return SYMGW_ERR_ARG;
+    }
+    info->flat_ao_size = L.flat_ao;
+    info->flat_ao_size_full = L.flat_ao_full;
+    info->flat_aux_size = L.flat_aux;
+    info->flat_aux_size_full = L.flat_aux_full;
+    info->flat_V_size = L.flat_V;
+    info->flat_orep_size = L.flat_orep;
+    info->flat_auxrep_size = L.flat_auxrep;
+    info->n_P0_blocks = W.n_p0_contractions;
+    info->n_Sigma_blocks = W.n_sig_contractions;
+    info->flop_P0 = W.flop_p0;
+    info->flop_Sigma = W.flop_sig;
+    info->device_bytes = footprint(L, W, 16);
+    return SYMGW_OK;
+  });
 }
 
 int symgw_layout_footprint(const symgw_layout_desc *desc, int precision, int64_t *bytes) {
-  if (!desc || !bytes) return SYMGW_ERR_ARG;
-  if (precision != SYMGW_PREC_FP64 && precision != SYMGW_PREC_FP32) {
-    g_create_error = "unknown precision mode";
-    return SYMGW_ERR_ARG;
-  }
-  const int eb = precision == SYMGW_PREC_FP64 ? 16 : 8;
-  Layout L;
-  WorkLists W;
-  std::string err;
-  if (!build_layout(desc, L, err) || !build_worklists(L, eb, W, err)) {
-    g_create_error = err;
-    return SYMGW_ERR_ARG;
-  }
-  *bytes = footprint(L, W, eb);
-  return SYMGW_OK;
+  return guarded(nullptr, [&]() -> int {
+    if (!desc || !bytes) return SYMGW_ERR_ARG;
+    if (precision != SYMGW_PREC_FP64 && precision != SYMGW_PREC_FP32) {
+      g_create_error = "unknown precision mode";
+      return SYMGW_ERR_ARG;
+    }
+    const int eb = precision == SYMGW_PREC_FP64 ? 16 : 8;
+    Layout L;
+    WorkLists W;
+    std::string err;
+    if (!build_layout(desc, L, err) || !build_worklists(L, eb, W, err)) {
+      g_create_error = err;
+      return SYMGW_ERR_ARG;
+    }
+    *bytes = footprint(L, W, eb);
+    return SYMGW_OK;
+  });
 }
 
 int symgw_device_memory(int device, int64_t *free_bytes, int64_t *total_bytes) {
-  if (!free_bytes || !total_bytes) return SYMGW_ERR_ARG;
-  int ndev = 0;
-  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
-    (void)hipGetLastError();
-    g_create_error = "no HIP device";
-    return SYMGW_ERR_NODEVICE;
-  }
-  if (device < 0 || device >= ndev) {
-    g_create_error = "device index out of range";
-    return SYMGW_ERR_ARG;
-  }
-  size_t fr = 0, tot = 0;
-  if (hipSetDevice(device) != hipSuccess || hipMemGetInfo(&fr, &tot) != hipSuccess) {
-    g_create_error = "hipMemGetInfo failed";
-    return SYMGW_ERR_HIP;
-  }
-  *free_bytes = (int64_t)fr;
-  *total_bytes = (int64_t)tot;
-  return SYMGW_OK;
+  return guarded(nullptr, [&]() -> int {
+    if (!free_bytes || !total_bytes) return SYMGW_ERR_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+      (void)hipGetLastError();
+      g_create_error = "no HIP device";
+      return SYMGW_ERR_NODEVICE;
+    }
+    if (device < 0 || device >= ndev) {
+      g_create_error = "device index out of range";
+      return SYMGW_ERR_ARG;
+    }
+    size_t fr = 0, tot = 0;
+    if (hipSetDevice(device) != hipSuccess || hipMemGetInfo(&fr, &tot) != hipSuccess) {
+      g_create_error = "hipMemGetInfo failed";
+      return SYMGW_ERR_HIP;
+    }
+    *free_bytes = (int64_t)fr;
+    *total_bytes = (int64_t)tot;
+    return SYMGW_OK;
+  });
 }
 
 int symgw_measure_peaks(int device, double *out) {
-  if (!out) return SYMGW_ERR_ARG;
-  int ndev = 0;
-  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
-    (void)hipGetLastError();
-    g_create_error = "no HIP device";
-    return SYMGW_ERR_NODEVICE;
-  }
-  if (device < 0 || device >= ndev) {
-    g_create_error = "device index out of range";
-    return SYMGW_ERR_ARG;
-  }
-  hipDeviceProp_t prop;
-  if (hipSetDevice(device) != hipSuccess || hipGetDeviceProperties(&prop, device) != hipSuccess) {
-    g_create_error = "device query failed";
-    return SYMGW_ERR_HIP;
-  }
-  const int n_wg = 4 * prop.multiProcessorCount;
-  size_t fr = 0, tot = 0;
-  if (hipMemGetInfo(&fr, &tot) != hipSuccess) {
-    g_create_error = "hipMemGetInfo failed";
-    return SYMGW_ERR_HIP;
-  }
-  // two copy buffers of up to 1 GiB each, at most a quarter of the free memory
-  const int64_t cbytes = std::min<int64_t>(int64_t(1) << 30, (int64_t)(fr / 4)) & ~int64_t(4095);
-  hipStream_t st = nullptr;
-  hipEvent_t e0 = nullptr, e1 = nullptr;
-  void *res = nullptr, *src = nullptr, *dst = nullptr;
-  int r = SYMGW_OK;
-  std::string what;
-  if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
-      hipEventCreate(&e1) != hipSuccess) {
-    r = SYMGW_ERR_HIP;
-    what = "stream/event creation failed";
-  }
-  if (!r && (dev_malloc(&res, (size_t)n_wg * mfma_peak_threads() * sizeof(double)) != hipSuccess ||
-             dev_malloc(&src, (size_t)cbytes) != hipSuccess || dev_malloc(&dst, (size_t)cbytes) != hipSuccess)) {
-    r = SYMGW_ERR_NOMEM;
-    what = "peak benchmark buffers";
-  }
-  // best of three runs of each (the first ones also bring the clocks up)
-  out[0] = out[1] = out[2] = 0.0;
-  for (int rep = 0; rep < 3 && !r; ++rep) {
-    double t64 = 0, t32 = 0;
-    if ((r = time_mfma_peak<double>(st, e0, e1, res, n_wg, 4096, &t64)) != SYMGW_OK) what = "FP64 MFMA peak";
-    if (!r && (r = time_mfma_peak<float>(st, e0, e1, res, n_wg, 8192, &t32)) != SYMGW_OK) what = "FP32 MFMA peak";
-    out[0] = std::max(out[0], t64);
-    out[1] = std::max(out[1], t32);
-  }
-  if (!r && hipMemsetAsync(src, 0, (size_t)cbytes, st) != hipSuccess) r = SYMGW_ERR_HIP;
-  for (int rep = 0; rep < 3 && !r; ++rep) {
-    const int reps = 20, cwg = 64 * prop.multiProcessorCount;
-    float ms = 0;
-    if (launch_hbm_copy(src, dst, cbytes, cwg, st) != hipSuccess || hipEventRecord(e0, st) != hipSuccess)
+  return guarded(nullptr, [&]() -> int {
+    if (!out) return SYMGW_ERR_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+      (void)hipGetLastError();
+      g_create_error = "no HIP device";
+      return SYMGW_ERR_NODEVICE;
+    }
+    if (device < 0 || device >= ndev) {
+      g_create_error = "device index out of range";
+      return SYMGW_ERR_ARG;
+    }
+    hipDeviceProp_t prop;
+    if (hipSetDevice(device) != hipSuccess || hipGetDeviceProperties(&prop, device) != hipSuccess) {
+      g_create_error = "device query failed";
+      return SYMGW_ERR_HIP;
+    }
+    const int n_wg = 4 * prop.multiProcessorCount;
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess) {
+      g_create_error = "hipMemGetInfo failed";
+      return SYMGW_ERR_HIP;
+    }
+    // two copy buffers of up to 1 GiB each, at most a quarter of the free memory
+    const int64_t cbytes = std::min<int64_t>(int64_t(1) << 30, (int64_t)(fr / 4)) & ~int64_t(4095);
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    void *res = nullptr, *src = nullptr, *dst = nullptr;
+    int r = SYMGW_OK;
+    std::string what;
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+        hipEventCreate(&e1) != hipSuccess) {
       r = SYMGW_ERR_HIP;
-    for (int i = 0; !r && i < reps; ++i)
-      if (launch_hbm_copy(src, dst, cbytes, cwg, st) != hipSuccess) r = SYMGW_ERR_HIP;
-    if (!r && (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-               hipEventElapsedTime(&ms, e0, e1) != hipSuccess || ms <= 0))
-      r = SYMGW_ERR_HIP;
-    if (r) what = "HBM copy";
-    else out[2] = std::max(out[2], 2.0 * (double)cbytes * reps / (ms * 1e-3) / 1e9);  // bytes read + written
-  }
-  if (st) (void)hipStreamSynchronize(st);
-  if (res) dev_free(res);
-  if (src) dev_free(src);
-  if (dst) dev_free(dst);
-  if (e0) (void)hipEventDestroy(e0);
-  if (e1) (void)hipEventDestroy(e1);
-  if (st) (void)hipStreamDestroy(st);
-  if (r) {
-    (void)hipGetLastError();
-    g_create_error = what + " failed";
-  }
-  return r;
+      what = "stream/event creation failed";
+    }
+    if (!r && (dev_malloc(&res, (size_t)n_wg * mfma_peak_threads() * sizeof(double)) != hipSuccess ||
+               dev_malloc(&src, (size_t)cbytes) != hipSuccess || dev_malloc(&dst, (size_t)cbytes) != hipSuccess)) {
+      r = SYMGW_ERR_NOMEM;
+      what = "peak benchmark buffers";
+    }
+    // best of three runs of each (the first ones also bring the clocks up)
+    out[0] = out[1] = out[2] = 0.0;
+    for (int rep = 0; rep < 3 && !r; ++rep) {
+      double t64 = 0, t32 = 0;
+      if ((r = time_mfma_peak<double>(st, e0, e1, res, n_wg, 4096, &t64)) != SYMGW_OK) what = "FP64 MFMA peak";
+      if (!r && (r = time_mfma_peak<float>(st, e0, e1, res, n_wg, 8192, &t32)) != SYMGW_OK) what = "FP32 MFMA peak";
+      out[0] = std::max(out[0], t64);
+      out[1] = std::max(out[1], t32);
+    }
+    if (!r && hipMemsetAsync(src, 0, (size_t)cbytes, st) != hipSuccess) r = SYMGW_ERR_HIP;
+    for (int rep = 0; rep < 3 && !r; ++rep) {
+      const int reps = 20, cwg = 64 * prop.multiProcessorCount;
+      float ms = 0;
+      if (launch_hbm_copy(src, dst, cbytes, cwg, st) != hipSuccess || hipEventRecord(e0, st) != hipSuccess)
+        r = SYMGW_ERR_HIP;
+      for (int i = 0; !r && i < reps; ++i)
+        if (launch_hbm_copy(src, dst, cbytes, cwg, st) != hipSuccess) r = SYMGW_ERR_HIP;
+      if (!r && (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                 hipEventElapsedTime(&ms, e0, e1) != hipSuccess || ms <= 0))
+        r = SYMGW_ERR_HIP;
+      if (r) what = "HBM copy";
+      else out[2] = std::max(out[2], 2.0 * (double)cbytes * reps / (ms * 1e-3) / 1e9);  // bytes read + written
+    }
+    if (st) (void)hipStreamSynchronize(st);
+    if (res) dev_free(res);
+    if (src) dev_free(src);
+    if (dst) dev_free(dst);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+    if (r) {
+      (void)hipGetLastError();
+      g_create_error = what + " failed";
+    }
+    return r;
+  });
 }
 
 int symgw_create(const symgw_layout_desc *desc, int device, int precision, symgw_ctx **out) {
-  if (!out) return SYMGW_ERR_ARG;
-  *out = nullptr;
-  if (precision != SYMGW_PREC_FP64 && precision != SYMGW_PREC_FP32) {
-    g_create_error = "unknown precision mode";
-    return SYMGW_ERR_ARG;
-  }
-  std::unique_ptr<symgw_ctx> c(new symgw_ctx());
-  c->prec = precision;
-  c->eb = precision == SYMGW_PREC_FP64 ? 16 : 8;
-  std::string err;
-  if (!build_layout(desc, c->L, err) || !build_worklists(c->L, c->eb, c->W, err)) {
-    g_create_error = err;
-    return SYMGW_ERR_ARG;
-  }
-  int ndev = 0;
-  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
-    (void)hipGetLastError();
-    g_create_error = "no HIP device";
-    return SYMGW_ERR_NODEVICE;
-  }
-  if (device < 0 || device >= ndev) {
-    g_create_error = "device index out of range";
-    return SYMGW_ERR_ARG;
-  }
-  hipDeviceProp_t prop;
-  if (hipGetDeviceProperties(&prop, device) != hipSuccess || std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
-    g_create_error = std::string("device is not gfx950: ") + prop.gcnArchName;
-    return SYMGW_ERR_NODEVICE;
-  }
-  c->device = device;
-  if (c->W.p0_lds > 160 * 1024 || *std::max_element(c->W.sig_lds, c->W.sig_lds + kNumSigDepths) > 160 * 1024) {
-    g_create_error = "LDS plan exceeds 160 KiB";
-    return SYMGW_ERR_ARG;
-  }
-  // the W / Dyson solvers keep one pivot per row in fixed-size LDS arrays
-  if (c->L.max_ao_block > 512 || c->L.max_aux_block > 512) {
-    g_create_error = "irrep blocks larger than 512 are not supported by the solvers";
-    return SYMGW_ERR_ARG;
-  }
-  {
-    std::lock_guard<std::mutex> lk(g_dev_mu);
-    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
-      g_create_error = "stream creation failed";
-      return SYMGW_ERR_HIP;
+  return guarded(nullptr, [&]() -> int {
+    if (!out) return SYMGW_ERR_ARG;
+    *out = nullptr;
+    if (precision != SYMGW_PREC_FP64 && precision != SYMGW_PREC_FP32) {
+      g_create_error = "unknown precision mode";
+      return SYMGW_ERR_ARG;
     }
-    for (auto &e : c->ev) (void)hipEventCreate(&e);
-  }
-  const Layout &L = c->L;
-  symgw_ctx *cp = c.get();
-  int r = 0;
-  const size_t eb = c->eb;
-  r = r ? r : dalloc_bytes(cp, &c->V, (size_t)L.flat_V * eb);
-  r = r ? r : dalloc(cp, &c->G_ibz, (size_t)L.nts * L.ns * L.flat_ao);
-  r = r ? r : dalloc_bytes(cp, &c->G_full, (size_t)L.nts * L.ns * L.flat_ao_full * eb);
-  r = r ? r : dalloc(cp, &c->P, (size_t)L.nts * L.flat_aux);
-  r = r ? r : dalloc_bytes(cp, &c->P_full, (size_t)L.nts * L.flat_aux_full * eb);
-  r = r ? r : dalloc(cp, &c->Sig, (size_t)L.nts * L.ns * L.flat_ao);
-  r = r ? r : dalloc(cp, &c->Pw, (size_t)std::max(1, L.nw_b) * L.flat_aux);
-  r = r ? r : dalloc(cp, &c->SGw, (size_t)std::max(1, L.nw) * L.ns * L.flat_ao);
-  r = r ? r : dalloc(cp, &c->F, (size_t)L.ns * L.flat_ao);
-  r = r ? r : dalloc(cp, &c->S, (size_t)L.ns * L.flat_ao);
-  r = r ? r : dalloc(cp, &c->orep, (size_t)L.flat_orep);
-  r = r ? r : dalloc(cp, &c->auxrep, (size_t)L.flat_auxrep);
-  r = r ? r : dalloc(cp, &c->Tnt, (size_t)L.nw * (L.nts - 2));
-  r = r ? r : dalloc(cp, &c->Ttn, (size_t)L.nts * L.nw);
-  r = r ? r : dalloc(cp, &c->Tnt_BF, (size_t)L.nw_b * (L.nts - 2));
-  r = r ? r : dalloc(cp, &c->Ttn_FB, (size_t)L.nts * L.nw_b);
-  r = r ? r : dalloc(cp, &c->iwn, (size_t)L.nw);
-  r = r ? r : dalloc(cp, &c->w_scratch, (size_t)std::max(1, L.nw_b) * c->W.w_scr_group);
-  r = r ? r : dalloc(cp, &c->d_scratch, (size_t)std::max(1, L.nw) * L.ns * c->W.d_scr_group);
-  r = r ? r : dalloc(cp, &c->kw, (size_t)L.ink);
-  r = r ? r : dalloc(cp, &c->nel_dev, 1);
-  // every data buffer starts at zero (outside any graph: plain memsets are fine here)
-  for (auto &b : c->owned)
-    if (!r && hipMemsetAsync(b.p, 0, b.bytes, c->st) != hipSuccess) r = fail(cp, SYMGW_ERR_HIP, "initial memset failed");
-  if (!r && hipStreamSynchronize(c->st) != hipSuccess) r = fail(cp, SYMGW_ERR_HIP, "initial memset failed");
-  // work lists
-  r = r ? r : upload(cp, &c->d_p0_items, c->W.p0_items);
-  r = r ? r : upload(cp, &c->d_p0_blks, c->W.p0_blks);
-  for (int d = 0; d < kNumSigDepths; ++d) r = r ? r : upload(cp, &c->d_sig_items[d], c->W.sig_items[d]);
-  r = r ? r : upload(cp, &c->d_sig_blks, c->W.sig_blks);
-  r = r ? r : upload(cp, &c->d_g_rot, c->W.g_rot);
-  r = r ? r : upload(cp, &c->d_p_rot, c->W.p_rot);
-  r = r ? r : upload(cp, &c->d_w_blocks, c->W.w_blocks);
-  r = r ? r : upload(cp, &c->d_d_blocks, c->W.d_blocks);
-  // rotation source/destination slices, per full-BZ target
-  {
-    std::vector<int64_t> gs(L.nk), gsl(L.nk), gd(L.nk), gds(L.nk), ps(L.nk), psl(L.nk), pd(L.nk), pds(L.nk);
-    for (int k = 0; k < L.nk; ++k) {
-      int ick = L.irre_index[L.conj_index[k]];
-      gs[k] = L.ns * L.flat_ao;  // G_ibz is ts_kij: one (t,s) row holds every iBZ k
-      gsl[k] = L.kao_off_irre[ick];
-      gd[k] = L.kao_off[k];
-      gds[k] = L.kao_size[k];
-      ps[k] = L.qaux_size_irre[ick];  // P is q_t_QP
-      psl[k] = (int64_t)L.nts * L.qaux_off_irre[ick];
-      pd[k] = L.qaux_off[k];
-      pds[k] = L.qaux_size[k];
+    std::unique_ptr<symgw_ctx> c(new symgw_ctx());
+    c->prec = precision;
+    c->eb = precision == SYMGW_PREC_FP64 ? 16 : 8;
+    std::string err;
+    if (!build_layout(desc, c->L, err) || !build_worklists(c->L, c->eb, c->W, err)) {
+      g_create_error = err;
+      return SYMGW_ERR_ARG;
     }
-    // G_ibz row stride is per (t, s): ts * flat_ao
-    for (int k = 0; k < L.nk; ++k) gs[k] = L.flat_ao;
-    r = r ? r : upload(cp, &c->d_g_src_stride, gs);
-    r = r ? r : upload(cp, &c->d_g_src_slice, gsl);
-    r = r ? r : upload(cp, &c->d_g_dst_slice, gd);
-    r = r ? r : upload(cp, &c->d_g_dst_size, gds);
-    r = r ? r : upload(cp, &c->d_p_src_stride, ps);
-    r = r ? r : upload(cp, &c->d_p_src_slice, psl);
-    r = r ? r : upload(cp, &c->d_p_dst_slice, pd);
-    r = r ? r : upload(cp, &c->d_p_dst_size, pds);
-  }
-  {
-    std::vector<int64_t> woff(L.ink), qsl(L.ink), qof(L.ink), doff(L.ink), ksl(L.ink), kof(L.ink);
-    std::vector<TBatch> wf(L.ink), wb(L.ink), df(L.ink), db(L.ink);
-    for (int i = 0; i < L.ink; ++i) {
-      woff[i] = (int64_t)L.nw_b * L.qaux_off_irre[i];
-      qsl[i] = L.qaux_size_irre[i];
-      qof[i] = L.qaux_off_irre[i];
-      doff[i] = (int64_t)L.nw * L.ns * L.kao_off_irre[i];
-      ksl[i] = L.kao_size_irre[i];
-      kof[i] = L.kao_off_irre[i];
-      c->max_qslice = std::max<int>(c->max_qslice, (int)qsl[i]);
-      c->max_kslice = std::max<int>(c->max_kslice, (int)ksl[i]);
-      // W forward: P (q_t_QP) -> Pw ([q][n][qslice])
-      wf[i] = TBatch{(int64_t)L.nts * L.qaux_off_irre[i], woff[i], qsl[i], qsl[i], 0, 0, (int32_t)qsl[i], 0};
-      // W back: Pw -> P
-      wb[i] = TBatch{woff[i], (int64_t)L.nts * L.qaux_off_irre[i], qsl[i], qsl[i], 0, 0, (int32_t)qsl[i], 0};
-      // Dyson forward: Sig (k_ts_ij) -> SGw ([k][w][s][kslice]); (s, x) contiguous
-      df[i] = TBatch{(int64_t)L.nts * L.ns * L.kao_off_irre[i], doff[i], L.ns * ksl[i], L.ns * ksl[i], 0, 0,
-                     (int32_t)(L.ns * ksl[i]), 0};
-      // Dyson back: SGw -> G_ibz (ts_kij): columns split per spin
-      db[i] = TBatch{doff[i], L.kao_off_irre[i], L.ns * ksl[i], L.ns * L.flat_ao, ksl[i], L.flat_ao,
-                     (int32_t)ksl[i], 0};
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+      (void)hipGetLastError();
+      g_create_error = "no HIP device";
+      return SYMGW_ERR_NODEVICE;
     }
-    r = r ? r : upload(cp, &c->d_w_off, woff);
-    r = r ? r : upload(cp, &c->d_qslice_irre, qsl);
-    r = r ? r : upload(cp, &c->d_qoff_irre, qof);
-    r = r ? r : upload(cp, &c->d_d_off, doff);
-    r = r ? r : upload(cp, &c->d_kslice_irre, ksl);
-    r = r ? r : upload(cp, &c->d_kao_off_irre, kof);
-    r = r ? r : upload(cp, &c->d_tb_wf, wf);
-    r = r ? r : upload(cp, &c->d_tb_wb, wb);
-    r = r ? r : upload(cp, &c->d_tb_df, df);
-    r = r ? r : upload(cp, &c->d_tb_db, db);
-  }
-  if (r) {
-    g_create_error = c->err;
-    std::lock_guard<std::mutex> lk(g_dev_mu);
-    for (auto &b : c->owned) (void)hipFree(b.p);
-    for (auto &e : c->ev) (void)hipEventDestroy(e);
-    (void)hipStreamDestroy(c->st);
-    return r;
-  }
-  *out = c.release();
-  return SYMGW_OK;
+    if (device < 0 || device >= ndev) {
+      g_create_error = "device index out of range";
+      return SYMGW_ERR_ARG;
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess || std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+      g_create_error = std::string("device is not gfx950: ") + prop.gcnArchName;
+      return SYMGW_ERR_NODEVICE;
+    }
+    c->device = device;
+    if (c->W.p0_lds > 160 * 1024 || *std::max_element(c->W.sig_lds, c->W.sig_lds + kNumSigDepths) > 160 * 1024) {
+      g_create_error = "LDS plan exceeds 160 KiB";
+      return SYMGW_ERR_ARG;
+    }
+    // the W / Dyson solvers keep one pivot per row in fixed-size LDS arrays
+    if (c->L.max_ao_block > 512 || c->L.max_aux_block > 512) {
+      g_create_error = "irrep blocks larger than 512 are not supported by the solvers";
+      return SYMGW_ERR_ARG;
+    }
+    {
+      std::lock_guard<std::mutex> lk(g_dev_mu);
+      if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
+        g_create_error = "stream creation failed";
+        return SYMGW_ERR_HIP;
+      }
+      for (auto &e : c->ev) (void)hipEventCreate(&e);
+    }
+    const Layout &L = c->L;
+    symgw_ctx *cp = c.get();
+    int r = 0;
+    const size_t eb = c->eb;
+    r = r ? r : dalloc_bytes(cp, &c->V, (size_t)L.flat_V * eb);
+    r = r ? r : dalloc(cp, &c->G_ibz, (size_t)L.nts * L.ns * L.flat_ao);
+    r = r ? r : dalloc_bytes(cp, &c->G_full, (size_t)L.nts * L.ns * L.flat_ao_full * eb);
+    r = r ? r : dalloc(cp, &c->P, (size_t)L.nts * L.flat_aux);
+    r = r ? r : dalloc_bytes(cp, &c->P_full, (size_t)L.nts * L.flat_aux_full * eb);
+    r = r ? r : dalloc(cp, &c->Sig, (size_t)L.nts * L.ns * L.flat_ao);
+    r = r ? r : dalloc(cp, &c->Pw, (size_t)std::max(1, L.nw_b) * L.flat_aux);
+    r = r ? r : dalloc(cp, &c->SGw, (size_t)std::max(1, L.nw) * L.ns * L.flat_ao);
+    r = r ? r : dalloc(cp, &c->F, (size_t)L.ns * L.flat_ao);
+    r = r ? r : dalloc(cp, &c->S, (size_t)L.ns * L.flat_ao);
+    r = r ? r : dalloc(cp, &c->orep, (size_t)L.flat_orep);
+    r = r ? r : dalloc(cp, &c->auxrep, (size_t)L.flat_auxrep);
+    r = r ? r : dalloc(cp, &c->Tnt, (size_t)L.nw * (L.nts - 2));
+    r = r ? r : dalloc(cp, &c->Ttn, (size_t)L.nts * L.nw);
+    r = r ? r : dalloc(cp, &c->Tnt_BF, (size_t)L.nw_b * (L.nts - 2));
+    r = r ? r : dalloc(cp, &c->Ttn_FB, (size_t)L.nts * L.nw_b);
+    r = r ? r : dalloc(cp, &c->iwn, (size_t)L.nw);
+    r = r ? r : dalloc(cp, &c->w_scratch, (size_t)std::max(1, L.nw_b) * c->W.w_scr_group);
+    r = r ? r : dalloc(cp, &c->d_scratch, (size_t)std::max(1, L.nw) * L.ns * c->W.d_scr_group);
+    r = r ? r : dalloc(cp, &c->kw, (size_t)L.ink);
+    r = r ? r : dalloc(cp, &c->nel_dev, 1);
+    // every data buffer starts at zero (outside any graph: plain memsets are fine here)
+    for (auto &b : c->owned)
+      if (!r && hipMemsetAsync(b.p, 0, b.bytes, c->st) != hipSuccess) r = fail(cp, SYMGW_ERR_HIP, "initial memset failed");
+    if (!r && hipStreamSynchronize(c->st) != hipSuccess) r = fail(cp, SYMGW_ERR_HIP, "initial memset failed");
+    // work lists
+    r = r ? r : upload(cp, &c->d_p0_items, c->W.p0_items);
+    r = r ? r : upload(cp, &c->d_p0_blks, c->W.p0_blks);
+    for (int d = 0; d < kNumSigDepths; ++d) r = r ? r : upload(cp, &c->d_sig_items[d], c->W.sig_items[d]);
+    r = r ? r : upload(cp, &c->d_sig_blks, c->W.sig_blks);
+    r = r ? r : upload(cp, &c->d_g_rot, c->W.g_rot);
+    r = r ? r : upload(cp, &c->d_p_rot, c->W.p_rot);
+    r = r ? r : upload(cp, &c->d_w_blocks, c->W.w_blocks);
+    r = r ? r : upload(cp, &c->d_d_blocks, c->W.d_blocks);
+    // rotation source/destination slices, per full-BZ target
+    {
+      std::vector<int64_t> gs(L.nk), gsl(L.nk), gd(L.nk), gds(L.nk), ps(L.nk), psl(L.nk), pd(L.nk), pds(L.nk);
+      for (int k = 0; k < L.nk; ++k) {
+        int ick = L.irre_index[L.conj_index[k]];
+        gs[k] = L.ns * L.flat_ao;  // G_ibz is ts_kij: one (t,s) row holds every iBZ k
+        gsl[k] = L.kao_off_irre[ick];
+        gd[k] = L.kao_off[k];
+        gds[k] = L.kao_size[k];
+        ps[k] = L.qaux_size_irre[ick];  // P is q_t_QP
+        psl[k] = (int64_t)L.nts * L.qaux_off_irre[ick];
+        pd[k] = L.qaux_off[k];
+        pds[k] = L.qaux_size[k];
+      }
+      // G_ibz row stride is per (t, s): ts * flat_ao
+      for (int k = 0; k < L.nk; ++k) gs[k] = L.flat_ao;
+      r = r ? r : upload(cp, &c->d_g_src_stride, gs);
+      r = r ? r : upload(cp, &c->d_g_src_slice, gsl);
+      r = r ? r : upload(cp, &c->d_g_dst_slice, gd);
+      r = r ? r : upload(cp, &c->d_g_dst_size, gds);
+      r = r ? r : upload(cp, &c->d_p_src_stride, ps);
+      r = r ? r : upload(cp, &c->d_p_src_slice, psl);
+      r = r ? r : upload(cp, &c->d_p_dst_slice, pd);
+      r = r ? r : upload(cp, &c->d_p_dst_size, pds);
+    }
+    {
+      std::vector<int64_t> woff(L.ink), qsl(L.ink), qof(L.ink), doff(L.ink), ksl(L.ink), kof(L.ink);
+      std::vector<TBatch> wf(L.ink), wb(L.ink), df(L.ink), db(L.ink);
+      for (int i = 0; i < L.ink; ++i) {
+        woff[i] = (int64_t)L.nw_b * L.qaux_off_irre[i];
+        qsl[i] = L.qaux_size_irre[i];
+        qof[i] = L.qaux_off_irre[i];
+        doff[i] = (int64_t)L.nw * L.ns * L.kao_off_irre[i];
+        ksl[i] = L.kao_size_irre[i];
+        kof[i] = L.kao_off_irre[i];
+        c->max_qslice = std::max<int>(c->max_qslice, (int)qsl[i]);
+        c->max_kslice = std::max<int>(c->max_kslice, (int)ksl[i]);
+        // W forward: P (q_t_QP) -> Pw ([q][n][qslice])
+        wf[i] = TBatch{(int64_t)L.nts * L.qaux_off_irre[i], woff[i], qsl[i], qsl[i], 0, 0, (int32_t)qsl[i], 0};
+        // W back: Pw -> P
+        wb[i] = TBatch{woff[i], (int64_t)L.nts * L.qaux_off_irre[i], qsl[i], qsl[i], 0, 0, (int32_t)qsl[i], 0};
+        // Dyson forward: Sig (k_ts_ij) -> SGw ([k][w][s][kslice]); (s, x) contiguous
+        df[i] = TBatch{(int64_t)L.nts * L.ns * L.kao_off_irre[i], doff[i], L.ns * ksl[i], L.ns * ksl[i], 0, 0,
+                       (int32_t)(L.ns * ksl[i]), 0};
+        // Dyson back: SGw -> G_ibz (ts_kij): columns split per spin
+        db[i] = TBatch{doff[i], L.kao_off_irre[i], L.ns * ksl[i], L.ns * L.flat_ao, ksl[i], L.flat_ao,
+                       (int32_t)ksl[i], 0};
+      }
+      r = r ? r : upload(cp, &c->d_w_off, woff);
+      r = r ? r : upload(cp, &c->d_qslice_irre, qsl);
+      r = r ? r : upload(cp, &c->d_qoff_irre, qof);
+      r = r ? r : upload(cp, &c->d_d_off, doff);
+      r = r ? r : upload(cp, &c->d_kslice_irre, ksl);
+      r = r ? r : upload(cp, &c->d_kao_off_irre, kof);
+      r = r ? r : upload(cp, &c->d_tb_wf, wf);
+      r = r ? r : upload(cp, &c->d_tb_wb, wb);
+      r = r ? r : upload(cp, &c->d_tb_df, df);
+      r = r ? r : upload(cp, &c->d_tb_db, db);
+    }
+    if (r) {
+      g_create_error = c->err;
+      std::lock_guard<std::mutex> lk(g_dev_mu);
+      for (auto &b : c->owned) (void)hipFree(b.p);
+      for (auto &e : c->ev) (void)hipEventDestroy(e);
+      (void)hipStreamDestroy(c->st);
+      return r;
+    }
+    *out = c.release();
+    return SYMGW_OK;
+  });
 }
 
 int symgw_destroy(symgw_ctx *ctx) {
-  if (!ctx) return SYMGW_OK;
-  (void)hipSetDevice(ctx->device);
-  (void)hipStreamSynchronize(ctx->st);
-  {
-    std::lock_guard<std::mutex> lk(g_dev_mu);
-    drop_graphs(ctx);
-    for (auto &b : ctx->owned) (void)hipFree(b.p);
-    for (auto &e : ctx->ev) (void)hipEventDestroy(e);
-    (void)hipStreamDestroy(ctx->st);
-  }
-  delete ctx;
-  return SYMGW_OK;
+  return guarded(ctx, [&]() -> int {
+    if (!ctx) return SYMGW_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->st);
+    {
+      std::lock_guard<std::mutex> lk(g_dev_mu);
+      drop_graphs(ctx);
+      for (auto &b : ctx->owned) (void)hipFree(b.p);
+      for (auto &e : ctx->ev) (void)hipEventDestroy(e);
+      (void)hipStreamDestroy(ctx->st);
+    }
+    delete ctx;
+    return SYMGW_OK;
+  });
 }
 
 int symgw_layout_info_ctx(const symgw_ctx *ctx, symgw_layout_info *info) {
-  if (!ctx || !info) return SYMGW_ERR_ARG;
-  const Layout &L = ctx->L;
-  info->flat_ao_size = L.flat_ao;
-  info->flat_ao_size_full = L.flat_ao_full;
-  info->flat_aux_size = L.flat_aux;
-  info->flat_aux_size_full = L.flat_aux_full;
-  info->flat_V_size = L.flat_V;
-  info->flat_orep_size = L.flat_orep;
-  info->flat_auxrep_size = L.flat_auxrep;
-  info->n_P0_blocks = ctx->W.n_p0_contractions;
-  info->n_Sigma_blocks = ctx->W.n_sig_contractions;
-  info->flop_P0 = ctx->W.flop_p0;
-  info->flop_Sigma = ctx->W.flop_sig;
-  info->device_bytes = footprint(L, ctx->W, ctx->eb);
-  return SYMGW_OK;
+  return guarded(const_cast<symgw_ctx *>(ctx), [&]() -> int {
+    if (!ctx || !info) return SYMGW_ERR_ARG;
+    const Layout &L = ctx->L;
+    info->flat_ao_size = L.flat_ao;
+    info->flat_ao_size_full = L.flat_ao_full;
+    info->flat_aux_size = L.flat_aux;
+    info->flat_aux_size_full = L.flat_aux_full;
+    info->flat_V_size = L.flat_V;
+    info->flat_orep_size = L.flat_orep;
+    info->flat_auxrep_size = L.flat_auxrep;
+    info->n_P0_blocks = ctx->W.n_p0_contractions;
+    info->n_Sigma_blocks = ctx->W.n_sig_contractions;
+    info->flop_P0 = ctx->W.flop_p0;
+    info->flop_Sigma = ctx->W.flop_sig;
+    info->device_bytes = footprint(L, ctx->W, ctx->eb);
+    return SYMGW_OK;
+  });
 }
 
 int64_t symgw_kpair_offset(const symgw_ctx *ctx, int k1, int k2) {
@@ -678,30 +717,32 @@ int64_t symgw_kpair_offset(const symgw_ctx *ctx, int k1, int k2) {
 }
 
 int symgw_set_V(symgw_ctx *ctx, const double *V_flat, int64_t n_complex) {
-  if (!ctx || !V_flat) return SYMGW_ERR_ARG;
-  if (n_complex != ctx->L.flat_V) return fail(ctx, SYMGW_ERR_ARG, "V length does not match the layout");
-  HIPCHK(ctx, hipSetDevice(ctx->device));
-  if (ctx->prec == SYMGW_PREC_FP64) {
-    HIPCHK(ctx, copy_sync(ctx, ctx->V, V_flat, (size_t)n_complex * 16, hipMemcpyHostToDevice));
-  } else {
-    // staged conversion in 64 MiB chunks
-    const int64_t chunk = 4 << 20;
-    cplx<double> *stage = nullptr;
-    HIPCHK(ctx, dev_malloc((void **)&stage, (size_t)chunk * 16));
-    for (int64_t o = 0; o < n_complex; o += chunk) {
-      int64_t n = std::min(chunk, n_complex - o);
-      hipError_t e = copy_sync(ctx, stage, V_flat + 2 * o, (size_t)n * 16, hipMemcpyHostToDevice);
-      if (e == hipSuccess) e = launch_convert<float>(stage, static_cast<cplx<float> *>(ctx->V) + o, n, ctx->st);
-      if (e == hipSuccess) e = hipStreamSynchronize(ctx->st);
-      if (e != hipSuccess) {
-        dev_free(stage);
-        return fail(ctx, SYMGW_ERR_HIP, std::string("V upload: ") + hipGetErrorString(e));
+  return guarded(ctx, [&]() -> int {
+    if (!ctx || !V_flat) return SYMGW_ERR_ARG;
+    if (n_complex != ctx->L.flat_V) return fail(ctx, SYMGW_ERR_ARG, "V length does not match the layout");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    if (ctx->prec == SYMGW_PREC_FP64) {
+      HIPCHK(ctx, copy_sync(ctx, ctx->V, V_flat, (size_t)n_complex * 16, hipMemcpyHostToDevice));
+    } else {
+      // staged conversion in 64 MiB chunks
+      const int64_t chunk = 4 << 20;
+      cplx<double> *stage = nullptr;
+      HIPCHK(ctx, dev_malloc((void **)&stage, (size_t)chunk * 16));
+      for (int64_t o = 0; o < n_complex; o += chunk) {
+        int64_t n = std::min(chunk, n_complex - o);
+        hipError_t e = copy_sync(ctx, stage, V_flat + 2 * o, (size_t)n * 16, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = launch_convert<float>(stage, static_cast<cplx<float> *>(ctx->V) + o, n, ctx->st);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->st);
+        if (e != hipSuccess) {
+          dev_free(stage);
+          return fail(ctx, SYMGW_ERR_HIP, std::string("V upload: ") + hipGetErrorString(e));
+        }
       }
+      dev_free(stage);
     }
-    dev_free(stage);
-  }
-  ctx->have_V = true;
-  return SYMGW_OK;
+    ctx->have_V = true;
+    return SYMGW_OK;
+  });
 }
 
 static void gen_blocks(const Layout &L, std::vector<GenBlk> &out, int only_pair) {
@@ -725,360 +766,402 @@ static void gen_blocks(const Layout &L, std::vector<GenBlk> &out, int only_pair)
 }
 
 int symgw_generate_V(symgw_ctx *ctx, uint64_t seed, double scale, const double *decay, int n_decay) {
-  if (!ctx || !decay || n_decay <= 0) return SYMGW_ERR_ARG;
-  HIPCHK(ctx, hipSetDevice(ctx->device));
-  std::vector<GenBlk> gb;
-  gen_blocks(ctx->L, gb, -1);
-  GenBlk *dgb = nullptr;
-  double *ddec = nullptr;
-  HIPCHK(ctx, dev_malloc((void **)&dgb, std::max<size_t>(1, gb.size()) * sizeof(GenBlk)));
-  hipError_t e0 = dev_malloc((void **)&ddec, (size_t)n_decay * sizeof(double));
-  if (e0 != hipSuccess) {
+  return guarded(ctx, [&]() -> int {
+    if (!ctx || !decay || n_decay <= 0) return SYMGW_ERR_ARG;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    std::vector<GenBlk> gb;
+    gen_blocks(ctx->L, gb, -1);
+    GenBlk *dgb = nullptr;
+    double *ddec = nullptr;
+    HIPCHK(ctx, dev_malloc((void **)&dgb, std::max<size_t>(1, gb.size()) * sizeof(GenBlk)));
+    hipError_t e0 = dev_malloc((void **)&ddec, (size_t)n_decay * sizeof(double));
+    if (e0 != hipSuccess) {
+      dev_free(dgb);
+      return fail(ctx, SYMGW_ERR_NOMEM, "V generation: allocation failed");
+    }
+    hipError_t e = copy_sync(ctx, dgb, gb.data(), gb.size() * sizeof(GenBlk), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = copy_sync(ctx, ddec, decay, (size_t)n_decay * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+      e = ctx->prec == SYMGW_PREC_FP64
+              ? launch_genv<double>(dgb, (int)gb.size(), seed, scale, ddec, n_decay, static_cast<cplx<double> *>(ctx->V),
+                                    ctx->st)
+              : launch_genv<float>(dgb, (int)gb.size(), seed, scale, ddec, n_decay, static_cast<cplx<float> *>(ctx->V),
+                                   ctx->st);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->st);
     dev_free(dgb);
-    return fail(ctx, SYMGW_ERR_NOMEM, "V generation: allocation failed");
-  }
-  hipError_t e = copy_sync(ctx, dgb, gb.data(), gb.size() * sizeof(GenBlk), hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = copy_sync(ctx, ddec, decay, (size_t)n_decay * sizeof(double), hipMemcpyHostToDevice);
-  if (e == hipSuccess) {
-    e = ctx->prec == SYMGW_PREC_FP64
-            ? launch_genv<double>(dgb, (int)gb.size(), seed, scale, ddec, n_decay, static_cast<cplx<double> *>(ctx->V),
-                                  ctx->st)
-            : launch_genv<float>(dgb, (int)gb.size(), seed, scale, ddec, n_decay, static_cast<cplx<float> *>(ctx->V),
-                                 ctx->st);
-  }
-  if (e == hipSuccess) e = hipStreamSynchronize(ctx->st);
-  dev_free(dgb);
-  dev_free(ddec);
-  if (e != hipSuccess) return fail(ctx, SYMGW_ERR_HIP, std::string("V generation: ") + hipGetErrorString(e));
-  ctx->have_V = true;
-  return SYMGW_OK;
+    dev_free(ddec);
+    if (e != hipSuccess) return fail(ctx, SYMGW_ERR_HIP, std::string("V generation: ") + hipGetErrorString(e));
+    ctx->have_V = true;
+    return SYMGW_OK;
+  });
 }
 
 int symgw_generate_V_pair_host(const symgw_layout_desc *desc, uint64_t seed, double scale, const double *decay,
                                int n_decay, int k1, int k2, double *out) {
-  if (!desc || !decay || n_decay <= 0 || !out) return SYMGW_ERR_ARG;
-  Layout L;
-  std::string err;
-  if (!build_layout(desc, L, err)) {
-    g_create_error = err;
-    return SYMGW_ERR_ARG;
-  }
-  if (k1 < 0 || k2 < 0 || k1 >= L.nk || k2 >= L.nk) return SYMGW_ERR_ARG;
-  std::vector<GenBlk> gb;
-  gen_blocks(L, gb, k1 * L.nk + k2);
-  for (const GenBlk &B : gb) {
-    const int64_t K = (int64_t)B.k1 * B.k2, n = (int64_t)B.q * K;
-    for (int64_t e = 0; e < n; ++e) {
-      const int64_t rem = e % K;
-      const int a = (int)(rem / B.k2), m = (int)(rem % B.k2);
-      int d = (B.a_base + a) - (B.b_base + m);
-      d = d < 0 ? -d : d;
-      d = d < n_decay ? d : n_decay - 1;
-      const double f = (scale * decay[d]) * 0.70710678118654752440;
-      double re, im;
-      gen_v_elem(seed, (uint32_t)B.pair, (uint64_t)(B.local_off + e), f, re, im);
-      out[2 * (B.local_off + e)] = re;
-      out[2 * (B.local_off + e) + 1] = im;
+  return guarded(nullptr, [&]() -> int {
+    if (!desc || !decay || n_decay <= 0 || !out) return SYMGW_ERR_ARG;
+    Layout L;
+    std::string err;
+    if (!build_layout(desc, L, err)) {
+      g_create_error = err;
+      return SYMGW_ERR_ARG;
     }
-  }
-  return SYMGW_OK;
+    if (k1 < 0 || k2 < 0 || k1 >= L.nk || k2 >= L.nk) return SYMGW_ERR_ARG;
+    std::vector<GenBlk> gb;
+    gen_blocks(L, gb, k1 * L.nk + k2);
+    for (const GenBlk &B : gb) {
+      const int64_t K = (int64_t)B.k1 * B.k2, n = (int64_t)B.q * K;
+      for (int64_t e = 0; e < n; ++e) {
+        const int64_t rem = e % K;
+        const int a = (int)(rem / B.k2), m = (int)(rem % B.k2);
+        int d = (B.a_base + a) - (B.b_base + m);
+        d = d < 0 ? -d : d;
+        d = d < n_decay ? d : n_decay - 1;
+        const double f = (scale * decay[d]) * 0.70710678118654752440;
+        double re, im;
+        gen_v_elem(seed, (uint32_t)B.pair, (uint64_t)(B.local_off + e), f, re, im);
+        out[2 * (B.local_off + e)] = re;
+        out[2 * (B.local_off + e) + 1] = im;
+      }
+    }
+    return SYMGW_OK;
+  });
 }
 
 int symgw_set_reps(symgw_ctx *ctx, const double *orep_flat, const double *auxrep_flat) {
-  if (!ctx) return SYMGW_ERR_ARG;
-  HIPCHK(ctx, hipSetDevice(ctx->device));
-  if (ctx->L.flat_orep > 0) {
-    if (!orep_flat) return fail(ctx, SYMGW_ERR_ARG, "orep missing");
-    HIPCHK(ctx, copy_sync(ctx, ctx->orep, orep_flat, (size_t)ctx->L.flat_orep * 16, hipMemcpyHostToDevice));
-  }
-  if (ctx->L.flat_auxrep > 0) {
-    if (!auxrep_flat) return fail(ctx, SYMGW_ERR_ARG, "auxrep missing");
-    HIPCHK(ctx, copy_sync(ctx, ctx->auxrep, auxrep_flat, (size_t)ctx->L.flat_auxrep * 16, hipMemcpyHostToDevice));
-  }
-  ctx->have_reps = true;
-  ctx->p_rotated = false;
-  return SYMGW_OK;
+  return guarded(ctx, [&]() -> int {
+    if (!ctx) return SYMGW_ERR_ARG;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    if (ctx->L.flat_orep > 0) {
+      if (!orep_flat) return fail(ctx, SYMGW_ERR_ARG, "orep missing");
+      HIPCHK(ctx, copy_sync(ctx, ctx->orep, orep_flat, (size_t)ctx->L.flat_orep * 16, hipMemcpyHostToDevice));
+    }
+    if (ctx->L.flat_auxrep > 0) {
+      if (!auxrep_flat) return fail(ctx, SYMGW_ERR_ARG, "auxrep missing");
+      HIPCHK(ctx, copy_sync(ctx, ctx->auxrep, auxrep_flat, (size_t)ctx->L.flat_auxrep * 16, hipMemcpyHostToDevice));
+    }
+    ctx->have_reps = true;
+    ctx->p_rotated = false;
+    return SYMGW_OK;
+  });
 }
 
 int symgw_upload_G(symgw_ctx *ctx, const double *G_ts_kij) {
-  if (!ctx || !G_ts_kij) return SYMGW_ERR_ARG;
-  HIPCHK(ctx, hipSetDevice(ctx->device));
-  const Layout &L = ctx->L;
-  HIPCHK(ctx, copy_sync(ctx, ctx->G_ibz, G_ts_kij, (size_t)L.nts * L.ns * L.flat_ao * 16, hipMemcpyHostToDevice));
-  ctx->have_G = true;
-  int r = rotate_G(ctx);
-  if (r) return r;
-  HIPCHK(ctx, hipStreamSynchronize(ctx->st));
-  return SYMGW_OK;
+  return guarded(ctx, [&]() -> int {
+    if (!ctx || !G_ts_kij) return SYMGW_ERR_ARG;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    const Layout &L = ctx->L;
+    HIPCHK(ctx, copy_sync(ctx, ctx->G_ibz, G_ts_kij, (size_t)L.nts * L.ns * L.flat_ao * 16, hipMemcpyHostToDevice));
+    ctx->have_G = true;
+    int r = rotate_G(ctx);
+    if (r) return r;
+    HIPCHK(ctx, hipStreamSynchronize(ctx->st));
+    return SYMGW_OK;
+  });
 }
 
 int symgw_download_G(symgw_ctx *ctx, double *G_ts_kij) {
-  if (!ctx || !G_ts_kij) return SYMGW_ERR_ARG;
-  HIPCHK(ctx, hipSetDevice(ctx->device));
-  HIPCHK(ctx, hipStreamSynchronize(ctx->st));
-  const Layout &L = ctx->L;
-  HIPCHK(ctx, copy_sync(ctx, G_ts_kij, ctx->G_ibz, (size_t)L.nts * L.ns * L.flat_ao * 16, hipMemcpyDeviceToHost));
-  return SYMGW_OK;
+  return guarded(ctx, [&]() -> int {
+    if (!ctx || !G_ts_kij) return SYMGW_ERR_ARG;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->st));
+    const Layout &L = ctx->L;
+    HIPCHK(ctx, copy_sync(ctx, G_ts_kij, ctx->G_ibz, (size_t)L.nts * L.ns * L.flat_ao * 16, hipMemcpyDeviceToHost));
+    return SYMGW_OK;
+  });
 }
 
 int symgw_solve_P0(symgw_ctx *ctx) {
-  if (!ctx) return SYMGW_ERR_ARG;
-  HIPCHK(ctx, hipSetDevice(ctx->device));
-  ev_record(ctx, 0);
-  int r = phase_P0(ctx);
-  if (r) return r;
-  ev_record(ctx, 1);
-  HIPCHK(ctx, hipStreamSynchronize(ctx->st));
-  if (ctx->timing) {
-    float ms = 0;
-    (void)hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]);
-    ctx->last_ms[0] = ms;
-  }
-  ctx->have_P = false;
-  return SYMGW_OK;
+  return guarded(ctx, [&]() -> int {
+    if (!ctx) return SYMGW_ERR_ARG;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    ev_record(ctx, 0);
+    int r = phase_P0(ctx);
+    if (r) return r;
+    ev_record(ctx, 1);
+    HIPCHK(ctx, hipStreamSynchronize(ctx->st));
+    if (ctx->timing) {
+      float ms = 0;
+      (void)hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]);
+      ctx->last_ms[0] = ms;
+    }
+    ctx->have_P = false;
+    return SYMGW_OK;
+  });
 }
 
 int symgw_copy_P0_to_host(const symgw_ctx *ctx, double *P0_q_t_QP) {
-  if (!ctx || !P0_q_t_QP) return SYMGW_ERR_ARG;
-  symgw_ctx *c = const_cast<symgw_ctx *>(ctx);
-  HIPCHK(c, hipSetDevice(ctx->device));
-  HIPCHK(c, hipStreamSynchronize(ctx->st));
-  const size_t n = (size_t)ctx->L.nts * ctx->L.flat_aux;
-  std::vector<double> tmp(2 * n);
-  HIPCHK(c, copy_sync(c, tmp.data(), ctx->P, n * 16, hipMemcpyDeviceToHost));
-  for (size_t i = 0; i < 2 * n; ++i) P0_q_t_QP[i] += tmp[i];
-  return SYMGW_OK;
+  return guarded(const_cast<symgw_ctx *>(ctx), [&]() -> int {
+    if (!ctx || !P0_q_t_QP) return SYMGW_ERR_ARG;
+    symgw_ctx *c = const_cast<symgw_ctx *>(ctx);
+    HIPCHK(c, hipSetDevice(ctx->device));
+    HIPCHK(c, hipStreamSynchronize(ctx->st));
+    const size_t n = (size_t)ctx->L.nts * ctx->L.flat_aux;
+    std::vector<double> tmp(2 * n);
+    HIPCHK(c, copy_sync(c, tmp.data(), ctx->P, n * 16, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < 2 * n; ++i) P0_q_t_QP[i] += tmp[i];
+    return SYMGW_OK;
+  });
 }
 
 int symgw_upload_P(symgw_ctx *ctx, const double *P_q_t_QP) {
-  if (!ctx || !P_q_t_QP) return SYMGW_ERR_ARG;
-  HIPCHK(ctx, hipSetDevice(ctx->device));
-  HIPCHK(ctx, copy_sync(ctx, ctx->P, P_q_t_QP, (size_t)ctx->L.nts * ctx->L.flat_aux * 16, hipMemcpyHostToDevice));
-  ctx->have_P = true;
-  ctx->p_rotated = false;
-  return SYMGW_OK;
+  return guarded(ctx, [&]() -> int {
+    if (!ctx || !P_q_t_QP) return SYMGW_ERR_ARG;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    HIPCHK(ctx, copy_sync(ctx, ctx->P, P_q_t_QP, (size_t)ctx->L.nts * ctx->L.flat_aux * 16, hipMemcpyHostToDevice));
+    ctx->have_P = true;
+    ctx->p_rotated = false;
+    return SYMGW_OK;
+  });
 }
 
 int symgw_set_transforms(symgw_ctx *ctx, const double *Tnt, const double *Ttn, const double *Tnt_BF,
                          const double *Ttn_FB, const double *iwn_fermi) {
-  if (!ctx || !Tnt || !Ttn || !Tnt_BF || !Ttn_FB || !iwn_fermi) return SYMGW_ERR_ARG;
-  HIPCHK(ctx, hipSetDevice(ctx->device));
-  const Layout &L = ctx->L;
-  HIPCHK(ctx, copy_sync(ctx, ctx->Tnt, Tnt, (size_t)L.nw * (L.nts - 2) * 16, hipMemcpyHostToDevice));
-  HIPCHK(ctx, copy_sync(ctx, ctx->Ttn, Ttn, (size_t)L.nts * L.nw * 16, hipMemcpyHostToDevice));
-  HIPCHK(ctx, copy_sync(ctx, ctx->Tnt_BF, Tnt_BF, (size_t)L.nw_b * (L.nts - 2) * 16, hipMemcpyHostToDevice));
-  HIPCHK(ctx, copy_sync(ctx, ctx->Ttn_FB, Ttn_FB, (size_t)L.nts * L.nw_b * 16, hipMemcpyHostToDevice));
-  HIPCHK(ctx, copy_sync(ctx, ctx->iwn, iwn_fermi, (size_t)L.nw * 16, hipMemcpyHostToDevice));
-  ctx->have_T = true;
-  return SYMGW_OK;
+  return guarded(ctx, [&]() -> int {
+    if (!ctx || !Tnt || !Ttn || !Tnt_BF || !Ttn_FB || !iwn_fermi) return SYMGW_ERR_ARG;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    const Layout &L = ctx->L;
+    HIPCHK(ctx, copy_sync(ctx, ctx->Tnt, Tnt, (size_t)L.nw * (L.nts - 2) * 16, hipMemcpyHostToDevice));
+    HIPCHK(ctx, copy_sync(ctx, ctx->Ttn, Ttn, (size_t)L.nts * L.nw * 16, hipMemcpyHostToDevice));
+    HIPCHK(ctx, copy_sync(ctx, ctx->Tnt_BF, Tnt_BF, (size_t)L.nw_b * (L.nts - 2) * 16, hipMemcpyHostToDevice));
+    HIPCHK(ctx, copy_sync(ctx, ctx->Ttn_FB, Ttn_FB, (size_t)L.nts * L.nw_b * 16, hipMemcpyHostToDevice));
+    HIPCHK(ctx, copy_sync(ctx, ctx->iwn, iwn_fermi, (size_t)L.nw * 16, hipMemcpyHostToDevice));
+    ctx->have_T = true;
+    return SYMGW_OK;
+  });
 }
 
 int symgw_solve_W(symgw_ctx *ctx) {
-  if (!ctx) return SYMGW_ERR_ARG;
-  HIPCHK(ctx, hipSetDevice(ctx->device));
-  int r = phase_W(ctx);
-  if (r) return r;
-  HIPCHK(ctx, hipStreamSynchronize(ctx->st));
-  return SYMGW_OK;
+  return guarded(ctx, [&]() -> int {
+    if (!ctx) return SYMGW_ERR_ARG;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    int r = phase_W(ctx);
+    if (r) return r;
+    HIPCHK(ctx, hipStreamSynchronize(ctx->st));
+    return SYMGW_OK;
+  });
 }
 
 int symgw_solve_Sigma(symgw_ctx *ctx) {
-  if (!ctx) return SYMGW_ERR_ARG;
-  HIPCHK(ctx, hipSetDevice(ctx->device));
-  ev_record(ctx, 2);
-  int r = phase_Sigma(ctx);
-  if (r) return r;
-  ev_record(ctx, 3);
-  HIPCHK(ctx, hipStreamSynchronize(ctx->st));
-  if (ctx->timing) {
-    float ms = 0;
-    (void)hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]);
-    ctx->last_ms[2] = ms;
-  }
-  return SYMGW_OK;
+  return guarded(ctx, [&]() -> int {
+    if (!ctx) return SYMGW_ERR_ARG;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    ev_record(ctx, 2);
+    int r = phase_Sigma(ctx);
+    if (r) return r;
+    ev_record(ctx, 3);
+    HIPCHK(ctx, hipStreamSynchronize(ctx->st));
+    if (ctx->timing) {
+      float ms = 0;
+      (void)hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]);
+      ctx->last_ms[2] = ms;
+    }
+    return SYMGW_OK;
+  });
 }
 
 int symgw_copy_Sigma_to_host(const symgw_ctx *ctx, double *Sigma_ts_kij) {
-  if (!ctx || !Sigma_ts_kij) return SYMGW_ERR_ARG;
-  symgw_ctx *c = const_cast<symgw_ctx *>(ctx);
-  HIPCHK(c, hipSetDevice(ctx->device));
-  HIPCHK(c, hipStreamSynchronize(ctx->st));
-  const Layout &L = ctx->L;
-  const size_t n = (size_t)L.nts * L.ns * L.flat_ao;
-  std::vector<double> tmp(2 * n);
-  HIPCHK(c, copy_sync(c, tmp.data(), ctx->Sig, n * 16, hipMemcpyDeviceToHost));
-  // k_ts_ij -> ts_kij with accumulation (cugw_routines_block.cu:257-273)
-  const int nts_ns = L.nts * L.ns;
-  for (int ik = 0; ik < L.ink; ++ik) {
-    const int64_t sz = L.kao_size_irre[ik], off = L.kao_off_irre[ik];
-    for (int ts = 0; ts < nts_ns; ++ts) {
-      const double *src = tmp.data() + 2 * (off * nts_ns + (int64_t)ts * sz);
-      double *dst = Sigma_ts_kij + 2 * ((int64_t)ts * L.flat_ao + off);
-      for (int64_t x = 0; x < 2 * sz; ++x) dst[x] += src[x];
+  return guarded(const_cast<symgw_ctx *>(ctx), [&]() -> int {
+    if (!ctx || !Sigma_ts_kij) return SYMGW_ERR_ARG;
+    symgw_ctx *c = const_cast<symgw_ctx *>(ctx);
+    HIPCHK(c, hipSetDevice(ctx->device));
+    HIPCHK(c, hipStreamSynchronize(ctx->st));
+    const Layout &L = ctx->L;
+    const size_t n = (size_t)L.nts * L.ns * L.flat_ao;
+    std::vector<double> tmp(2 * n);
+    HIPCHK(c, copy_sync(c, tmp.data(), ctx->Sig, n * 16, hipMemcpyDeviceToHost));
+    // k_ts_ij -> ts_kij with accumulation (cugw_routines_block.cu:257-273)
+    const int nts_ns = L.nts * L.ns;
+    for (int ik = 0; ik < L.ink; ++ik) {
+      const int64_t sz = L.kao_size_irre[ik], off = L.kao_off_irre[ik];
+      for (int ts = 0; ts < nts_ns; ++ts) {
+        const double *src = tmp.data() + 2 * (off * nts_ns + (int64_t)ts * sz);
+        double *dst = Sigma_ts_kij + 2 * ((int64_t)ts * L.flat_ao + off);
+        for (int64_t x = 0; x < 2 * sz; ++x) dst[x] += src[x];
+      }
     }
-  }
-  return SYMGW_OK;
+    return SYMGW_OK;
+  });
 }
 
 int symgw_upload_Sigma(symgw_ctx *ctx, const double *Sigma_ts_kij) {
-  if (!ctx || !Sigma_ts_kij) return SYMGW_ERR_ARG;
-  HIPCHK(ctx, hipSetDevice(ctx->device));
-  const Layout &L = ctx->L;
-  const size_t n = (size_t)L.nts * L.ns * L.flat_ao;
-  std::vector<double> tmp(2 * n);
-  const int nts_ns = L.nts * L.ns;
-  for (int ik = 0; ik < L.ink; ++ik) {
-    const int64_t sz = L.kao_size_irre[ik], off = L.kao_off_irre[ik];
-    for (int ts = 0; ts < nts_ns; ++ts) {
-      const double *src = Sigma_ts_kij + 2 * ((int64_t)ts * L.flat_ao + off);
-      double *dst = tmp.data() + 2 * (off * nts_ns + (int64_t)ts * sz);
-      std::memcpy(dst, src, (size_t)sz * 16);
+  return guarded(ctx, [&]() -> int {
+    if (!ctx || !Sigma_ts_kij) return SYMGW_ERR_ARG;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    const Layout &L = ctx->L;
+    const size_t n = (size_t)L.nts * L.ns * L.flat_ao;
+    std::vector<double> tmp(2 * n);
+    const int nts_ns = L.nts * L.ns;
+    for (int ik = 0; ik < L.ink; ++ik) {
+      const int64_t sz = L.kao_size_irre[ik], off = L.kao_off_irre[ik];
+      for (int ts = 0; ts < nts_ns; ++ts) {
+        const double *src = Sigma_ts_kij + 2 * ((int64_t)ts * L.flat_ao + off);
+        double *dst = tmp.data() + 2 * (off * nts_ns + (int64_t)ts * sz);
+        std::memcpy(dst, src, (size_t)sz * 16);
+      }
     }
-  }
-  HIPCHK(ctx, copy_sync(ctx, ctx->Sig, tmp.data(), n * 16, hipMemcpyHostToDevice));
-  ctx->have_Sig = true;
-  return SYMGW_OK;
+    HIPCHK(ctx, copy_sync(ctx, ctx->Sig, tmp.data(), n * 16, hipMemcpyHostToDevice));
+    ctx->have_Sig = true;
+    return SYMGW_OK;
+  });
 }
 
 int symgw_set_fock(symgw_ctx *ctx, const double *F, const double *S) {
-  if (!ctx || !F || !S) return SYMGW_ERR_ARG;
-  HIPCHK(ctx, hipSetDevice(ctx->device));
-  const size_t n = (size_t)ctx->L.ns * ctx->L.flat_ao;
-  HIPCHK(ctx, copy_sync(ctx, ctx->F, F, n * 16, hipMemcpyHostToDevice));
-  HIPCHK(ctx, copy_sync(ctx, ctx->S, S, n * 16, hipMemcpyHostToDevice));
-  ctx->have_F = true;
-  return SYMGW_OK;
+  return guarded(ctx, [&]() -> int {
+    if (!ctx || !F || !S) return SYMGW_ERR_ARG;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    const size_t n = (size_t)ctx->L.ns * ctx->L.flat_ao;
+    HIPCHK(ctx, copy_sync(ctx, ctx->F, F, n * 16, hipMemcpyHostToDevice));
+    HIPCHK(ctx, copy_sync(ctx, ctx->S, S, n * 16, hipMemcpyHostToDevice));
+    ctx->have_F = true;
+    return SYMGW_OK;
+  });
 }
 
 int symgw_dyson(symgw_ctx *ctx, double mu, double sigma_scale) {
-  if (!ctx) return SYMGW_ERR_ARG;
-  HIPCHK(ctx, hipSetDevice(ctx->device));
-  int r = phase_Dyson(ctx, mu, sigma_scale);
-  if (r) return r;
-  HIPCHK(ctx, hipStreamSynchronize(ctx->st));
-  return SYMGW_OK;
+  return guarded(ctx, [&]() -> int {
+    if (!ctx) return SYMGW_ERR_ARG;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    int r = phase_Dyson(ctx, mu, sigma_scale);
+    if (r) return r;
+    HIPCHK(ctx, hipStreamSynchronize(ctx->st));
+    return SYMGW_OK;
+  });
 }
 
 int symgw_density(symgw_ctx *ctx, const double *kweights, double *nel) {
-  if (!ctx || !kweights || !nel) return SYMGW_ERR_ARG;
-  HIPCHK(ctx, hipSetDevice(ctx->device));
-  if (!ctx->have_G || !ctx->have_F) return fail(ctx, SYMGW_ERR_STATE, "G or S missing");
-  const Layout &L = ctx->L;
-  HIPCHK(ctx, hipMemcpyAsync(ctx->kw, kweights, (size_t)L.ink * sizeof(double), hipMemcpyHostToDevice, ctx->st));
-  const cplx<double> *Gbeta = ctx->G_ibz + (int64_t)(L.nts - 1) * L.ns * L.flat_ao;
-  HIPCHK(ctx, launch_density(ctx->d_d_blocks, (int)ctx->W.d_blocks.size(), L.ns, Gbeta, ctx->S, ctx->d_kao_off_irre,
-                             L.flat_ao, ctx->kw, ctx->nel_dev, ctx->st));
-  double v = 0;
-  HIPCHK(ctx, hipMemcpyAsync(&v, ctx->nel_dev, sizeof(double), hipMemcpyDeviceToHost, ctx->st));
-  HIPCHK(ctx, hipStreamSynchronize(ctx->st));
-  // dmr = -G(beta) * (2 / ns) (sc_loop_gf_block.cpp:18-19), per k weighted by w_k / nk
-  *nel = -(2.0 / L.ns) * v / L.nk;
-  return SYMGW_OK;
+  return guarded(ctx, [&]() -> int {
+    if (!ctx || !kweights || !nel) return SYMGW_ERR_ARG;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    if (!ctx->have_G || !ctx->have_F) return fail(ctx, SYMGW_ERR_STATE, "G or S missing");
+    const Layout &L = ctx->L;
+    HIPCHK(ctx, hipMemcpyAsync(ctx->kw, kweights, (size_t)L.ink * sizeof(double), hipMemcpyHostToDevice, ctx->st));
+    const cplx<double> *Gbeta = ctx->G_ibz + (int64_t)(L.nts - 1) * L.ns * L.flat_ao;
+    HIPCHK(ctx, launch_density(ctx->d_d_blocks, (int)ctx->W.d_blocks.size(), L.ns, Gbeta, ctx->S, ctx->d_kao_off_irre,
+                               L.flat_ao, ctx->kw, ctx->nel_dev, ctx->st));
+    double v = 0;
+    HIPCHK(ctx, hipMemcpyAsync(&v, ctx->nel_dev, sizeof(double), hipMemcpyDeviceToHost, ctx->st));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->st));
+    // dmr = -G(beta) * (2 / ns) (sc_loop_gf_block.cpp:18-19), per k weighted by w_k / nk
+    *nel = -(2.0 / L.ns) * v / L.nk;
+    return SYMGW_OK;
+  });
 }
 
 int symgw_mu_search(symgw_ctx *ctx, double nel_target, const double *kweights, double sigma_scale, double mu_lo,
                     double mu_hi, double tol, double *mu_out, double *nel_out) {
-  if (!ctx || !kweights || !mu_out || mu_hi <= mu_lo || tol <= 0) return SYMGW_ERR_ARG;
-  double lo = mu_lo, hi = mu_hi, mu = 0.5 * (lo + hi), n = 0, nlo = 0, nhi = 0;
-  int r;
-  auto count = [&](double m, double &nn) {
-    int rr = symgw_dyson(ctx, m, sigma_scale);
-    return rr ? rr : symgw_density(ctx, kweights, &nn);
-  };
-  // widen [lo, hi] (doubling its width on the short side) until it brackets the target
-  if ((r = count(lo, nlo)) || (r = count(hi, nhi))) return r;
-  for (int ex = 0; ex < 8 && (nlo > nel_target || nhi < nel_target); ++ex) {
-    const double w = hi - lo;
-    if (nlo > nel_target && (r = count(lo -= w, nlo))) return r;
-    if (nhi < nel_target && (r = count(hi += w, nhi))) return r;
-  }
-  if (nlo > nel_target || nhi < nel_target)
-    return fail(ctx, SYMGW_ERR_ARG, "mu_search: the electron count is not bracketed");
-  for (int it = 0; it < 200; ++it) {
-    mu = 0.5 * (lo + hi);
-    if ((r = symgw_dyson(ctx, mu, sigma_scale))) return r;
-    if ((r = symgw_density(ctx, kweights, &n))) return r;
-    // N(mu) increases with mu
-    if (n < nel_target) lo = mu; else hi = mu;
-    if (std::fabs(n - nel_target) < tol || hi - lo < 1e-14) break;
-  }
-  *mu_out = mu;
-  if (nel_out) *nel_out = n;
-  return SYMGW_OK;
+  return guarded(ctx, [&]() -> int {
+    if (!ctx || !kweights || !mu_out || mu_hi <= mu_lo || tol <= 0) return SYMGW_ERR_ARG;
+    double lo = mu_lo, hi = mu_hi, mu = 0.5 * (lo + hi), n = 0, nlo = 0, nhi = 0;
+    int r;
+    auto count = [&](double m, double &nn) {
+      int rr = symgw_dyson(ctx, m, sigma_scale);
+      return rr ? rr : symgw_density(ctx, kweights, &nn);
+    };
+    // widen [lo, hi] (doubling its width on the short side) until it brackets the target
+    if ((r = count(lo, nlo)) || (r = count(hi, nhi))) return r;
+    for (int ex = 0; ex < 8 && (nlo > nel_target || nhi < nel_target); ++ex) {
+      const double w = hi - lo;
+      if (nlo > nel_target && (r = count(lo -= w, nlo))) return r;
+      if (nhi < nel_target && (r = count(hi += w, nhi))) return r;
+    }
+    if (nlo > nel_target || nhi < nel_target)
+      return fail(ctx, SYMGW_ERR_ARG, "mu_search: the electron count is not bracketed");
+    for (int it = 0; it < 200; ++it) {
+      mu = 0.5 * (lo + hi);
+      if ((r = symgw_dyson(ctx, mu, sigma_scale))) return r;
+      if ((r = symgw_density(ctx, kweights, &n))) return r;
+      // N(mu) increases with mu
+      if (n < nel_target) lo = mu; else hi = mu;
+      if (std::fabs(n - nel_target) < tol || hi - lo < 1e-14) break;
+    }
+    *mu_out = mu;
+    if (nel_out) *nel_out = n;
+    return SYMGW_OK;
+  });
 }
 
 int symgw_gw_iteration(symgw_ctx *ctx, double mu, int use_graph) {
-  if (!ctx) return SYMGW_ERR_ARG;
-  HIPCHK(ctx, hipSetDevice(ctx->device));
-  if (!ctx->have_V || !ctx->have_G || !ctx->have_T || !ctx->have_F)
-    return fail(ctx, SYMGW_ERR_STATE, "V, G, transforms and Fock must be set first");
-  ctx->have_P = true;   // produced inside the sequence
-  ctx->have_Sig = true;
-  if (!use_graph) {
-    int r = iteration_body(ctx, mu);
-    if (r) return r;
-  } else {
-    std::unique_lock<std::mutex> lk(g_dev_mu);
-    if (ctx->graph_exec[0] && ctx->graph_mu != mu) drop_graphs(ctx);
-    if (!ctx->graph_exec[0]) {
-      const bool tm = ctx->timing;
-      ctx->timing = false;
-      for (int ph = 0; ph < 4; ++ph) {
-        hipGraph_t g = nullptr;
-        hipError_t e = hipStreamBeginCapture(ctx->st, hipStreamCaptureModeThreadLocal);
-        if (e != hipSuccess) {
-          ctx->timing = tm;
-          drop_graphs(ctx);
-          return fail(ctx, SYMGW_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+  return guarded(ctx, [&]() -> int {
+    if (!ctx) return SYMGW_ERR_ARG;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    if (!ctx->have_V || !ctx->have_G || !ctx->have_T || !ctx->have_F)
+      return fail(ctx, SYMGW_ERR_STATE, "V, G, transforms and Fock must be set first");
+    ctx->have_P = true;   // produced inside the sequence
+    ctx->have_Sig = true;
+    if (!use_graph) {
+      int r = iteration_body(ctx, mu);
+      if (r) return r;
+    } else {
+      std::unique_lock<std::mutex> lk(g_dev_mu);
+      if (ctx->graph_exec[0] && ctx->graph_mu != mu) drop_graphs(ctx);
+      if (!ctx->graph_exec[0]) {
+        const bool tm = ctx->timing;
+        ctx->timing = false;
+        for (int ph = 0; ph < 4; ++ph) {
+          hipGraph_t g = nullptr;
+          hipError_t e = hipStreamBeginCapture(ctx->st, hipStreamCaptureModeThreadLocal);
+          if (e != hipSuccess) {
+            ctx->timing = tm;
+            drop_graphs(ctx);
+            return fail(ctx, SYMGW_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+          }
+          int r = run_phase(ctx, ph, mu);
+          e = hipStreamEndCapture(ctx->st, &g);
+          if (!r && e == hipSuccess) e = hipGraphInstantiate(&ctx->graph_exec[ph], g, nullptr, nullptr, 0);
+          if (g) (void)hipGraphDestroy(g);
+          if (r || e != hipSuccess) {
+            ctx->timing = tm;
+            drop_graphs(ctx);
+            return r ? r : fail(ctx, SYMGW_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+          }
         }
-        int r = run_phase(ctx, ph, mu);
-        e = hipStreamEndCapture(ctx->st, &g);
-        if (!r && e == hipSuccess) e = hipGraphInstantiate(&ctx->graph_exec[ph], g, nullptr, nullptr, 0);
-        if (g) (void)hipGraphDestroy(g);
-        if (r || e != hipSuccess) {
-          ctx->timing = tm;
-          drop_graphs(ctx);
-          return r ? r : fail(ctx, SYMGW_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
-        }
+        ctx->timing = tm;
+        ctx->graph_mu = mu;
       }
-      ctx->timing = tm;
-      ctx->graph_mu = mu;
+      lk.unlock();
+      ev_record(ctx, 0);
+      for (int ph = 0; ph < 4; ++ph) {
+        HIPCHK(ctx, hipGraphLaunch(ctx->graph_exec[ph], ctx->st));
+        ev_record(ctx, ph + 1);
+      }
     }
-    lk.unlock();
-    ev_record(ctx, 0);
-    for (int ph = 0; ph < 4; ++ph) {
-      HIPCHK(ctx, hipGraphLaunch(ctx->graph_exec[ph], ctx->st));
-      ev_record(ctx, ph + 1);
-    }
-  }
-  return SYMGW_OK;
+    return SYMGW_OK;
+  });
 }
 
 int symgw_sync(symgw_ctx *ctx) {
-  if (!ctx) return SYMGW_ERR_ARG;
-  HIPCHK(ctx, hipSetDevice(ctx->device));
-  HIPCHK(ctx, hipStreamSynchronize(ctx->st));
-  if (ctx->timing) {
-    for (int i = 0; i < 5; ++i) {
-      float ms = 0;
-      const int a = i < 4 ? i : 0, b = i < 4 ? i + 1 : 4;
-      if (hipEventElapsedTime(&ms, ctx->ev[a], ctx->ev[b]) == hipSuccess) ctx->last_ms[i] = ms;
-      else (void)hipGetLastError();
+  return guarded(ctx, [&]() -> int {
+    if (!ctx) return SYMGW_ERR_ARG;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->st));
+    if (ctx->timing) {
+      for (int i = 0; i < 5; ++i) {
+        float ms = 0;
+        const int a = i < 4 ? i : 0, b = i < 4 ? i + 1 : 4;
+        if (hipEventElapsedTime(&ms, ctx->ev[a], ctx->ev[b]) == hipSuccess) ctx->last_ms[i] = ms;
+        else (void)hipGetLastError();
+      }
     }
-  }
-  return SYMGW_OK;
+    return SYMGW_OK;
+  });
 }
 
 int symgw_last_times(const symgw_ctx *ctx, double *ms5) {
-  if (!ctx || !ms5) return SYMGW_ERR_ARG;
-  for (int i = 0; i < 5; ++i) ms5[i] = ctx->last_ms[i];
-  return SYMGW_OK;
+  return guarded(const_cast<symgw_ctx *>(ctx), [&]() -> int {
+    if (!ctx || !ms5) return SYMGW_ERR_ARG;
+    for (int i = 0; i < 5; ++i) ms5[i] = ctx->last_ms[i];
+    return SYMGW_OK;
+  });
 }
 
 int symgw_set_timing(symgw_ctx *ctx, int on) {
-  if (!ctx) return SYMGW_ERR_ARG;
-  ctx->timing = on != 0;
-  return SYMGW_OK;
+  return guarded(ctx, [&]() -> int {
+    if (!ctx) return SYMGW_ERR_ARG;
+    ctx->timing = on != 0;
+    return SYMGW_OK;
+  });
 }
 
 }  // extern "C"

@@ -17,8 +17,11 @@
 
 namespace symgw {
 
-// LDS of one rotation workgroup (the W = X O^H column chunk).
-constexpr int kRotLds = 64 * 1024;
+// LDS of one rotation workgroup (the W = X O^H column chunk); the planner
+// rejects source blocks above kRotMaxBlock, so a chunk is never narrower than
+// 16 columns.
+constexpr int kRotLds = kRotLdsBytes;
+static_assert(kRotMaxBlock * 16 * 16 <= kRotLds, "rotation chunk of 16 columns must fit the LDS");
 
 namespace {
 

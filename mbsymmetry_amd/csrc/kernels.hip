@@ -178,8 +178,10 @@ __global__ __launch_bounds__(NT) void wsolve_kernel(const SolveBlk *__restrict__
 // Dyson per (k_ir, w, s, ao block) (sc_type_block.cpp:159-168):
 //   G(iw) = [(iw + mu) S - F - sigma_scale * Sigma(iw)]^-1
 // Sw/Gw layout: [k_ir][w][s][kslice] at d_off[k_ir] + (w*ns + s)*kslice + off.
+// Sw and Gw may be the same buffer (the Dyson phase solves in place): each
+// workgroup reads its Sigma block before the barrier and writes G after it.
 __global__ __launch_bounds__(NT) void dyson_kernel(const SolveBlk *__restrict__ blks, int nblk, int nw, int ns,
-                                                  const cplx<double> *__restrict__ Sw, cplx<double> *__restrict__ Gw,
+                                                  const cplx<double> *Sw, cplx<double> *Gw,
                                                   const int64_t *__restrict__ d_off,
                                                   const int64_t *__restrict__ kslice,
                                                   const int64_t *__restrict__ f_off,

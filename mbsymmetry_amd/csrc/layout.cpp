@@ -518,6 +518,12 @@ bool build_worklists(const Layout &L, int eb, WorkLists &W, std::string &err) {
         int64_t base = which == 0 ? L.korep_off[ck] : L.kauxrep_off[ck];
         for (size_t b = 0; b < irr.size(); ++b) {
           int a1 = irr[b][0], a2 = irr[b][1];
+          // the rotation kernel keeps a column chunk of at least 16 columns of
+          // X O^H (n rows) in its LDS
+          if (szr[a2] > kRotMaxBlock) {
+            err = fmt(which == 0 ? "rotated ao block larger than 256" : "rotated aux block larger than 256", szr[a2]);
+            return false;
+          }
           RotBlk R{so[a1], sor[a2], base + off[b], sz[a1], szr[a2], k, k != ck ? 1 : 0};
           out.push_back(R);
         }

@@ -165,6 +165,12 @@ struct RotBlk {
   int32_t conj;        // 1 => complex-conjugate the result (k != conj_index[k])
 };
 
+// Largest source block a rotation reads (O is m x n with n <= kRotMaxBlock):
+// the rotation workgroup holds 16 columns of X O^H, n x 16 complex128, in
+// its kRotLdsBytes of LDS.
+constexpr int kRotLdsBytes = 64 * 1024;
+constexpr int kRotMaxBlock = kRotLdsBytes / 16 / 16;
+
 // Contiguous aux-block of one W solve or ao-block of one Dyson solve.
 struct SolveBlk {
   int64_t off;         // flat offset inside the slice

@@ -97,8 +97,9 @@ def fermi_g(eps, tau, beta):
     eps = np.asarray(eps)[None, :]
     tau = np.asarray(tau)[:, None]
     pos = eps >= 0
-    out = np.where(pos, -np.exp(-eps * tau) / (1.0 + np.exp(-beta * np.abs(eps))),
-                   -np.exp((beta - tau) * eps) / (1.0 + np.exp(beta * eps)))
+    with np.errstate(over="ignore", invalid="ignore"):   # np.where evaluates both branches
+        out = np.where(pos, -np.exp(-eps * tau) / (1.0 + np.exp(-beta * np.abs(eps))),
+                       -np.exp((beta - tau) * eps) / (1.0 + np.exp(beta * eps)))
     return out
 
 
