@@ -191,6 +191,12 @@ int symgw_generate_V(symgw_ctx *ctx, uint64_t seed, double scale,
 int symgw_generate_V_pair_host(const symgw_layout_desc *desc, uint64_t seed, double scale,
                                const double *decay, int n_decay, int k1, int k2, double *out);
 
+/* The same for npairs stored pairs at once, spread over the host's threads:
+ * pair i = (pairs[2i], pairs[2i+1]) is written at out + 2 * out_off[i]
+ * (complex128, its kpair slice size elements). */
+int symgw_generate_V_pairs_host(const symgw_layout_desc *desc, uint64_t seed, double scale, const double *decay,
+                                int n_decay, int npairs, const int32_t *pairs, const int64_t *out_off, double *out);
+
 /* copy_ORep_host_to_device / copy_AuxRep_host_to_device
  * (cugw_routines_block.cu:214-234): flat rotation blocks, complex128. */
 int symgw_set_reps(symgw_ctx *ctx, const double *orep_flat, const double *auxrep_flat);

@@ -56,6 +56,8 @@ SIGNATURES = {
     "symgw_generate_V": (C.c_int, [C.c_void_p, C.c_uint64, C.c_double, _dp, C.c_int]),
     "symgw_generate_V_pair_host": (C.c_int, [C.POINTER(LayoutDesc), C.c_uint64, C.c_double, _dp, C.c_int,
                                              C.c_int, C.c_int, _dp]),
+    "symgw_generate_V_pairs_host": (C.c_int, [C.POINTER(LayoutDesc), C.c_uint64, C.c_double, _dp, C.c_int, C.c_int,
+                                              _i32p, C.POINTER(C.c_int64), _dp]),
     "symgw_set_reps": (C.c_int, [C.c_void_p, _dp, _dp]),
     "symgw_upload_G": (C.c_int, [C.c_void_p, _dp]),
     "symgw_download_G": (C.c_int, [C.c_void_p, _dp]),
@@ -359,3 +361,27 @@ def ir_uhat(basis, n):
     if rc != SYMGW_OK:
         raise SymGWError(f"symgw_ir_uhat failed ({rc})")
     return out[..., 0] + 1j * out[..., 1]
+
+
+def generate_V_pairs_host(system, seed, scale, decay, pairs):
+    """Host twin of the device generator for a list of stored pairs (k1, k2):
+    returns (V, offsets) with pair i's flat slice at V[offsets[i]:]."""
+    lib = load()
+    offs = system.kpair_offsets()
+    ps = np.ascontiguousarray(np.asarray(pairs, dtype=np.int32).reshape(-1, 2))
+    sizes = []
+    for k1, k2 in ps:
+        p = int(k1) * system.nk + int(k2)
+        if p not in offs:
+            raise ValueError(f"generate_V_pairs_host: pair ({k1}, {k2}) is not stored")
+        sizes.append(offs[p][1])
+    start = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64) if sizes else np.zeros(0, np.int64)
+    out = np.zeros(int(sum(sizes)), dtype=np.complex128)
+    decay = np.ascontiguousarray(decay, dtype=np.float64)
+    d = system.desc()
+    rc = lib.symgw_generate_V_pairs_host(C.byref(d), seed, scale, dptr(decay), decay.size, len(ps),
+                                         ps.ctypes.data_as(_i32p), start.ctypes.data_as(C.POINTER(C.c_int64)),
+                                         dptr(out) if out.size else None)
+    if rc != SYMGW_OK:
+        raise SymGWError(f"symgw_generate_V_pairs_host failed: {lib.symgw_create_error().decode()}")
+    return out, start

@@ -8,6 +8,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
+#include <thread>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -825,6 +827,58 @@ int symgw_generate_V_pair_host(const symgw_layout_desc *desc, uint64_t seed, dou
         out[2 * (B.local_off + e) + 1] = im;
       }
     }
+    return SYMGW_OK;
+  });
+}
+
+int symgw_generate_V_pairs_host(const symgw_layout_desc *desc, uint64_t seed, double scale, const double *decay,
+                                int n_decay, int npairs, const int32_t *pairs, const int64_t *out_off, double *out) {
+  return guarded(nullptr, [&]() -> int {
+    if (!desc || !decay || n_decay <= 0 || npairs < 0 || (npairs > 0 && (!pairs || !out_off || !out)))
+      return SYMGW_ERR_ARG;
+    Layout L;
+    std::string err;
+    if (!build_layout(desc, L, err)) {
+      g_create_error = err;
+      return SYMGW_ERR_ARG;
+    }
+    for (int i = 0; i < npairs; ++i) {
+      const int k1 = pairs[2 * i], k2 = pairs[2 * i + 1];
+      if (k1 < 0 || k2 < 0 || k1 >= L.nk || k2 >= L.nk || L.V_irreps[(size_t)k1 * L.nk + k2].empty()) {
+        g_create_error = "generate_V_pairs_host: pair not stored";
+        return SYMGW_ERR_ARG;
+      }
+    }
+    // pairs are independent: spread them over the host threads
+    const int nth = std::max(1, std::min<int>(npairs, (int)std::thread::hardware_concurrency()));
+    std::atomic<int> next(0);
+    auto work = [&]() {
+      std::vector<GenBlk> gb;
+      for (int i = next++; i < npairs; i = next++) {
+        gb.clear();
+        gen_blocks(L, gb, pairs[2 * i] * L.nk + pairs[2 * i + 1]);
+        double *o = out + 2 * out_off[i];
+        for (const GenBlk &B : gb) {
+          const int64_t K = (int64_t)B.k1 * B.k2, n = (int64_t)B.q * K;
+          for (int64_t e = 0; e < n; ++e) {
+            const int64_t rem = e % K;
+            const int a = (int)(rem / B.k2), m = (int)(rem % B.k2);
+            int d = (B.a_base + a) - (B.b_base + m);
+            d = d < 0 ? -d : d;
+            d = d < n_decay ? d : n_decay - 1;
+            const double f = (scale * decay[d]) * 0.70710678118654752440;
+            double re, im;
+            gen_v_elem(seed, (uint32_t)B.pair, (uint64_t)(B.local_off + e), f, re, im);
+            o[2 * (B.local_off + e)] = re;
+            o[2 * (B.local_off + e) + 1] = im;
+          }
+        }
+      }
+    };
+    std::vector<std::thread> th;
+    for (int t = 1; t < nth; ++t) th.emplace_back(work);
+    work();
+    for (auto &t : th) t.join();
     return SYMGW_OK;
   });
 }

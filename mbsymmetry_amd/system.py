@@ -172,13 +172,19 @@ class System:
             return self.V
         offs = self.kpair_offsets()
         out = np.zeros(self.flat_V(), dtype=np.complex128)
-        dec = self.decay_table()
-        for p, (o, sz) in offs.items():
-            if pairs is not None and p not in pairs:
-                continue
-            k1, k2 = divmod(p, self.nk)
-            out[o:o + sz] = _lib.generate_V_pair_host(self, self.v_seed, self.v_scale, dec, k1, k2, sz)
+        sel = [p for p in offs if pairs is None or p in pairs]
+        Vs, st = _lib.generate_V_pairs_host(self, self.v_seed, self.v_scale, self.decay_table(),
+                                            [divmod(p, self.nk) for p in sel])
+        for p, o in zip(sel, st):
+            out[offs[p][0]:offs[p][0] + offs[p][1]] = Vs[o:o + offs[p][1]]
         return out
+
+    def host_V_pairs(self, pairs):
+        """V of the listed stored pairs only: (compact array, {pair: offset})."""
+        sel = sorted(set(int(p) for p in pairs))
+        Vs, st = _lib.generate_V_pairs_host(self, self.v_seed, self.v_scale, self.decay_table(),
+                                            [divmod(p, self.nk) for p in sel])
+        return Vs, {p: int(o) for p, o in zip(sel, st)}
 
     def decay_table(self):
         nao = max(sum(s) for s in self.ao_sizes)
@@ -416,12 +422,22 @@ def lih_fixture(z, transforms: Transforms) -> System:
 SEED_BASE = 20241203
 
 
+def coulomb_scale(aux_sizes) -> float:
+    """V scale of the benchmark systems: 1/sqrt(naux), so that the bare
+    Coulomb integrals (ij|kl) = sum_Q V^Q_ij conj(V^Q_kl) of unit-variance
+    entries are O(1) Ha (unscaled entries make them O(naux) Ha, a self-energy
+    of hundreds of Ha and a Dyson step no imaginary-time grid resolves)."""
+    return 1.0 / float(np.sqrt(sum(aux_sizes)))
+
+
 def config1(transforms: Optional[Transforms] = None) -> System:
     """2x2x2 mesh, 4 iBZ points, nao 26 {8,6,6,4,2}, naux 128 {40,32,32,24},
     beta = 50, 60-node Chebyshev grid, ns = 1."""
     from .transforms import chebyshev
     tr = transforms or chebyshev(60, 50.0)
-    return synthetic("config1", 2, 4, [8, 6, 6, 4, 2], [40, 32, 32, 24], 1, tr, SEED_BASE + 1, all_pairs=True)
+    aux = [40, 32, 32, 24]
+    return synthetic("config1", 2, 4, [8, 6, 6, 4, 2], aux, 1, tr, SEED_BASE + 1, v_scale=coulomb_scale(aux),
+                     all_pairs=True)
 
 
 def config2_sizes():
@@ -431,14 +447,16 @@ def config2_sizes():
     return ao, aux
 
 
-def config2(transforms: Optional[Transforms] = None, v_scale: float = 1.0) -> System:
+def config2(transforms: Optional[Transforms] = None, v_scale: Optional[float] = None) -> System:
     """6x6x6 mesh reduced to 16 iBZ points; nao 104 in 8 blocks log-uniform
-    [4, 32]; naux 400 in 8 blocks log-uniform [16, 128]; beta = 100; 104 tau
-    points (nts = 106)."""
-    from .transforms import chebyshev
-    tr = transforms or chebyshev(104, 100.0)
+    [4, 32]; naux 400 in 8 blocks log-uniform [16, 128]; beta = 100; IR grid
+    of cutoff Lambda = 1e4 with 104 fermionic (nts = 106) and 103 bosonic
+    points, generated by the library."""
+    from .transforms import ir
+    tr = transforms or ir(100.0, 1e4, 104)
     ao, aux = config2_sizes()
-    return synthetic("config2", 6, 16, ao, aux, 1, tr, SEED_BASE + 2, v_scale=v_scale)
+    return synthetic("config2", 6, 16, ao, aux, 1, tr, SEED_BASE + 2,
+                     v_scale=coulomb_scale(aux) if v_scale is None else v_scale)
 
 
 def tiny(ns: int = 1, n: int = 2, ink: int = 4, ao=(3, 2), aux=(4, 3), ni: int = 8, beta: float = 10.0,
@@ -467,6 +485,7 @@ def serving_mix(n_requests: int = 32, seed: int = SEED_BASE + 5, meshes=(2, 4), 
         ao = split_sizes(rng, nao, nb, max(2, nao // (2 * nb)), max(4, 2 * nao // nb))
         aux = split_sizes(rng, 4 * nao, nb, max(4, 4 * nao // (2 * nb)), max(8, 8 * nao // nb))
         ink = {2: 4, 4: 8, 6: 16}[n]
-        sysm = synthetic(f"mix{i}-n{n}-nao{nao}", n, ink, ao, aux, 1, tr, seed * 100 + i)
+        sysm = synthetic(f"mix{i}-n{n}-nao{nao}", n, ink, ao, aux, 1, tr, seed * 100 + i,
+                         v_scale=coulomb_scale(aux))
         out.append((sysm, iterations))
     return out

@@ -370,6 +370,186 @@ int orc_Sigma(const orc_desc *d, const double *V, const double *G_full, const do
   return 0;
 }
 
+/* ---- single (q, t) / (k, t, s) slices summed over the whole mesh ----------
+ * The same loop nests as orc_P0 / orc_Sigma (gw_block.cpp:6-84, :86-158) for
+ * one output slice, with the k (P0) or q (Sigma) sum split over threads in
+ * contiguous ranges and the partial sums added in thread order.  V is given
+ * per pair: the slice of pair p = k1*nk + k2 starts at V + 2*voff[p]. */
+static int nthreads_of(int n) {
+#ifdef _OPENMP
+  return n < 1 ? 1 : n;
+#else
+  (void)n;
+  return 1;
+#endif
+}
+
+int orc_P0_at(const orc_desc *d, const double *V, const int64_t *voff, const double *G_full, int iqr, int t,
+              double *out, int nthreads) {
+  tables T; make_tables(d, &T);
+  const int nk = d->nk, ns = d->ns, nts = d->nts;
+  if (iqr < 0 || iqr >= d->ink || t < 0 || t >= nts) { free_tables(&T); return 1; }
+  const int q = d->irre_list[iqr];
+  const int64_t qsl = T.qaux_sz[q];
+  const double scale = -1.0 / nk * (ns == 1 ? 2.0 : 1.0);
+  const int nth = nthreads_of(nthreads);
+  double *part = calloc(2 * (size_t)qsl * nth, sizeof(double));
+  int bad = 0;
+#ifdef _OPENMP
+#pragma omp parallel num_threads(nth) reduction(| : bad)
+#endif
+  {
+#ifdef _OPENMP
+    const int th = omp_get_thread_num(), nt = omp_get_num_threads();
+#else
+    const int th = 0, nt = 1;
+#endif
+    double *acc = part + 2 * (size_t)qsl * th;
+    size_t cap = 0;
+    double *X1 = NULL, *X2 = NULL;
+    const int kb = (int)((int64_t)nk * th / nt), ke = (int)((int64_t)nk * (th + 1) / nt);
+    for (int k1 = kb; k1 < ke; ++k1) {
+      const int k2 = d->kpq[k1 * nk + q];
+      const int p = k1 * nk + k2;
+      if (T.kp_off[p] < 0) continue;
+      if (voff[p] < 0) { bad = 1; continue; }
+      const double *Vp = V + 2 * voff[p];
+      for (int s = 0; s < ns; ++s)
+        for (int b = 0; b < T.kp_nb[p]; ++b) {
+          const int iq = T.kp_irr[p][3 * b], i1 = T.kp_irr[p][3 * b + 1], i2 = T.kp_irr[p][3 * b + 2];
+          const int qs = T.aux_sz[q][iq], a1 = T.ao_sz[k1][i1], a2 = T.ao_sz[k2][i2];
+          const size_t K = (size_t)a1 * a2, sz = K * qs;
+          if (sz > cap) { cap = sz; X1 = realloc(X1, 16 * cap); X2 = realloc(X2, 16 * cap); }
+          const double *Vb = Vp + 2 * T.kp_boff[p][b];
+          const double *Gtp = G_full + 2 * (((size_t)(nts - 1 - t) * ns + s) * T.flat_ao_full + T.kao_off[k1] + T.ao_off[k1][i1]);
+          const double *G1 = G_full + 2 * (((size_t)t * ns + s) * T.flat_ao_full + T.kao_off[k2] + T.ao_off[k2][i2]);
+          /* X1[Q][p][m] = sum_a Gtp[p][a] V[Q][a][m]  (G(beta - t) V) */
+          for (int Q = 0; Q < qs; ++Q) {
+            zgemm(a1, a2, a1, Gtp, a1, 0, Vb + 2 * (size_t)Q * K, a2, 0, X2, a2, 0);
+            memcpy(X1 + 2 * (size_t)Q * K, X2, 16 * K);
+          }
+          /* Y[P][p][n] = sum_m conj(V[P][p][m]) G1[n][m]  (conj(V) G(t)^T, as orc_P0_sample) */
+          double *Y = X2;
+          zgemm(qs * a1, a2, a2, Vb, a2, 2, G1, a2, 1, Y, a2, 0);
+          /* P0[Q][P] += sum_x Y[P][x] X1[Q][x] over x = (p, m) */
+          for (int Q = 0; Q < qs; ++Q)
+            for (int P = 0; P < qs; ++P) {
+              double re = 0, im = 0;
+              const double *x1 = X1 + 2 * (size_t)Q * K, *x2 = Y + 2 * (size_t)P * K;
+              for (size_t x = 0; x < K; ++x) {
+                re += x2[2 * x] * x1[2 * x] - x2[2 * x + 1] * x1[2 * x + 1];
+                im += x2[2 * x] * x1[2 * x + 1] + x2[2 * x + 1] * x1[2 * x];
+              }
+              double *o = acc + 2 * (T.aux_off[q][iq] + (size_t)Q * qs + P);
+              o[0] += re;
+              o[1] += im;
+            }
+        }
+    }
+    free(X1); free(X2);
+  }
+  for (int64_t x = 0; x < 2 * qsl; ++x) {
+    double v = 0;
+    for (int th = 0; th < nth; ++th) v += part[2 * (size_t)qsl * th + x];
+    out[x] = scale * v;
+  }
+  free(part);
+  free_tables(&T);
+  return bad ? 2 : 0;
+}
+
+int orc_rotate_P_t(const orc_desc *d, const double *P_ibz, const double *auxrep, int t, double *P_full_t) {
+  tables T; make_tables(d, &T);
+  const int nk = d->nk, nts = d->nts;
+  for (int q = 0; q < nk; ++q) {
+    int cq = d->conj_index[q], icq = d->irre_index[cq], qr = d->irre_list[icq];
+    int cj = (q != cq);
+    const double *src = P_ibz + 2 * ((size_t)nts * T.qaux_off_i[icq] + (size_t)t * T.qaux_sz[qr]);
+    double *dst = P_full_t + 2 * T.qaux_off[q];
+    if (d->index[cq] == cq) {
+      for (int64_t x = 0; x < T.qaux_sz[q]; ++x) { dst[2 * x] = src[2 * x]; dst[2 * x + 1] = cj ? -src[2 * x + 1] : src[2 * x + 1]; }
+    } else {
+      int nb = d->auxrep_nblocks[cq];
+      int64_t pre = 0;
+      for (int kk = 0; kk < cq; ++kk) pre += d->auxrep_nblocks[kk];
+      for (int b = 0; b < nb; ++b) {
+        int a1 = d->auxrep_irreps[2 * (pre + b)], a2 = d->auxrep_irreps[2 * (pre + b) + 1];
+        int m = T.aux_sz[cq][a1], n = T.aux_sz[qr][a2];
+        const double *O = auxrep + 2 * (T.auxrep_koff[cq] + T.auxrep_boff[cq][b]);
+        rotate_block(m, n, O, src + 2 * T.aux_off[qr][a2], dst + 2 * T.aux_off[cq][a1], cj);
+      }
+    }
+  }
+  free_tables(&T);
+  return 0;
+}
+
+int orc_Sigma_at(const orc_desc *d, const double *V, const int64_t *voff, const double *G_full, const double *P_full_t,
+                 int ikr, int t, int s, double *out, int nthreads) {
+  tables T; make_tables(d, &T);
+  const int nk = d->nk, ns = d->ns, nts = d->nts;
+  if (ikr < 0 || ikr >= d->ink || t < 0 || t >= nts || s < 0 || s >= ns) { free_tables(&T); return 1; }
+  const int k1 = d->irre_list[ikr];
+  const int64_t ksl = T.kao_sz[k1];
+  const int nth = nthreads_of(nthreads);
+  double *part = calloc(2 * (size_t)ksl * nth, sizeof(double));
+  int bad = 0;
+#ifdef _OPENMP
+#pragma omp parallel num_threads(nth) reduction(| : bad)
+#endif
+  {
+#ifdef _OPENMP
+    const int th = omp_get_thread_num(), nt = omp_get_num_threads();
+#else
+    const int th = 0, nt = 1;
+#endif
+    double *acc = part + 2 * (size_t)ksl * th;
+    size_t cap = 0;
+    double *Y1 = NULL, *Y2 = NULL;
+    const int qb = (int)((int64_t)nk * th / nt), qe = (int)((int64_t)nk * (th + 1) / nt);
+    for (int q = qb; q < qe; ++q) {
+      const int k2 = d->kmq[k1 * nk + q];
+      const int p = k1 * nk + k2;
+      if (T.kp_off[p] < 0) continue;
+      if (voff[p] < 0) { bad = 1; continue; }
+      const double *Vp = V + 2 * voff[p];
+      for (int b = 0; b < T.kp_nb[p]; ++b) {
+        const int iq = T.kp_irr[p][3 * b], i1 = T.kp_irr[p][3 * b + 1], i2 = T.kp_irr[p][3 * b + 2];
+        const int qs = T.aux_sz[q][iq], a1 = T.ao_sz[k1][i1], a2 = T.ao_sz[k2][i2];
+        const size_t K = (size_t)a1 * a2, sz = K * qs;
+        if (sz > cap) { cap = sz; Y1 = realloc(Y1, 16 * cap); Y2 = realloc(Y2, 16 * cap); }
+        const double *Vb = Vp + 2 * T.kp_boff[p][b];
+        const double *G1 = G_full + 2 * (((size_t)t * ns + s) * T.flat_ao_full + T.kao_off[k2] + T.ao_off[k2][i2]);
+        const double *Pqp = P_full_t + 2 * (T.qaux_off[q] + T.aux_off[q][iq]);
+        /* Y1 (q k1 x k2) = V G1;  Y2 (k1 k2 x q) = Y1^T P;  Sigma += Y2 conj(V) (gw_block.cpp:86-158) */
+        zgemm(qs * a1, a2, a2, Vb, a2, 0, G1, a2, 0, Y1, a2, 0);
+        zgemm((int)K, qs, qs, Y1, (int)K, 1, Pqp, qs, 0, Y2, qs, 0);
+        double *Sg = acc + 2 * T.ao_off[k1][i1];
+        for (int i = 0; i < a1; ++i)
+          for (int n = 0; n < a2; ++n)
+            for (int P = 0; P < qs; ++P) {
+              const double yr = Y2[2 * ((size_t)(i * a2 + n) * qs + P)], yi = Y2[2 * ((size_t)(i * a2 + n) * qs + P) + 1];
+              const double *vrow = Vb + 2 * ((size_t)P * K);
+              for (int j = 0; j < a1; ++j) {
+                const double vr = vrow[2 * (j * a2 + n)], vi = -vrow[2 * (j * a2 + n) + 1];
+                Sg[2 * (i * a1 + j)] += yr * vr - yi * vi;
+                Sg[2 * (i * a1 + j) + 1] += yr * vi + yi * vr;
+              }
+            }
+      }
+    }
+    free(Y1); free(Y2);
+  }
+  for (int64_t x = 0; x < 2 * ksl; ++x) {
+    double v = 0;
+    for (int th = 0; th < nth; ++th) v += part[2 * (size_t)ksl * th + x];
+    out[x] = -v / nk;
+  }
+  free(part);
+  free_tables(&T);
+  return bad ? 2 : 0;
+}
+
 /* ---- dense inverses -------------------------------------------------------- */
 int orc_inverse_lu(int n, const double *A, double *Ainv) {
   /* LU with partial pivoting (Doolittle), then solve for each unit vector */

@@ -67,6 +67,19 @@ int orc_P0_sample(const orc_desc *d, const double *V, const double *G_full, int 
 int orc_Sigma(const orc_desc *d, const double *V, const double *G_full, const double *P_full, int t0, int t1,
               double *Sigma, int nthreads);
 
+/* One output slice summed over the whole mesh, for parity checks of systems
+ * whose full sweeps are too large for the CPU: P0 of iBZ q index iqr at time t
+ * (out: qslice complex, scaled as orc_P0), Sigma of iBZ k index ikr at (t, s)
+ * (out: kslice complex, scaled as orc_Sigma).  V is given per pair: pair
+ * p = k1*nk + k2 at V + 2*voff[p]; a needed pair with voff[p] < 0 returns 2.
+ * P_full_t: one time slice of P over the full BZ ([q][qslice(q)]). */
+int orc_P0_at(const orc_desc *d, const double *V, const int64_t *voff, const double *G_full, int iqr, int t,
+              double *out, int nthreads);
+int orc_Sigma_at(const orc_desc *d, const double *V, const int64_t *voff, const double *G_full, const double *P_full_t,
+                 int ikr, int t, int s, double *out, int nthreads);
+/* Time slice t of orc_rotate_P. */
+int orc_rotate_P_t(const orc_desc *d, const double *P_ibz, const double *auxrep, int t, double *P_full_t);
+
 /* W: P0 (q_t_QP, already /nk, rows t < nts/2 used) -> P (q_t_QP all nts). */
 int orc_W(const orc_desc *d, const double *P0, const double *Tnt_BF, const double *Ttn_FB, double *P);
 /* Dyson: Sigma (ts_kij, final) -> G (ts_kij). iwn: nw complex i*omega_n. */

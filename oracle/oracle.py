@@ -39,6 +39,10 @@ def load():
             "orc_W": (C.c_int, [vp, _dp, _dp, _dp, _dp]),
             "orc_dyson": (C.c_int, [vp, _dp, _dp, _dp, C.c_double, _dp, _dp, _dp, _dp]),
             "orc_density": (C.c_double, [vp, _dp, _dp, _dp]),
+            "orc_P0_at": (C.c_int, [vp, _dp, C.POINTER(C.c_int64), _dp, C.c_int, C.c_int, _dp, C.c_int]),
+            "orc_Sigma_at": (C.c_int, [vp, _dp, C.POINTER(C.c_int64), _dp, _dp, C.c_int, C.c_int, C.c_int, _dp,
+                                       C.c_int]),
+            "orc_rotate_P_t": (C.c_int, [vp, _dp, _dp, C.c_int, _dp]),
             "orc_inverse_lu": (C.c_int, [C.c_int, _dp, _dp]),
             "orc_inverse_chol": (C.c_int, [C.c_int, _dp, _dp]),
         }
@@ -104,6 +108,45 @@ class Oracle:
         out = s.zeros_ts_kij() if out is None else out
         self.lib.orc_Sigma(self.dp, _p(V), _p(np.ascontiguousarray(G_full)), _p(np.ascontiguousarray(P_full)), t0, t1,
                            _p(out), nthreads)
+        return out
+
+    def _voff(self, pairs_off):
+        """pair -> complex offset map as the int64[nk*nk] table orc_*_at read."""
+        voff = np.full(self.s.nk * self.s.nk, -1, dtype=np.int64)
+        for p, o in pairs_off.items():
+            voff[p] = o
+        return voff
+
+    def P0_at(self, V, pairs_off, G_full, iqr, t, nthreads=1):
+        """P0 of iBZ q index iqr at time t summed over every k (qslice
+        complex, scaled as P0); V holds pair p's slice at pairs_off[p]."""
+        s = self.s
+        out = np.zeros(s.qslice(int(s.irre_list[iqr])), dtype=np.complex128)
+        voff = self._voff(pairs_off)
+        rc = self.lib.orc_P0_at(self.dp, _p(V), voff.ctypes.data_as(C.POINTER(C.c_int64)),
+                                _p(np.ascontiguousarray(G_full)), iqr, t, _p(out), nthreads)
+        if rc:
+            raise RuntimeError(f"orc_P0_at failed ({rc})")
+        return out
+
+    def rotate_P_t(self, P_ibz, t):
+        s = self.s
+        out = np.zeros(self.flat_sizes()["flat_aux_full"], dtype=np.complex128)
+        aux = np.ascontiguousarray(s.auxrep if s.auxrep.size else np.zeros(1, np.complex128))
+        self.lib.orc_rotate_P_t(self.dp, _p(np.ascontiguousarray(P_ibz)), _p(aux), t, _p(out))
+        return out
+
+    def Sigma_at(self, V, pairs_off, G_full, P_full_t, ikr, t, spin=0, nthreads=1):
+        """Sigma of iBZ k index ikr at (t, spin) summed over every q (kslice
+        complex, scaled as Sigma); P_full_t from rotate_P_t."""
+        s = self.s
+        out = np.zeros(s.kslice(int(s.irre_list[ikr])), dtype=np.complex128)
+        voff = self._voff(pairs_off)
+        rc = self.lib.orc_Sigma_at(self.dp, _p(V), voff.ctypes.data_as(C.POINTER(C.c_int64)),
+                                   _p(np.ascontiguousarray(G_full)), _p(np.ascontiguousarray(P_full_t)), ikr, t, spin,
+                                   _p(out), nthreads)
+        if rc:
+            raise RuntimeError(f"orc_Sigma_at failed ({rc})")
         return out
 
     def W(self, P0):

@@ -373,3 +373,39 @@ def test_sigma_blocks_hermitian(name, request):
     _, _, Sig, _ = orc.gw_iteration(G0, 0.0)
     assert np.abs(Sig).max() > 0
     assert H.sigma_hermiticity_error(sysm, Sig) < 1e-12
+
+
+@pytest.mark.parametrize("name", ["tiny_rot", "tiny_tr"])
+def test_sampled_slices_equal_full_sweeps(name, request):
+    """orc_P0_at / orc_Sigma_at / orc_rotate_P_t (one output slice summed
+    over the whole mesh, used for the parity of systems too large for full
+    CPU sweeps) give the same numbers as the full sweeps."""
+    sysm = request.getfixturevalue(name)
+    orc = Oracle(sysm)
+    G0 = orc.dyson(sysm.zeros_ts_kij(), 0.0)
+    V = sysm.host_V()
+    Gf = orc.rotate_G(G0)
+    P0 = orc.P0(V, Gf)
+    P = orc.W(P0)
+    Pf = orc.rotate_P(P)
+    Sg = orc.Sigma(V, Gf, Pf)
+    offs = {p: o for p, (o, sz) in sysm.kpair_offsets().items()}
+    qoff, koff = sysm.qaux_off_irre(), sysm.kao_off_irre()
+    nfull = orc.flat_sizes()["flat_aux_full"]
+    for iqr in range(sysm.ink):
+        qs = sysm.qslice(int(sysm.irre_list[iqr]))
+        for t in (0, sysm.nts // 2 - 1):
+            base = sysm.nts * qoff[iqr] + t * qs
+            assert np.abs(orc.P0_at(V, offs, Gf, iqr, t, nthreads=3) - P0[base:base + qs]).max() < 1e-13
+    for t in (0, sysm.nts - 1):
+        Pt = orc.rotate_P_t(P, t)
+        for q in range(sysm.nk):
+            o = sum(sysm.qslice(qq) for qq in range(q))
+            ref = Pf[sysm.nts * o + t * sysm.qslice(q): sysm.nts * o + (t + 1) * sysm.qslice(q)]
+            assert np.abs(Pt[o:o + sysm.qslice(q)] - ref).max() < 1e-15
+        assert Pt.size == nfull
+        for ikr in range(sysm.ink):
+            ks = sysm.kslice(int(sysm.irre_list[ikr]))
+            for sp in range(sysm.ns):
+                got = orc.Sigma_at(V, offs, Gf, Pt, ikr, t, sp, nthreads=4)
+                assert np.abs(got - Sg[t, sp, koff[ikr]:koff[ikr] + ks]).max() < 1e-13
