@@ -24,7 +24,8 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAK = {"fp64": 78.6, "fp32": 157.3, "hbm_gbps": 8000.0}  # MI355X datasheet: dense matrix TFLOP/s, HBM3E GB/s
+PEAK = {"fp64": 78.6, "fp32": 157.3, "int8": 5000.0, "hbm_gbps": 8000.0}  # MI355X datasheet: dense matrix
+# TFLOP/s (INT8 TOPS), HBM3E GB/s
 
 
 def log(*a):
@@ -92,8 +93,34 @@ def run_path(sysm, precision, steps, warmup, mu, G0):
     log(f"timed {steps} iterations: {dt:.3f} s")
     info = ctx.info()
     ctx.close()
+    from mbsymmetry_amd import _lib
+    from mbsymmetry_amd.transforms import leakage
     return dict(sec_per_iter=dt / steps, phase_ms=phase / steps, step_s=step_s, flop_p0=info.flop_P0,
-                flop_sig=info.flop_Sigma, sigma=sig, G1=G1)
+                flop_sig=info.flop_Sigma, sigma=sig, G1=G1, leakage=leakage(sysm.transforms, G1),
+                int8_ops=int8_ops(sysm) if precision == _lib.PREC_MIXED else 0.0)
+
+
+def int8_ops(sysm):
+    """INT8 MFMA operations one mixed-mode Sigma sweep executes (executed, with
+    the padding of 16-wide tiles and 64-deep slice-pair chunks): per Z job
+    tile, 3 real products x sum_{d<5} ceil((d + 1) kp / 64) instructions of
+    2 * 16 * 16 * 64 ops, for the blocks above the INT8 threshold."""
+    q_min = int(os.environ.get("SYMGW_Z8_MIN_Q", "32"))
+    tot = 0.0
+    nts = sysm.nts
+    for k in sysm.irre_list:
+        k = int(k)
+        for q in range(sysm.nk):
+            k2 = int(sysm.kmq[k, q])
+            for (qs, a1, a2) in sysm.V_block_sizes(k * sysm.nk + k2):
+                if qs <= q_min:
+                    continue
+                kp = (qs + 15) // 16 * 16
+                inst = 3 * sum((d * kp + 63) // 64 for d in range(1, 6))
+                # Z rows: times x aux rows; columns: a1 x a2, one pass per slab band of a1
+                tiles = ((nts * qs + 15) // 16) * ((a1 * a2 + 15) // 16)
+                tot += sysm.ns * tiles * inst * 2 * 16 * 16 * 64
+    return tot
 
 
 def cpu_baseline(sysm, G0, budget_s=15.0):
@@ -203,6 +230,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="config2")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--fp32", action="store_true", help="also time the all-FP32 contraction path")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mu", type=float, default=0.0)
     ap.add_argument("--no-serving", action="store_true")
@@ -250,10 +278,13 @@ def main():
 
     plain = run_path(sysm, _lib.PREC_FP64, args.steps, args.warmup, args.mu, G0)
     log("plain fp64", plain["sec_per_iter"], "s/iter; phases ms", plain["phase_ms"][:4])
-    mixed = None
+    mixed = fp32 = None
     if not args.no_mixed:
-        mixed = run_path(sysm, _lib.PREC_FP32, args.steps, args.warmup, args.mu, G0)
-        log("fp32", mixed["sec_per_iter"], "s/iter; phases ms", mixed["phase_ms"][:4])
+        mixed = run_path(sysm, _lib.PREC_MIXED, args.steps, args.warmup, args.mu, G0)
+        log("mixed (int8 Z)", mixed["sec_per_iter"], "s/iter; phases ms", mixed["phase_ms"][:4])
+        if args.fp32:
+            fp32 = run_path(sysm, _lib.PREC_FP32, args.steps, args.warmup, args.mu, G0)
+            log("fp32", fp32["sec_per_iter"], "s/iter; phases ms", fp32["phase_ms"][:4])
 
     p0_ms, sig_ms = float(plain["phase_ms"][0]), float(plain["phase_ms"][2])
     dom = "Sigma" if sig_ms >= p0_ms else "P0"
@@ -303,22 +334,32 @@ def main():
         "kernel_tflops": {"P0": plain["flop_p0"] / (p0_ms * 1e-3) / 1e12,
                           "Sigma": plain["flop_sig"] / (sig_ms * 1e-3) / 1e12},
     }
+    out["physics"] = {"max_abs_Sigma": float(np.abs(plain["sigma"]).max()),
+                      "max_abs_G_after_iteration": float(np.abs(plain["G1"]).max()),
+                      "causal_bound_|G|<=1": bool(np.abs(plain["G1"]).max() <= 1.0 + 1e-9),
+                      "dyson_leakage": plain["leakage"]}
+
+    def compare(res, name):
+        dsig = float(np.abs(res["sigma"] - plain["sigma"]).max())
+        dg = float(np.abs(res["G1"] - plain["G1"]).max())
+        rp0, rsig = float(res["phase_ms"][0]), float(res["phase_ms"][2])
+        kt = {"P0": res["flop_p0"] / (rp0 * 1e-3) / 1e12, "Sigma": res["flop_sig"] / (rsig * 1e-3) / 1e12}
+        return {"mode": name, "iter_per_s": 1.0 / res["sec_per_iter"],
+                "speedup_vs_plain": plain["sec_per_iter"] / res["sec_per_iter"],
+                "max_abs_dSigma_vs_plain": dsig, "max_abs_dG_vs_plain": dg, "meets_1e-8": dsig <= 1e-8 and dg <= 1e-8,
+                "phase_ms": list(map(float, res["phase_ms"][:4])), "kernel_tflops": kt,
+                "p0_sigma_tflops": flops_iter / (1e-3 * (rp0 + rsig)) / 1e12}
+
     if mixed is not None:
-        dsig = float(np.abs(mixed["sigma"] - plain["sigma"]).max())
-        dg = float(np.abs(mixed["G1"] - plain["G1"]).max())
-        out["mixed_fp32"] = {"iter_per_s": 1.0 / mixed["sec_per_iter"], "speedup_vs_plain":
-                             plain["sec_per_iter"] / mixed["sec_per_iter"], "max_abs_dSigma_vs_plain": dsig,
-                             "max_abs_dG_vs_plain": dg, "meets_1e-8": dsig <= 1e-8,
-                             "max_abs_Sigma_plain": float(np.abs(plain["sigma"]).max()),
-                             "max_abs_G_plain": float(np.abs(plain["G1"]).max()),
-                             "phase_ms": list(map(float, mixed["phase_ms"][:4]))}
-        # the FP32 path against the FP32 matrix-core roofline
-        mp0, msig = float(mixed["phase_ms"][0]), float(mixed["phase_ms"][2])
-        mt = {"P0": mixed["flop_p0"] / (mp0 * 1e-3) / 1e12, "Sigma": mixed["flop_sig"] / (msig * 1e-3) / 1e12}
-        out["mixed_fp32"]["kernel_tflops"] = mt
-        out["mixed_fp32"]["p0_sigma_tflops"] = flops_iter / (1e-3 * (mp0 + msig)) / 1e12
-        out["mixed_fp32"]["frac_fp32_peak"] = {k: v / PEAK["fp32"] for k, v in mt.items()}
-        out["mixed_fp32"]["frac_measured_fp32_peak"] = {k: v / peaks["fp32_mfma_tflops"] for k, v in mt.items()}
+        out["mixed"] = compare(mixed, "Sigma's q x q x k1k2 product (67% of its FLOPs) on INT8 matrix cores: 5 "
+                                      "slices per operand, exact int32 sums, FP64 reconstruction; P0, the other "
+                                      "products, W, Dyson and the transforms FP64")
+        out["mixed"]["int8_ops_Sigma"] = mixed["int8_ops"]
+        out["mixed"]["int8_rate_lower_bound_TOPS"] = mixed["int8_ops"] / (mixed["phase_ms"][2] * 1e-3) / 1e12
+        out["mixed"]["frac_int8_peak_lower_bound"] = out["mixed"]["int8_rate_lower_bound_TOPS"] / PEAK["int8"]
+    if fp32 is not None:
+        out["fp32"] = compare(fp32, "every contraction product in FP32 (3M), FP32 accumulation")
+        out["fp32"]["frac_fp32_peak"] = {k: v / PEAK["fp32"] for k, v in out["fp32"]["kernel_tflops"].items()}
     if not args.no_uneven:
         out["uneven_blocks"] = uneven_run()
     if not args.no_serving:

@@ -36,6 +36,13 @@ extern "C" {
  * the mu search always run in FP64. */
 #define SYMGW_PREC_FP64 0      /* plain: complex FP64                          */
 #define SYMGW_PREC_FP32 1      /* complex FP32 operands, FP32 accumulation   */
+/* Mixed: the P0 contraction and the two short products of Sigma in FP64; the
+ * q x q x k1 k2 product of Sigma (Z = P^H V) on INT8 matrix cores as exact
+ * int32 products of 7-bit slices (Ozaki scheme, 5 slices per operand, a
+ * power-of-two exponent per row of P and per column of V), reconstructed in
+ * FP64.  V and P are also kept as slices: V8 once per set_V / generate_V,
+ * P8 after each rotation of P. */
+#define SYMGW_PREC_MIXED 2
 
 /*
  * Block-sparse layout of one system.  Mirrors the tables that

@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(HERE, "libsymgw.so")
 SYMGW_OK = 0
 PREC_FP64 = 0
 PREC_FP32 = 1
+PREC_MIXED = 2
 
 _i32p = C.POINTER(C.c_int32)
 _dp = C.POINTER(C.c_double)

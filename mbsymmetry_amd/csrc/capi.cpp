@@ -59,6 +59,12 @@ struct symgw_ctx {
   cplx<double> *Tnt = nullptr, *Ttn = nullptr, *Tnt_BF = nullptr, *Ttn_FB = nullptr, *iwn = nullptr;
   cplx<double> *w_scratch = nullptr, *d_scratch = nullptr;
   double *kw = nullptr, *nel_dev = nullptr;
+  // mixed mode: INT8 slice copies of V (Sigma's blocks) and of P (full BZ)
+  int8_t *V8 = nullptr, *P8 = nullptr;
+  int *VE = nullptr, *PE = nullptr;
+  Split8Blk *d_v8_split = nullptr, *d_p8_split = nullptr;
+  int64_t *d_v8_prefix = nullptr, *d_p8_prefix = nullptr;
+  int64_t v8_rows = 0, p8_rows = 0;
 
   // device tables
   P0Item *d_p0_items = nullptr;
@@ -188,7 +194,30 @@ int64_t footprint(const Layout &L, const WorkLists &W, int eb) {
   b += (int64_t)L.nw_b * L.flat_aux * 16 + (int64_t)L.nw * L.ns * L.flat_ao * 16;
   b += (int64_t)L.nw_b * W.w_scr_group * 16 + (int64_t)L.nw * L.ns * W.d_scr_group * 16;
   b += (int64_t)(W.p0_blks.size() * sizeof(P0Blk) + W.sig_blks.size() * sizeof(SigBlk));
+  // mixed mode: INT8 slices, exponents and split tables
+  b += W.v8_bytes + W.p8_bytes + 4 * (W.ve_count + W.pe_count);
+  b += (int64_t)(W.v8_split.size() + W.p8_split.size()) * (sizeof(Split8Blk) + 8);
   return b;
+}
+
+bool valid_precision(int p) { return p == SYMGW_PREC_FP64 || p == SYMGW_PREC_FP32 || p == SYMGW_PREC_MIXED; }
+// bytes of one complex element of V, G and P in the contractions
+int precision_bytes(int p) { return p == SYMGW_PREC_FP32 ? 8 : 16; }
+
+// INT8 slices of V (after it is set) and of P_full (after each rotation).
+int split_V8(symgw_ctx *c) {
+  if (c->prec != SYMGW_PREC_MIXED) return SYMGW_OK;
+  hipError_t e = launch_split8(c->d_v8_split, (int)c->W.v8_split.size(), c->d_v8_prefix, c->v8_rows,
+                               static_cast<const cplx<double> *>(c->V), c->V8, c->VE, c->st);
+  if (e != hipSuccess) return fail(c, SYMGW_ERR_HIP, std::string("V split: ") + hipGetErrorString(e));
+  return SYMGW_OK;
+}
+int split_P8(symgw_ctx *c) {
+  if (c->prec != SYMGW_PREC_MIXED) return SYMGW_OK;
+  hipError_t e = launch_split8(c->d_p8_split, (int)c->W.p8_split.size(), c->d_p8_prefix, c->p8_rows,
+                               static_cast<const cplx<double> *>(c->P_full), c->P8, c->PE, c->st);
+  if (e != hipSuccess) return fail(c, SYMGW_ERR_HIP, std::string("P split: ") + hipGetErrorString(e));
+  return SYMGW_OK;
 }
 
 template <typename T>
@@ -203,7 +232,7 @@ int run_rotate_G(symgw_ctx *c) {
 
 int rotate_G(symgw_ctx *c) {
   if (!c->have_reps && c->L.flat_orep > 0) return fail(c, SYMGW_ERR_STATE, "rotation matrices not set");
-  return c->prec == SYMGW_PREC_FP64 ? run_rotate_G<double>(c) : run_rotate_G<float>(c);
+  return c->eb == 16 ? run_rotate_G<double>(c) : run_rotate_G<float>(c);
 }
 
 template <typename T>
@@ -231,11 +260,17 @@ template <typename T>
 int run_sigma(symgw_ctx *c) {
   const Layout &L = c->L;
   hipError_t e = hipSuccess;
+  Z8Ops z8;
+  z8.P8 = c->P8;
+  z8.V8 = c->V8;
+  z8.PE = c->PE;
+  z8.VE = c->VE;
+  const Z8Ops *pz8 = c->prec == SYMGW_PREC_MIXED ? &z8 : nullptr;
   for (int d = 0; d < kNumSigDepths && e == hipSuccess; ++d)
     e = launch_sigma<T>(c->d_sig_items[d], c->d_sig_blks, (int)c->W.sig_items[d].size(), L.nts, L.ns,
                         static_cast<const cplx<T> *>(c->V), static_cast<const cplx<T> *>(c->G_full),
                         static_cast<const cplx<T> *>(c->P_full), c->Sig, -1.0, c->W.sig_lds[d], c->st,
-                        kSigDepths[d]);
+                        kSigDepths[d], pz8);
   if (e != hipSuccess) return fail(c, SYMGW_ERR_HIP, std::string("Sigma kernel: ") + hipGetErrorString(e));
   return SYMGW_OK;
 }
@@ -243,8 +278,9 @@ int run_sigma(symgw_ctx *c) {
 // P (iBZ q) -> P_full (every q of the full BZ).
 int rotate_P(symgw_ctx *c) {
   if (!c->have_reps && c->L.flat_auxrep > 0) return fail(c, SYMGW_ERR_STATE, "rotation matrices not set");
-  int r = c->prec == SYMGW_PREC_FP64 ? run_rotate_P<double>(c) : run_rotate_P<float>(c);
+  int r = c->eb == 16 ? run_rotate_P<double>(c) : run_rotate_P<float>(c);
   if (r) return r;
+  if ((r = split_P8(c))) return r;
   c->p_rotated = true;
   return SYMGW_OK;
 }
@@ -253,7 +289,7 @@ int phase_P0(symgw_ctx *c) {
   if (!c->have_V) return fail(c, SYMGW_ERR_STATE, "V not set");
   if (!c->have_G) return fail(c, SYMGW_ERR_STATE, "G not uploaded");
   const Layout &L = c->L;
-  int r = c->prec == SYMGW_PREC_FP64 ? run_p0<double>(c) : run_p0<float>(c);
+  int r = c->eb == 16 ? run_p0<double>(c) : run_p0<float>(c);
   if (r) return r;
   (void)L;
   c->p_rotated = false;
@@ -282,7 +318,7 @@ int phase_Sigma(symgw_ctx *c) {
   if (!c->have_P) return fail(c, SYMGW_ERR_STATE, "P not available");
   int r = c->p_rotated ? SYMGW_OK : rotate_P(c);
   if (r) return r;
-  r = c->prec == SYMGW_PREC_FP64 ? run_sigma<double>(c) : run_sigma<float>(c);
+  r = c->eb == 16 ? run_sigma<double>(c) : run_sigma<float>(c);
   if (r) return r;
   c->have_Sig = true;
   return SYMGW_OK;
@@ -396,15 +432,15 @@ int symgw_layout_query(const symgw_layout_desc *desc, symgw_layout_info *info) {
 int symgw_layout_footprint(const symgw_layout_desc *desc, int precision, int64_t *bytes) {
   return guarded(nullptr, [&]() -> int {
     if (!desc || !bytes) return SYMGW_ERR_ARG;
-    if (precision != SYMGW_PREC_FP64 && precision != SYMGW_PREC_FP32) {
+    if (!valid_precision(precision)) {
       g_create_error = "unknown precision mode";
       return SYMGW_ERR_ARG;
     }
-    const int eb = precision == SYMGW_PREC_FP64 ? 16 : 8;
+    const int eb = precision_bytes(precision);
     Layout L;
     WorkLists W;
     std::string err;
-    if (!build_layout(desc, L, err) || !build_worklists(L, eb, W, err)) {
+    if (!build_layout(desc, L, err) || !build_worklists(L, eb, W, err, precision == SYMGW_PREC_MIXED)) {
       g_create_error = err;
       return SYMGW_ERR_ARG;
     }
@@ -520,15 +556,16 @@ int symgw_create(const symgw_layout_desc *desc, int device, int precision, symgw
   return guarded(nullptr, [&]() -> int {
     if (!out) return SYMGW_ERR_ARG;
     *out = nullptr;
-    if (precision != SYMGW_PREC_FP64 && precision != SYMGW_PREC_FP32) {
+    if (!valid_precision(precision)) {
       g_create_error = "unknown precision mode";
       return SYMGW_ERR_ARG;
     }
     std::unique_ptr<symgw_ctx> c(new symgw_ctx());
     c->prec = precision;
-    c->eb = precision == SYMGW_PREC_FP64 ? 16 : 8;
+    c->eb = precision_bytes(precision);
     std::string err;
-    if (!build_layout(desc, c->L, err) || !build_worklists(c->L, c->eb, c->W, err)) {
+    if (!build_layout(desc, c->L, err) ||
+        !build_worklists(c->L, c->eb, c->W, err, precision == SYMGW_PREC_MIXED)) {
       g_create_error = err;
       return SYMGW_ERR_ARG;
     }
@@ -590,6 +627,12 @@ int symgw_create(const symgw_layout_desc *desc, int device, int precision, symgw
     r = r ? r : dalloc(cp, &c->d_scratch, (size_t)std::max(1, L.nw) * L.ns * c->W.d_scr_group);
     r = r ? r : dalloc(cp, &c->kw, (size_t)L.ink);
     r = r ? r : dalloc(cp, &c->nel_dev, 1);
+    if (precision == SYMGW_PREC_MIXED) {
+      r = r ? r : dalloc(cp, &c->V8, (size_t)c->W.v8_bytes);
+      r = r ? r : dalloc(cp, &c->P8, (size_t)c->W.p8_bytes);
+      r = r ? r : dalloc(cp, &c->VE, (size_t)c->W.ve_count);
+      r = r ? r : dalloc(cp, &c->PE, (size_t)c->W.pe_count);
+    }
     // every data buffer starts at zero (outside any graph: plain memsets are fine here)
     for (auto &b : c->owned)
       if (!r && hipMemsetAsync(b.p, 0, b.bytes, c->st) != hipSuccess) r = fail(cp, SYMGW_ERR_HIP, "initial memset failed");
@@ -599,6 +642,18 @@ int symgw_create(const symgw_layout_desc *desc, int device, int precision, symgw
     r = r ? r : upload(cp, &c->d_p0_blks, c->W.p0_blks);
     for (int d = 0; d < kNumSigDepths; ++d) r = r ? r : upload(cp, &c->d_sig_items[d], c->W.sig_items[d]);
     r = r ? r : upload(cp, &c->d_sig_blks, c->W.sig_blks);
+    if (precision == SYMGW_PREC_MIXED) {
+      auto prefix = [](const std::vector<Split8Blk> &v, int64_t &tot) {
+        std::vector<int64_t> p(v.size() + 1, 0);
+        for (size_t i = 0; i < v.size(); ++i) p[i + 1] = p[i] + (int64_t)v[i].nsub * v[i].nrows;
+        tot = p.back();
+        return p;
+      };
+      r = r ? r : upload(cp, &c->d_v8_split, c->W.v8_split);
+      r = r ? r : upload(cp, &c->d_p8_split, c->W.p8_split);
+      r = r ? r : upload(cp, &c->d_v8_prefix, prefix(c->W.v8_split, c->v8_rows));
+      r = r ? r : upload(cp, &c->d_p8_prefix, prefix(c->W.p8_split, c->p8_rows));
+    }
     r = r ? r : upload(cp, &c->d_g_rot, c->W.g_rot);
     r = r ? r : upload(cp, &c->d_p_rot, c->W.p_rot);
     r = r ? r : upload(cp, &c->d_w_blocks, c->W.w_blocks);
@@ -723,7 +778,7 @@ int symgw_set_V(symgw_ctx *ctx, const double *V_flat, int64_t n_complex) {
     if (!ctx || !V_flat) return SYMGW_ERR_ARG;
     if (n_complex != ctx->L.flat_V) return fail(ctx, SYMGW_ERR_ARG, "V length does not match the layout");
     HIPCHK(ctx, hipSetDevice(ctx->device));
-    if (ctx->prec == SYMGW_PREC_FP64) {
+    if (ctx->eb == 16) {
       HIPCHK(ctx, copy_sync(ctx, ctx->V, V_flat, (size_t)n_complex * 16, hipMemcpyHostToDevice));
     } else {
       // staged conversion in 64 MiB chunks
@@ -742,6 +797,9 @@ int symgw_set_V(symgw_ctx *ctx, const double *V_flat, int64_t n_complex) {
       }
       dev_free(stage);
     }
+    int r = split_V8(ctx);
+    if (r) return r;
+    HIPCHK(ctx, hipStreamSynchronize(ctx->st));
     ctx->have_V = true;
     return SYMGW_OK;
   });
@@ -784,7 +842,7 @@ int symgw_generate_V(symgw_ctx *ctx, uint64_t seed, double scale, const double *
     hipError_t e = copy_sync(ctx, dgb, gb.data(), gb.size() * sizeof(GenBlk), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = copy_sync(ctx, ddec, decay, (size_t)n_decay * sizeof(double), hipMemcpyHostToDevice);
     if (e == hipSuccess) {
-      e = ctx->prec == SYMGW_PREC_FP64
+      e = ctx->eb == 16
               ? launch_genv<double>(dgb, (int)gb.size(), seed, scale, ddec, n_decay, static_cast<cplx<double> *>(ctx->V),
                                     ctx->st)
               : launch_genv<float>(dgb, (int)gb.size(), seed, scale, ddec, n_decay, static_cast<cplx<float> *>(ctx->V),
@@ -794,6 +852,9 @@ int symgw_generate_V(symgw_ctx *ctx, uint64_t seed, double scale, const double *
     dev_free(dgb);
     dev_free(ddec);
     if (e != hipSuccess) return fail(ctx, SYMGW_ERR_HIP, std::string("V generation: ") + hipGetErrorString(e));
+    int r = split_V8(ctx);
+    if (r) return r;
+    HIPCHK(ctx, hipStreamSynchronize(ctx->st));
     ctx->have_V = true;
     return SYMGW_OK;
   });

@@ -275,9 +275,9 @@ __device__ __forceinline__ void mma_rowshare_all(Tile<T> (&acc)[NA], int n, int 
   mma_rowshare<0, NG < MT ? NG : MT, CB>(acc, n, kb, ke, arow, la, lb);
   if (NG < MT) mma_rowshare<NG < MT ? NG : 0, NG < MT ? MT - NG : 1, CB>(acc, n, kb, ke, arow, la, lb);
 }
-template <bool CB, typename T, typename LA, typename LB>
-__device__ __forceinline__ void mma_multi_all(Acc<T> &acc, int n, int kb, int ke, const LA &la, const LB &lb) {
-  constexpr int NG = Group<T>::n, MT = Cfg<T>::MT;
+template <bool CB, typename T, int MT, typename LA, typename LB>
+__device__ __forceinline__ void mma_multi_all(Tile<T> (&acc)[MT], int n, int kb, int ke, const LA &la, const LB &lb) {
+  constexpr int NG = Group<T>::n;
   mma_multi<0, NG < MT ? NG : MT, CB>(acc, n, kb, ke, la, lb);
   if (NG < MT) mma_multi<NG < MT ? NG : 0, NG < MT ? MT - NG : 1, CB>(acc, n, kb, ke, la, lb);
 }
@@ -370,10 +370,9 @@ __device__ __forceinline__ void job_store(const Tile3<T> &acc, const F &f) {
 // (row tile, group of NG column tiles) sharing the A fragment of each k-step;
 // column-share jobs are (group of NG row tiles, column tile) sharing B.
 // la(row, k), lb(k, col), st(row, col, value) (st only for row < M, col < N).
-template <typename T, int NG, bool CB, int AH, typename LA, typename LB, typename ST>
+template <typename T, int NG, bool CB, int AH, int NW, typename LA, typename LB, typename ST>
 __device__ __forceinline__ int gemm_jobs_rs_g(int M, int N, int K, int first, const LA &la, const LB &lb,
                                               const ST &st) {
-  constexpr int NW = Cfg<T>::NW;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int ntm = (M + 15) / 16, ntn = (N + 15) / 16, ngr = (ntn + NG - 1) / NG, njobs = ntm * ngr;
   // job j = mt * ngr + gi, advanced by NW per round without dividing
@@ -397,10 +396,9 @@ __device__ __forceinline__ int gemm_jobs_rs_g(int M, int N, int K, int first, co
   return njobs;
 }
 
-template <typename T, int NG, bool CB, int AH, typename LA, typename LB, typename ST>
+template <typename T, int NG, bool CB, int AH, int NW, typename LA, typename LB, typename ST>
 __device__ __forceinline__ int gemm_jobs_cs_g(int M, int N, int K, int first, const LA &la, const LB &lb,
                                               const ST &st) {
-  constexpr int NW = Cfg<T>::NW;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int ntm = (M + 15) / 16, ntn = (N + 15) / 16, ngr = (ntm + NG - 1) / NG, njobs = ngr * ntn;
   // job j = gt * ntn + nt, advanced by NW per round without dividing
@@ -425,25 +423,191 @@ __device__ __forceinline__ int gemm_jobs_cs_g(int M, int N, int K, int first, co
 }
 
 // Grouped jobs when there are enough of them to keep every wave busy, else
-// single-tile jobs (less operand sharing, shorter critical path).
-template <typename T, bool CB = false, int AH = kAheadJob<T>, typename LA, typename LB, typename ST>
+// single-tile jobs (less operand sharing, shorter critical path).  NW: waves
+// of the calling workgroup.
+template <typename T, bool CB = false, int AH = kAheadJob<T>, int NW = Cfg<T>::NW, typename LA, typename LB,
+          typename ST>
 __device__ __forceinline__ int gemm_jobs_rs(int M, int N, int K, int first, const LA &la, const LB &lb,
                                             const ST &st) {
   constexpr int NG = Group<T>::n;
   const int ntm = (M + 15) / 16, ntn = (N + 15) / 16;
-  if (ntm * ((ntn + NG - 1) / NG) < Cfg<T>::NW) return gemm_jobs_rs_g<T, 1, CB, AH>(M, N, K, first, la, lb, st);
-  return gemm_jobs_rs_g<T, NG, CB, AH>(M, N, K, first, la, lb, st);
+  if (ntm * ((ntn + NG - 1) / NG) < NW) return gemm_jobs_rs_g<T, 1, CB, AH, NW>(M, N, K, first, la, lb, st);
+  return gemm_jobs_rs_g<T, NG, CB, AH, NW>(M, N, K, first, la, lb, st);
 }
-template <typename T, bool CB = false, int AH = kAheadJob<T>, typename LA, typename LB, typename ST>
+template <typename T, bool CB = false, int AH = kAheadJob<T>, int NW = Cfg<T>::NW, typename LA, typename LB,
+          typename ST>
 __device__ __forceinline__ int gemm_jobs_cs(int M, int N, int K, int first, const LA &la, const LB &lb,
                                             const ST &st) {
   constexpr int NG = Group<T>::n;
   const int ntm = (M + 15) / 16, ntn = (N + 15) / 16;
-  if (((ntm + NG - 1) / NG) * ntn < Cfg<T>::NW) return gemm_jobs_cs_g<T, 1, CB, AH>(M, N, K, first, la, lb, st);
-  return gemm_jobs_cs_g<T, NG, CB, AH>(M, N, K, first, la, lb, st);
+  if (((ntm + NG - 1) / NG) * ntn < NW) return gemm_jobs_cs_g<T, 1, CB, AH, NW>(M, N, K, first, la, lb, st);
+  return gemm_jobs_cs_g<T, NG, CB, AH, NW>(M, N, K, first, la, lb, st);
+}
+
+// ---- mixed precision: the Z product of Sigma on INT8 slices ----------------
+// Ozaki scheme (Ozaki, Ogita, Oishi, Rump, Numer. Algorithms 59 (2012) 95):
+// each real operand row (A) or column (B) is x = 2^e sum_s x_s 128^-s with
+// int8 slices x_s (split8_kernel); a real product sum_k A B is then
+// 2^(eA + eB) sum_{a + b < kZ8Slices} 128^-(a + b) sum_k A_a B_b, each inner sum
+// exact in int32 on v_mfma_i32_16x16x64_i8.  The slice pairs of one
+// diagonal d = a + b are concatenated along K (K' = (d + 1) kp), so short K
+// still fills the 64-deep instruction.  The three real products of the 3M
+// complex scheme use their own slices: A comps (re, im, re + im), B comps
+// (re, im, re - im) for A conj(B).
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+// sum_d 128^-d R_d of the five diagonal accumulators, times 2^ex, as
+// 128^-1 (head + tail / 128): the head 128 R0 + R1 is exact in int32 for
+// K <= 256 (|R_d| <= (d + 1) K 127^2); the tail R2 + R3/128 + R4/128^2 is
+// formed in FP32, whose rounding (~1e-7 of the tail, ~1e-11 of the value)
+// stays below the 128^-5 truncation of the slices.
+static_assert(kZ8Slices == 5, "z8_value combines five diagonals");
+__device__ __forceinline__ double z8_value(const v4i (&acc)[kZ8Slices], int e, int ex) {
+  const int head = acc[0][e] * 128 + acc[1][e];
+  const float tail = fmaf(fmaf((float)acc[4][e], 1.0f / 128.0f, (float)acc[3][e]), 1.0f / 128.0f, (float)acc[2][e]);
+  return ldexp(fma((double)tail, 1.0 / 128.0, (double)head), ex - 7);
+}
+
+// Z jobs of one chunk: Z[row][col] = conj(sum_P P[row][P] conj(V[P][col])),
+// rows (tb, r) -> P row Q0 + r at time t0 + tb, cols (jj, l) -> V column
+// (J0 + jj) k2 + n0 + l.  Single-tile jobs dealt as gemm_jobs_cs does
+// (job j to wave (first + j) % NW); returns the job count.  Per real product
+// and 64-wide chunk of K the five A and five B slices are loaded once and the
+// 15 slice pairs with a + b < 5 issued from registers.
+template <int NW, typename T>
+__device__ __forceinline__ int z8_jobs(int M, int N, int first, const int8_t *__restrict__ P8,
+                                       const int *__restrict__ PE, int64_t p8_tstride, int pe_tstride, int q,
+                                       int kp, int Q0, FDiv dnr, int nr, const int8_t *__restrict__ V8,
+                                       const int *__restrict__ VE, int ncol, int J0, int n0, int k2, FDiv dnn,
+                                       int nn, int Jb, int ldz, cplx<T> *bufZ) {
+  constexpr int NS = kZ8Slices;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int ntm = (M + 15) / 16, ntn = (N + 15) / 16, njobs = ntm * ntn;
+  const int64_t ss_a = (int64_t)q * kp, ss_b = (int64_t)ncol * kp;  // slice strides
+  const int nch = (kp + 63) / 64, kg = 16 * (lane >> 4);
+  for (int j = (wave - first % NW + NW) % NW; j < njobs; j += NW) {
+    const int mt = j / ntn, nt = j - mt * ntn;
+    // this lane's operand row and column
+    const int arow = min(mt * 16 + (lane & 15), M - 1), acol = min(nt * 16 + (lane & 15), N - 1);
+    const int atb = dnr.div(arow), ar = arow - atb * nr;
+    const int bjj = dnn.div(acol), bl = acol - bjj * nn;
+    const int8_t *Arow = P8 + atb * p8_tstride + (int64_t)(Q0 + ar) * kp + kg;
+    const int8_t *Bcol = V8 + (int64_t)((J0 + bjj) * k2 + n0 + bl) * kp + kg;
+    // output element e of this lane: row 4 (lane / 16) + e, column lane % 16
+    const int col = nt * 16 + (lane & 15), cc = min(col, N - 1), ojj = dnn.div(cc), ol = cc - ojj * nn;
+    const int vcol = (J0 + ojj) * k2 + n0 + ol;
+    // Z = conj(A conj(B)) = (P1 + P2) - i (P3 - P1 + P2), accumulated product by
+    // product in this lane's own Z elements (no other lane touches them)
+#pragma unroll 1
+    for (int c = 0; c < kZ8Comps; ++c) {
+      v4i acc[NS];
+#pragma unroll
+      for (int d = 0; d < NS; ++d) acc[d] = v4i{0, 0, 0, 0};
+      const int8_t *Ac = Arow + c * NS * ss_a, *Bc = Bcol + c * NS * ss_b;
+      for (int ch = 0; ch < nch; ++ch) {
+        // unconditional loads (k beyond kp reads the row's first bytes and is
+        // zeroed at use), all ten issued before the MFMAs wait on them
+        const bool in = ch * 64 + kg < kp;
+        const int off = in ? ch * 64 : -kg;
+        v4i a[NS], b[NS];
+#pragma unroll
+        for (int sl = 0; sl < NS; ++sl) {
+          a[sl] = *reinterpret_cast<const v4i *>(Ac + sl * ss_a + off);
+          b[sl] = *reinterpret_cast<const v4i *>(Bc + sl * ss_b + off);
+        }
+#pragma unroll
+        for (int sl = 0; sl < NS; ++sl)
+          if (!in) a[sl] = v4i{0, 0, 0, 0};
+#pragma unroll
+        for (int sa = 0; sa < NS; ++sa)
+#pragma unroll
+          for (int sb = 0; sa + sb < NS; ++sb)
+            acc[sa + sb] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[sa], b[sb], acc[sa + sb], 0, 0, 0);
+      }
+      const int eb = VE[c * ncol + vcol];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int row = mt * 16 + 4 * (lane >> 4) + e;
+        if (col >= N || row >= M) continue;
+        const int otb = dnr.div(row), orr = row - otb * nr;
+        const double v = z8_value(acc, e, PE[otb * pe_tstride + c * q + Q0 + orr] + eb);
+        cplx<T> &z = bufZ[(otb * Jb + ojj) * ldz + orr * nn + ol];
+        if (c == 0) z = cplx<T>{(T)v, (T)v};                  // P1: re P1, -im = P1
+        else if (c == 1) z = cplx<T>{z.re + (T)v, z.im - (T)v};  // P2: re + P2, -im - P2
+        else z.im -= (T)v;                                       // P3: -im - P3
+      }
+    }
+  }
+  return njobs;
 }
 
 }  // namespace
+
+// Split of real-matrix rows into INT8 slices (Split8Blk), one wave per row:
+// per component the row maximum m gives e with m 2^-e <= 127, then
+// x 2^-e = sum_{s < kZ8Slices} x_s 128^-s + r with |r| <= 128^-(kZ8Slices-1)/2:
+// rint, an exact subtraction and an exact scaling per slice.
+__global__ __launch_bounds__(256) void split8_kernel(const Split8Blk *__restrict__ sb, int nsb,
+                                                     const int64_t *__restrict__ row_prefix, int64_t nrows_total,
+                                                     const cplx<double> *__restrict__ src, int8_t *__restrict__ dst,
+                                                     int *__restrict__ ex) {
+  const int lane = threadIdx.x & 63;
+  for (int64_t row = ((int64_t)blockIdx.x * 256 + threadIdx.x) >> 6; row < nrows_total;
+       row += ((int64_t)gridDim.x * 256) >> 6) {
+    int lo = 0, hi = nsb - 1;  // block: row_prefix[b] <= row < row_prefix[b + 1]
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (row_prefix[mid] <= row) lo = mid; else hi = mid - 1;
+    }
+    const Split8Blk S = sb[lo];
+    const int64_t local = row - row_prefix[lo];
+    const int u = (int)(local / S.nrows), r = (int)(local - (int64_t)u * S.nrows);
+    const cplx<double> *x = src + S.src_off + (int64_t)u * S.sub_stride + (int64_t)r * S.row_stride;
+    const double sg = S.kind == 0 ? 1.0 : -1.0;
+    double m[kZ8Comps] = {0.0, 0.0, 0.0};
+    for (int k = lane; k < S.K; k += 64) {
+      const cplx<double> v = x[(int64_t)k * S.k_stride];
+      m[0] = fmax(m[0], fabs(v.re));
+      m[1] = fmax(m[1], fabs(v.im));
+      m[2] = fmax(m[2], fabs(v.re + sg * v.im));
+    }
+#pragma unroll
+    for (int c = 0; c < kZ8Comps; ++c)
+      for (int o = 32; o > 0; o >>= 1) m[c] = fmax(m[c], __shfl_xor(m[c], o));
+    int e[kZ8Comps];
+#pragma unroll
+    for (int c = 0; c < kZ8Comps; ++c) {
+      int xe = 0;
+      if (m[c] > 0.0) (void)frexp(m[c] / 127.0, &xe);
+      e[c] = xe;
+    }
+    int8_t *d = dst + S.dst_off + (int64_t)u * kZ8Comps * kZ8Slices * S.nrows * S.kp + (int64_t)r * S.kp;
+    if (lane < kZ8Comps) ex[S.e_off + (int64_t)u * 3 * S.nrows + (int64_t)lane * S.nrows + r] = e[lane];
+    for (int k = lane; k < S.kp; k += 64) {
+      cplx<double> v{0.0, 0.0};
+      if (k < S.K) v = x[(int64_t)k * S.k_stride];
+#pragma unroll
+      for (int c = 0; c < kZ8Comps; ++c) {
+        double t = ldexp(c == 0 ? v.re : c == 1 ? v.im : v.re + sg * v.im, -e[c]);
+#pragma unroll
+        for (int sl = 0; sl < kZ8Slices; ++sl) {
+          const double w = rint(t);
+          d[(int64_t)(c * kZ8Slices + sl) * S.nrows * S.kp + k] = (int8_t)(int)w;
+          t = (t - w) * 128.0;
+        }
+      }
+    }
+  }
+}
+
+hipError_t launch_split8(const Split8Blk *sb, int nsb, const int64_t *row_prefix, int64_t nrows_total,
+                         const cplx<double> *src, int8_t *dst, int *ex, hipStream_t st) {
+  if (nsb == 0 || nrows_total == 0) return hipSuccess;
+  const int64_t waves = nrows_total, grid = std::min<int64_t>((waves + 3) / 4, 65536);
+  hipLaunchKernelGGL(split8_kernel, dim3((unsigned)grid), dim3(256), 0, st, sb, nsb, row_prefix, nrows_total, src,
+                     dst, ex);
+  return hipGetLastError();
+}
 
 // ---------------------------------------------------------------------------
 // P0.  Slab = rows [Q0, Q0 + nband) (nband <= 16) of aux block iq of iBZ point
@@ -615,19 +779,29 @@ __global__ __launch_bounds__(Cfg<T>::NT) void p0_mfma_kernel(const P0Item *__res
 // share every V fragment: their P rows are stacked as rows of one Z product
 // and their G^{k-q} blocks as columns of one E product.
 // ---------------------------------------------------------------------------
-template <typename T, int TB>
-__global__ __launch_bounds__(Cfg<T>::NT) void sigma_mfma_kernel(const SigItem *__restrict__ items,
+// Sigma workgroup shape: the contraction shape of T, or, with the INT8 Z
+// product, kZ8Waves waves of kZ8Tiles accumulator tiles (its slices and
+// integer accumulators need the registers of three waves per SIMD).
+template <typename T, bool Z8>
+struct SigCfg {
+  static constexpr int NW = Z8 ? kZ8Waves : Cfg<T>::NW;
+  static constexpr int MT = Z8 ? kZ8Tiles : Cfg<T>::MT;
+  static constexpr int NT = 64 * NW;
+};
+
+template <typename T, int TB, bool Z8>
+__global__ __launch_bounds__((SigCfg<T, Z8>::NT)) void sigma_mfma_kernel(const SigItem *__restrict__ items,
                                                        const SigBlk *__restrict__ blks, int64_t n_work, int nts,
                                                        int ns, const cplx<T> *__restrict__ V,
                                                        const cplx<T> *__restrict__ G, const cplx<T> *__restrict__ Pf,
-                                                       cplx<double> *__restrict__ Sig, double pref) {
+                                                       cplx<double> *__restrict__ Sig, double pref, Z8Ops z8) {
   const int64_t w = xcd_logical(blockIdx.x, gridDim.x);
   if (w >= n_work) return;
   const int np = (nts + TB - 1) / TB, per = np * ns;
   const SigItem it = items[w / per];
   const int ts = (int)(w % per), t0 = (ts / ns) * TB, s = ts - (ts / ns) * ns;
   const int tbn = min(TB, nts - t0);  // times of this slab
-  constexpr int NW = Cfg<T>::NW, MT = Cfg<T>::MT;
+  constexpr int NW = SigCfg<T, Z8>::NW, MT = SigCfg<T, Z8>::MT, NT = SigCfg<T, Z8>::NT;
   const int tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cplx<T> *lds = reinterpret_cast<cplx<T> *>(smem_raw);
@@ -637,7 +811,7 @@ __global__ __launch_bounds__(Cfg<T>::NT) void sigma_mfma_kernel(const SigItem *_
   // (Q, n) reduction
   const int S = choose_split<NW, MT>(nall), TG = NW / S, kp = wave % S, tg = wave / S;
   const int nmine = tg < TG && nall > tg ? (nall - tg + TG - 1) / TG : 0;  // tiles tg, tg + TG, ...
-  Acc<T> acc;
+  Tile<T> acc[MT];
 #pragma unroll
   for (int c = 0; c < MT; ++c) tile_zero(acc[c]);
 
@@ -672,7 +846,7 @@ __global__ __launch_bounds__(Cfg<T>::NT) void sigma_mfma_kernel(const SigItem *_
         // also ends the previous chunk's final product, the reader of bufZ and bufE
         {
           const FDiv dNC(NC);
-          for (int e = tid; e < k2 * NC; e += Cfg<T>::NT) {
+          for (int e = tid; e < k2 * NC; e += NT) {
             const int m = dNC.div(e), c = e - m * NC, tb = dnn.div(c), l = c - tb * nn;
             bufG[e] = G2[(int64_t)tb * g2ts + m * k2 + n0 + l];
           }
@@ -699,8 +873,17 @@ __global__ __launch_bounds__(Cfg<T>::NT) void sigma_mfma_kernel(const SigItem *_
           };
           // several row tiles: share each V fragment among them (always in
           // FP64, where one job layout per product saves 4 spilled registers)
-          if (kSigOneLayout<T> || tbn * nr > 16) nj = gemm_jobs_cs<T, true>(tbn * nr, zc, q, 0, la, lb, st);
-          else nj = gemm_jobs_rs<T, true>(tbn * nr, zc, q, 0, la, lb, st);
+          if (Z8 && B.kp > 0) {
+            // mixed precision: INT8 slices of P (this slab's first time) and V
+            const int64_t p8t = (int64_t)kZ8Comps * kZ8Slices * q * B.kp;
+            nj = z8_jobs<NW>(tbn * nr, zc, 0, z8.P8 + B.p8_off + t0 * p8t, z8.PE + B.pe_off + (int64_t)t0 * 3 * q,
+                             p8t, 3 * q, q, B.kp, Q0, dnr, nr, z8.V8 + B.v8_off, z8.VE + B.ve_off, K, J0, n0, k2,
+                             dnn, nn, Jb, ldz, bufZ);
+          } else if (kSigOneLayout<T> || tbn * nr > 16) {
+            nj = gemm_jobs_cs<T, true, kAheadJob<T>, NW>(tbn * nr, zc, q, 0, la, lb, st);
+          } else {
+            nj = gemm_jobs_rs<T, true, kAheadJob<T>, NW>(tbn * nr, zc, q, 0, la, lb, st);
+          }
         }
         // E[(r,i)][(tb,l)] = sum_m V[Q0+r][I0+i][m] G2_tb[m][n0+l]  (jobs dealt on after Z's)
         {
@@ -714,8 +897,8 @@ __global__ __launch_bounds__(Cfg<T>::NT) void sigma_mfma_kernel(const SigItem *_
             const int r = dIb.div(row), ii = row - r * Ib, tb = dnn.div(col), l = col - tb * nn;
             bufE[(tb * Ib + ii) * lde + r * nn + l] = v;
           };
-          if (kSigOneLayout<T> || tbn * nn > 16) gemm_jobs_rs<T>(nr * Ib, tbn * nn, k2, nj, la, lb, st);
-          else gemm_jobs_cs<T>(nr * Ib, tbn * nn, k2, nj, la, lb, st);
+          if (kSigOneLayout<T> || tbn * nn > 16) gemm_jobs_rs<T, false, kAheadJob<T>, NW>(nr * Ib, tbn * nn, k2, nj, la, lb, st);
+          else gemm_jobs_cs<T, false, kAheadJob<T>, NW>(nr * Ib, tbn * nn, k2, nj, la, lb, st);
         }
         __syncthreads();
         // acc[i][j] += sum_y E_tb[i][y] conj(Z_tb[j][y]) over this wave's part of y
@@ -847,27 +1030,39 @@ hipError_t launch_p0(const P0Item *items, const P0Blk *blks, int n_items, int nt
   return launch_p0_k<T, 2>(items, blks, n_items, nts, ns, V, G, P0, pref, lds, st);
 }
 
-template <typename T, int TB>
+template <typename T, int TB, bool Z8>
 hipError_t launch_sigma_tb(const SigItem *items, const SigBlk *blks, int n_items, int nts, int ns, const cplx<T> *V,
                            const cplx<T> *G, const cplx<T> *Pf, cplx<double> *Sig, double pref, size_t lds,
-                           hipStream_t st) {
+                           hipStream_t st, const Z8Ops &z8) {
   const int64_t n_work = (int64_t)n_items * ((nts + TB - 1) / TB) * ns;
   if (n_work == 0) return hipSuccess;
-  (void)hipFuncSetAttribute((const void *)sigma_mfma_kernel<T, TB>, hipFuncAttributeMaxDynamicSharedMemorySize,
+  (void)hipFuncSetAttribute((const void *)sigma_mfma_kernel<T, TB, Z8>, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)lds);
-  hipLaunchKernelGGL((sigma_mfma_kernel<T, TB>), dim3((unsigned)xcd_grid(n_work)), dim3(Cfg<T>::NT), lds, st, items,
-                     blks, n_work, nts, ns, V, G, Pf, Sig, pref);
+  hipLaunchKernelGGL((sigma_mfma_kernel<T, TB, Z8>), dim3((unsigned)xcd_grid(n_work)), dim3(SigCfg<T, Z8>::NT), lds, st,
+                     items, blks, n_work, nts, ns, V, G, Pf, Sig, pref, z8);
   return hipGetLastError();
+}
+
+template <typename T, bool Z8>
+hipError_t launch_sigma_z(const SigItem *items, const SigBlk *blks, int n_items, int nts, int ns, const cplx<T> *V,
+                          const cplx<T> *G, const cplx<T> *Pf, cplx<double> *Sig, double pref, size_t lds,
+                          hipStream_t st, int times, const Z8Ops &z8) {
+  if (times == 4) return launch_sigma_tb<T, 4, Z8>(items, blks, n_items, nts, ns, V, G, Pf, Sig, pref, lds, st, z8);
+  if (times == 2) return launch_sigma_tb<T, 2, Z8>(items, blks, n_items, nts, ns, V, G, Pf, Sig, pref, lds, st, z8);
+  if (times == 1) return launch_sigma_tb<T, 1, Z8>(items, blks, n_items, nts, ns, V, G, Pf, Sig, pref, lds, st, z8);
+  return hipErrorInvalidValue;
 }
 
 template <typename T>
 hipError_t launch_sigma(const SigItem *items, const SigBlk *blks, int n_items, int nts, int ns, const cplx<T> *V,
                         const cplx<T> *G, const cplx<T> *Pf, cplx<double> *Sig, double pref, size_t lds,
-                        hipStream_t st, int times) {
-  if (times == 4) return launch_sigma_tb<T, 4>(items, blks, n_items, nts, ns, V, G, Pf, Sig, pref, lds, st);
-  if (times == 2) return launch_sigma_tb<T, 2>(items, blks, n_items, nts, ns, V, G, Pf, Sig, pref, lds, st);
-  if (times == 1) return launch_sigma_tb<T, 1>(items, blks, n_items, nts, ns, V, G, Pf, Sig, pref, lds, st);
-  return hipErrorInvalidValue;
+                        hipStream_t st, int times, const Z8Ops *z8) {
+  if constexpr (sizeof(T) == 8) {
+    if (z8) return launch_sigma_z<T, true>(items, blks, n_items, nts, ns, V, G, Pf, Sig, pref, lds, st, times, *z8);
+  } else {
+    if (z8) return hipErrorInvalidValue;  // the INT8 Z product reads FP64 P and V
+  }
+  return launch_sigma_z<T, false>(items, blks, n_items, nts, ns, V, G, Pf, Sig, pref, lds, st, times, Z8Ops{});
 }
 
 template hipError_t launch_p0<double>(const P0Item *, const P0Blk *, int, int, int, const cplx<double> *,
@@ -876,9 +1071,9 @@ template hipError_t launch_p0<float>(const P0Item *, const P0Blk *, int, int, in
                                      const cplx<float> *, cplx<double> *, double, size_t, hipStream_t, int);
 template hipError_t launch_sigma<double>(const SigItem *, const SigBlk *, int, int, int, const cplx<double> *,
                                          const cplx<double> *, const cplx<double> *, cplx<double> *, double, size_t,
-                                         hipStream_t, int);
+                                         hipStream_t, int, const Z8Ops *);
 template hipError_t launch_sigma<float>(const SigItem *, const SigBlk *, int, int, int, const cplx<float> *,
                                         const cplx<float> *, const cplx<float> *, cplx<double> *, double, size_t,
-                                        hipStream_t, int);
+                                        hipStream_t, int, const Z8Ops *);
 
 }  // namespace symgw

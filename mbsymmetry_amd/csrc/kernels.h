@@ -31,10 +31,21 @@ struct GenBlk {
 template <typename T>
 hipError_t launch_p0(const P0Item *items, const P0Blk *blks, int n_items, int nts, int ns, const cplx<T> *V,
                      const cplx<T> *G, cplx<double> *P0, double pref, size_t lds, hipStream_t st, int mode);
+// INT8 slice copies read by the mixed-precision Z product of Sigma.
+struct Z8Ops {
+  const int8_t *P8 = nullptr, *V8 = nullptr;
+  const int *PE = nullptr, *VE = nullptr;
+};
+// z8 == nullptr: every Z product on the matrix cores of T; otherwise the
+// blocks with SigBlk::kp > 0 run it on INT8 slices (T = double only).
 template <typename T>
 hipError_t launch_sigma(const SigItem *items, const SigBlk *blks, int n_items, int nts, int ns, const cplx<T> *V,
                         const cplx<T> *G, const cplx<T> *Pf, cplx<double> *Sig, double pref, size_t lds,
-                        hipStream_t st, int times);
+                        hipStream_t st, int times, const Z8Ops *z8);
+// Split rows of complex128 matrices into INT8 slices (Split8Blk);
+// row_prefix[b] is the first flat row of block b (nsb + 1 entries).
+hipError_t launch_split8(const Split8Blk *sb, int nsb, const int64_t *row_prefix, int64_t nrows_total,
+                         const cplx<double> *src, int8_t *dst, int *ex, hipStream_t st);
 // peaks.hip: device peak microbenchmarks
 template <typename T>
 hipError_t launch_mfma_peak(void *out, int n_wg, int iters, hipStream_t st);

@@ -7,6 +7,7 @@
 #include "layout.h"
 
 #include <algorithm>
+#include <cstdlib>
 #include <map>
 #include <sstream>
 
@@ -350,7 +351,7 @@ static int p0_band(int qs, int band_cap, int budget, int eb, int TB, const std::
   return best_band;
 }
 
-bool build_worklists(const Layout &L, int eb, WorkLists &W, std::string &err) {
+bool build_worklists(const Layout &L, int eb, WorkLists &W, std::string &err, bool z8) {
   W = WorkLists();
   const int nk = L.nk, nts = L.nts, ns = L.ns;
   const int budget = contract_lds(eb) / eb;  // complex elements of LDS per workgroup
@@ -432,6 +433,13 @@ bool build_worklists(const Layout &L, int eb, WorkLists &W, std::string &err) {
   // ---- Sigma: items per (k_ir, ik1, row band, column band); blocks in (q, V-block) order ----
   size_t sig_lds[kNumSigDepths] = {};
   std::map<std::array<int, 4>, std::array<int, 2>> sig_plans;
+  // mixed mode: one INT8 split copy per V block and per P block the Sigma
+  // sweep reads, keyed by their FP64 offsets
+  std::map<int64_t, size_t> v8_of, p8_of;
+  // aux blocks above this size take the INT8 Z product (SYMGW_Z8_MIN_Q
+  // overrides the default, e.g. 0 to put every block on it)
+  int z8_min_q = kZ8MinQ;
+  if (const char *ev = std::getenv("SYMGW_Z8_MIN_Q")) z8_min_q = std::atoi(ev);
   for (int ikr = 0; ikr < L.ink; ++ikr) {
     int k = L.irre_list[ikr];
     for (size_t i1 = 0; i1 < L.ao_sizes[k].size(); ++i1) {
@@ -459,6 +467,34 @@ bool build_worklists(const Layout &L, int eb, WorkLists &W, std::string &err) {
           }
           B.ldz = odd_ld(B.qsub * B.nsub);
           B.lde = odd_ld(B.qsub * B.nsub);
+          B.kp = 0;
+          B.v8_off = B.ve_off = B.p8_off = B.pe_off = 0;
+          if (z8 && B.q > z8_min_q) {
+            B.kp = (B.q + 15) / 16 * 16;
+            const int ncol = k1 * B.k2;
+            auto iv = v8_of.find(B.v_off);
+            if (iv == v8_of.end()) {
+              // B operand of Z: columns (a, b) of V, K = P (stride k1 k2)
+              Split8Blk S8{B.v_off, 0, W.v8_bytes, W.ve_count, 1, ncol, ncol, B.q, B.kp, 1, 1, 0};
+              iv = v8_of.emplace(B.v_off, W.v8_split.size()).first;
+              W.v8_split.push_back(S8);
+              W.v8_bytes += (int64_t)kZ8Comps * kZ8Slices * ncol * B.kp;
+              W.ve_count += (int64_t)kZ8Comps * ncol;
+            }
+            B.v8_off = W.v8_split[iv->second].dst_off;
+            B.ve_off = W.v8_split[iv->second].e_off;
+            auto ip = p8_of.find(B.p_base);
+            if (ip == p8_of.end()) {
+              // A operand of Z: rows (t, Q) of P, K = P (contiguous)
+              Split8Blk S8{B.p_base, B.p_tstride, W.p8_bytes, W.pe_count, B.q, 1, B.q, B.q, B.kp, nts, 0, 0};
+              ip = p8_of.emplace(B.p_base, W.p8_split.size()).first;
+              W.p8_split.push_back(S8);
+              W.p8_bytes += (int64_t)nts * kZ8Comps * kZ8Slices * B.q * B.kp;
+              W.pe_count += (int64_t)nts * kZ8Comps * B.q;
+            }
+            B.p8_off = W.p8_split[ip->second].dst_off;
+            B.pe_off = W.p8_split[ip->second].e_off;
+          }
           size_t &lds_t = sig_lds[sig_depth_index(TB)];
           lds_t = std::max(lds_t, (size_t)TB * (band * (B.ldz + B.lde) + B.k2 * B.nsub) * eb);
           W.sig_blks.push_back(B);
@@ -491,7 +527,9 @@ bool build_worklists(const Layout &L, int eb, WorkLists &W, std::string &err) {
         }
     }
   }
-  for (int d = 0; d < kNumSigDepths; ++d) W.sig_lds[d] = W.sig_items[d].empty() ? 0 : std::max(sig_lds[d], red_bytes);
+  // the INT8-Z Sigma kernel reduces split-K parts over its own wave / tile shape
+  const size_t red_sig = z8 ? (size_t)kZ8Waves * kZ8Tiles * 64 * 8 * (eb / 2) : red_bytes;
+  for (int d = 0; d < kNumSigDepths; ++d) W.sig_lds[d] = W.sig_items[d].empty() ? 0 : std::max(sig_lds[d], red_sig);
   for (int d = 0; d < kNumSigDepths; ++d)
     if (W.sig_lds[d] > (size_t)contract_lds(eb)) { err = fmt("Sigma LDS plan exceeds the CU's LDS (bytes)", (int)W.sig_lds[d]); return false; }
   if (W.p0_lds > (size_t)contract_lds(eb)) { err = fmt("P0 LDS plan exceeds the CU's LDS (bytes)", (int)W.p0_lds); return false; }

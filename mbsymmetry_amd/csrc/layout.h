@@ -142,6 +142,13 @@ struct SigBlk {
   int32_t qsub;        // Q rows per LDS chunk
   int32_t ldz, lde;    // LDS row strides of the Z and E chunks
   int32_t nsub;        // orbitals n of k - q per LDS chunk
+  // mixed mode (Z product on INT8 slices): split copies of this V block
+  // (columns (a, b) over P) and of P of aux block iq of q (rows (t, Q) over P)
+  int32_t kp;          // q rounded up to 16 (row length of the split copies)
+  int64_t v8_off;      // bytes: V8 of this block, [comp][slice][col][kp]
+  int64_t ve_off;      // int32 index: column exponents, [comp][col]
+  int64_t p8_off;      // bytes: P8 of (q, iq) at t = 0, [t][comp][slice][Q][kp]
+  int64_t pe_off;      // int32 index: row exponents at t = 0, [t][comp][Q]
 };
 // Rows [I0, I0 + Ib) x columns [J0, J0 + Jb) of ao block ik1 of iBZ point
 // k_ir, summed over all q.
@@ -163,6 +170,34 @@ struct RotBlk {
   int32_t m, n;        // target block m x m, source block n x n (O is m x n)
   int32_t target;      // full-BZ index of the target
   int32_t conj;        // 1 => complex-conjugate the result (k != conj_index[k])
+};
+
+// INT8 slice split of one real-matrix family for the mixed-precision Z
+// product of Sigma (Ozaki scheme: x = 2^e sum_s x_s 128^-s, x_s int8,
+// s < kZ8Slices).
+// Rows of `nsub` sub-blocks of `nrows` rows each, K elements per row (row r
+// of sub-block u at src_off + u * sub_stride + r * row_stride, element k at
+// + k * k_stride, complex128).  Output per sub-block u:
+// dst_off + u * kZ8Comps * kZ8Slices * nrows * kp bytes, [comp][slice][row][kp] (k >= K zero);
+// exponents e_off + u * 3 * nrows, [comp][row].  comps: re, im and
+// re + im (kind 0, the A operand of the 3M product) or re - im (kind 1, the
+// conjugated B operand).
+// Five slices: the truncation error of a product is ~2^-35 of |A|max |B|max
+// per term (four slices, 2^-28, left the Dyson step of the LiH fixture at
+// |dG| ~ 5e-8).
+constexpr int kZ8Slices = 5;
+constexpr int kZ8Comps = 3;
+// Aux blocks up to this size keep the FP64 Z product in the mixed mode: the
+// exact reconstruction of the four int32 accumulators costs about as much as
+// a short FP64 reduction (measured per block size on MI355X).
+constexpr int kZ8MinQ = 32;
+// Workgroup of the Sigma kernel with the INT8 Z product: 12 waves of three
+// accumulator tiles (36 tiles, as many as sig_shape plans for complex128).
+constexpr int kZ8Waves = 12;
+constexpr int kZ8Tiles = 3;
+struct Split8Blk {
+  int64_t src_off, sub_stride, dst_off, e_off;
+  int32_t row_stride, k_stride, nrows, K, kp, nsub, kind, pad;
 };
 
 // Largest source block a rotation reads (O is m x n with n <= kRotMaxBlock):
@@ -194,10 +229,14 @@ struct WorkLists {
   int64_t n_p0_contractions = 0, n_sig_contractions = 0;
   double flop_p0 = 0, flop_sig = 0;
   size_t p0_lds = 0, sig_lds[kNumSigDepths] = {};  // dynamic LDS bytes of the contraction kernels
+  // mixed mode: INT8 split tables (V blocks read by Sigma; P blocks of every q)
+  std::vector<Split8Blk> v8_split, p8_split;
+  int64_t v8_bytes = 0, ve_count = 0, p8_bytes = 0, pe_count = 0;
 };
 
-// elem_bytes: bytes of one complex element in the contraction precision.
-bool build_worklists(const Layout &L, int elem_bytes, WorkLists &W, std::string &err);
+// elem_bytes: bytes of one complex element in the contraction precision;
+// z8: also plan the INT8 slice copies of the mixed-precision Z product.
+bool build_worklists(const Layout &L, int elem_bytes, WorkLists &W, std::string &err, bool z8 = false);
 
 // Per-iteration FLOPs of P0 and Sigma, 8*M*N*K per complex GEMM
 // (analysis.cpp:229-255 and :271-298 without the +12MN epilogue).
