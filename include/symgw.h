@@ -238,6 +238,18 @@ int symgw_set_transforms(symgw_ctx *ctx, const double *Tnt, const double *Ttn,
  * Ttn_FB.  Overwrites the device P0 buffer with P (as the reference aliases). */
 int symgw_solve_W(symgw_ctx *ctx);
 
+/* symgw_solve_W without allocation: no counterpart in the reference; for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same results as
+ * symgw_solve_W, bit for bit; the solver scratch comes from a 32 MiB array
+ * that the library holds on each GPU it runs on until the process ends, and
+ * the bosonic frequencies are solved in groups that fit it.  A system whose
+ * solves of one frequency need more than 32 MiB (2 n^2 complex128 per aux
+ * block of every iBZ q) gets SYMGW_ERR_ARG.  The calls of one process are
+ * serialised (they share each GPU's array); allocates nothing, neither on the
+ * device nor through a library routine. */
+int symgw_solve_W_noalloc(symgw_ctx *ctx);
+
 /* cugw_routine_block::solve_Sigma (cugw_routines_block.cu:296-318): rotates P
  * to every q of the full BZ and contracts Sigma for the iBZ k, all t;
  * prefactor -1, NOT divided by nk (cugw_solver_block.cpp:50-52). */

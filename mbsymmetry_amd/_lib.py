@@ -67,6 +67,7 @@ SIGNATURES = {
     "symgw_upload_P": (C.c_int, [C.c_void_p, _dp]),
     "symgw_set_transforms": (C.c_int, [C.c_void_p, _dp, _dp, _dp, _dp, _dp]),
     "symgw_solve_W": (C.c_int, [C.c_void_p]),
+    "symgw_solve_W_noalloc": (C.c_int, [C.c_void_p]),
     "symgw_solve_Sigma": (C.c_int, [C.c_void_p]),
     "symgw_copy_Sigma_to_host": (C.c_int, [C.c_void_p, _dp]),
     "symgw_upload_Sigma": (C.c_int, [C.c_void_p, _dp]),
@@ -221,6 +222,9 @@ class Context:
 
     def solve_W(self):
         self._chk(self.lib.symgw_solve_W(self.h), "solve_W")
+
+    def solve_W_noalloc(self):
+        self._chk(self.lib.symgw_solve_W_noalloc(self.h), "solve_W_noalloc")
 
     def solve_Sigma(self):
         self._chk(self.lib.symgw_solve_Sigma(self.h), "solve_Sigma")

@@ -296,15 +296,19 @@ int phase_P0(symgw_ctx *c) {
   return SYMGW_OK;
 }
 
-int phase_W(symgw_ctx *c) {
+// The W phase with the solver scratch at `scratch`, which holds `per_launch`
+// frequency groups (the solves run in launches of at most that many
+// frequencies).
+int phase_W_with(symgw_ctx *c, cplx<double> *scratch, int64_t per_launch) {
   if (!c->have_T) return fail(c, SYMGW_ERR_STATE, "transforms not set");
   const Layout &L = c->L;
   const int KT = L.nts - 2;
   // P0(iW_n) = Tnt_BF * P0(t interior), mirrored, / nk (cugw_solver_block.cpp:395-400)
   HIPCHK(c, launch_tgemm(c->d_tb_wf, L.ink, c->max_qslice, L.nw_b, KT, 1, c->Tnt_BF, c->P, c->Pw, 1, 1, L.nts,
                          1.0 / L.nk, 0, 0, c->st));
-  HIPCHK(c, launch_wsolve(c->d_w_blocks, (int)c->W.w_blocks.size(), L.nw_b, c->Pw, c->d_w_off, c->d_qslice_irre,
-                          c->w_scratch, c->W.w_scr_group, c->st));
+  for (int n0 = 0; n0 < L.nw_b; n0 += (int)per_launch)
+    HIPCHK(c, launch_wsolve(c->d_w_blocks, (int)c->W.w_blocks.size(), n0, (int)std::min<int64_t>(per_launch, L.nw_b - n0),
+                            c->Pw, c->d_w_off, c->d_qslice_irre, scratch, c->W.w_scr_group, c->st));
   // P(t) = Ttn_FB * Pt(iW_n) (cugw_solver_block.cpp:428)
   HIPCHK(c, launch_tgemm(c->d_tb_wb, L.ink, c->max_qslice, L.nts, L.nw_b, 1, c->Ttn_FB, c->Pw, c->P, 0, 0, L.nts,
                          1.0, 0, 0, c->st));
@@ -312,6 +316,8 @@ int phase_W(symgw_ctx *c) {
   // P of every full-BZ q for the Sigma contraction (cugw_qkpt_block.cu:287-340)
   return rotate_P(c);
 }
+
+int phase_W(symgw_ctx *c) { return phase_W_with(c, c->w_scratch, std::max(1, c->L.nw_b)); }
 
 int phase_Sigma(symgw_ctx *c) {
   if (!c->have_V || !c->have_G) return fail(c, SYMGW_ERR_STATE, "V or G missing");
@@ -391,6 +397,22 @@ int time_mfma_peak(hipStream_t st, hipEvent_t e0, hipEvent_t e1, void *out, int 
   *tflops = flops / (ms * 1e-3) / 1e12;
   return SYMGW_OK;
 }
+}  // namespace
+
+// Scratch of symgw_solve_W_noalloc: a file-scope device array, one copy per
+// GPU the library runs on, held until the process ends.  Host code reaches it
+// only through the address hipGetSymbolAddress returns for the current device
+// (kept per device in g_noalloc_scratch); the array's name in host code is a
+// stand-in, not a device address.
+constexpr size_t kNoAllocScratchBytes = size_t(32) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kNoAllocMaxDevices = 64;
+// Serialises the allocation-free calls of this process (they share each GPU's
+// array) and guards the per-device addresses.
+std::mutex g_noalloc_mu;
+void *g_noalloc_scratch[kNoAllocMaxDevices] = {};
 }  // namespace
 
 extern "C" {
@@ -1052,6 +1074,28 @@ int symgw_solve_W(symgw_ctx *ctx) {
     if (!ctx) return SYMGW_ERR_ARG;
     HIPCHK(ctx, hipSetDevice(ctx->device));
     int r = phase_W(ctx);
+    if (r) return r;
+    HIPCHK(ctx, hipStreamSynchronize(ctx->st));
+    return SYMGW_OK;
+  });
+}
+
+int symgw_solve_W_noalloc(symgw_ctx *ctx) {
+  return guarded(ctx, [&]() -> int {
+    if (!ctx) return SYMGW_ERR_ARG;
+    if (ctx->device < 0 || ctx->device >= kNoAllocMaxDevices)
+      return fail(ctx, SYMGW_ERR_ARG, "solve_W_noalloc: device index above the supported range");
+    const int64_t cap = (int64_t)(kNoAllocScratchBytes / sizeof(cplx<double>));
+    const int64_t group = ctx->W.w_scr_group;  // complex elements of one bosonic frequency's solves
+    if (group > cap)
+      return fail(ctx, SYMGW_ERR_ARG,
+                  "solve_W_noalloc: the solves of one bosonic frequency need " + std::to_string(group * 16) +
+                      " bytes of scratch, above the 32 MiB of the allocation-free path");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    void *&ptr = g_noalloc_scratch[ctx->device];
+    if (!ptr) HIPCHK(ctx, hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    int r = phase_W_with(ctx, static_cast<cplx<double> *>(ptr), std::max<int64_t>(1, cap / std::max<int64_t>(1, group)));
     if (r) return r;
     HIPCHK(ctx, hipStreamSynchronize(ctx->st));
     return SYMGW_OK;

@@ -58,8 +58,11 @@ hipError_t launch_rotate(const RotBlk *rb, int n_rb, int nts_ns, const cplx<doub
 hipError_t launch_tgemm(const TBatch *tb, int nbatch, int max_cols, int M, int KT, int nsplit,
                         const cplx<double> *Tm, const cplx<double> *in, cplx<double> *out, int mode, int row_shift,
                         int nts, double alpha, int64_t in_split_stride, int64_t out_split_stride, hipStream_t st);
-hipError_t launch_wsolve(const SolveBlk *blks, int nblk, int nw_b, cplx<double> *Pw, const int64_t *w_off,
-                         const int64_t *qslice, cplx<double> *scratch, int64_t scr_group, hipStream_t st);
+// W solves of the bosonic frequencies [nw_first, nw_first + nw_count);
+// scratch holds nw_count groups of scr_group complex elements.
+hipError_t launch_wsolve(const SolveBlk *blks, int nblk, int nw_first, int nw_count, cplx<double> *Pw,
+                         const int64_t *w_off, const int64_t *qslice, cplx<double> *scratch, int64_t scr_group,
+                         hipStream_t st);
 hipError_t launch_dyson(const SolveBlk *blks, int nblk, int nw, int ns, const cplx<double> *Sw, cplx<double> *Gw,
                         const int64_t *d_off, const int64_t *kslice, const int64_t *f_off, const cplx<double> *F,
                         const cplx<double> *S, int64_t flat_ao, const cplx<double> *iwn, double mu,

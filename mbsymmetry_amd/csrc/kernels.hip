@@ -133,12 +133,14 @@ __device__ void gj_invert(cplx<double> *A, int n, int *piv, cplx<double> *fcol, 
 // W solve per (q_ir, n, aux block) (cugw_solver_block.cpp:412-424):
 //   A = I - (X + X^H)/2,  A <- A^-1,  A <- (A + A^H)/2,  X <- A X
 // Pw layout: [q_ir][n][qslice], block at w_off[q_ir] + n*qslice + off.
-__global__ __launch_bounds__(NT) void wsolve_kernel(const SolveBlk *__restrict__ blks, int nblk, int nw_b,
-                                                   cplx<double> *__restrict__ Pw,
+// Frequencies [nw_first, nw_first + nw_count); frequency n uses scratch group
+// n - nw_first.
+__global__ __launch_bounds__(NT) void wsolve_kernel(const SolveBlk *__restrict__ blks, int nblk, int nw_first,
+                                                   int nw_count, cplx<double> *__restrict__ Pw,
                                                    const int64_t *__restrict__ w_off,
                                                    const int64_t *__restrict__ qslice,
                                                    cplx<double> *__restrict__ scratch, int64_t scr_group) {
-  const int b = blockIdx.x / nw_b, nn = blockIdx.x % nw_b;
+  const int b = blockIdx.x / nw_count, nn = nw_first + blockIdx.x % nw_count;
   if (b >= nblk) return;
   const SolveBlk S = blks[b];
   const int n = S.n, tid = threadIdx.x;
@@ -147,7 +149,7 @@ __global__ __launch_bounds__(NT) void wsolve_kernel(const SolveBlk *__restrict__
   __shared__ double red_v[NT];
   __shared__ int red_i[NT];
   cplx<double> *X = Pw + w_off[S.slice] + (int64_t)nn * qslice[S.slice] + S.off;
-  cplx<double> *A = scratch + (int64_t)nn * scr_group + S.scr;
+  cplx<double> *A = scratch + (int64_t)(nn - nw_first) * scr_group + S.scr;
   cplx<double> *Xc = A + (int64_t)n * n;
   for (int e = tid; e < n * n; e += NT) {
     const int i = e / n, j = e - i * n;
@@ -305,11 +307,12 @@ hipError_t launch_tgemm(const TBatch *tb, int nbatch, int max_cols, int M, int K
   return hipGetLastError();
 }
 
-hipError_t launch_wsolve(const SolveBlk *blks, int nblk, int nw_b, cplx<double> *Pw, const int64_t *w_off,
-                         const int64_t *qslice, cplx<double> *scratch, int64_t scr_group, hipStream_t st) {
-  if (nblk == 0 || nw_b == 0) return hipSuccess;
-  hipLaunchKernelGGL(wsolve_kernel, dim3(nblk * nw_b), dim3(NT), 0, st, blks, nblk, nw_b, Pw, w_off, qslice, scratch,
-                     scr_group);
+hipError_t launch_wsolve(const SolveBlk *blks, int nblk, int nw_first, int nw_count, cplx<double> *Pw,
+                         const int64_t *w_off, const int64_t *qslice, cplx<double> *scratch, int64_t scr_group,
+                         hipStream_t st) {
+  if (nblk == 0 || nw_count <= 0) return hipSuccess;
+  hipLaunchKernelGGL(wsolve_kernel, dim3(nblk * nw_count), dim3(NT), 0, st, blks, nblk, nw_first, nw_count, Pw, w_off,
+                     qslice, scratch, scr_group);
   return hipGetLastError();
 }
 
