@@ -43,6 +43,15 @@ extern "C" {
  * FP64.  V and P are also kept as slices: V8 once per set_V / generate_V,
  * P8 after each rotation of P. */
 #define SYMGW_PREC_MIXED 2
+/* Flag OR-ed into the precision argument of symgw_create /
+ * symgw_layout_footprint: V is not kept resident.  symgw_generate_V records
+ * the generator (symgw_set_V is refused) and every P0 / Sigma sweep
+ * regenerates V on the device one batch at a time, just ahead of that batch's
+ * contraction: the pairs (k, k + q) of one iBZ q for P0, (k, k - q) of one
+ * iBZ k for Sigma, into a buffer of the largest batch.  Values are those of
+ * the resident V bit for bit; the device needs the largest batch instead of
+ * the whole V (for systems whose V exceeds HBM).  Not with SYMGW_PREC_MIXED. */
+#define SYMGW_FLAG_STREAM_V 0x100
 
 /*
  * Block-sparse layout of one system.  Mirrors the tables that
@@ -287,6 +296,15 @@ int symgw_mu_search(symgw_ctx *ctx, double nel_target, const double *kweights,
  * when mu changes) and replays the four graphs; with timing on, HIP events
  * between the replays give the per-phase device times. */
 int symgw_gw_iteration(symgw_ctx *ctx, double mu, int use_graph);
+
+/* Restrict the P0 and Sigma sweeps to the iBZ points with index in
+ * [first, first + count): P0 of those q, Sigma of those k (all k, resp. q,
+ * summed); the other rows of P0 / Sigma are left as they are.  This is one
+ * GPU's share when the iBZ points are divided over GPUs (the reference
+ * strides (q, k) pairs over devices instead, cugw_routines_block.cu:279,302);
+ * W, Dyson and the rotations still cover every point.  Default: every
+ * point.  Drops the captured iteration graphs. */
+int symgw_set_shard(symgw_ctx *ctx, int first, int count);
 
 /* Wait for all work queued on the context's stream. */
 int symgw_sync(symgw_ctx *ctx);

@@ -15,6 +15,7 @@ SYMGW_OK = 0
 PREC_FP64 = 0
 PREC_FP32 = 1
 PREC_MIXED = 2
+FLAG_STREAM_V = 0x100
 
 _i32p = C.POINTER(C.c_int32)
 _dp = C.POINTER(C.c_double)
@@ -77,6 +78,7 @@ SIGNATURES = {
     "symgw_mu_search": (C.c_int, [C.c_void_p, C.c_double, _dp, C.c_double, C.c_double, C.c_double, C.c_double,
                                   _dp, _dp]),
     "symgw_gw_iteration": (C.c_int, [C.c_void_p, C.c_double, C.c_int]),
+    "symgw_set_shard": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "symgw_sync": (C.c_int, [C.c_void_p]),
     "symgw_last_times": (C.c_int, [C.c_void_p, _dp]),
     "symgw_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
@@ -261,6 +263,9 @@ class Context:
 
     def gw_iteration(self, mu: float, use_graph: bool = True):
         self._chk(self.lib.symgw_gw_iteration(self.h, mu, 1 if use_graph else 0), "gw_iteration")
+
+    def set_shard(self, first: int, count: int):
+        self._chk(self.lib.symgw_set_shard(self.h, first, count), "set_shard")
 
     def sync(self):
         self._chk(self.lib.symgw_sync(self.h), "sync")

@@ -59,6 +59,16 @@ struct symgw_ctx {
   cplx<double> *Tnt = nullptr, *Ttn = nullptr, *Tnt_BF = nullptr, *Ttn_FB = nullptr, *iwn = nullptr;
   cplx<double> *w_scratch = nullptr, *d_scratch = nullptr;
   double *kw = nullptr, *nel_dev = nullptr;
+  // shard: the P0 / Sigma sweeps cover iBZ q / k indices [shard_first, shard_end)
+  int shard_first = 0, shard_end = 0;
+  // streamed V: generator parameters and the per-batch generator blocks
+  bool stream = false;
+  uint64_t gen_seed = 0;
+  double gen_scale = 0.0;
+  double *d_decay = nullptr;
+  int n_decay = 0;
+  GenBlk *d_p0_gen = nullptr, *d_sig_gen = nullptr;
+  std::vector<int> p0_gen_begin, sig_gen_begin;  // [ink + 1] prefixes into the generator block lists
   // mixed mode: INT8 slice copies of V (Sigma's blocks) and of P (full BZ)
   int8_t *V8 = nullptr, *P8 = nullptr;
   int *VE = nullptr, *PE = nullptr;
@@ -186,7 +196,7 @@ int upload(symgw_ctx *c, X **dst, const std::vector<X> &v) {
 
 int64_t footprint(const Layout &L, const WorkLists &W, int eb) {
   int64_t b = 0;
-  b += L.flat_V * eb;
+  b += (W.stream ? W.batch_V_max : L.flat_V) * eb;  // resident V, or the largest streamed batch
   b += (int64_t)L.nts * L.ns * L.flat_ao * 16 * 2;       // G_ibz, Sig
   b += (int64_t)L.nts * L.ns * L.flat_ao_full * eb;      // G_full
   b += (int64_t)L.nts * L.flat_aux * 16;                 // P
@@ -200,9 +210,49 @@ int64_t footprint(const Layout &L, const WorkLists &W, int eb) {
   return b;
 }
 
-bool valid_precision(int p) { return p == SYMGW_PREC_FP64 || p == SYMGW_PREC_FP32 || p == SYMGW_PREC_MIXED; }
+// precision argument: a SYMGW_PREC_* mode, optionally with SYMGW_FLAG_STREAM_V
+int prec_mode(int p) { return p & 0xFF; }
+bool prec_stream(int p) { return (p & SYMGW_FLAG_STREAM_V) != 0; }
+bool valid_precision(int p) {
+  const int m = prec_mode(p);
+  return (p & ~(0xFF | SYMGW_FLAG_STREAM_V)) == 0 &&
+         (m == SYMGW_PREC_FP64 || m == SYMGW_PREC_FP32 || m == SYMGW_PREC_MIXED);
+}
 // bytes of one complex element of V, G and P in the contractions
-int precision_bytes(int p) { return p == SYMGW_PREC_FP32 ? 8 : 16; }
+int precision_bytes(int p) { return prec_mode(p) == SYMGW_PREC_FP32 ? 8 : 16; }
+
+// Generator blocks of the pairs of every streamed batch (V offsets inside the
+// batch buffer, counters of the resident layout so the values are the same).
+void batch_gen_blocks(const Layout &L, const std::vector<std::pair<int, int64_t>> &pairs,
+                      const std::vector<int> &pair_begin, std::vector<GenBlk> &out, std::vector<int> &begin) {
+  begin.assign(1, 0);
+  for (size_t bt = 0; bt + 1 < pair_begin.size(); ++bt) {
+    for (int i = pair_begin[bt]; i < pair_begin[bt + 1]; ++i) {
+      const int p = pairs[i].first, k1 = p / L.nk, k2 = p % L.nk;
+      for (size_t b = 0; b < L.V_irreps[p].size(); ++b) {
+        const auto &t = L.V_irreps[p][b];
+        GenBlk g;
+        g.v_off = pairs[i].second + L.V_offsets[p][b];
+        g.local_off = L.V_offsets[p][b];
+        g.q = L.V_qsize[p][b];
+        g.k1 = L.ao_sizes[k1][t[1]];
+        g.k2 = L.ao_sizes[k2][t[2]];
+        g.a_base = L.ao_block_offsets[k1][t[1]];
+        g.b_base = L.ao_block_offsets[k2][t[2]];
+        g.pair = p;
+        out.push_back(g);
+      }
+    }
+    begin.push_back((int)out.size());
+  }
+}
+
+// Streamed V: generate batch `bt` of a contraction into the batch buffer.
+template <typename T>
+hipError_t gen_batch(symgw_ctx *c, const GenBlk *d_gen, const std::vector<int> &begin, int bt) {
+  return launch_genv<T>(d_gen + begin[bt], begin[bt + 1] - begin[bt], c->gen_seed, c->gen_scale, c->d_decay,
+                        c->n_decay, static_cast<cplx<T> *>(c->V), c->st);
+}
 
 // INT8 slices of V (after it is set) and of P_full (after each rotation).
 int split_V8(symgw_ctx *c) {
@@ -249,9 +299,23 @@ template <typename T>
 int run_p0(symgw_ctx *c) {
   const Layout &L = c->L;
   const double pref = L.ns == 1 ? -2.0 : -1.0;  // cugw_qkpt_block.cu:210
-  hipError_t e = launch_p0<T>(c->d_p0_items, c->d_p0_blks, (int)c->W.p0_items.size(), L.nts, L.ns,
-                              static_cast<const cplx<T> *>(c->V), static_cast<const cplx<T> *>(c->G_full), c->P, pref,
-                              c->W.p0_lds, c->st, c->W.p0_mode);
+  hipError_t e = hipSuccess;
+  const auto &ib = c->W.p0_batch_items;
+  if (!c->stream) {
+    // the items of the shard's iBZ q, in one launch
+    e = launch_p0<T>(c->d_p0_items + ib[c->shard_first], c->d_p0_blks, ib[c->shard_end] - ib[c->shard_first], L.nts,
+                     L.ns, static_cast<const cplx<T> *>(c->V), static_cast<const cplx<T> *>(c->G_full), c->P, pref,
+                     c->W.p0_lds, c->st, c->W.p0_mode);
+  } else {
+    // one batch per iBZ q: its V pairs generated, then its items contracted
+    for (int bt = c->shard_first; bt < c->shard_end && e == hipSuccess; ++bt) {
+      e = gen_batch<T>(c, c->d_p0_gen, c->p0_gen_begin, bt);
+      if (e == hipSuccess)
+        e = launch_p0<T>(c->d_p0_items + ib[bt], c->d_p0_blks, ib[bt + 1] - ib[bt], L.nts, L.ns,
+                         static_cast<const cplx<T> *>(c->V), static_cast<const cplx<T> *>(c->G_full), c->P, pref,
+                         c->W.p0_lds, c->st, c->W.p0_mode);
+    }
+  }
   if (e != hipSuccess) return fail(c, SYMGW_ERR_HIP, std::string("P0 kernel: ") + hipGetErrorString(e));
   return SYMGW_OK;
 }
@@ -266,11 +330,28 @@ int run_sigma(symgw_ctx *c) {
   z8.PE = c->PE;
   z8.VE = c->VE;
   const Z8Ops *pz8 = c->prec == SYMGW_PREC_MIXED ? &z8 : nullptr;
-  for (int d = 0; d < kNumSigDepths && e == hipSuccess; ++d)
-    e = launch_sigma<T>(c->d_sig_items[d], c->d_sig_blks, (int)c->W.sig_items[d].size(), L.nts, L.ns,
-                        static_cast<const cplx<T> *>(c->V), static_cast<const cplx<T> *>(c->G_full),
-                        static_cast<const cplx<T> *>(c->P_full), c->Sig, -1.0, c->W.sig_lds[d], c->st,
-                        kSigDepths[d], pz8);
+  if (!c->stream) {
+    // the items of the shard's iBZ k, one launch per slab depth
+    for (int d = 0; d < kNumSigDepths && e == hipSuccess; ++d) {
+      const auto &ib = c->W.sig_batch_items[d];
+      e = launch_sigma<T>(c->d_sig_items[d] + ib[c->shard_first], c->d_sig_blks, ib[c->shard_end] - ib[c->shard_first],
+                          L.nts, L.ns, static_cast<const cplx<T> *>(c->V), static_cast<const cplx<T> *>(c->G_full),
+                          static_cast<const cplx<T> *>(c->P_full), c->Sig, -1.0, c->W.sig_lds[d], c->st,
+                          kSigDepths[d], pz8);
+    }
+  } else {
+    // one batch per iBZ k: its V pairs generated, then its items of every slab depth
+    for (int bt = c->shard_first; bt < c->shard_end && e == hipSuccess; ++bt) {
+      e = gen_batch<T>(c, c->d_sig_gen, c->sig_gen_begin, bt);
+      for (int d = 0; d < kNumSigDepths && e == hipSuccess; ++d) {
+        const auto &ib = c->W.sig_batch_items[d];
+        e = launch_sigma<T>(c->d_sig_items[d] + ib[bt], c->d_sig_blks, ib[bt + 1] - ib[bt], L.nts, L.ns,
+                            static_cast<const cplx<T> *>(c->V), static_cast<const cplx<T> *>(c->G_full),
+                            static_cast<const cplx<T> *>(c->P_full), c->Sig, -1.0, c->W.sig_lds[d], c->st,
+                            kSigDepths[d], pz8);
+      }
+    }
+  }
   if (e != hipSuccess) return fail(c, SYMGW_ERR_HIP, std::string("Sigma kernel: ") + hipGetErrorString(e));
   return SYMGW_OK;
 }
@@ -462,7 +543,8 @@ int symgw_layout_footprint(const symgw_layout_desc *desc, int precision, int64_t
     Layout L;
     WorkLists W;
     std::string err;
-    if (!build_layout(desc, L, err) || !build_worklists(L, eb, W, err, precision == SYMGW_PREC_MIXED)) {
+    if (!build_layout(desc, L, err) ||
+        !build_worklists(L, eb, W, err, prec_mode(precision) == SYMGW_PREC_MIXED, prec_stream(precision))) {
       g_create_error = err;
       return SYMGW_ERR_ARG;
     }
@@ -583,11 +665,13 @@ int symgw_create(const symgw_layout_desc *desc, int device, int precision, symgw
       return SYMGW_ERR_ARG;
     }
     std::unique_ptr<symgw_ctx> c(new symgw_ctx());
-    c->prec = precision;
+    c->prec = prec_mode(precision);
+    c->stream = prec_stream(precision);
     c->eb = precision_bytes(precision);
+    c->shard_first = 0;
     std::string err;
     if (!build_layout(desc, c->L, err) ||
-        !build_worklists(c->L, c->eb, c->W, err, precision == SYMGW_PREC_MIXED)) {
+        !build_worklists(c->L, c->eb, c->W, err, c->prec == SYMGW_PREC_MIXED, c->stream)) {
       g_create_error = err;
       return SYMGW_ERR_ARG;
     }
@@ -625,10 +709,11 @@ int symgw_create(const symgw_layout_desc *desc, int device, int precision, symgw
       for (auto &e : c->ev) (void)hipEventCreate(&e);
     }
     const Layout &L = c->L;
+    c->shard_end = L.ink;
     symgw_ctx *cp = c.get();
     int r = 0;
     const size_t eb = c->eb;
-    r = r ? r : dalloc_bytes(cp, &c->V, (size_t)L.flat_V * eb);
+    r = r ? r : dalloc_bytes(cp, &c->V, (size_t)(c->stream ? c->W.batch_V_max : L.flat_V) * eb);
     r = r ? r : dalloc(cp, &c->G_ibz, (size_t)L.nts * L.ns * L.flat_ao);
     r = r ? r : dalloc_bytes(cp, &c->G_full, (size_t)L.nts * L.ns * L.flat_ao_full * eb);
     r = r ? r : dalloc(cp, &c->P, (size_t)L.nts * L.flat_aux);
@@ -649,7 +734,7 @@ int symgw_create(const symgw_layout_desc *desc, int device, int precision, symgw
     r = r ? r : dalloc(cp, &c->d_scratch, (size_t)std::max(1, L.nw) * L.ns * c->W.d_scr_group);
     r = r ? r : dalloc(cp, &c->kw, (size_t)L.ink);
     r = r ? r : dalloc(cp, &c->nel_dev, 1);
-    if (precision == SYMGW_PREC_MIXED) {
+    if (c->prec == SYMGW_PREC_MIXED) {
       r = r ? r : dalloc(cp, &c->V8, (size_t)c->W.v8_bytes);
       r = r ? r : dalloc(cp, &c->P8, (size_t)c->W.p8_bytes);
       r = r ? r : dalloc(cp, &c->VE, (size_t)c->W.ve_count);
@@ -664,7 +749,14 @@ int symgw_create(const symgw_layout_desc *desc, int device, int precision, symgw
     r = r ? r : upload(cp, &c->d_p0_blks, c->W.p0_blks);
     for (int d = 0; d < kNumSigDepths; ++d) r = r ? r : upload(cp, &c->d_sig_items[d], c->W.sig_items[d]);
     r = r ? r : upload(cp, &c->d_sig_blks, c->W.sig_blks);
-    if (precision == SYMGW_PREC_MIXED) {
+    if (c->stream) {
+      std::vector<GenBlk> g0, g1;
+      batch_gen_blocks(L, c->W.p0_batch_pairs, c->W.p0_batch_pair_begin, g0, c->p0_gen_begin);
+      batch_gen_blocks(L, c->W.sig_batch_pairs, c->W.sig_batch_pair_begin, g1, c->sig_gen_begin);
+      r = r ? r : upload(cp, &c->d_p0_gen, g0);
+      r = r ? r : upload(cp, &c->d_sig_gen, g1);
+    }
+    if (c->prec == SYMGW_PREC_MIXED) {
       auto prefix = [](const std::vector<Split8Blk> &v, int64_t &tot) {
         std::vector<int64_t> p(v.size() + 1, 0);
         for (size_t i = 0; i < v.size(); ++i) p[i + 1] = p[i] + (int64_t)v[i].nsub * v[i].nrows;
@@ -798,6 +890,7 @@ int64_t symgw_kpair_offset(const symgw_ctx *ctx, int k1, int k2) {
 int symgw_set_V(symgw_ctx *ctx, const double *V_flat, int64_t n_complex) {
   return guarded(ctx, [&]() -> int {
     if (!ctx || !V_flat) return SYMGW_ERR_ARG;
+    if (ctx->stream) return fail(ctx, SYMGW_ERR_STATE, "a context with a streamed V takes symgw_generate_V only");
     if (n_complex != ctx->L.flat_V) return fail(ctx, SYMGW_ERR_ARG, "V length does not match the layout");
     HIPCHK(ctx, hipSetDevice(ctx->device));
     if (ctx->eb == 16) {
@@ -851,6 +944,20 @@ int symgw_generate_V(symgw_ctx *ctx, uint64_t seed, double scale, const double *
   return guarded(ctx, [&]() -> int {
     if (!ctx || !decay || n_decay <= 0) return SYMGW_ERR_ARG;
     HIPCHK(ctx, hipSetDevice(ctx->device));
+    if (ctx->stream) {
+      // streamed V: keep the generator; every P0 / Sigma batch regenerates its pairs
+      if (ctx->d_decay && ctx->n_decay != n_decay) return fail(ctx, SYMGW_ERR_ARG, "decay table length changed");
+      if (!ctx->d_decay) {
+        int r = dalloc(ctx, &ctx->d_decay, (size_t)n_decay);
+        if (r) return r;
+      }
+      HIPCHK(ctx, copy_sync(ctx, ctx->d_decay, decay, (size_t)n_decay * sizeof(double), hipMemcpyHostToDevice));
+      ctx->n_decay = n_decay;
+      ctx->gen_seed = seed;
+      ctx->gen_scale = scale;
+      ctx->have_V = true;
+      return SYMGW_OK;
+    }
     std::vector<GenBlk> gb;
     gen_blocks(ctx->L, gb, -1);
     GenBlk *dgb = nullptr;
@@ -1286,6 +1393,19 @@ int symgw_gw_iteration(symgw_ctx *ctx, double mu, int use_graph) {
         ev_record(ctx, ph + 1);
       }
     }
+    return SYMGW_OK;
+  });
+}
+
+int symgw_set_shard(symgw_ctx *ctx, int first, int count) {
+  return guarded(ctx, [&]() -> int {
+    if (!ctx) return SYMGW_ERR_ARG;
+    if (first < 0 || count < 1 || first + count > ctx->L.ink)
+      return fail(ctx, SYMGW_ERR_ARG, "shard outside the irreducible points");
+    ctx->shard_first = first;
+    ctx->shard_end = first + count;
+    std::lock_guard<std::mutex> lk(g_dev_mu);
+    drop_graphs(ctx);  // captured sweeps cover the previous shard
     return SYMGW_OK;
   });
 }

@@ -351,8 +351,35 @@ static int p0_band(int qs, int band_cap, int budget, int eb, int TB, const std::
   return best_band;
 }
 
-bool build_worklists(const Layout &L, int eb, WorkLists &W, std::string &err, bool z8) {
+bool build_worklists(const Layout &L, int eb, WorkLists &W, std::string &err, bool z8, bool stream) {
   W = WorkLists();
+  W.stream = stream;
+  if (stream && z8) {
+    err = "the mixed-precision mode needs a resident V";
+    return false;
+  }
+  // streamed V: the batch buffer offset of every pair of a batch (pairs of iBZ
+  // q: (k, k + q) for P0; of iBZ k: (k, k - q) for Sigma), in k / q order
+  std::vector<int64_t> batch_off(stream ? (size_t)L.nk * L.nk : 0, -1);
+  auto open_batch = [&](std::vector<std::pair<int, int64_t>> &pairs, std::vector<int> &begin, int kr, bool p0) {
+    begin.push_back((int)pairs.size());
+    int64_t o = 0;
+    for (int x = 0; x < L.nk; ++x) {
+      const size_t p = p0 ? (size_t)x * L.nk + L.kpq[(size_t)x * L.nk + kr]
+                          : (size_t)kr * L.nk + L.kmq[(size_t)kr * L.nk + x];
+      if (L.V_irreps[p].empty() || batch_off[p] >= 0) continue;  // a pair may recur (k + q == k' + q)
+      batch_off[p] = o;
+      pairs.emplace_back((int)p, o);
+      o += L.kpair_size[p];
+    }
+    W.batch_V_max = std::max(W.batch_V_max, o);
+  };
+  auto close_batch = [&](const std::vector<std::pair<int, int64_t>> &pairs, int from) {
+    for (size_t i = from; i < pairs.size(); ++i) batch_off[pairs[i].first] = -1;
+  };
+  auto v_offset = [&](size_t p, size_t b) {
+    return stream ? batch_off[p] + L.V_offsets[p][b] : L.kpair_off[p] + L.V_offsets[p][b];
+  };
   const int nk = L.nk, nts = L.nts, ns = L.ns;
   const int budget = contract_lds(eb) / eb;  // complex elements of LDS per workgroup
   flop_model(L, W.flop_p0, W.flop_sig, W.n_p0_contractions, W.n_sig_contractions);
@@ -367,6 +394,9 @@ bool build_worklists(const Layout &L, int eb, WorkLists &W, std::string &err, bo
   const int p0_tb = p0_times(eb, W.p0_mode);
   for (int iqr = 0; iqr < L.ink; ++iqr) {
     int q = L.irre_list[iqr];
+    const int pair_from = (int)W.p0_batch_pairs.size();
+    W.p0_batch_items.push_back((int)W.p0_items.size());
+    if (stream) open_batch(W.p0_batch_pairs, W.p0_batch_pair_begin, q, true);
     for (size_t iq = 0; iq < L.aux_sizes[q].size(); ++iq) {
       int qs = L.aux_sizes[q][iq];
       if (qs > 256) { err = fmt("aux block larger than 256", qs); return false; }
@@ -394,7 +424,7 @@ bool build_worklists(const Layout &L, int eb, WorkLists &W, std::string &err, bo
           auto &t = L.V_irreps[p][b];
           if (t[0] != (int)iq) continue;
           P0Blk B;
-          B.v_off = L.kpair_off[p] + L.V_offsets[p][b];
+          B.v_off = v_offset(p, b);
           B.k1 = L.ao_sizes[k][t[1]];
           B.k2 = L.ao_sizes[k2][t[2]];
           B.g1_base = L.kao_off[k] * ns * nts + L.ao_offsets[k][t[1]];
@@ -422,7 +452,10 @@ bool build_worklists(const Layout &L, int eb, WorkLists &W, std::string &err, bo
         W.p0_items.push_back(it);
       }
     }
+    if (stream) close_batch(W.p0_batch_pairs, pair_from);
   }
+  W.p0_batch_items.push_back((int)W.p0_items.size());
+  if (stream) W.p0_batch_pair_begin.push_back((int)W.p0_batch_pairs.size());
   // the split-K epilogue passes partial tiles through LDS: one slot per wave of
   // contract_tiles tiles x 64 lanes x 8 reals
   const size_t red_bytes = (size_t)contract_waves(eb) * contract_tiles(eb) * 64 * 8 * (eb / 2);
@@ -442,6 +475,9 @@ bool build_worklists(const Layout &L, int eb, WorkLists &W, std::string &err, bo
   if (const char *ev = std::getenv("SYMGW_Z8_MIN_Q")) z8_min_q = std::atoi(ev);
   for (int ikr = 0; ikr < L.ink; ++ikr) {
     int k = L.irre_list[ikr];
+    const int pair_from = (int)W.sig_batch_pairs.size();
+    for (int d = 0; d < kNumSigDepths; ++d) W.sig_batch_items[d].push_back((int)W.sig_items[d].size());
+    if (stream) open_batch(W.sig_batch_pairs, W.sig_batch_pair_begin, k, false);
     for (size_t i1 = 0; i1 < L.ao_sizes[k].size(); ++i1) {
       int k1 = L.ao_sizes[k][i1];
       const SigShape sh = sig_shape(eb, k1);
@@ -454,7 +490,7 @@ bool build_worklists(const Layout &L, int eb, WorkLists &W, std::string &err, bo
           auto &t = L.V_irreps[p][b];
           if (t[1] != (int)i1) continue;
           SigBlk B;
-          B.v_off = L.kpair_off[p] + L.V_offsets[p][b];
+          B.v_off = v_offset(p, b);
           B.q = L.aux_sizes[q][t[0]];
           B.k2 = L.ao_sizes[k2i][t[2]];
           B.g2_base = L.kao_off[k2i] * ns * nts + L.ao_offsets[k2i][t[2]];
@@ -526,7 +562,10 @@ bool build_worklists(const Layout &L, int eb, WorkLists &W, std::string &err, bo
           W.sig_items[sig_depth_index(TB)].push_back(it);
         }
     }
+    if (stream) close_batch(W.sig_batch_pairs, pair_from);
   }
+  for (int d = 0; d < kNumSigDepths; ++d) W.sig_batch_items[d].push_back((int)W.sig_items[d].size());
+  if (stream) W.sig_batch_pair_begin.push_back((int)W.sig_batch_pairs.size());
   // the INT8-Z Sigma kernel reduces split-K parts over its own wave / tile shape
   const size_t red_sig = z8 ? (size_t)kZ8Waves * kZ8Tiles * 64 * 8 * (eb / 2) : red_bytes;
   for (int d = 0; d < kNumSigDepths; ++d) W.sig_lds[d] = W.sig_items[d].empty() ? 0 : std::max(sig_lds[d], red_sig);

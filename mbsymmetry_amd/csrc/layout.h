@@ -232,11 +232,24 @@ struct WorkLists {
   // mixed mode: INT8 split tables (V blocks read by Sigma; P blocks of every q)
   std::vector<Split8Blk> v8_split, p8_split;
   int64_t v8_bytes = 0, ve_count = 0, p8_bytes = 0, pe_count = 0;
+  // streamed V (not resident): P0 runs one batch per iBZ q, Sigma one per iBZ
+  // k; each batch's V pairs are generated into one buffer just ahead of its
+  // contraction, and the blocks' v_off are offsets in that buffer.
+  bool stream = false;
+  // items of iBZ q (P0) / iBZ k (Sigma) index b: [prefix[b], prefix[b + 1])
+  // (every mode: they also delimit the shards of symgw_set_shard)
+  std::vector<int> p0_batch_items;                 // [ink + 1] item prefix
+  std::vector<int> sig_batch_items[kNumSigDepths]; // [ink + 1] item prefix per slab depth
+  // pairs of each batch with their offsets in the batch buffer
+  std::vector<std::pair<int, int64_t>> p0_batch_pairs, sig_batch_pairs;
+  std::vector<int> p0_batch_pair_begin, sig_batch_pair_begin;  // [ink + 1]
+  int64_t batch_V_max = 0;                         // complex elements of the largest batch
 };
 
 // elem_bytes: bytes of one complex element in the contraction precision;
 // z8: also plan the INT8 slice copies of the mixed-precision Z product.
-bool build_worklists(const Layout &L, int elem_bytes, WorkLists &W, std::string &err, bool z8 = false);
+bool build_worklists(const Layout &L, int elem_bytes, WorkLists &W, std::string &err, bool z8 = false,
+                     bool stream = false);
 
 // Per-iteration FLOPs of P0 and Sigma, 8*M*N*K per complex GEMM
 // (analysis.cpp:229-255 and :271-298 without the +12MN epilogue).

@@ -459,6 +459,79 @@ def config2(transforms: Optional[Transforms] = None, v_scale: Optional[float] = 
                      v_scale=coulomb_scale(aux) if v_scale is None else v_scale)
 
 
+def config3(transforms: Optional[Transforms] = None, n: int = 4, seed: int = SEED_BASE + 3) -> System:
+    """Uneven-batching stress system: a Gamma-centred 4x4x4 mesh whose 64
+    points are all irreducible (no rotations), each with its own irreps:
+    2..10 orbital and 2..10 auxiliary blocks per point, sizes log-uniform in
+    [1, 256].  V-block rule (k-dependent blocks): for the pair (k1, k2) with
+    momentum transfer q, one block per aux block iq of q, coupling ao block
+    ik1 = iq mod n_ao(k1) of k1 to ik2 = (ik1 + iq) mod n_ao(k2) of k2.
+    q and -q share their aux blocks.  V at the Coulomb scale 1/sqrt(naux(q))
+    of the largest q; beta = 20 on a 6-node Chebyshev grid unless transforms
+    are given."""
+    from .transforms import chebyshev
+    rng = np.random.default_rng(seed)
+    pts, kpq, kmq = cubic_mesh(n)
+    nk = len(pts)
+
+    def blocks():
+        nb = int(rng.integers(2, 11))
+        return [int(v) for v in np.clip(np.round(np.exp(rng.uniform(0.0, np.log(256.0), nb))), 1, 256)]
+    ao = [blocks() for _ in range(nk)]
+    # q and -q share their auxiliary irreps (the Sigma sweep reads a pair with
+    # the aux blocks of -q, as P(-q) = P(q)^* relates them)
+    aux = [None] * nk
+    for q in range(nk):
+        mq = int(kmq[0, q])
+        if aux[q] is None:
+            aux[q] = blocks()
+            aux[mq] = list(aux[q])
+    V_irreps = {}
+    for k1 in range(nk):
+        for q in range(nk):
+            k2 = int(kpq[k1, q])
+            na1, na2 = len(ao[k1]), len(ao[k2])
+            V_irreps[k1 * nk + k2] = [(iq, iq % na1, (iq % na1 + iq) % na2) for iq in range(len(aux[q]))]
+    tr = transforms or chebyshev(6, 20.0)
+    irre = np.arange(nk, dtype=np.int32)
+    F = []
+    for k in range(nk):
+        for b in ao[k]:
+            U = haar_unitary(rng, b)
+            F.append((U @ np.diag(rng.uniform(-1.0, 1.0, b)) @ U.conj().T).ravel())
+    F = np.concatenate(F)[None, :]
+    S = np.concatenate([np.eye(b).ravel() for k in range(nk) for b in ao[k]])[None, :].astype(np.complex128)
+    return System(nk=nk, ink=nk, ns=1, transforms=tr, ao_sizes=ao, aux_sizes=aux, irre_list=irre, index=irre.copy(),
+                  irre_index=irre.copy(), conj_index=irre.copy(), kpq=kpq, kmq=kmq,
+                  orep_irreps=[[] for _ in range(nk)], auxrep_irreps=[[] for _ in range(nk)], V_irreps=V_irreps,
+                  kweights=np.ones(nk), orep=np.zeros(0, np.complex128), auxrep=np.zeros(0, np.complex128),
+                  F=F, S=S, H=np.zeros_like(F), name="config3", v_seed=seed,
+                  v_scale=1.0 / float(np.sqrt(max(sum(a) for a in aux))))
+
+
+def config4_sizes():
+    """nao 256 in 6 blocks log-uniform in [16, 80]; naux 1200 in 6 blocks
+    log-uniform in [150, 250], redrawn until every block is at most 256 (the
+    widest aux block the contraction kernels take)."""
+    rng = np.random.default_rng(SEED_BASE + 4)
+    ao = split_sizes(rng, 256, 6, 16, 80)
+    while True:
+        aux = split_sizes(rng, 1200, 6, 150, 250)
+        if max(aux) <= 256:
+            return ao, aux
+
+
+def config4(transforms: Optional[Transforms] = None, n: int = 8, ink: int = 29) -> System:
+    """8x8x8 mesh reduced to 29 iBZ points (the 35 O_h stars merged as for
+    6x6x6); nao 256 in 6 blocks, naux 1200 in 6 blocks; beta = 200 on the IR
+    grid of Lambda = 1e5 with 144 points (nts = 146) unless transforms are
+    given.  Its V (~5 TB) is streamed (SYMGW_FLAG_STREAM_V)."""
+    from .transforms import ir
+    tr = transforms or ir(200.0, 1e5, 144, 143)
+    ao, aux = config4_sizes()
+    return synthetic("config4", n, ink, ao, aux, 1, tr, SEED_BASE + 4, v_scale=coulomb_scale(aux))
+
+
 def tiny(ns: int = 1, n: int = 2, ink: int = 4, ao=(3, 2), aux=(4, 3), ni: int = 8, beta: float = 10.0,
          seed: int = 7, all_pairs: bool = True) -> System:
     """Small synthetic system for brute-force checks."""
