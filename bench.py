@@ -60,19 +60,28 @@ def setup_ctx(sysm, precision, mu):
     return ctx
 
 
-def run_path(sysm, precision, steps, warmup, mu, G0):
+def run_path(sysm, precision, steps, warmup, mu, G0, keep_P=False):
     """Time `steps` GW iterations replayed from per-phase hipGraphs.
 
     The first iteration from G0 is untimed (it is warmup step 1) and gives the
     Sigma and G compared across precisions.  In the timed region HIP events sit
     between the phase graphs on the context stream, so the per-phase device
-    times are averages over exactly the timed iterations."""
+    times are averages over exactly the timed iterations.  keep_P: also return
+    that iteration's P0 and P (recomputed from G0 after it, outside timing)."""
     ctx = setup_ctx(sysm, precision, mu)
     ctx.upload_G(G0)
     ctx.gw_iteration(mu, use_graph=True)
     ctx.sync()
     sig = ctx.copy_Sigma_to_host(sysm.zeros_ts_kij()) / sysm.nk
     G1 = ctx.download_G(G0.shape)
+    P0 = P = None
+    if keep_P:
+        ctx.upload_G(G0)
+        ctx.solve_P0()
+        P0 = ctx.copy_P0_to_host(sysm.zeros_q_t_QP()) / sysm.nk
+        ctx.solve_W()
+        P = ctx.copy_P0_to_host(sysm.zeros_q_t_QP())
+        ctx.upload_G(G1)
     log("warmup 1 done (iteration from G0)")
     for i in range(1, warmup):
         ctx.gw_iteration(mu, use_graph=True)
@@ -96,7 +105,7 @@ def run_path(sysm, precision, steps, warmup, mu, G0):
     from mbsymmetry_amd import _lib
     from mbsymmetry_amd.transforms import leakage
     return dict(sec_per_iter=dt / steps, phase_ms=phase / steps, step_s=step_s, flop_p0=info.flop_P0,
-                flop_sig=info.flop_Sigma, sigma=sig, G1=G1, leakage=leakage(sysm.transforms, G1),
+                flop_sig=info.flop_Sigma, sigma=sig, G1=G1, P0=P0, P=P, leakage=leakage(sysm.transforms, G1),
                 int8_ops=int8_ops(sysm) if precision == _lib.PREC_MIXED else 0.0)
 
 
@@ -123,103 +132,149 @@ def int8_ops(sysm):
     return tot
 
 
-def cpu_baseline(sysm, G0, budget_s=15.0):
-    """The FP64 CPU oracle (the reference's CPU loop nest restated) timed on a
-    bounded sample of the same workload: P0 for min(cores, ink) iBZ q points
-    (one per thread), the first nkk k points and the first ntau tau points,
-    sized to about `budget_s` seconds; scaled to GW iterations/s with the FLOP
-    model."""
+def cpu_baseline(sysm, G0, P0, P, Sigma, max_workers=16):
+    """The reference's CPU path restated (the oracle), timed on the host on a
+    sample of the same iteration:
+      * P0 and Sigma: one output slice each (iBZ point 0, tau 0), every block
+        product a zgemm of numpy's OpenBLAS (oracle/blas.py); on 1 thread over
+        the first nk/8 k (q) points of the slice's sum, and over the whole sum
+        split across `workers` single-thread processes (the host's cores, at
+        most max_workers; time = the slowest process's);
+      * W ((1 - P0)^-1 P0 per bosonic frequency, with the tau <-> i Omega
+        transforms), Dyson and the rotation of G to the full zone: whole, in
+        the oracle's C loops, one thread.
+    A GW iteration is then flop_P0 / rate_P0 + flop_Sigma / rate_Sigma + W +
+    Dyson + rotation.  The whole-sum slices are also compared with the device's
+    P0 and Sigma of the same iteration (the full-size parity check)."""
+    from mbsymmetry_amd import _lib
+    from oracle import blas as B
     from oracle.oracle import Oracle
-    cores = len(os.sched_getaffinity(0))
+    nproc = os.cpu_count()
+    workers = max(1, min(len(os.sched_getaffinity(0)), max_workers))
     orc = Oracle(sysm)
-    nk = sysm.nk
-
-    def flops(q, nkk):
-        f = 0.0
-        for k in range(nkk):
-            p = k * nk + int(sysm.kpq[k, q])
-            for (qs, a1, a2) in sysm.V_block_sizes(p):
-                f += sysm.ns * 8.0 * (a2 * qs * a1 * a1 + qs * a1 * a2 * a2 + qs * qs * a1 * a2)
-        return f
-
-    nq = min(cores, sysm.ink)
-    qs_ = [int(sysm.irre_list[i]) for i in range(nq)]
-    # at most ~256 k-pairs of V are generated on the host (V of config 2 is 8 MiB per pair)
-    nkk = min(nk, max(1, 256 // nq))
-    pairs = {k * nk + int(sysm.kpq[k, q]) for q in qs_ for k in range(nkk)}
-    V = sysm.host_V(pairs=pairs)
+    T = B.Tables(sysm)
+    nk, nts = sysm.nk, sysm.nts
+    q = int(sysm.irre_list[0])
+    k = q
+    p0_pairs = [kk * nk + int(sysm.kpq[kk, q]) for kk in range(nk)]
+    sg_pairs = [k * nk + int(sysm.kmq[k, qq]) for qq in range(nk)]
+    V, off = sysm.host_V_pairs(p0_pairs + sg_pairs)
+    log(f"cpu baseline: host V of {len(set(p0_pairs + sg_pairs))} k-pairs ready ({V.nbytes / 2**30:.2f} GiB)")
+    t0 = time.perf_counter()
     Gf = orc.rotate_G(G0)
-    log(f"cpu baseline: host V of {len(pairs)} k-pairs ready")
-    # calibrate on one k of the first q (one thread), then size the sample
+    t_rot = time.perf_counter() - t0
+    Pt = orc.rotate_P_t(P, 0)
+    sub = list(range(max(1, nk // 8)))
+    res = {}
+    for nthr, sel in ((1, sub), (workers, None)):
+        tm = {}
+        a = B.p0_at(T, V, off, Gf, 0, 0, ks=sel, workers=nthr, timing=tm)
+        tp = tm["seconds"]
+        fp = B.flops_p0_at(T, 0, sel)
+        log(f"cpu baseline: P0 slice ({len(sel) if sel else nk} k) on {nthr} threads: {tp:.2f} s, "
+            f"{fp / tp / 1e9:.2f} GFLOP/s")
+        b = B.sigma_at(T, V, off, Gf, Pt, 0, 0, 0, qs_sel=sel, workers=nthr, timing=tm)
+        ts = tm["seconds"]
+        fs = B.flops_sigma_at(T, 0, sel)
+        log(f"cpu baseline: Sigma slice ({len(sel) if sel else nk} q) on {nthr} threads: {ts:.2f} s, "
+            f"{fs / ts / 1e9:.2f} GFLOP/s")
+        res[nthr] = dict(rate_p0=fp / tp, rate_sig=fs / ts, seconds=tp + ts, p0=a, sig=b)
     t0 = time.perf_counter()
-    orc.P0_sample(V, Gf, 0, 1, 1, 1, nthreads=1)
-    rate1 = flops(qs_[0], 1) / max(time.perf_counter() - t0, 1e-6)
-    per_tau = max(flops(q, nkk) for q in qs_) / rate1      # seconds for nkk k, one tau, per thread
-    if per_tau >= budget_s:
-        nkk, ntau = max(1, int(nkk * budget_s / per_tau)), 1
-    else:
-        ntau = int(min(sysm.nts // 2, max(1, budget_s // per_tau)))
-    log(f"cpu baseline: timing {nq} q x {nkk} k x {ntau} tau on {nq} threads")
+    orc.W(P0)
+    t_w = time.perf_counter() - t0
     t0 = time.perf_counter()
-    orc.P0_sample(V, Gf, 0, ntau, nq, nkk, nthreads=nq)
-    dt = time.perf_counter() - t0
-    fl = ntau * sum(flops(q, nkk) for q in qs_)
-    return dict(rate=fl / dt, seconds=dt, cores=nq, nkk=nkk, nq=nq, ntau=ntau, flops=fl)
+    orc.dyson(Sigma, 0.0)
+    t_d = time.perf_counter() - t0
+    log(f"cpu baseline: W {t_w:.2f} s, Dyson {t_d:.2f} s, G rotation {t_rot:.2f} s (1 thread)")
+    info = _lib.layout_query(sysm)
+    full = res[workers]
+    qs = sysm.qslice(q)
+    ks = sysm.kslice(k)
+    d_p0 = float(np.abs(full["p0"] - P0[0:qs]).max())
+    d_sig = float(np.abs(full["sig"] - Sigma[0, 0, 0:ks]).max())
+    out = {}
+    for nthr, r in res.items():
+        it = info.flop_P0 / r["rate_p0"] + info.flop_Sigma / r["rate_sig"] + t_w + t_d + t_rot
+        out[nthr] = dict(iter_s=it, gflops_p0=r["rate_p0"] / 1e9, gflops_sigma=r["rate_sig"] / 1e9,
+                         timed_s=r["seconds"])
+    return dict(by_threads=out, workers=workers, nproc=nproc, W_s=t_w, dyson_s=t_d, rot_s=t_rot,
+                gpu_vs_cpu_max_abs_dP0=d_p0, gpu_vs_cpu_max_abs_dSigma=d_sig,
+                max_abs_P0=float(np.abs(full["p0"]).max()), max_abs_Sigma=float(np.abs(full["sig"]).max()),
+                sample_k=len(sub))
 
 
-def uneven_run():
-    """Irrep blocks of every size from 1 to 256 in one system (the uneven-batching
-    stress case at a size that fits one test): P0 and Sigma kernel rates from
-    HIP events, after one untimed pass, in both precisions."""
+def config3_run():
+    """The uneven-batching system (64 irreducible points, each with its own
+    2..10 irreps of sizes 1..256; the whole sweep of each contraction in one
+    launch per slab depth): P0 and Sigma kernel time and rate from HIP events
+    after one untimed pass, in the three precision modes, with max|dSigma| of
+    FP32 and mixed against FP64."""
     from mbsymmetry_amd import _lib, system as S
-    from mbsymmetry_amd.transforms import chebyshev
-    ao, aux = [1, 3, 9, 27, 81, 256], [1, 4, 16, 64, 256]
-    sysm = S.synthetic("uneven-1-256", 2, 4, ao, aux, 1, chebyshev(20, 20.0), S.SEED_BASE + 3)
-    out = {"workload": f"2x2x2 k-mesh (4 iBZ points), ao blocks {ao}, aux blocks {aux}, nts {sysm.nts}"}
-    for prec, name in ((_lib.PREC_FP64, "fp64"), (_lib.PREC_FP32, "fp32")):
+    sysm = S.config3()
+    nb = [len(b) for b in sysm.ao_sizes]
+    out = {"workload": (f"config3: 4x4x4 mesh, all 64 points irreducible, 2..10 irreps per point "
+                        f"(ao blocks {min(min(b) for b in sysm.ao_sizes)}..{max(max(b) for b in sysm.ao_sizes)}, "
+                        f"mean {np.mean(nb):.1f} per point), nts {sysm.nts}")}
+    sig = {}
+    for prec, name in ((_lib.PREC_FP64, "fp64"), (_lib.PREC_FP32, "fp32"), (_lib.PREC_MIXED, "mixed")):
         ctx = setup_ctx(sysm, prec, 0.0)
         ctx.upload_G(sysm.zeros_ts_kij())
         ctx.dyson(0.0, 0.0)
+        G0 = ctx.download_G((sysm.nts, sysm.ns, sysm.flat_ao))
         for timed in (False, True):
+            ctx.upload_G(G0)
             ctx.set_timing(timed)
             ctx.solve_P0(); ctx.solve_W(); ctx.solve_Sigma()
             ctx.sync()
         lt, inf = ctx.last_times(), ctx.info()
+        sig[name] = ctx.copy_Sigma_to_host(sysm.zeros_ts_kij()) / sysm.nk
         ctx.close()
         out[name] = {"P0_ms": float(lt[0]), "Sigma_ms": float(lt[2]), "P0_tflops": float(inf.flop_P0 / lt[0] / 1e9),
                      "Sigma_tflops": float(inf.flop_Sigma / lt[2] / 1e9)}
-        log("uneven", name, out[name])
+        if name != "fp64":
+            out[name]["max_abs_dSigma_vs_fp64"] = float(np.abs(sig[name] - sig["fp64"]).max())
+        log("config3", name, out[name])
+    out["max_abs_Sigma"] = float(np.abs(sig["fp64"]).max())
+    out["flop_P0"], out["flop_Sigma"] = float(inf.flop_P0), float(inf.flop_Sigma)
     return out
 
 
-def _ints(text):
-    return [int(x) for x in text.split(",") if x]
-
-
-def serving_run(n_requests, rate, with_fp32, naos=(26, 52, 104), meshes=(2, 4)):
-    """The serving loop on a seeded request mix: Poisson arrivals, HBM-footprint
-    admission, continuous batching over per-system contexts; latency from
-    arrival to the final G on the host."""
+def serving_run(window_s, rate, deadline_s, precisions=("fp64", "mixed"), n_systems=32):
+    """The serving loop on the 32-system mix (nao in {26, 52, 104, 156, 256},
+    meshes 2^3 / 4^3 / 6^3): systems resident on the device (loaded before the
+    clock; V streamed for the largest), GW-iteration requests for uniformly
+    drawn systems arriving as a Poisson stream for `window_s` seconds, admitted
+    by HBM footprint, continuously batched over up to 8 concurrent systems;
+    no request starts that would end after `deadline_s` by the FLOP estimate.
+    Latency = arrival -> new G on the device."""
     from mbsymmetry_amd import _lib, serving as SV
     from mbsymmetry_amd import system as S
-    mix = S.serving_mix(n_requests, meshes=tuple(meshes), naos=tuple(naos))
-    arr = SV.poisson_arrivals(len(mix), rate, seed=S.SEED_BASE + 5)
-    reqs = [SV.Request(i, s, float(arr[i]), iterations=it) for i, (s, it) in enumerate(mix)]
-    flops = sum(it * (_lib.layout_query(s).flop_P0 + _lib.layout_query(s).flop_Sigma) for s, it in mix)
-    out = {"requests": len(reqs), "arrival_rate_per_s": rate,
-           "mix": f"synthetic systems on {' / '.join(f'{m}^3' for m in meshes)} meshes, nao in "
-                  f"{{{', '.join(map(str, naos))}}} (naux = 4 nao), nts 32, 2 GW iterations each",
-           "P0_Sigma_TFLOP": flops / 1e12}
-    for name, prec in (("fp64", _lib.PREC_FP64), ("fp32", _lib.PREC_FP32)):
-        if name == "fp32" and not with_fp32:
-            continue
-        r = SV.serve(reqs, device=0, precision=prec, max_active=8)
+    t0 = time.perf_counter()
+    mix = S.config5(n_systems)
+    log(f"serving: {len(mix)} systems built in {time.perf_counter() - t0:.1f} s")
+    rng = np.random.default_rng(S.SEED_BASE + 55)
+    arr = SV.poisson_arrivals(int(rate * window_s * 3) + 16, rate, seed=S.SEED_BASE + 5)
+    arr = arr[arr <= window_s]
+    reqs = [SV.IterRequest(i, int(rng.integers(0, len(mix))), float(a)) for i, a in enumerate(arr)]
+    sizes = {}
+    for s_ in mix:
+        sizes[SV._key(s_)] = sizes.get(SV._key(s_), 0) + 1
+    out = {"systems": len(mix), "system_sizes": dict(sorted(sizes.items())), "requests": len(reqs),
+           "arrival_rate_per_s": rate, "arrival_window_s": window_s, "deadline_s": deadline_s,
+           "mix": "config5: nao uniform in {26,52,104,156,256} (naux = 4 nao, <= 8 irreps), mesh uniform in "
+                  "{2^3,4^3,6^3}, IR grid Lambda=1e3 (nts 50), beta 50; one GW iteration per request"}
+    modes = {"fp64": _lib.PREC_FP64, "mixed": _lib.PREC_MIXED, "fp32": _lib.PREC_FP32}
+    for name in precisions:
+        r = SV.serve_resident(mix, reqs, device=0, precision=modes[name], max_active=8, time_limit=window_s,
+                              deadline=deadline_s)
         sm = r["summary"]
-        sm["peak_active"] = r["peak_active"]
-        sm["peak_footprint_GiB"] = r["peak_bytes"] / 2 ** 30
+        sm.update(peak_active=r["peak_active"], peak_resident_GiB=r["peak_bytes"] / 2 ** 30,
+                  budget_GiB=r["budget"] / 2 ** 30, setup_s=r["setup_s"], loads=r["loads"],
+                  evictions=r["evictions"], systems_with_streamed_V=r["streamed"])
         out[name] = sm
-        log(f"serving {name}: p50 {sm['p50_latency_s']:.3f} s  p99 {sm['p99_latency_s']:.3f} s  "
-            f"{sm['gw_iter_per_s']:.2f} iter/s")
+        log(f"serving {name}: {sm['completed']}/{sm['requests']} done, p50 {sm['p50_latency_s']:.3f} s, "
+            f"p99 {sm['p99_latency_s']:.3f} s, {sm['gw_iter_per_s']:.2f} iter/s, "
+            f"{sm['P0_Sigma_tflops']:.1f} TFLOP/s, unserved {sm['unserved']}, failed {sm['failed']}")
     return out
 
 
@@ -234,15 +289,19 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mu", type=float, default=0.0)
     ap.add_argument("--no-serving", action="store_true")
-    ap.add_argument("--no-uneven", action="store_true", help="skip the blocks-1-to-256 stress case")
-    ap.add_argument("--serving-naos", default="26,52,104", help="nao values of the serving mix")
-    ap.add_argument("--serving-meshes", default="2,4", help="k-mesh edges of the serving mix")
+    ap.add_argument("--no-config3", action="store_true", help="skip the uneven-block (config 3) leg")
     ap.add_argument("--serving-only", action="store_true", help="run only the serving mix and print its JSON")
-    ap.add_argument("--serving-requests", type=int, default=32)
+    ap.add_argument("--serving-window", type=float, default=2.0, help="seconds of Poisson arrivals")
+    ap.add_argument("--serving-deadline", type=float, default=30.0,
+                    help="no serving request starts that is estimated to end later (s)")
     ap.add_argument("--serving-rate", type=float, default=50.0, help="Poisson arrival rate (requests/s)")
+    ap.add_argument("--serving-modes", default="fp64,mixed", help="precision modes of the serving leg")
     ap.add_argument("--copy-calib", action="store_true",
                     help="run only a 4 GiB device-to-device copy kernel (scale reference for FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
+    # one hardware queue per serving slot (8), so that a short request's kernels
+    # never queue behind a long request's on a shared queue; set before HIP starts
+    os.environ["GPU_MAX_HW_QUEUES"] = "8"
     if args.copy_calib:
         import torch
         n = 4 << 30
@@ -260,8 +319,8 @@ def main():
     from mbsymmetry_amd import _lib
 
     if args.serving_only:
-        print(json.dumps({"serving": serving_run(args.serving_requests, args.serving_rate, not args.no_mixed,
-                                                 _ints(args.serving_naos), _ints(args.serving_meshes))}),
+        print(json.dumps({"serving": serving_run(args.serving_window, args.serving_rate, args.serving_deadline,
+                                                 args.serving_modes.split(","))}),
               flush=True)
         return
     sysm = make_system(args.config)
@@ -276,7 +335,7 @@ def main():
     ctx.close()
     log("G0 ready")
 
-    plain = run_path(sysm, _lib.PREC_FP64, args.steps, args.warmup, args.mu, G0)
+    plain = run_path(sysm, _lib.PREC_FP64, args.steps, args.warmup, args.mu, G0, keep_P=not args.no_cpu)
     log("plain fp64", plain["sec_per_iter"], "s/iter; phases ms", plain["phase_ms"][:4])
     mixed = fp32 = None
     if not args.no_mixed:
@@ -360,18 +419,31 @@ def main():
     if fp32 is not None:
         out["fp32"] = compare(fp32, "every contraction product in FP32 (3M), FP32 accumulation")
         out["fp32"]["frac_fp32_peak"] = {k: v / PEAK["fp32"] for k, v in out["fp32"]["kernel_tflops"].items()}
-    if not args.no_uneven:
-        out["uneven_blocks"] = uneven_run()
+    if not args.no_config3:
+        out["config3"] = config3_run()
     if not args.no_serving:
-        out["serving"] = serving_run(args.serving_requests, args.serving_rate, not args.no_mixed,
-                                     _ints(args.serving_naos), _ints(args.serving_meshes))
+        out["serving"] = serving_run(args.serving_window, args.serving_rate, args.serving_deadline,
+                                     args.serving_modes.split(","))
     if not args.no_cpu:
-        cb = cpu_baseline(sysm, G0)
-        out["cpu_baseline"] = {"value": cb["rate"] / flops_iter, "unit": "GW iterations/s", "cores": cb["cores"],
-                               "kind": "port",
-                               "sample": f"FP64 oracle P0 loop nest (gw_block.cpp:6-84 restated), {cb['nq']} iBZ q x "
-                                         f"{cb['nkk']} k x {cb['ntau']} tau, {cb['flops'] / 1e9:.1f} GFLOP in {cb['seconds']:.1f} s "
-                                         f"({cb['rate'] / 1e9:.2f} GFLOP/s), scaled by the FLOP model"}
+        cb = cpu_baseline(sysm, G0, plain["P0"], plain["P"], plain["sigma"])
+        w, one = cb["workers"], cb["by_threads"][1]
+        many = cb["by_threads"][w]
+        out["cpu_baseline"] = {
+            "value": 1.0 / many["iter_s"], "unit": "GW iterations/s", "cores": w, "kind": "port",
+            "nproc": cb["nproc"], "value_1_thread": 1.0 / one["iter_s"],
+            "sample": (f"the oracle (gw_block.cpp loop nest restated, block products on numpy's OpenBLAS zgemm) on "
+                       f"one P0 slice (q_ir 0, tau 0) and one Sigma slice (k_ir 0, tau 0): over the whole "
+                       f"k / q sum on {w} threads ({many['timed_s']:.1f} s; {many['gflops_p0']:.1f} / "
+                       f"{many['gflops_sigma']:.1f} GFLOP/s), over {cb['sample_k']} of {sysm.nk} points on 1 thread "
+                       f"({one['timed_s']:.1f} s; {one['gflops_p0']:.2f} / {one['gflops_sigma']:.2f} GFLOP/s); "
+                       f"W {cb['W_s']:.2f} s, Dyson {cb['dyson_s']:.2f} s and G rotation {cb['rot_s']:.2f} s whole "
+                       f"(C loops, 1 thread); an iteration = flop_P0 / rate_P0 + flop_Sigma / rate_Sigma + "
+                       f"W + Dyson + rotation"),
+            "iteration_s": {"1_thread": one["iter_s"], f"{w}_threads": many["iter_s"]},
+            "gpu_vs_cpu_full_size_slices": {"max_abs_dP0": cb["gpu_vs_cpu_max_abs_dP0"],
+                                            "max_abs_P0": cb["max_abs_P0"],
+                                            "max_abs_dSigma": cb["gpu_vs_cpu_max_abs_dSigma"],
+                                            "max_abs_Sigma": cb["max_abs_Sigma"]}}
     print(json.dumps(out), flush=True)
 
 

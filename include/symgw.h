@@ -50,7 +50,9 @@ extern "C" {
  * contraction: the pairs (k, k + q) of one iBZ q for P0, (k, k - q) of one
  * iBZ k for Sigma, into a buffer of the largest batch.  Values are those of
  * the resident V bit for bit; the device needs the largest batch instead of
- * the whole V (for systems whose V exceeds HBM).  Not with SYMGW_PREC_MIXED. */
+ * the whole V (for systems whose V exceeds HBM).  With SYMGW_PREC_MIXED the
+ * INT8 slices of each Sigma batch's V are made right after it is generated,
+ * into a buffer of the largest batch's slices. */
 #define SYMGW_FLAG_STREAM_V 0x100
 
 /*

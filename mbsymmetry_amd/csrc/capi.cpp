@@ -75,6 +75,7 @@ struct symgw_ctx {
   Split8Blk *d_v8_split = nullptr, *d_p8_split = nullptr;
   int64_t *d_v8_prefix = nullptr, *d_p8_prefix = nullptr;
   int64_t v8_rows = 0, p8_rows = 0;
+  std::vector<int64_t> v8_batch_rows;  // streamed V: split rows of each Sigma batch
 
   // device tables
   P0Item *d_p0_items = nullptr;
@@ -165,15 +166,45 @@ void dev_free(void *p) {
   (void)hipFree(p);
 }
 
+// Context buffers come from the device's stream-ordered pool, allocated and
+// freed on the context's own stream: hipFree waits for the whole device, so a
+// context set up or released while others run (serving) would stall them all;
+// hipFreeAsync waits for nothing but its stream.  The pool is trimmed after a
+// release so the memory goes back to the device.
+hipError_t ctx_malloc(symgw_ctx *c, void **p, size_t bytes) { return hipMallocAsync(p, bytes, c->st); }
+
+void ctx_free(symgw_ctx *c, void *p) { (void)hipFreeAsync(p, c->st); }
+
+void trim_pool(int device) {
+  hipMemPool_t pool = nullptr;
+  if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess && pool) (void)hipMemPoolTrimTo(pool, 0);
+}
+
+void drop_graphs(symgw_ctx *c);
+
+// Release every buffer of a context, its graphs, events and stream.
+void release_ctx(symgw_ctx *c) {
+  for (auto &b : c->owned) ctx_free(c, b.p);
+  c->owned.clear();
+  (void)hipStreamSynchronize(c->st);
+  {
+    std::lock_guard<std::mutex> lk(g_dev_mu);
+    drop_graphs(c);
+    for (auto &e : c->ev) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(c->st);
+  }
+  trim_pool(c->device);
+}
+
 template <typename X>
 int dalloc(symgw_ctx *c, X **p, size_t count) {
   *p = nullptr;
   if (count == 0) count = 1;
   void *q = nullptr;
-  hipError_t e = dev_malloc(&q, count * sizeof(X));
+  hipError_t e = ctx_malloc(c, &q, count * sizeof(X));
   if (e != hipSuccess) {
     (void)hipGetLastError();
-    return fail(c, SYMGW_ERR_NOMEM, "hipMalloc of " + std::to_string(count * sizeof(X)) + " bytes failed");
+    return fail(c, SYMGW_ERR_NOMEM, "hipMallocAsync of " + std::to_string(count * sizeof(X)) + " bytes failed");
   }
   c->owned.push_back(DevBuf{q, count * sizeof(X)});
   *p = static_cast<X *>(q);
@@ -256,7 +287,7 @@ hipError_t gen_batch(symgw_ctx *c, const GenBlk *d_gen, const std::vector<int> &
 
 // INT8 slices of V (after it is set) and of P_full (after each rotation).
 int split_V8(symgw_ctx *c) {
-  if (c->prec != SYMGW_PREC_MIXED) return SYMGW_OK;
+  if (c->prec != SYMGW_PREC_MIXED || c->stream) return SYMGW_OK;  // streamed: split per batch
   hipError_t e = launch_split8(c->d_v8_split, (int)c->W.v8_split.size(), c->d_v8_prefix, c->v8_rows,
                                static_cast<const cplx<double> *>(c->V), c->V8, c->VE, c->st);
   if (e != hipSuccess) return fail(c, SYMGW_ERR_HIP, std::string("V split: ") + hipGetErrorString(e));
@@ -307,11 +338,15 @@ int run_p0(symgw_ctx *c) {
                      L.ns, static_cast<const cplx<T> *>(c->V), static_cast<const cplx<T> *>(c->G_full), c->P, pref,
                      c->W.p0_lds, c->st, c->W.p0_mode);
   } else {
-    // one batch per iBZ q: its V pairs generated, then its items contracted
-    for (int bt = c->shard_first; bt < c->shard_end && e == hipSuccess; ++bt) {
-      e = gen_batch<T>(c, c->d_p0_gen, c->p0_gen_begin, bt);
+    // one batch per group of iBZ q: its V pairs generated, then the items of
+    // its points inside the shard contracted in one launch
+    const auto &gf = c->W.p0_group_first;
+    for (size_t g = 0; g + 1 < gf.size() && e == hipSuccess; ++g) {
+      const int a = std::max(gf[g], c->shard_first), b = std::min(gf[g + 1], c->shard_end);
+      if (a >= b) continue;
+      e = gen_batch<T>(c, c->d_p0_gen, c->p0_gen_begin, (int)g);
       if (e == hipSuccess)
-        e = launch_p0<T>(c->d_p0_items + ib[bt], c->d_p0_blks, ib[bt + 1] - ib[bt], L.nts, L.ns,
+        e = launch_p0<T>(c->d_p0_items + ib[a], c->d_p0_blks, ib[b] - ib[a], L.nts, L.ns,
                          static_cast<const cplx<T> *>(c->V), static_cast<const cplx<T> *>(c->G_full), c->P, pref,
                          c->W.p0_lds, c->st, c->W.p0_mode);
     }
@@ -340,12 +375,22 @@ int run_sigma(symgw_ctx *c) {
                           kSigDepths[d], pz8);
     }
   } else {
-    // one batch per iBZ k: its V pairs generated, then its items of every slab depth
-    for (int bt = c->shard_first; bt < c->shard_end && e == hipSuccess; ++bt) {
-      e = gen_batch<T>(c, c->d_sig_gen, c->sig_gen_begin, bt);
+    // one batch per group of iBZ k: its V pairs generated (and split, in the
+    // mixed mode), then the items of its points inside the shard, per slab depth
+    const auto &gf = c->W.sig_group_first;
+    for (size_t g = 0; g + 1 < gf.size() && e == hipSuccess; ++g) {
+      const int a = std::max(gf[g], c->shard_first), b = std::min(gf[g + 1], c->shard_end);
+      if (a >= b) continue;
+      e = gen_batch<T>(c, c->d_sig_gen, c->sig_gen_begin, (int)g);
+      if (e == hipSuccess && pz8) {
+        const int i0 = c->W.v8_batch_begin[g];
+        e = launch_split8(c->d_v8_split + i0, c->W.v8_batch_begin[g + 1] - i0, c->d_v8_prefix + i0,
+                          c->v8_batch_rows[g], static_cast<const cplx<double> *>(c->V), c->V8, c->VE, c->st);
+      }
       for (int d = 0; d < kNumSigDepths && e == hipSuccess; ++d) {
         const auto &ib = c->W.sig_batch_items[d];
-        e = launch_sigma<T>(c->d_sig_items[d] + ib[bt], c->d_sig_blks, ib[bt + 1] - ib[bt], L.nts, L.ns,
+        if (ib[b] == ib[a]) continue;
+        e = launch_sigma<T>(c->d_sig_items[d] + ib[a], c->d_sig_blks, ib[b] - ib[a], L.nts, L.ns,
                             static_cast<const cplx<T> *>(c->V), static_cast<const cplx<T> *>(c->G_full),
                             static_cast<const cplx<T> *>(c->P_full), c->Sig, -1.0, c->W.sig_lds[d], c->st,
                             kSigDepths[d], pz8);
@@ -765,7 +810,24 @@ int symgw_create(const symgw_layout_desc *desc, int device, int precision, symgw
       };
       r = r ? r : upload(cp, &c->d_v8_split, c->W.v8_split);
       r = r ? r : upload(cp, &c->d_p8_split, c->W.p8_split);
-      r = r ? r : upload(cp, &c->d_v8_prefix, prefix(c->W.v8_split, c->v8_rows));
+      if (!c->stream) {
+        r = r ? r : upload(cp, &c->d_v8_prefix, prefix(c->W.v8_split, c->v8_rows));
+      } else {
+        // row prefixes restarted at every batch, so each batch's split launch
+        // reads its own slice of the table
+        const auto &vb = c->W.v8_batch_begin;
+        std::vector<int64_t> pre(c->W.v8_split.size() + 1, 0);
+        c->v8_batch_rows.assign(vb.empty() ? 0 : vb.size() - 1, 0);
+        for (size_t bt = 0; bt + 1 < vb.size(); ++bt) {
+          int64_t acc = 0;
+          for (int i = vb[bt]; i < vb[bt + 1]; ++i) {
+            pre[i] = acc;
+            acc += (int64_t)c->W.v8_split[i].nsub * c->W.v8_split[i].nrows;
+          }
+          c->v8_batch_rows[bt] = acc;
+        }
+        r = r ? r : upload(cp, &c->d_v8_prefix, pre);
+      }
       r = r ? r : upload(cp, &c->d_p8_prefix, prefix(c->W.p8_split, c->p8_rows));
     }
     r = r ? r : upload(cp, &c->d_g_rot, c->W.g_rot);
@@ -833,10 +895,7 @@ int symgw_create(const symgw_layout_desc *desc, int device, int precision, symgw
     }
     if (r) {
       g_create_error = c->err;
-      std::lock_guard<std::mutex> lk(g_dev_mu);
-      for (auto &b : c->owned) (void)hipFree(b.p);
-      for (auto &e : c->ev) (void)hipEventDestroy(e);
-      (void)hipStreamDestroy(c->st);
+      release_ctx(c.get());
       return r;
     }
     *out = c.release();
@@ -849,13 +908,7 @@ int symgw_destroy(symgw_ctx *ctx) {
     if (!ctx) return SYMGW_OK;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->st);
-    {
-      std::lock_guard<std::mutex> lk(g_dev_mu);
-      drop_graphs(ctx);
-      for (auto &b : ctx->owned) (void)hipFree(b.p);
-      for (auto &e : ctx->ev) (void)hipEventDestroy(e);
-      (void)hipStreamDestroy(ctx->st);
-    }
+    release_ctx(ctx);
     delete ctx;
     return SYMGW_OK;
   });
@@ -899,18 +952,18 @@ int symgw_set_V(symgw_ctx *ctx, const double *V_flat, int64_t n_complex) {
       // staged conversion in 64 MiB chunks
       const int64_t chunk = 4 << 20;
       cplx<double> *stage = nullptr;
-      HIPCHK(ctx, dev_malloc((void **)&stage, (size_t)chunk * 16));
+      HIPCHK(ctx, ctx_malloc(ctx, (void **)&stage, (size_t)chunk * 16));
       for (int64_t o = 0; o < n_complex; o += chunk) {
         int64_t n = std::min(chunk, n_complex - o);
         hipError_t e = copy_sync(ctx, stage, V_flat + 2 * o, (size_t)n * 16, hipMemcpyHostToDevice);
         if (e == hipSuccess) e = launch_convert<float>(stage, static_cast<cplx<float> *>(ctx->V) + o, n, ctx->st);
         if (e == hipSuccess) e = hipStreamSynchronize(ctx->st);
         if (e != hipSuccess) {
-          dev_free(stage);
+          ctx_free(ctx, stage);
           return fail(ctx, SYMGW_ERR_HIP, std::string("V upload: ") + hipGetErrorString(e));
         }
       }
-      dev_free(stage);
+      ctx_free(ctx, stage);
     }
     int r = split_V8(ctx);
     if (r) return r;
@@ -962,10 +1015,10 @@ int symgw_generate_V(symgw_ctx *ctx, uint64_t seed, double scale, const double *
     gen_blocks(ctx->L, gb, -1);
     GenBlk *dgb = nullptr;
     double *ddec = nullptr;
-    HIPCHK(ctx, dev_malloc((void **)&dgb, std::max<size_t>(1, gb.size()) * sizeof(GenBlk)));
-    hipError_t e0 = dev_malloc((void **)&ddec, (size_t)n_decay * sizeof(double));
+    HIPCHK(ctx, ctx_malloc(ctx, (void **)&dgb, std::max<size_t>(1, gb.size()) * sizeof(GenBlk)));
+    hipError_t e0 = ctx_malloc(ctx, (void **)&ddec, (size_t)n_decay * sizeof(double));
     if (e0 != hipSuccess) {
-      dev_free(dgb);
+      ctx_free(ctx, dgb);
       return fail(ctx, SYMGW_ERR_NOMEM, "V generation: allocation failed");
     }
     hipError_t e = copy_sync(ctx, dgb, gb.data(), gb.size() * sizeof(GenBlk), hipMemcpyHostToDevice);
@@ -978,8 +1031,8 @@ int symgw_generate_V(symgw_ctx *ctx, uint64_t seed, double scale, const double *
                                    ctx->st);
     }
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->st);
-    dev_free(dgb);
-    dev_free(ddec);
+    ctx_free(ctx, dgb);
+    ctx_free(ctx, ddec);
     if (e != hipSuccess) return fail(ctx, SYMGW_ERR_HIP, std::string("V generation: ") + hipGetErrorString(e));
     int r = split_V8(ctx);
     if (r) return r;

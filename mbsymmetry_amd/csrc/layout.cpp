@@ -354,28 +354,59 @@ static int p0_band(int qs, int band_cap, int budget, int eb, int TB, const std::
 bool build_worklists(const Layout &L, int eb, WorkLists &W, std::string &err, bool z8, bool stream) {
   W = WorkLists();
   W.stream = stream;
-  if (stream && z8) {
-    err = "the mixed-precision mode needs a resident V";
-    return false;
-  }
   // streamed V: the batch buffer offset of every pair of a batch (pairs of iBZ
   // q: (k, k + q) for P0; of iBZ k: (k, k - q) for Sigma), in k / q order
   std::vector<int64_t> batch_off(stream ? (size_t)L.nk * L.nk : 0, -1);
-  auto open_batch = [&](std::vector<std::pair<int, int64_t>> &pairs, std::vector<int> &begin, int kr, bool p0) {
-    begin.push_back((int)pairs.size());
-    int64_t o = 0;
-    for (int x = 0; x < L.nk; ++x) {
-      const size_t p = p0 ? (size_t)x * L.nk + L.kpq[(size_t)x * L.nk + kr]
-                          : (size_t)kr * L.nk + L.kmq[(size_t)kr * L.nk + x];
-      if (L.V_irreps[p].empty() || batch_off[p] >= 0) continue;  // a pair may recur (k + q == k' + q)
-      batch_off[p] = o;
-      pairs.emplace_back((int)p, o);
-      o += L.kpair_size[p];
+  int64_t batch_fill = 0;  // complex elements of the open batch
+  auto pair_of = [&](int kr, int x, bool p0) {
+    return p0 ? (size_t)x * L.nk + L.kpq[(size_t)x * L.nk + kr] : (size_t)kr * L.nk + L.kmq[(size_t)kr * L.nk + x];
+  };
+  // consecutive iBZ points per batch while their V stays within
+  // kStreamBatchBytes (SYMGW_STREAM_BATCH_BYTES overrides it, e.g. 1 for one
+  // point per batch)
+  int64_t batch_cap = kStreamBatchBytes;
+  if (const char *ev = std::getenv("SYMGW_STREAM_BATCH_BYTES")) batch_cap = std::atoll(ev);
+  auto groups = [&](bool p0, std::vector<int> &first) {
+    first.assign(1, 0);
+    int64_t acc = 0;
+    for (int ir = 0; ir < L.ink; ++ir) {
+      int64_t sz = 0;
+      for (int x = 0; x < L.nk; ++x) sz += L.kpair_size[pair_of(L.irre_list[ir], x, p0)];
+      if (ir > first.back() && (acc + sz) * eb > batch_cap) {
+        first.push_back(ir);
+        acc = 0;
+      }
+      acc += sz;
     }
-    W.batch_V_max = std::max(W.batch_V_max, o);
+    first.push_back(L.ink);
+  };
+  if (stream) {
+    groups(true, W.p0_group_first);
+    groups(false, W.sig_group_first);
+  }
+  auto open_batch = [&](std::vector<std::pair<int, int64_t>> &pairs, std::vector<int> &begin) {
+    begin.push_back((int)pairs.size());
+    batch_fill = 0;
+  };
+  auto add_point = [&](std::vector<std::pair<int, int64_t>> &pairs, int kr, bool p0) {
+    for (int x = 0; x < L.nk; ++x) {
+      const size_t p = pair_of(kr, x, p0);
+      if (L.V_irreps[p].empty() || batch_off[p] >= 0) continue;  // a pair may recur (k + q == k' + q)
+      batch_off[p] = batch_fill;
+      pairs.emplace_back((int)p, batch_fill);
+      batch_fill += L.kpair_size[p];
+    }
+    W.batch_V_max = std::max(W.batch_V_max, batch_fill);
   };
   auto close_batch = [&](const std::vector<std::pair<int, int64_t>> &pairs, int from) {
     for (size_t i = from; i < pairs.size(); ++i) batch_off[pairs[i].first] = -1;
+  };
+  // is iBZ index ir the first / last point of its group
+  auto group_starts = [&](const std::vector<int> &first, int ir) {
+    return std::binary_search(first.begin(), first.end() - 1, ir);
+  };
+  auto group_ends = [&](const std::vector<int> &first, int ir) {
+    return std::binary_search(first.begin() + 1, first.end(), ir + 1);
   };
   auto v_offset = [&](size_t p, size_t b) {
     return stream ? batch_off[p] + L.V_offsets[p][b] : L.kpair_off[p] + L.V_offsets[p][b];
@@ -392,11 +423,17 @@ bool build_worklists(const Layout &L, int eb, WorkLists &W, std::string &err, bo
     for (int qs : L.aux_sizes[L.irre_list[iqr]]) (p0_stacked(eb, qs) ? n_stacked : n_unstacked) += 1;
   W.p0_mode = n_unstacked == 0 ? 1 : n_stacked == 0 ? 0 : 2;
   const int p0_tb = p0_times(eb, W.p0_mode);
+  int p0_pair_from = 0;
   for (int iqr = 0; iqr < L.ink; ++iqr) {
     int q = L.irre_list[iqr];
-    const int pair_from = (int)W.p0_batch_pairs.size();
     W.p0_batch_items.push_back((int)W.p0_items.size());
-    if (stream) open_batch(W.p0_batch_pairs, W.p0_batch_pair_begin, q, true);
+    if (stream) {
+      if (group_starts(W.p0_group_first, iqr)) {
+        p0_pair_from = (int)W.p0_batch_pairs.size();
+        open_batch(W.p0_batch_pairs, W.p0_batch_pair_begin);
+      }
+      add_point(W.p0_batch_pairs, q, true);
+    }
     for (size_t iq = 0; iq < L.aux_sizes[q].size(); ++iq) {
       int qs = L.aux_sizes[q][iq];
       if (qs > 256) { err = fmt("aux block larger than 256", qs); return false; }
@@ -452,7 +489,7 @@ bool build_worklists(const Layout &L, int eb, WorkLists &W, std::string &err, bo
         W.p0_items.push_back(it);
       }
     }
-    if (stream) close_batch(W.p0_batch_pairs, pair_from);
+    if (stream && group_ends(W.p0_group_first, iqr)) close_batch(W.p0_batch_pairs, p0_pair_from);
   }
   W.p0_batch_items.push_back((int)W.p0_items.size());
   if (stream) W.p0_batch_pair_begin.push_back((int)W.p0_batch_pairs.size());
@@ -473,11 +510,25 @@ bool build_worklists(const Layout &L, int eb, WorkLists &W, std::string &err, bo
   // overrides the default, e.g. 0 to put every block on it)
   int z8_min_q = kZ8MinQ;
   if (const char *ev = std::getenv("SYMGW_Z8_MIN_Q")) z8_min_q = std::atoi(ev);
+  // V split bytes / exponents of the current batch (streamed V) or of the whole V
+  int64_t v8_cur = 0, ve_cur = 0;
+  int sig_pair_from = 0;
   for (int ikr = 0; ikr < L.ink; ++ikr) {
     int k = L.irre_list[ikr];
-    const int pair_from = (int)W.sig_batch_pairs.size();
     for (int d = 0; d < kNumSigDepths; ++d) W.sig_batch_items[d].push_back((int)W.sig_items[d].size());
-    if (stream) open_batch(W.sig_batch_pairs, W.sig_batch_pair_begin, k, false);
+    if (stream) {
+      if (group_starts(W.sig_group_first, ikr)) {
+        sig_pair_from = (int)W.sig_batch_pairs.size();
+        open_batch(W.sig_batch_pairs, W.sig_batch_pair_begin);
+        if (z8) {
+          // batch-relative V offsets repeat from batch to batch: a fresh split buffer
+          v8_of.clear();
+          v8_cur = ve_cur = 0;
+          W.v8_batch_begin.push_back((int)W.v8_split.size());
+        }
+      }
+      add_point(W.sig_batch_pairs, k, false);
+    }
     for (size_t i1 = 0; i1 < L.ao_sizes[k].size(); ++i1) {
       int k1 = L.ao_sizes[k][i1];
       const SigShape sh = sig_shape(eb, k1);
@@ -511,11 +562,13 @@ bool build_worklists(const Layout &L, int eb, WorkLists &W, std::string &err, bo
             auto iv = v8_of.find(B.v_off);
             if (iv == v8_of.end()) {
               // B operand of Z: columns (a, b) of V, K = P (stride k1 k2)
-              Split8Blk S8{B.v_off, 0, W.v8_bytes, W.ve_count, 1, ncol, ncol, B.q, B.kp, 1, 1, 0};
+              Split8Blk S8{B.v_off, 0, v8_cur, ve_cur, 1, ncol, ncol, B.q, B.kp, 1, 1, 0};
               iv = v8_of.emplace(B.v_off, W.v8_split.size()).first;
               W.v8_split.push_back(S8);
-              W.v8_bytes += (int64_t)kZ8Comps * kZ8Slices * ncol * B.kp;
-              W.ve_count += (int64_t)kZ8Comps * ncol;
+              v8_cur += (int64_t)kZ8Comps * kZ8Slices * ncol * B.kp;
+              ve_cur += (int64_t)kZ8Comps * ncol;
+              W.v8_bytes = std::max(W.v8_bytes, v8_cur);
+              W.ve_count = std::max(W.ve_count, ve_cur);
             }
             B.v8_off = W.v8_split[iv->second].dst_off;
             B.ve_off = W.v8_split[iv->second].e_off;
@@ -562,10 +615,11 @@ bool build_worklists(const Layout &L, int eb, WorkLists &W, std::string &err, bo
           W.sig_items[sig_depth_index(TB)].push_back(it);
         }
     }
-    if (stream) close_batch(W.sig_batch_pairs, pair_from);
+    if (stream && group_ends(W.sig_group_first, ikr)) close_batch(W.sig_batch_pairs, sig_pair_from);
   }
   for (int d = 0; d < kNumSigDepths; ++d) W.sig_batch_items[d].push_back((int)W.sig_items[d].size());
   if (stream) W.sig_batch_pair_begin.push_back((int)W.sig_batch_pairs.size());
+  if (stream && z8) W.v8_batch_begin.push_back((int)W.v8_split.size());
   // the INT8-Z Sigma kernel reduces split-K parts over its own wave / tile shape
   const size_t red_sig = z8 ? (size_t)kZ8Waves * kZ8Tiles * 64 * 8 * (eb / 2) : red_bytes;
   for (int d = 0; d < kNumSigDepths; ++d) W.sig_lds[d] = W.sig_items[d].empty() ? 0 : std::max(sig_lds[d], red_sig);

@@ -191,6 +191,11 @@ constexpr int kZ8Comps = 3;
 // exact reconstruction of the four int32 accumulators costs about as much as
 // a short FP64 reduction (measured per block size on MI355X).
 constexpr int kZ8MinQ = 32;
+// streamed V: target size of one batch's V buffer (a batch of one iBZ point
+// may exceed it).  Several points per batch keep each launch's grid large: one
+// point of a 6x6x6 mesh gives a few hundred workgroups per Sigma launch, under
+// one per CU, and runs at a quarter of the rate of a whole-sweep launch.
+constexpr int64_t kStreamBatchBytes = int64_t(32) << 30;
 // Workgroup of the Sigma kernel with the INT8 Z product: 12 waves of three
 // accumulator tiles (36 tiles, as many as sig_shape plans for complex128).
 constexpr int kZ8Waves = 12;
@@ -231,18 +236,25 @@ struct WorkLists {
   size_t p0_lds = 0, sig_lds[kNumSigDepths] = {};  // dynamic LDS bytes of the contraction kernels
   // mixed mode: INT8 split tables (V blocks read by Sigma; P blocks of every q)
   std::vector<Split8Blk> v8_split, p8_split;
+  // with a streamed V the V splits of each Sigma batch start at
+  // v8_batch_begin[batch] and write a batch-sized slice buffer (v8_bytes and
+  // ve_count are then the largest batch's)
+  std::vector<int> v8_batch_begin;
   int64_t v8_bytes = 0, ve_count = 0, p8_bytes = 0, pe_count = 0;
-  // streamed V (not resident): P0 runs one batch per iBZ q, Sigma one per iBZ
-  // k; each batch's V pairs are generated into one buffer just ahead of its
-  // contraction, and the blocks' v_off are offsets in that buffer.
+  // streamed V (not resident): P0 and Sigma run one batch per group of
+  // consecutive iBZ points (q for P0, k for Sigma; as many as keep the group's
+  // V within kStreamBatchBytes, at least one); each batch's V pairs are
+  // generated into one buffer just ahead of its contraction, and the blocks'
+  // v_off are offsets in that buffer.
   bool stream = false;
+  std::vector<int> p0_group_first, sig_group_first;  // [ngroups + 1] first iBZ index of each group
   // items of iBZ q (P0) / iBZ k (Sigma) index b: [prefix[b], prefix[b + 1])
   // (every mode: they also delimit the shards of symgw_set_shard)
   std::vector<int> p0_batch_items;                 // [ink + 1] item prefix
   std::vector<int> sig_batch_items[kNumSigDepths]; // [ink + 1] item prefix per slab depth
   // pairs of each batch with their offsets in the batch buffer
   std::vector<std::pair<int, int64_t>> p0_batch_pairs, sig_batch_pairs;
-  std::vector<int> p0_batch_pair_begin, sig_batch_pair_begin;  // [ink + 1]
+  std::vector<int> p0_batch_pair_begin, sig_batch_pair_begin;  // [ngroups + 1]
   int64_t batch_V_max = 0;                         // complex elements of the largest batch
 };
 

@@ -539,6 +539,43 @@ def tiny(ns: int = 1, n: int = 2, ink: int = 4, ao=(3, 2), aux=(4, 3), ni: int =
     return synthetic("tiny", n, ink, list(ao), list(aux), ns, chebyshev(ni, beta), seed, all_pairs=all_pairs)
 
 
+CONFIG5_NAOS = (26, 52, 104, 156, 256)
+CONFIG5_MESHES = (2, 4, 6)
+
+
+def config5_system(i: int, n: int, nao: int, transforms: Transforms, seed: int = SEED_BASE + 5) -> System:
+    """System `i` of the serving mix: a Gamma-centred n^3 mesh reduced to 4 / 8
+    / 16 iBZ points (n = 2 / 4 / 6); nao in max(2, min(8, nao // 13)) blocks
+    log-uniform in [nao / 2nb, 2 nao / nb]; naux = 4 nao in as many blocks,
+    each at most 256; V at the Coulomb scale."""
+    rng = np.random.default_rng(seed * 1000 + i)
+    nb = max(2, min(8, nao // 13))
+    ao = split_sizes(rng, nao, nb, max(1, nao // (2 * nb)), max(4, 2 * nao // nb))
+    while True:
+        aux = split_sizes(rng, 4 * nao, nb, max(4, 4 * nao // (2 * nb)), min(256, max(8, 8 * nao // nb)))
+        if max(aux) <= 256:
+            break
+    ink = {2: 4, 4: 8, 6: 16}[n]
+    return synthetic(f"c5-{i}-{n}^3-nao{nao}", n, ink, ao, aux, 1, transforms, seed * 1000 + i,
+                     v_scale=coulomb_scale(aux))
+
+
+def config5(n_systems: int = 32, seed: int = SEED_BASE + 5, transforms: Optional[Transforms] = None,
+            naos=CONFIG5_NAOS, meshes=CONFIG5_MESHES) -> List[System]:
+    """The serving mix: `n_systems` systems, nao uniform in {26, 52, 104, 156,
+    256} and k-mesh uniform in {2^3, 4^3, 6^3}, each built by config5_system;
+    beta = 50 on the IR grid of Lambda = 1e3 with 48 points (nts 50), shared."""
+    from .transforms import ir
+    tr = transforms or ir(50.0, 1e3, 48)
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n_systems):
+        n = int(rng.choice(meshes))
+        nao = int(rng.choice(naos))
+        out.append(config5_system(i, n, nao, tr, seed))
+    return out
+
+
 def serving_mix(n_requests: int = 32, seed: int = SEED_BASE + 5, meshes=(2, 4), naos=(26, 52, 104),
                 iterations: int = 2):
     """Seeded request mix for the serving loop: each request is a synthetic

@@ -6,6 +6,8 @@ Config 4 itself (8x8x8 mesh, V of 5.5 TB) is only timed on a sample; its
 block structure (nao 256 in 6 blocks, naux 1200 in 6 blocks) is checked here
 on a 2x2x2 mesh, where V also fits resident so the streamed sweeps can be
 compared with the resident ones bit for bit, and against the oracle."""
+import os
+
 import numpy as np
 import pytest
 
@@ -22,8 +24,19 @@ def _say(*a):
     print("[case]", *a, flush=True)
 
 
-def _ctx(sysm, precision):
-    ctx = _lib.Context(sysm, 0, precision)
+def _ctx(sysm, precision, batch_bytes=None):
+    """Context with V generated; batch_bytes overrides the streamed-batch size
+    (1: one iBZ point per batch)."""
+    old = os.environ.get("SYMGW_STREAM_BATCH_BYTES")
+    try:
+        if batch_bytes is not None:
+            os.environ["SYMGW_STREAM_BATCH_BYTES"] = str(batch_bytes)
+        ctx = _lib.Context(sysm, 0, precision)
+    finally:
+        if old is None:
+            os.environ.pop("SYMGW_STREAM_BATCH_BYTES", None)
+        else:
+            os.environ["SYMGW_STREAM_BATCH_BYTES"] = old
     ctx.generate_V(sysm.v_seed, sysm.v_scale, sysm.decay_table())
     ctx.set_reps(sysm.orep, sysm.auxrep)
     ctx.set_transforms(sysm.transforms)
@@ -56,17 +69,19 @@ def test_streamed_V_equals_resident_V(c4small):
     sysm = c4small
     orc = Oracle(sysm)
     G0 = orc.dyson(sysm.zeros_ts_kij(), 0.0)
-    for prec in (_lib.PREC_FP64, _lib.PREC_FP32):
+    for prec in (_lib.PREC_FP64, _lib.PREC_FP32, _lib.PREC_MIXED):
         _say("stream vs resident, config-4 blocks on 2x2x2, precision", prec)
         a = _ctx(sysm, prec)
         ra = _sweeps(a, sysm, G0)
         a.close()
-        b = _ctx(sysm, prec | _lib.FLAG_STREAM_V)
-        assert b.info().device_bytes < _lib.layout_footprint(sysm, prec)
-        rb = _sweeps(b, sysm, G0)
-        b.close()
-        for x, y, name in zip(ra, rb, ("P0", "P", "Sigma", "G")):
-            assert np.array_equal(x, y), (prec, name)
+        # all iBZ points in one batch (the default size), and one point per batch
+        for bb in (None, 1):
+            b = _ctx(sysm, prec | _lib.FLAG_STREAM_V, bb)
+            assert b.info().device_bytes < _lib.layout_footprint(sysm, prec)
+            rb = _sweeps(b, sysm, G0)
+            b.close()
+            for x, y, name in zip(ra, rb, ("P0", "P", "Sigma", "G")):
+                assert np.array_equal(x, y), (prec, bb, name)
 
 
 def test_config4_blocks_against_the_oracle(c4small):
@@ -99,13 +114,16 @@ def test_config4_blocks_against_the_oracle(c4small):
     assert err < 1e-10
 
 
-def test_shard_recomputes_its_rows(config1):
+@pytest.mark.parametrize("flags,batch_bytes", [(0, None), (_lib.FLAG_STREAM_V, None), (_lib.FLAG_STREAM_V, 1),
+                                               (_lib.FLAG_STREAM_V, 5 << 20)])
+def test_shard_recomputes_its_rows(config1, flags, batch_bytes):
     """A shard's P0 and Sigma rows equal those of the full sweep bit for bit;
-    the rows outside it are left untouched."""
+    the rows outside it are left untouched.  Resident V, and V streamed in
+    batches of all four, one, or two iBZ points (a shard cutting a batch)."""
     sysm = config1
     orc = Oracle(sysm)
     G0 = orc.dyson(sysm.zeros_ts_kij(), 0.0)
-    ctx = _ctx(sysm, _lib.PREC_FP64)
+    ctx = _ctx(sysm, _lib.PREC_FP64 | flags, batch_bytes)
     P0, P, Sg, _ = _sweeps(ctx, sysm, G0)
     ctx.set_shard(1, 2)
     ctx.upload_G(G0)

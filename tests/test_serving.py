@@ -1,5 +1,7 @@
-"""Serving loop: admission policy on the CPU (stand-in runner), and on the
-GPU the same bits for a system alone and in a concurrent mix."""
+"""Serving loops: admission policy on the CPU (stand-in runners and
+backends), and on the GPU the same bits for a system alone and in a
+concurrent mix, for both request models (resident systems, and requests that
+bring their own system)."""
 import threading
 import time
 
@@ -67,6 +69,171 @@ def test_poisson_arrivals_rate():
     a = SV.poisson_arrivals(20000, 50.0, seed=3)
     assert np.all(np.diff(a) > 0)
     assert abs(a[-1] / 20000 - 1 / 50.0) < 0.002
+
+
+class _FakeBackend:
+    """Backend that sleeps instead of computing; records loads and overlaps."""
+
+    def __init__(self, runtime, fail_load=()):
+        self.runtime = runtime
+        self.fail_load = set(fail_load)
+        self.lock = threading.Lock()
+        self.loaded = set()
+        self.load_calls = []
+        self.running = {"n": 0, "max": 0}
+        self.per_system = {}
+
+    def load(self, i):
+        if i in self.fail_load:
+            raise RuntimeError(f"cannot load {i}")
+        with self.lock:
+            assert i not in self.loaded
+            self.loaded.add(i)
+            self.load_calls.append(i)
+        return i
+
+    def iterate(self, h, n, keep_G):
+        with self.lock:
+            assert h in self.loaded
+            self.running["n"] += 1
+            self.running["max"] = max(self.running["max"], self.running["n"])
+            self.per_system[h] = self.per_system.get(h, 0) + 1
+            assert self.per_system[h] == 1, "two requests of one system at once"
+        time.sleep(self.runtime[h] * n)
+        with self.lock:
+            self.running["n"] -= 1
+            self.per_system[h] -= 1
+        return np.full(2, float(h)) if keep_G else None
+
+    def unload(self, h):
+        with self.lock:
+            self.loaded.remove(h)
+
+
+def test_server_resident_systems_budget_and_lru():
+    fp = [400, 300, 300, 500, 2000]          # system 4 exceeds the budget
+    be = _FakeBackend({i: 0.02 for i in range(5)})
+    srv = SV.Server(fp, budget=1000, backend=be, max_active=3)
+    assert srv.preload([0, 1, 2, 3]) >= 0.0
+    assert be.load_calls == [0, 1, 2] and srv.used == 1000     # 3 does not fit
+    reqs = [SV.IterRequest(0, 0, 0.0), SV.IterRequest(1, 1, 0.0), SV.IterRequest(2, 0, 0.0),
+            SV.IterRequest(3, 3, 0.05), SV.IterRequest(4, 4, 0.05), SV.IterRequest(5, 2, 0.2, iterations=2)]
+    res = srv.run(reqs, keep_G=True)
+    assert [r.rid for r in res] == list(range(6))
+    assert res[4].error and "exceeds" in res[4].error
+    ok = [r for r in res if r.error is None]
+    assert len(ok) == 5
+    # system 3 was loaded cold, evicting least recently used idle systems
+    assert res[3].cold and not res[0].cold and not res[1].cold
+    assert srv.evictions >= 1 and srv.peak_used <= 1000
+    # requests of one system never overlap; different systems do
+    assert be.running["max"] >= 2
+    assert res[2].started >= res[0].finished - 1e-6
+    # system 2 was evicted for system 3, so its later request reloads it
+    assert res[5].cold and res[5].iterations == 2 and np.array_equal(res[5].G, [2.0, 2.0])
+    srv.close()
+    assert not be.loaded and srv.used == 0
+    s = SV.summarize_iter(res, 1.0, [type("Sys", (), {"nk": 8, "ao_sizes": [[i + 1]], "irre_list": [0]})()
+                                     for i in range(5)])
+    assert s["completed"] == 5 and s["failed"] == 1 and s["cold_starts"] == 2
+    assert s["p99_latency_s"] >= s["p50_latency_s"] and s["unserved"] == 0
+
+
+def test_server_oldest_cold_request_claims_memory_first():
+    """A large cold system at the head of the queue is not overtaken by later
+    small cold systems (no starvation); warm systems keep running meanwhile."""
+    fp = [600, 300, 300, 300]
+    be = _FakeBackend({0: 0.15, 1: 0.01, 2: 0.01, 3: 0.01})
+    srv = SV.Server(fp, budget=1000, backend=be, max_active=4)
+    srv.preload([1])
+    reqs = [SV.IterRequest(0, 2, 0.0), SV.IterRequest(1, 3, 0.0), SV.IterRequest(2, 0, 0.001),
+            SV.IterRequest(3, 2, 0.002)]
+    res = srv.run(reqs)
+    assert all(r.error is None for r in res)
+    # 2 and 3 fill the budget (300 + 300 + 300 resident); 0 (600) needs evictions
+    # and starts once they finish, before any later cold claim
+    assert res[2].cold and res[2].started >= min(res[0].finished, res[1].finished) - 1e-6
+    assert srv.peak_used <= 1000
+    srv.close()
+
+
+def test_server_deadline_leaves_long_requests_unserved():
+    fp = [10, 10]
+    be = _FakeBackend({0: 0.01, 1: 0.01})
+    est = {0: 0.01, 1: 5.0}
+    srv = SV.Server(fp, budget=100, backend=be, max_active=2, estimate=lambda i: est[i])
+    srv.preload([0, 1])
+    reqs = [SV.IterRequest(i, i % 2, 0.001 * i) for i in range(6)]
+    res = srv.run(reqs, deadline=0.5)
+    assert all(r.served for r in res if r.system == 0) and all(r.error is None for r in res if r.system == 0)
+    assert all(not r.served and "deadline" in r.error for r in res if r.system == 1)
+    s = SV.summarize_iter(res, 0.5, [type("Sys", (), {"nk": 8, "ao_sizes": [[4]], "irre_list": [0]})()] * 2,
+                          deadline=0.5)
+    assert s["unserved"] == 3 and s["completed"] == 3
+    assert s["p99_latency_all_issued_lower_bound_s"] >= 0.49
+    srv.close()
+
+
+def test_server_long_requests_leave_slots_to_short_ones():
+    fp = [10] * 6
+    be = _FakeBackend({0: 0.2, 1: 0.2, 2: 0.2, 3: 0.01, 4: 0.01, 5: 0.01})
+    est = {0: 5.0, 1: 5.0, 2: 5.0, 3: 0.01, 4: 0.01, 5: 0.01}
+    srv = SV.Server(fp, budget=100, backend=be, max_active=4, estimate=lambda i: est[i], long_s=1.0, max_long=2)
+    srv.preload(list(range(6)))
+    reqs = [SV.IterRequest(i, i, 0.0) for i in range(3)] + [SV.IterRequest(3 + i, 3 + i, 0.01) for i in range(3)]
+    res = srv.run(reqs)
+    assert all(r.error is None for r in res)
+    # two long requests ran first, the third waited for one of them; the short
+    # ones arriving later took the open slots at once
+    longs = sorted(res[:3], key=lambda r: r.started)
+    assert longs[2].started >= min(longs[0].finished, longs[1].finished) - 1e-6
+    for r in res[3:]:
+        assert r.started - r.arrival < 0.1 and r.finished < longs[0].finished
+    srv.close()
+
+
+def test_server_failed_load_releases_its_memory():
+    be = _FakeBackend({0: 0.01, 1: 0.01}, fail_load=[0])
+    srv = SV.Server([600, 600], budget=1000, backend=be, max_active=2)
+    res = srv.run([SV.IterRequest(0, 0, 0.0), SV.IterRequest(1, 1, 0.01)])
+    assert res[0].error and "cannot load" in res[0].error
+    assert res[1].error is None and res[1].cold
+    assert srv.used == 600
+    srv.close()
+
+
+@pytest.mark.gpu
+def test_resident_same_bits_alone_and_in_the_mix():
+    """Resident serving: a system's G after its requests (interleaved with the
+    other systems' on concurrent contexts) equals the G of the same number of
+    iterations run alone, bit for bit.  One system is of serving-mix size:
+    4x4x4 mesh (8 iBZ points), nao 156 in 8 blocks, naux 624, nts 50."""
+    from mbsymmetry_amd import _lib
+    from mbsymmetry_amd import system as S
+    from mbsymmetry_amd.transforms import ir
+    big = S.config5_system(2, 4, 156, ir(50.0, 1e3, 48))
+    systems = [big, S.config1(),
+               S.time_reversal_square(ns=2, ao=(5, 3, 2), aux=(12, 8, 6), ni=10, beta=12.0, seed=13),
+               S.tiny(ns=2, n=2, ink=4, ao=(3, 2), aux=(4, 3), ni=8, beta=10.0, seed=11)]
+    be = SV.DeviceBackend(systems, 0, _lib.PREC_FP64)
+    alone = []
+    for i in range(len(systems)):
+        print(f"[case] resident alone: system {i} ({systems[i].name})", flush=True)
+        h = be.load(i)
+        alone.append(be.iterate(h, 2, True))
+        be.unload(h)
+    # two one-iteration requests per system, interleaved
+    reqs = [SV.IterRequest(j, j % 4, 0.0 if j < 4 else 0.05) for j in range(8)]
+    fp = [_lib.layout_footprint(s, _lib.PREC_FP64) for s in systems]
+    srv = SV.Server(fp, budget=sum(fp) + (1 << 30), backend=be, max_active=4)
+    print("[case] resident mix", flush=True)
+    srv.preload(list(range(4)))
+    res = srv.run(reqs, keep_G=True)
+    srv.close()
+    assert all(r.error is None for r in res), [r.error for r in res]
+    assert srv.peak_active >= 2
+    for r in res[4:]:
+        assert np.array_equal(r.G, alone[r.system]), r.system
 
 
 @pytest.mark.gpu
