@@ -73,5 +73,23 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
+def build_abi_consumer(verbose: bool = False) -> str:
+    """tests/abi_consumer: a C11 program built with gcc against include/symgw.h
+    and the library alone (no HIP headers), as a C or C++ host of the
+    reference would bind it."""
+    root = os.path.dirname(HERE)
+    out = os.path.join(root, "tests", "abi_consumer")
+    cmd = ["gcc", "-std=c11", "-O1", "-Wall", "-Wextra", "-Werror", "-I", os.path.join(root, "include"),
+           os.path.join(root, "tests", "abi_consumer.c"), "-o", out + ".tmp", "-L", HERE,
+           "-l:" + os.path.basename(LIB), "-Wl,-rpath,$ORIGIN/../mbsymmetry_amd"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
+    if r.returncode != 0:
+        raise RuntimeError("abi_consumer build failed:\n" + r.stdout.decode(errors="replace"))
+    os.replace(out + ".tmp", out)
+    return out
+
+
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))

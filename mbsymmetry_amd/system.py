@@ -313,6 +313,101 @@ def time_reversal_square(ns: int = 1, ao=(3, 2), aux=(4, 3), ni: int = 8, beta: 
                      list(aux), ns, chebyshev(ni, beta), seed, 1.0, all_pairs)
 
 
+def permuted_blocks(n: int = 3, ns: int = 1, ni: int = 8, beta: float = 10.0, seed: int = 17,
+                    max_blocks: int = 4, max_size: int = 9) -> System:
+    """Rotations with permuted block maps and k-dependent block structure: on
+    the n^3 mesh with its O_h stars, every star has its own irreps (2 to
+    max_blocks orbital and auxiliary blocks of sizes 1..max_size), and every
+    non-representative point k holds its representative's blocks in its own
+    order: block b of k is block sigma_k(b) of the representative, so the
+    rotation maps (orep_irreps / auxrep_irreps, cugw_routines_block.cu:134-186)
+    pair blocks a1 = b with a2 = sigma_k(b) != b.  The auxiliary order of q and
+    -q is the same (P(-q) = P(q)^* reads one layout).  V blocks of pair
+    (k1, k2) with transfer q: aux block iq couples ao block iq mod n(k1) of k1
+    to (iq + iq mod n(k1)) mod n(k2) of k2."""
+    from .transforms import chebyshev
+    rng = np.random.default_rng(seed)
+    pts, kpq, kmq = cubic_mesh(n)
+    nk = len(pts)
+    stars = cubic_stars(pts, n)
+    ink = len(stars)
+    irre_list = np.array([st[0] for st in stars], dtype=np.int32)
+    index = np.zeros(nk, dtype=np.int32)
+    irre_index = np.zeros(nk, dtype=np.int32)
+    for i, st in enumerate(stars):
+        for k in st:
+            index[k] = st[0]
+            irre_index[k] = i
+
+    def sizes():
+        nb = int(rng.integers(2, max_blocks + 1))
+        return [int(v) for v in rng.integers(1, max_size + 1, nb)]
+    rep_ao = {int(r): sizes() for r in irre_list}
+    rep_aux = {int(r): sizes() for r in irre_list}
+    ao, aux = [None] * nk, [None] * nk
+    omap, amap = [[] for _ in range(nk)], [[] for _ in range(nk)]
+    for k in range(nk):
+        r = int(index[k])
+        if k == r:
+            ao[k], aux[k] = list(rep_ao[r]), list(rep_aux[r])
+            continue
+        so = rng.permutation(len(rep_ao[r]))
+        ao[k] = [rep_ao[r][int(s)] for s in so]
+        omap[k] = [(b, int(so[b])) for b in range(len(so))]
+    for q in range(nk):
+        r = int(index[q])
+        if q == r or aux[q] is not None:
+            continue
+        mq = int(kmq[0, q])
+        sa = rng.permutation(len(rep_aux[r]))
+        for x in {q, mq}:
+            if x != int(index[x]):
+                aux[x] = [rep_aux[r][int(s)] for s in sa]
+                amap[x] = [(b, int(sa[b])) for b in range(len(sa))]
+    for q in range(nk):                     # q and -q: one auxiliary order
+        mq = int(kmq[0, q])
+        assert aux[q] == aux[mq], (q, mq)
+    orep, auxrep = [], []
+    for k in range(nk):
+        for a1, _ in omap[k]:
+            orep.append(haar_unitary(rng, ao[k][a1]).ravel())
+        for a1, _ in amap[k]:
+            auxrep.append(haar_unitary(rng, aux[k][a1]).ravel())
+    need = set()
+    for q in irre_list:
+        for k in range(nk):
+            need.add(k * nk + int(kpq[k, q]))
+    for k in irre_list:
+        for q in range(nk):
+            need.add(int(k) * nk + int(kmq[k, q]))
+    V_irreps = {}
+    for p in sorted(need):
+        k1, k2 = divmod(p, nk)
+        q = int(np.nonzero(kpq[k1] == k2)[0][0])
+        n1, n2 = len(ao[k1]), len(ao[k2])
+        V_irreps[p] = [(iq, iq % n1, (iq + iq % n1) % n2) for iq in range(len(aux[q]))]
+    F, S = [], []
+    for s in range(ns):
+        fs, ss = [], []
+        for r in irre_list:
+            for b in ao[int(r)]:
+                U = haar_unitary(rng, b)
+                fs.append((U @ np.diag(rng.uniform(-1.0, 1.0, b)) @ U.conj().T).ravel())
+                ss.append(np.eye(b).ravel())
+        F.append(np.concatenate(fs))
+        S.append(np.concatenate(ss))
+    F = np.array(F)
+    S = np.array(S).astype(np.complex128)
+    kweights = np.array([len(st) for st in stars], dtype=np.float64)
+    return System(nk=nk, ink=ink, ns=ns, transforms=chebyshev(ni, beta), ao_sizes=ao, aux_sizes=aux,
+                  irre_list=irre_list, index=index, irre_index=irre_index, conj_index=np.arange(nk, dtype=np.int32),
+                  kpq=kpq, kmq=kmq, orep_irreps=omap, auxrep_irreps=amap, V_irreps=V_irreps, kweights=kweights,
+                  orep=np.concatenate(orep) if orep else np.zeros(0, np.complex128),
+                  auxrep=np.concatenate(auxrep) if auxrep else np.zeros(0, np.complex128),
+                  F=F, S=S, H=np.zeros_like(F), name="permuted_blocks", v_seed=seed,
+                  v_scale=1.0 / float(np.sqrt(max(sum(a) for a in aux))))
+
+
 def _assemble(name, rng, kpq, kmq, irre_list, index, irre_index, conj_index, kweights, ao_sizes, aux_sizes, ns,
               transforms, seed, v_scale, all_pairs) -> System:
     """Rotation blocks, V pair list, Fock matrices and the System record of a

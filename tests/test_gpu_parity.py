@@ -24,6 +24,8 @@ def systems(lih, config1):
         "tiny_rot_ns2": S.tiny(ns=2, n=2, ink=4, ao=(3, 2), aux=(4, 3), ni=8, beta=10.0, seed=11),
         "tiny_3cube": S.tiny(ns=1, n=3, ink=4, ao=(2, 3, 1), aux=(3, 2), ni=6, beta=8.0, seed=5),
         "tr_square": S.time_reversal_square(ns=2, ao=(5, 3, 2), aux=(12, 8, 6), ni=10, beta=12.0, seed=13),
+        # rotations with permuted block maps (a1 != a2), k-dependent blocks
+        "permuted": S.permuted_blocks(n=3, ns=2),
         "config1": config1,
         "lih": lih,
     }
@@ -48,7 +50,7 @@ def _rel(a, b):
     return np.abs(a - b).max() / max(1.0, np.abs(b).max())
 
 
-@pytest.mark.parametrize("name", ["tiny_rot_ns2", "tiny_3cube", "tr_square", "config1", "lih"])
+@pytest.mark.parametrize("name", ["tiny_rot_ns2", "tiny_3cube", "tr_square", "permuted", "config1", "lih"])
 def test_P0_parity(name, systems):
     sysm = systems[name]
     _say("P0", name)
@@ -70,7 +72,7 @@ def test_P0_parity(name, systems):
     ctx.close()
 
 
-@pytest.mark.parametrize("name", ["tiny_rot_ns2", "tiny_3cube", "tr_square", "config1", "lih"])
+@pytest.mark.parametrize("name", ["tiny_rot_ns2", "tiny_3cube", "tr_square", "permuted", "config1", "lih"])
 def test_W_and_Sigma_parity(name, systems):
     sysm = systems[name]
     _say("W+Sigma", name)
@@ -92,7 +94,7 @@ def test_W_and_Sigma_parity(name, systems):
     ctx.close()
 
 
-@pytest.mark.parametrize("name", ["tiny_rot_ns2", "tr_square", "config1", "lih"])
+@pytest.mark.parametrize("name", ["tiny_rot_ns2", "tr_square", "permuted", "config1", "lih"])
 def test_dyson_parity(name, systems):
     sysm = systems[name]
     _say("Dyson", name)
@@ -109,7 +111,7 @@ def test_dyson_parity(name, systems):
     ctx.close()
 
 
-@pytest.mark.parametrize("name", ["tiny_rot_ns2", "tr_square", "config1"])
+@pytest.mark.parametrize("name", ["tiny_rot_ns2", "tr_square", "permuted", "config1"])
 def test_gw_iteration_parity_and_graph(name, systems):
     sysm = systems[name]
     _say("iteration", name)
