@@ -60,16 +60,23 @@ def setup_ctx(sysm, precision, mu):
     return ctx
 
 
-def run_path(sysm, precision, steps, warmup, mu, G0, keep_P=False):
+def run_path(sysm, precision, steps, warmup, mu, G0, keep_P=False, tune=True):
     """Time `steps` GW iterations replayed from per-phase hipGraphs.
 
     The first iteration from G0 is untimed (it is warmup step 1) and gives the
     Sigma and G compared across precisions.  In the timed region HIP events sit
     between the phase graphs on the context stream, so the per-phase device
     times are averages over exactly the timed iterations.  keep_P: also return
-    that iteration's P0 and P (recomputed from G0 after it, outside timing)."""
+    that iteration's P0 and P (recomputed from G0 after it, outside timing).
+    tune: pick the work-list plan with the autotuner first (outside timing)."""
     ctx = setup_ctx(sysm, precision, mu)
     ctx.upload_G(G0)
+    tuned = None
+    if tune:
+        from mbsymmetry_amd import autotune
+        tuned = autotune.tune(ctx, sysm, precision, use_cache=False, log=log)
+        tuned = {k: tuned[k] for k in ("plan", "Sigma", "P0", "seconds")}
+        ctx.upload_G(G0)
     ctx.gw_iteration(mu, use_graph=True)
     ctx.sync()
     sig = ctx.copy_Sigma_to_host(sysm.zeros_ts_kij()) / sysm.nk
@@ -106,6 +113,7 @@ def run_path(sysm, precision, steps, warmup, mu, G0, keep_P=False):
     from mbsymmetry_amd.transforms import leakage
     return dict(sec_per_iter=dt / steps, phase_ms=phase / steps, step_s=step_s, flop_p0=info.flop_P0,
                 flop_sig=info.flop_Sigma, sigma=sig, G1=G1, P0=P0, P=P, leakage=leakage(sysm.transforms, G1),
+                tuned=tuned,
                 int8_ops=int8_ops(sysm) if precision == _lib.PREC_MIXED else 0.0)
 
 
@@ -203,7 +211,7 @@ def cpu_baseline(sysm, G0, P0, P, Sigma, max_workers=16):
                 sample_k=len(sub))
 
 
-def config3_run():
+def config3_run(tune=True):
     """The uneven-batching system (64 irreducible points, each with its own
     2..10 irreps of sizes 1..256; the whole sweep of each contraction in one
     launch per slab depth): P0 and Sigma kernel time and rate from HIP events
@@ -221,6 +229,12 @@ def config3_run():
         ctx.upload_G(sysm.zeros_ts_kij())
         ctx.dyson(0.0, 0.0)
         G0 = ctx.download_G((sysm.nts, sysm.ns, sysm.flat_ao))
+        tuned = None
+        if tune:
+            from mbsymmetry_amd import autotune
+            ctx.upload_G(G0)
+            rec = autotune.tune(ctx, sysm, prec, sample_s=0.3, use_cache=False, log=log)
+            tuned = {k: rec[k] for k in ("plan", "Sigma", "P0", "seconds")}
         for timed in (False, True):
             ctx.upload_G(G0)
             ctx.set_timing(timed)
@@ -230,7 +244,7 @@ def config3_run():
         sig[name] = ctx.copy_Sigma_to_host(sysm.zeros_ts_kij()) / sysm.nk
         ctx.close()
         out[name] = {"P0_ms": float(lt[0]), "Sigma_ms": float(lt[2]), "P0_tflops": float(inf.flop_P0 / lt[0] / 1e9),
-                     "Sigma_tflops": float(inf.flop_Sigma / lt[2] / 1e9)}
+                     "Sigma_tflops": float(inf.flop_Sigma / lt[2] / 1e9), "autotune": tuned}
         if name != "fp64":
             out[name]["max_abs_dSigma_vs_fp64"] = float(np.abs(sig[name] - sig["fp64"]).max())
         log("config3", name, out[name])
@@ -290,6 +304,7 @@ def main():
     ap.add_argument("--mu", type=float, default=0.0)
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-config3", action="store_true", help="skip the uneven-block (config 3) leg")
+    ap.add_argument("--no-tune", action="store_true", help="keep the built-in work-list plan (no autotuner)")
     ap.add_argument("--serving-only", action="store_true", help="run only the serving mix and print its JSON")
     ap.add_argument("--serving-window", type=float, default=2.0, help="seconds of Poisson arrivals")
     ap.add_argument("--serving-deadline", type=float, default=30.0,
@@ -335,14 +350,15 @@ def main():
     ctx.close()
     log("G0 ready")
 
-    plain = run_path(sysm, _lib.PREC_FP64, args.steps, args.warmup, args.mu, G0, keep_P=not args.no_cpu)
+    plain = run_path(sysm, _lib.PREC_FP64, args.steps, args.warmup, args.mu, G0, keep_P=not args.no_cpu,
+                     tune=not args.no_tune)
     log("plain fp64", plain["sec_per_iter"], "s/iter; phases ms", plain["phase_ms"][:4])
     mixed = fp32 = None
     if not args.no_mixed:
-        mixed = run_path(sysm, _lib.PREC_MIXED, args.steps, args.warmup, args.mu, G0)
+        mixed = run_path(sysm, _lib.PREC_MIXED, args.steps, args.warmup, args.mu, G0, tune=not args.no_tune)
         log("mixed (int8 Z)", mixed["sec_per_iter"], "s/iter; phases ms", mixed["phase_ms"][:4])
         if args.fp32:
-            fp32 = run_path(sysm, _lib.PREC_FP32, args.steps, args.warmup, args.mu, G0)
+            fp32 = run_path(sysm, _lib.PREC_FP32, args.steps, args.warmup, args.mu, G0, tune=not args.no_tune)
             log("fp32", fp32["sec_per_iter"], "s/iter; phases ms", fp32["phase_ms"][:4])
 
     p0_ms, sig_ms = float(plain["phase_ms"][0]), float(plain["phase_ms"][2])
@@ -393,6 +409,7 @@ def main():
         "kernel_tflops": {"P0": plain["flop_p0"] / (p0_ms * 1e-3) / 1e12,
                           "Sigma": plain["flop_sig"] / (sig_ms * 1e-3) / 1e12},
     }
+    out["autotune"] = {"fp64": plain["tuned"], "mixed": mixed["tuned"] if mixed else None}
     out["physics"] = {"max_abs_Sigma": float(np.abs(plain["sigma"]).max()),
                       "max_abs_G_after_iteration": float(np.abs(plain["G1"]).max()),
                       "causal_bound_|G|<=1": bool(np.abs(plain["G1"]).max() <= 1.0 + 1e-9),
@@ -420,7 +437,7 @@ def main():
         out["fp32"] = compare(fp32, "every contraction product in FP32 (3M), FP32 accumulation")
         out["fp32"]["frac_fp32_peak"] = {k: v / PEAK["fp32"] for k, v in out["fp32"]["kernel_tflops"].items()}
     if not args.no_config3:
-        out["config3"] = config3_run()
+        out["config3"] = config3_run(tune=not args.no_tune)
     if not args.no_serving:
         out["serving"] = serving_run(args.serving_window, args.serving_rate, args.serving_deadline,
                                      args.serving_modes.split(","))

@@ -308,6 +308,28 @@ int symgw_gw_iteration(symgw_ctx *ctx, double mu, int use_graph);
  * point.  Drops the captured iteration graphs. */
 int symgw_set_shard(symgw_ctx *ctx, int first, int count);
 
+/* Rules of the work-list planner that splits every P0 and Sigma block
+ * contraction into LDS chunks and slabs: no counterpart in the reference
+ * (which hands each block to cuBLAS, cugw_qkpt_block.cu:233-260); the search
+ * space of the autotuner (mbsymmetry_amd/autotune.py), which times candidate
+ * plans on the device for a system's block-size histogram.  Results do not
+ * depend on the plan beyond rounding (the order of the k sums is fixed by
+ * the plan).  symgw_plan_defaults gives the built-in rules; symgw_replan
+ * rebuilds the work lists of a context under another plan (the data on the
+ * device is kept; SYMGW_ERR_ARG if the plan is out of range or would change
+ * a buffer size); symgw_get_plan returns the context's plan. */
+typedef struct symgw_plan {
+  double sig_chunk_cost;   /* fixed cost of one Sigma LDS chunk, MFMA FLOPs (default 2.5e5) */
+  int32_t sig_rows;        /* most aux rows of one Sigma chunk, 1..64 (64)                   */
+  int32_t sig_max_times;   /* most times per Sigma slab: 1, 2 or 4 (4)                       */
+  double p0_chunk_cost;    /* fixed cost of one P0 LDS chunk (2.5e5)                          */
+  int32_t p0_band_cap;     /* most output rows of one P0 slab, 1..16 (16)                    */
+  int32_t reserved;        /* 0 */
+} symgw_plan;
+int symgw_plan_defaults(symgw_plan *plan);
+int symgw_replan(symgw_ctx *ctx, const symgw_plan *plan);
+int symgw_get_plan(const symgw_ctx *ctx, symgw_plan *plan);
+
 /* Wait for all work queued on the context's stream. */
 int symgw_sync(symgw_ctx *ctx);
 /* Device time (ms) of the phases of the last timed symgw_gw_iteration,

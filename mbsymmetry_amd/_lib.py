@@ -42,8 +42,22 @@ class LayoutInfo(C.Structure):
     ]
 
 
+class Plan(C.Structure):
+    """symgw_plan: rules of the work-list planner (the autotuner's knobs)."""
+    _fields_ = [
+        ("sig_chunk_cost", C.c_double), ("sig_rows", C.c_int32), ("sig_max_times", C.c_int32),
+        ("p0_chunk_cost", C.c_double), ("p0_band_cap", C.c_int32), ("reserved", C.c_int32),
+    ]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved"}
+
+
 # every entry point declared in include/symgw.h: name -> (restype, argtypes)
 SIGNATURES = {
+    "symgw_plan_defaults": (C.c_int, [C.POINTER(Plan)]),
+    "symgw_replan": (C.c_int, [C.c_void_p, C.POINTER(Plan)]),
+    "symgw_get_plan": (C.c_int, [C.c_void_p, C.POINTER(Plan)]),
     "symgw_layout_query": (C.c_int, [C.POINTER(LayoutDesc), C.POINTER(LayoutInfo)]),
     "symgw_kpair_offset": (C.c_int64, [C.c_void_p, C.c_int, C.c_int]),
     "symgw_layout_footprint": (C.c_int, [C.POINTER(LayoutDesc), C.c_int, C.POINTER(C.c_int64)]),
@@ -267,6 +281,21 @@ class Context:
     def set_shard(self, first: int, count: int):
         self._chk(self.lib.symgw_set_shard(self.h, first, count), "set_shard")
 
+    def replan(self, **rules):
+        """Rebuild the work lists under the default plan with `rules` changed
+        (sig_chunk_cost, sig_rows, sig_max_times, p0_chunk_cost, p0_band_cap)."""
+        p = plan_defaults()
+        for k, v in rules.items():
+            if k not in p.as_dict():
+                raise KeyError(f"unknown plan rule {k}")
+            setattr(p, k, v)
+        self._chk(self.lib.symgw_replan(self.h, C.byref(p)), "replan")
+
+    def plan(self) -> dict:
+        p = Plan()
+        self._chk(self.lib.symgw_get_plan(self.h, C.byref(p)), "get_plan")
+        return p.as_dict()
+
     def sync(self):
         self._chk(self.lib.symgw_sync(self.h), "sync")
 
@@ -277,6 +306,14 @@ class Context:
         out = np.zeros(5)
         self._chk(self.lib.symgw_last_times(self.h, dptr(out)), "last_times")
         return out
+
+
+def plan_defaults() -> Plan:
+    lib = load()
+    p = Plan()
+    if lib.symgw_plan_defaults(C.byref(p)) != 0:
+        raise SymGWError("symgw_plan_defaults failed")
+    return p
 
 
 def layout_query(system) -> LayoutInfo:

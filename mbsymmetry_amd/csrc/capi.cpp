@@ -76,6 +76,7 @@ struct symgw_ctx {
   int64_t *d_v8_prefix = nullptr, *d_p8_prefix = nullptr;
   int64_t v8_rows = 0, p8_rows = 0;
   std::vector<int64_t> v8_batch_rows;  // streamed V: split rows of each Sigma batch
+  Plan plan;                           // planner rules of the work lists
 
   // device tables
   P0Item *d_p0_items = nullptr;
@@ -541,6 +542,71 @@ std::mutex g_noalloc_mu;
 void *g_noalloc_scratch[kNoAllocMaxDevices] = {};
 }  // namespace
 
+// Replace a device table: the old buffer (if any) is released on the
+// context's stream, the new one uploaded.
+template <typename X>
+static int reupload(symgw_ctx *c, X **dst, const std::vector<X> &v) {
+  if (*dst) {
+    void *old = *dst;
+    for (size_t i = 0; i < c->owned.size(); ++i)
+      if (c->owned[i].p == old) {
+        c->owned.erase(c->owned.begin() + (long)i);
+        break;
+      }
+    ctx_free(c, old);
+    *dst = nullptr;
+  }
+  return upload(c, dst, v);
+}
+
+// The device tables that follow from the plan: P0 and Sigma items and
+// blocks, and in the mixed mode the INT8 split tables and their row prefixes.
+static int upload_plan_tables(symgw_ctx *c) {
+  int r = reupload(c, &c->d_p0_items, c->W.p0_items);
+  r = r ? r : reupload(c, &c->d_p0_blks, c->W.p0_blks);
+  for (int d = 0; d < kNumSigDepths; ++d) r = r ? r : reupload(c, &c->d_sig_items[d], c->W.sig_items[d]);
+  r = r ? r : reupload(c, &c->d_sig_blks, c->W.sig_blks);
+  if (r || c->prec != SYMGW_PREC_MIXED) return r;
+  auto prefix = [](const std::vector<Split8Blk> &v, int64_t &tot) {
+    std::vector<int64_t> p(v.size() + 1, 0);
+    for (size_t i = 0; i < v.size(); ++i) p[i + 1] = p[i] + (int64_t)v[i].nsub * v[i].nrows;
+    tot = p.back();
+    return p;
+  };
+  r = reupload(c, &c->d_v8_split, c->W.v8_split);
+  r = r ? r : reupload(c, &c->d_p8_split, c->W.p8_split);
+  if (!c->stream) {
+    r = r ? r : reupload(c, &c->d_v8_prefix, prefix(c->W.v8_split, c->v8_rows));
+  } else {
+    // row prefixes restarted at every batch, so each batch's split launch
+    // reads its own slice of the table
+    const auto &vb = c->W.v8_batch_begin;
+    std::vector<int64_t> pre(c->W.v8_split.size() + 1, 0);
+    c->v8_batch_rows.assign(vb.empty() ? 0 : vb.size() - 1, 0);
+    for (size_t bt = 0; bt + 1 < vb.size(); ++bt) {
+      int64_t acc = 0;
+      for (int i = vb[bt]; i < vb[bt + 1]; ++i) {
+        pre[i] = acc;
+        acc += (int64_t)c->W.v8_split[i].nsub * c->W.v8_split[i].nrows;
+      }
+      c->v8_batch_rows[bt] = acc;
+    }
+    r = r ? r : reupload(c, &c->d_v8_prefix, pre);
+  }
+  r = r ? r : reupload(c, &c->d_p8_prefix, prefix(c->W.p8_split, c->p8_rows));
+  return r;
+}
+
+static Plan plan_of(const symgw_plan &p) {
+  Plan q;
+  q.sig_chunk_cost = p.sig_chunk_cost;
+  q.sig_rows = p.sig_rows;
+  q.sig_max_times = p.sig_max_times;
+  q.p0_chunk_cost = p.p0_chunk_cost;
+  q.p0_band_cap = p.p0_band_cap;
+  return q;
+}
+
 extern "C" {
 
 const char *symgw_create_error(void) { return g_create_error.c_str(); }
@@ -789,11 +855,7 @@ int symgw_create(const symgw_layout_desc *desc, int device, int precision, symgw
     for (auto &b : c->owned)
       if (!r && hipMemsetAsync(b.p, 0, b.bytes, c->st) != hipSuccess) r = fail(cp, SYMGW_ERR_HIP, "initial memset failed");
     if (!r && hipStreamSynchronize(c->st) != hipSuccess) r = fail(cp, SYMGW_ERR_HIP, "initial memset failed");
-    // work lists
-    r = r ? r : upload(cp, &c->d_p0_items, c->W.p0_items);
-    r = r ? r : upload(cp, &c->d_p0_blks, c->W.p0_blks);
-    for (int d = 0; d < kNumSigDepths; ++d) r = r ? r : upload(cp, &c->d_sig_items[d], c->W.sig_items[d]);
-    r = r ? r : upload(cp, &c->d_sig_blks, c->W.sig_blks);
+    // work lists (the contraction tables come with the split tables, below)
     if (c->stream) {
       std::vector<GenBlk> g0, g1;
       batch_gen_blocks(L, c->W.p0_batch_pairs, c->W.p0_batch_pair_begin, g0, c->p0_gen_begin);
@@ -801,35 +863,7 @@ int symgw_create(const symgw_layout_desc *desc, int device, int precision, symgw
       r = r ? r : upload(cp, &c->d_p0_gen, g0);
       r = r ? r : upload(cp, &c->d_sig_gen, g1);
     }
-    if (c->prec == SYMGW_PREC_MIXED) {
-      auto prefix = [](const std::vector<Split8Blk> &v, int64_t &tot) {
-        std::vector<int64_t> p(v.size() + 1, 0);
-        for (size_t i = 0; i < v.size(); ++i) p[i + 1] = p[i] + (int64_t)v[i].nsub * v[i].nrows;
-        tot = p.back();
-        return p;
-      };
-      r = r ? r : upload(cp, &c->d_v8_split, c->W.v8_split);
-      r = r ? r : upload(cp, &c->d_p8_split, c->W.p8_split);
-      if (!c->stream) {
-        r = r ? r : upload(cp, &c->d_v8_prefix, prefix(c->W.v8_split, c->v8_rows));
-      } else {
-        // row prefixes restarted at every batch, so each batch's split launch
-        // reads its own slice of the table
-        const auto &vb = c->W.v8_batch_begin;
-        std::vector<int64_t> pre(c->W.v8_split.size() + 1, 0);
-        c->v8_batch_rows.assign(vb.empty() ? 0 : vb.size() - 1, 0);
-        for (size_t bt = 0; bt + 1 < vb.size(); ++bt) {
-          int64_t acc = 0;
-          for (int i = vb[bt]; i < vb[bt + 1]; ++i) {
-            pre[i] = acc;
-            acc += (int64_t)c->W.v8_split[i].nsub * c->W.v8_split[i].nrows;
-          }
-          c->v8_batch_rows[bt] = acc;
-        }
-        r = r ? r : upload(cp, &c->d_v8_prefix, pre);
-      }
-      r = r ? r : upload(cp, &c->d_p8_prefix, prefix(c->W.p8_split, c->p8_rows));
-    }
+    r = r ? r : upload_plan_tables(cp);
     r = r ? r : upload(cp, &c->d_g_rot, c->W.g_rot);
     r = r ? r : upload(cp, &c->d_p_rot, c->W.p_rot);
     r = r ? r : upload(cp, &c->d_w_blocks, c->W.w_blocks);
@@ -1459,6 +1493,64 @@ int symgw_set_shard(symgw_ctx *ctx, int first, int count) {
     ctx->shard_end = first + count;
     std::lock_guard<std::mutex> lk(g_dev_mu);
     drop_graphs(ctx);  // captured sweeps cover the previous shard
+    return SYMGW_OK;
+  });
+}
+
+int symgw_plan_defaults(symgw_plan *plan) {
+  return guarded(nullptr, [&]() -> int {
+    if (!plan) return SYMGW_ERR_ARG;
+    const Plan d;
+    plan->sig_chunk_cost = d.sig_chunk_cost;
+    plan->sig_rows = d.sig_rows;
+    plan->sig_max_times = d.sig_max_times;
+    plan->p0_chunk_cost = d.p0_chunk_cost;
+    plan->p0_band_cap = d.p0_band_cap;
+    plan->reserved = 0;
+    return SYMGW_OK;
+  });
+}
+
+int symgw_get_plan(const symgw_ctx *ctx, symgw_plan *plan) {
+  return guarded(nullptr, [&]() -> int {
+    if (!ctx || !plan) return SYMGW_ERR_ARG;
+    plan->sig_chunk_cost = ctx->plan.sig_chunk_cost;
+    plan->sig_rows = ctx->plan.sig_rows;
+    plan->sig_max_times = ctx->plan.sig_max_times;
+    plan->p0_chunk_cost = ctx->plan.p0_chunk_cost;
+    plan->p0_band_cap = ctx->plan.p0_band_cap;
+    plan->reserved = 0;
+    return SYMGW_OK;
+  });
+}
+
+int symgw_replan(symgw_ctx *ctx, const symgw_plan *plan) {
+  return guarded(ctx, [&]() -> int {
+    if (!ctx || !plan) return SYMGW_ERR_ARG;
+    const Plan p = plan_of(*plan);
+    if (!valid_plan(p)) return fail(ctx, SYMGW_ERR_ARG, "plan parameters out of range");
+    WorkLists W2;
+    std::string err;
+    if (!build_worklists(ctx->L, ctx->eb, W2, err, ctx->prec == SYMGW_PREC_MIXED, ctx->stream, p))
+      return fail(ctx, SYMGW_ERR_ARG, err);
+    const WorkLists &W = ctx->W;
+    // the data buffers sized from the work lists must stay valid
+    if (W2.batch_V_max != W.batch_V_max || W2.v8_bytes != W.v8_bytes || W2.p8_bytes != W.p8_bytes ||
+        W2.ve_count != W.ve_count || W2.pe_count != W.pe_count || W2.w_scr_group != W.w_scr_group ||
+        W2.d_scr_group != W.d_scr_group || W2.p0_group_first != W.p0_group_first ||
+        W2.sig_group_first != W.sig_group_first || W2.v8_batch_begin != W.v8_batch_begin)
+      return fail(ctx, SYMGW_ERR_ARG, "the plan changes buffer sizes of the context");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->st));
+    {
+      std::lock_guard<std::mutex> lk(g_dev_mu);
+      drop_graphs(ctx);  // captured sweeps read the old tables
+    }
+    ctx->W = std::move(W2);
+    ctx->plan = p;
+    int r = upload_plan_tables(ctx);
+    if (r) return r;
+    HIPCHK(ctx, hipStreamSynchronize(ctx->st));
     return SYMGW_OK;
   });
 }

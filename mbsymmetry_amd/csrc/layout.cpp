@@ -256,7 +256,7 @@ int64_t pad_to(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
 // Among the shapes that fit, pick the one with the fewest executed MFMA
 // FLOPs (16x16x4 tiles, padding included) plus a fixed cost per chunk for
 // its barriers and staging.
-bool sig_chunks(int q, int k1, int k2, int band, int budget, int TB,
+bool sig_chunks(int q, int k1, int k2, int band, int budget, int TB, double chunk_cost, int max_rows,
                 std::map<std::array<int, 4>, std::array<int, 2>> &memo, int &qsub, int &nsub) {
   const std::array<int, 4> key{q, k1, k2, band};
   auto f = memo.find(key);
@@ -265,9 +265,9 @@ bool sig_chunks(int q, int k1, int k2, int band, int budget, int TB,
     nsub = f->second[1];
     return true;
   }
-  const double chunk_cost = 2.5e5;  // ~2000 cycles of a CU's FP64 MFMA rate
+  // chunk_cost: the default 2.5e5 is ~2000 cycles of a CU's FP64 MFMA rate
   double best = -1;
-  for (int qs = 1; qs <= std::min(q, kSigRows); ++qs) {
+  for (int qs = 1; qs <= std::min(q, max_rows); ++qs) {
     const int rem = budget - 2 * TB * band;
     if (rem <= 0) break;
     // per orbital of the chunk: Z and E columns of every slab row, and a G column of every orbital m
@@ -320,10 +320,10 @@ static std::vector<std::array<int, 2>> p0_dims(const Layout &L, int q, int iq) {
 // band_cap, pick the one with the fewest executed MFMA FLOPs (16x16x4 tiles,
 // padding included) plus a fixed cost per chunk, summed over the bands of
 // the block.
-static int p0_band(int qs, int band_cap, int budget, int eb, int TB, const std::vector<std::array<int, 2>> &dims) {
+static int p0_band(int qs, int band_cap, int budget, int eb, int TB, double chunk_cost,
+                   const std::vector<std::array<int, 2>> &dims) {
   const bool stack = p0_stacked(eb, qs);  // the times share one pass
   const int tb = stack ? TB : 1, passes = stack ? 1 : TB;
-  const double chunk_cost = 2.5e5;
   int best_band = balanced(qs, band_cap);
   double best = -1;
   for (int cap = band_cap; cap >= 1; --cap) {
@@ -351,8 +351,19 @@ static int p0_band(int qs, int band_cap, int budget, int eb, int TB, const std::
   return best_band;
 }
 
-bool build_worklists(const Layout &L, int eb, WorkLists &W, std::string &err, bool z8, bool stream) {
+bool valid_plan(const Plan &p) {
+  return p.sig_chunk_cost >= 0 && p.sig_chunk_cost < 1e12 && p.sig_rows >= 1 && p.sig_rows <= kSigRows &&
+         (p.sig_max_times == 1 || p.sig_max_times == 2 || p.sig_max_times == 4) && p.p0_chunk_cost >= 0 &&
+         p.p0_chunk_cost < 1e12 && p.p0_band_cap >= 1 && p.p0_band_cap <= kP0Band;
+}
+
+bool build_worklists(const Layout &L, int eb, WorkLists &W, std::string &err, bool z8, bool stream,
+                     const Plan &plan) {
   W = WorkLists();
+  if (!valid_plan(plan)) {
+    err = "plan parameters out of range";
+    return false;
+  }
   W.stream = stream;
   // streamed V: the batch buffer offset of every pair of a batch (pairs of iBZ
   // q: (k, k + q) for P0; of iBZ k: (k, k - q) for Sigma), in k / q order
@@ -450,9 +461,9 @@ bool build_worklists(const Layout &L, int eb, WorkLists &W, std::string &err, bo
         for (auto &t : L.V_irreps[p])
           if (t[0] == (int)iq) maxk1 = std::max(maxk1, L.ao_sizes[k][t[1]]);
       }
-      const int band_cap = std::min(kP0Band, budget / (2 * p0_tb * (maxk1 + 1)));
+      const int band_cap = std::min(plan.p0_band_cap, budget / (2 * p0_tb * (maxk1 + 1)));
       if (band_cap < 1) { err = fmt("P0 ao block too large for the LDS plan (k1)", maxk1); return false; }
-      const int band = p0_band(qs, band_cap, budget, eb, p0_tb, p0_dims(L, q, (int)iq));
+      const int band = p0_band(qs, band_cap, budget, eb, p0_tb, plan.p0_chunk_cost, p0_dims(L, q, (int)iq));
       int b0 = (int)W.p0_blks.size();
       for (int k = 0; k < nk; ++k) {
         int k2 = L.kpq[(size_t)k * nk + q];
@@ -531,7 +542,8 @@ bool build_worklists(const Layout &L, int eb, WorkLists &W, std::string &err, bo
     }
     for (size_t i1 = 0; i1 < L.ao_sizes[k].size(); ++i1) {
       int k1 = L.ao_sizes[k][i1];
-      const SigShape sh = sig_shape(eb, k1);
+      SigShape sh = sig_shape(eb, k1);
+      sh.times = std::min(sh.times, plan.sig_max_times);
       const int TB = sh.times, band = sh.band;
       int b0 = (int)W.sig_blks.size();
       for (int q = 0; q < nk; ++q) {
@@ -548,7 +560,8 @@ bool build_worklists(const Layout &L, int eb, WorkLists &W, std::string &err, bo
           B.g2_tstride = (int32_t)(ns * L.kao_size[k2i]); B.g2_sstride = (int32_t)L.kao_size[k2i];
           B.p_base = L.qaux_off[q] * nts + L.aux_offsets[q][t[0]];
           B.p_tstride = (int32_t)L.qaux_size[q];
-          if (!sig_chunks(B.q, k1, B.k2, band, budget, TB, sig_plans, B.qsub, B.nsub)) {
+          if (!sig_chunks(B.q, k1, B.k2, band, budget, TB, plan.sig_chunk_cost, plan.sig_rows, sig_plans, B.qsub,
+                          B.nsub)) {
             err = fmt("Sigma block too large for the LDS plan (q, k2)", B.q, B.k2);
             return false;
           }

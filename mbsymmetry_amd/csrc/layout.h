@@ -260,8 +260,18 @@ struct WorkLists {
 
 // elem_bytes: bytes of one complex element in the contraction precision;
 // z8: also plan the INT8 slice copies of the mixed-precision Z product.
+// Tunable rules of the work-list planner (the autotuner's search space;
+// the defaults are the hand-measured rules).
+struct Plan {
+  double sig_chunk_cost = 2.5e5;  // fixed cost of one Sigma LDS chunk (barriers, staging), in MFMA FLOPs
+  int sig_rows = kSigRows;        // most aux rows Q of one Sigma chunk (1..kSigRows)
+  int sig_max_times = 4;          // most times per Sigma slab (1, 2 or 4)
+  double p0_chunk_cost = 2.5e5;   // the same for one P0 chunk
+  int p0_band_cap = kP0Band;      // most output rows of one P0 slab (1..kP0Band)
+};
+bool valid_plan(const Plan &p);
 bool build_worklists(const Layout &L, int elem_bytes, WorkLists &W, std::string &err, bool z8 = false,
-                     bool stream = false);
+                     bool stream = false, const Plan &plan = Plan());
 
 // Per-iteration FLOPs of P0 and Sigma, 8*M*N*K per complex GEMM
 // (analysis.cpp:229-255 and :271-298 without the +12MN epilogue).
