@@ -160,7 +160,11 @@ def tune(ctx, sysm, precision: int, sample_s: float = 0.4, use_cache: bool = Tru
     ctx.set_shard(0, ink)
     if best_plan and validate_s > 0:
         # the shard's choice against the built-in rules on whole sweeps (a
-        # shard's grid is smaller: its tail can favour other shapes)
+        # shard's grid is smaller: its tail can favour other shapes); the P0
+        # sweeps above overwrote P, so the Sigma sweeps get it back first
+        ctx.replan()
+        ctx.solve_P0()
+        ctx.solve_W()
         for which in ("Sigma", "P0"):
             picked = {k: v for k, v in best_plan.items() if (k.startswith("sig") == (which == "Sigma"))}
             if not picked:
