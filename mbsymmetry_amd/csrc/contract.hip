@@ -149,6 +149,10 @@ constexpr int kAhead = 2;
 // FP64 (no spills; Sigma +2.3%) and in P0 (FP32 +3%), two in the FP32 Sigma.
 template <typename T>
 constexpr int kAheadJob = sizeof(T) == 8 ? 1 : 2;
+// P0's E and C jobs under the dynamic deal (FP32: P0 +1.6%) or the static
+// round-robin one (FP64: the dynamic deal measured 2.5% slower).
+template <typename T>
+constexpr bool kP0Dyn = sizeof(T) == 4;
 // Sigma's Z and E products with one job layout each (column- and row-share)
 // instead of a per-chunk choice: FP64 (Sigma +1.2%, 5 -> 1 spilled registers).
 template <typename T>
@@ -364,12 +368,48 @@ __device__ __forceinline__ void job_store(const Tile3<T> &acc, const F &f) {
   }
 }
 
-// C (M x N) = A (M x K) B (K x N) into LDS as independent jobs dealt to the
-// waves round-robin, job j to wave (first + j) % NW; returns the job count so
-// that a following product can continue the deal.  Loaders clamp (see above).  Row-share jobs are
-// (row tile, group of NG column tiles) sharing the A fragment of each k-step;
-// column-share jobs are (group of NG row tiles, column tile) sharing B.
-// la(row, k), lb(k, col), st(row, col, value) (st only for row < M, col < N).
+// C (M x N) = A (M x K) B (K x N) into LDS as independent jobs.  Row-share
+// jobs are (row tile, group of NG column tiles) sharing the A fragment of each
+// k-step; column-share jobs are (group of NG row tiles, column tile) sharing
+// B.  la(row, k), lb(k, col), st(row, col, value) (st only for row < M,
+// col < N); loaders clamp (see above).
+template <typename T, int NG, bool CB, int AH, typename LA, typename LB, typename ST>
+__device__ __forceinline__ void job_rs(int M, int N, int K, int mt, int gi, int ntn, const LA &la, const LB &lb,
+                                       const ST &st) {
+  const int g = gi * NG, n = min(NG, ntn - g), m0 = mt * 16;
+  Tile3<T> acc[NG];
+#pragma unroll
+  for (int c = 0; c < NG; ++c) tile_zero(acc[c]);
+  mma_rowshare<0, NG, CB, AH>(acc, n, 0, K, true, [&](int i, int k) { return la(m0 + i, k); },
+                             [&](int c, int k, int jj) { return lb(k, (g + c) * 16 + jj); });
+#pragma unroll
+  for (int c = 0; c < NG; ++c)
+    if (c < n)
+      job_store<CB>(acc[c], [&](int i, int jj, cplx<T> v) {
+        const int row = m0 + i, col = (g + c) * 16 + jj;
+        if (row < M && col < N) st(row, col, v);
+      });
+}
+template <typename T, int NG, bool CB, int AH, typename LA, typename LB, typename ST>
+__device__ __forceinline__ void job_cs(int M, int N, int K, int gt, int nt, int ntm, const LA &la, const LB &lb,
+                                       const ST &st) {
+  const int g = gt * NG, n = min(NG, ntm - g), n0 = nt * 16;
+  Tile3<T> acc[NG];
+#pragma unroll
+  for (int c = 0; c < NG; ++c) tile_zero(acc[c]);
+  mma_colshare<0, NG, CB, AH>(acc, n, 0, K, [&](int c, int i, int k) { return la((g + c) * 16 + i, k); },
+                             [&](int k, int jj) { return lb(k, n0 + jj); });
+#pragma unroll
+  for (int c = 0; c < NG; ++c)
+    if (c < n)
+      job_store<CB>(acc[c], [&](int i, int jj, cplx<T> v) {
+        const int row = (g + c) * 16 + i, col = n0 + jj;
+        if (row < M && col < N) st(row, col, v);
+      });
+}
+
+// Static round-robin deal: job j to wave (first + j) % NW; returns the job
+// count so that a following product can continue the deal.
 template <typename T, int NG, bool CB, int AH, int NW, typename LA, typename LB, typename ST>
 __device__ __forceinline__ int gemm_jobs_rs_g(int M, int N, int K, int first, const LA &la, const LB &lb,
                                               const ST &st) {
@@ -378,21 +418,8 @@ __device__ __forceinline__ int gemm_jobs_rs_g(int M, int N, int K, int first, co
   // job j = mt * ngr + gi, advanced by NW per round without dividing
   int j = (wave - first % NW + NW) % NW, mt = j / ngr, gi = j - mt * ngr;
   const int dmt = NW / ngr, dgi = NW - dmt * ngr;
-  for (; j < njobs; j += NW, gi += dgi, mt += dmt + (gi >= ngr ? 1 : 0), gi -= gi >= ngr ? ngr : 0) {
-    const int g = gi * NG, n = min(NG, ntn - g), m0 = mt * 16;
-    Tile3<T> acc[NG];
-#pragma unroll
-    for (int c = 0; c < NG; ++c) tile_zero(acc[c]);
-    mma_rowshare<0, NG, CB, AH>(acc, n, 0, K, true, [&](int i, int k) { return la(m0 + i, k); },
-                               [&](int c, int k, int jj) { return lb(k, (g + c) * 16 + jj); });
-#pragma unroll
-    for (int c = 0; c < NG; ++c)
-      if (c < n)
-        job_store<CB>(acc[c], [&](int i, int jj, cplx<T> v) {
-          const int row = m0 + i, col = (g + c) * 16 + jj;
-          if (row < M && col < N) st(row, col, v);
-        });
-  }
+  for (; j < njobs; j += NW, gi += dgi, mt += dmt + (gi >= ngr ? 1 : 0), gi -= gi >= ngr ? ngr : 0)
+    job_rs<T, NG, CB, AH>(M, N, K, mt, gi, ntn, la, lb, st);
   return njobs;
 }
 
@@ -404,21 +431,8 @@ __device__ __forceinline__ int gemm_jobs_cs_g(int M, int N, int K, int first, co
   // job j = gt * ntn + nt, advanced by NW per round without dividing
   int j = (wave - first % NW + NW) % NW, gt = j / ntn, nt = j - gt * ntn;
   const int dgt = NW / ntn, dnt = NW - dgt * ntn;
-  for (; j < njobs; j += NW, nt += dnt, gt += dgt + (nt >= ntn ? 1 : 0), nt -= nt >= ntn ? ntn : 0) {
-    const int g = gt * NG, n = min(NG, ntm - g), n0 = nt * 16;
-    Tile3<T> acc[NG];
-#pragma unroll
-    for (int c = 0; c < NG; ++c) tile_zero(acc[c]);
-    mma_colshare<0, NG, CB, AH>(acc, n, 0, K, [&](int c, int i, int k) { return la((g + c) * 16 + i, k); },
-                               [&](int k, int jj) { return lb(k, n0 + jj); });
-#pragma unroll
-    for (int c = 0; c < NG; ++c)
-      if (c < n)
-        job_store<CB>(acc[c], [&](int i, int jj, cplx<T> v) {
-          const int row = (g + c) * 16 + i, col = n0 + jj;
-          if (row < M && col < N) st(row, col, v);
-        });
-  }
+  for (; j < njobs; j += NW, nt += dnt, gt += dgt + (nt >= ntn ? 1 : 0), nt -= nt >= ntn ? ntn : 0)
+    job_cs<T, NG, CB, AH>(M, N, K, gt, nt, ntm, la, lb, st);
   return njobs;
 }
 
@@ -442,6 +456,66 @@ __device__ __forceinline__ int gemm_jobs_cs(int M, int N, int K, int first, cons
   const int ntm = (M + 15) / 16, ntn = (N + 15) / 16;
   if (((ntm + NG - 1) / NG) * ntn < NW) return gemm_jobs_cs_g<T, 1, CB, AH, NW>(M, N, K, first, la, lb, st);
   return gemm_jobs_cs_g<T, NG, CB, AH, NW>(M, N, K, first, la, lb, st);
+}
+
+// Dynamic deal of the jobs of one phase (the products between two barriers),
+// numbered across the phase's products in order: a wave takes its next job
+// index from a counter in LDS, one atomic per job, so a wave whose jobs ran
+// short (fewer k-steps, cache hits) takes more and the waves reach the
+// phase's barrier together.  Which wave runs a job changes nothing in its
+// arithmetic, so the results are the same bits as with any other deal.  The
+// counter is set to zero by one thread before the barrier that opens the
+// phase (no wave takes jobs of the phase before it, and every wave has
+// taken its last job of the previous phase before that barrier).
+struct Deal {
+  int *ctr;  // the counter, in LDS
+  int j;     // this wave's current job index (wave-uniform)
+  __device__ __forceinline__ int take() {
+    int v = 0;
+    if ((threadIdx.x & 63) == 0) v = atomicAdd(ctr, 1);
+    return __builtin_amdgcn_readfirstlane(v);
+  }
+  __device__ __forceinline__ void open() { j = take(); }
+};
+// The product's jobs are [base, base + njobs) of the phase; returns njobs.
+template <typename T, int NG, bool CB, int AH, typename LA, typename LB, typename ST>
+__device__ __forceinline__ int gemm_jobs_rs_dyn(int M, int N, int K, int base, Deal &d, const LA &la, const LB &lb,
+                                                const ST &st) {
+  const int ntm = (M + 15) / 16, ntn = (N + 15) / 16, ngr = (ntn + NG - 1) / NG, njobs = ntm * ngr;
+  for (; d.j - base < njobs; d.j = d.take()) {
+    const int jj = d.j - base, mt = jj / ngr;
+    job_rs<T, NG, CB, AH>(M, N, K, mt, jj - mt * ngr, ntn, la, lb, st);
+  }
+  return njobs;
+}
+template <typename T, int NG, bool CB, int AH, typename LA, typename LB, typename ST>
+__device__ __forceinline__ int gemm_jobs_cs_dyn(int M, int N, int K, int base, Deal &d, const LA &la, const LB &lb,
+                                                const ST &st) {
+  const int ntm = (M + 15) / 16, ntn = (N + 15) / 16, ngr = (ntm + NG - 1) / NG, njobs = ngr * ntn;
+  for (; d.j - base < njobs; d.j = d.take()) {
+    const int jj = d.j - base, gt = jj / ntn;
+    job_cs<T, NG, CB, AH>(M, N, K, gt, jj - gt * ntn, ntm, la, lb, st);
+  }
+  return njobs;
+}
+// Grouped or single-tile jobs, as gemm_jobs_rs / gemm_jobs_cs choose them.
+template <typename T, bool CB = false, int AH = kAheadJob<T>, int NW = Cfg<T>::NW, typename LA, typename LB,
+          typename ST>
+__device__ __forceinline__ int gemm_jobs_rs_d(int M, int N, int K, int base, Deal &d, const LA &la, const LB &lb,
+                                              const ST &st) {
+  constexpr int NG = Group<T>::n;
+  const int ntm = (M + 15) / 16, ntn = (N + 15) / 16;
+  if (ntm * ((ntn + NG - 1) / NG) < NW) return gemm_jobs_rs_dyn<T, 1, CB, AH>(M, N, K, base, d, la, lb, st);
+  return gemm_jobs_rs_dyn<T, NG, CB, AH>(M, N, K, base, d, la, lb, st);
+}
+template <typename T, bool CB = false, int AH = kAheadJob<T>, int NW = Cfg<T>::NW, typename LA, typename LB,
+          typename ST>
+__device__ __forceinline__ int gemm_jobs_cs_d(int M, int N, int K, int base, Deal &d, const LA &la, const LB &lb,
+                                              const ST &st) {
+  constexpr int NG = Group<T>::n;
+  const int ntm = (M + 15) / 16, ntn = (N + 15) / 16;
+  if (((ntm + NG - 1) / NG) * ntn < NW) return gemm_jobs_cs_dyn<T, 1, CB, AH>(M, N, K, base, d, la, lb, st);
+  return gemm_jobs_cs_dyn<T, NG, CB, AH>(M, N, K, base, d, la, lb, st);
 }
 
 // ---- mixed precision: the Z product of Sigma on INT8 slices ----------------
@@ -470,23 +544,23 @@ __device__ __forceinline__ double z8_value(const v4i (&acc)[kZ8Slices], int e, i
 
 // Z jobs of one chunk: Z[row][col] = conj(sum_P P[row][P] conj(V[P][col])),
 // rows (tb, r) -> P row Q0 + r at time t0 + tb, cols (jj, l) -> V column
-// (J0 + jj) k2 + n0 + l.  Single-tile jobs dealt as gemm_jobs_cs does
-// (job j to wave (first + j) % NW); returns the job count.  Per real product
+// (J0 + jj) k2 + n0 + l.  Single-tile jobs, the first product of the phase
+// (jobs [0, njobs) of the dynamic deal); returns the job count.  Per real product
 // and 64-wide chunk of K the five A and five B slices are loaded once and the
 // 15 slice pairs with a + b < 5 issued from registers.
-template <int NW, typename T>
-__device__ __forceinline__ int z8_jobs(int M, int N, int first, const int8_t *__restrict__ P8,
+template <typename T>
+__device__ __forceinline__ int z8_jobs(int M, int N, Deal &d, const int8_t *__restrict__ P8,
                                        const int *__restrict__ PE, int64_t p8_tstride, int pe_tstride, int q,
                                        int kp, int Q0, FDiv dnr, int nr, const int8_t *__restrict__ V8,
                                        const int *__restrict__ VE, int ncol, int J0, int n0, int k2, FDiv dnn,
                                        int nn, int Jb, int ldz, cplx<T> *bufZ) {
   constexpr int NS = kZ8Slices;
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int lane = threadIdx.x & 63;
   const int ntm = (M + 15) / 16, ntn = (N + 15) / 16, njobs = ntm * ntn;
   const int64_t ss_a = (int64_t)q * kp, ss_b = (int64_t)ncol * kp;  // slice strides
   const int nch = (kp + 63) / 64, kg = 16 * (lane >> 4);
-  for (int j = (wave - first % NW + NW) % NW; j < njobs; j += NW) {
-    const int mt = j / ntn, nt = j - mt * ntn;
+  for (; d.j < njobs; d.j = d.take()) {
+    const int j = d.j, mt = j / ntn, nt = j - mt * ntn;
     // this lane's operand row and column
     const int arow = min(mt * 16 + (lane & 15), M - 1), acol = min(nt * 16 + (lane & 15), N - 1);
     const int atb = dnr.div(arow), ar = arow - atb * nr;
@@ -639,7 +713,10 @@ __global__ __launch_bounds__(Cfg<T>::NT) void p0_mfma_kernel(const P0Item *__res
   constexpr int MTP = TB > MT ? TB : MT;
   const int tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  cplx<T> *lds = reinterpret_cast<cplx<T> *>(smem_raw);
+  // the job counter of the dynamic deal, then the data (from the start of
+  // the LDS under the static deal)
+  Deal deal{reinterpret_cast<int *>(smem_raw), 0};
+  cplx<T> *lds = reinterpret_cast<cplx<T> *>(smem_raw + (kP0Dyn<T> ? kDealBytes : 0));
   const int q = it.q, nband = it.nband, nct = (q + 15) / 16;
   // MODE 1: every slab stacked, 0: none, 2: per slab
   const bool stack = MODE == 2 ? nct <= NW : MODE == 1;
@@ -672,7 +749,14 @@ __global__ __launch_bounds__(Cfg<T>::NT) void p0_mfma_kernel(const P0Item *__res
         for (int n0 = 0; n0 < k2; n0 += nsub) {
           const int nn = min(nsub, k2 - n0), X = k1 * nn;
           const FDiv dnn(nn);
+          // under the dynamic deal the E and C jobs of the chunk are one run
+          // of it (C's continue E's numbering after the barrier between them)
+          if constexpr (kP0Dyn<T>) {
+            if (tid == 0) *deal.ctr = 0;
+          }
           __syncthreads();
+          if constexpr (kP0Dyn<T>) deal.open();
+          int ne;
           // E_tb[(r,a)][l] = sum_m V[Q0+r][a][m] G2_tb[m][n0+l], columns (tb, l)
           {
             auto la = [&](int row, int m) {
@@ -688,29 +772,35 @@ __global__ __launch_bounds__(Cfg<T>::NT) void p0_mfma_kernel(const P0Item *__res
               const int r = dk1.div(row), a = row - r * k1, tb = dnn.div(col), l = col - tb * nn;
               bufE[(tb * nband + r) * lde + a * nn + l] = v;
             };
-            if (tbn * nn > 16) gemm_jobs_rs<T, false, 1>(nband * k1, tbn * nn, k2, 0, la, lb, st);
-            else gemm_jobs_cs<T, false, 1>(nband * k1, tbn * nn, k2, 0, la, lb, st);
+            if constexpr (kP0Dyn<T>) {
+              if (tbn * nn > 16) ne = gemm_jobs_rs_d<T, false, 1>(nband * k1, tbn * nn, k2, 0, deal, la, lb, st);
+              else ne = gemm_jobs_cs_d<T, false, 1>(nband * k1, tbn * nn, k2, 0, deal, la, lb, st);
+            } else {
+              if (tbn * nn > 16) ne = gemm_jobs_rs<T, false, 1>(nband * k1, tbn * nn, k2, 0, la, lb, st);
+              else ne = gemm_jobs_cs<T, false, 1>(nband * k1, tbn * nn, k2, 0, la, lb, st);
+            }
           }
           __syncthreads();
-          // C_tb[r][p][l] = sum_a G1_tb[p][a] E_tb[r][a][l]  (the times' jobs dealt in turn)
+          // C_tb[r][p][l] = sum_a G1_tb[p][a] E_tb[r][a][l]  (the times' jobs numbered in
+          // turn; the static deal starts over at wave 0)
           {
-            int first = 0;
+            int base = kP0Dyn<T> ? ne : 0;
             for (int tb = 0; tb < tbn; ++tb) {
               const cplx<T> *G1t = G1 - (int64_t)tb * g1ts;
               const cplx<T> *Et = bufE + tb * nband * lde;
               cplx<T> *Ct = bufC + tb * nband * lde;
-              first += gemm_jobs_rs<T, false, 1>(
-                  k1, nband * nn, k1, first,
-                  [&](int p, int a) { return G1t[min(p, k1 - 1) * k1 + min(a, k1 - 1)]; },
-                  [&](int a, int col) {
-                    col = min(col, nband * nn - 1);
-                    const int r = dnn.div(col), l = col - r * nn;
-                    return Et[r * lde + min(a, k1 - 1) * nn + l];
-                  },
-                  [&](int p, int col, cplx<T> v) {
-                    const int r = dnn.div(col), l = col - r * nn;
-                    Ct[r * lde + p * nn + l] = v;
-                  });
+              auto la = [&](int p, int a) { return G1t[min(p, k1 - 1) * k1 + min(a, k1 - 1)]; };
+              auto lb = [&](int a, int col) {
+                col = min(col, nband * nn - 1);
+                const int r = dnn.div(col), l = col - r * nn;
+                return Et[r * lde + min(a, k1 - 1) * nn + l];
+              };
+              auto st = [&](int p, int col, cplx<T> v) {
+                const int r = dnn.div(col), l = col - r * nn;
+                Ct[r * lde + p * nn + l] = v;
+              };
+              if constexpr (kP0Dyn<T>) base += gemm_jobs_rs_d<T, false, 1>(k1, nband * nn, k1, base, deal, la, lb, st);
+              else base += gemm_jobs_rs<T, false, 1>(k1, nband * nn, k1, base, la, lb, st);
             }
           }
           __syncthreads();
@@ -804,7 +894,9 @@ __global__ __launch_bounds__((SigCfg<T, Z8>::NT)) void sigma_mfma_kernel(const S
   constexpr int NW = SigCfg<T, Z8>::NW, MT = SigCfg<T, Z8>::MT, NT = SigCfg<T, Z8>::NT;
   const int tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  cplx<T> *lds = reinterpret_cast<cplx<T> *>(smem_raw);
+  // the job counter of the dynamic deal, then the data
+  Deal deal{reinterpret_cast<int *>(smem_raw), 0};
+  cplx<T> *lds = reinterpret_cast<cplx<T> *>(smem_raw + kDealBytes);
   const int k1 = it.k1, I0 = it.I0, Ib = it.Ib, J0 = it.J0, Jb = it.Jb;
   const int nti = (Ib + 15) / 16, ntj = (Jb + 15) / 16, ntile = nti * ntj, nall = tbn * ntile;
   // output tiles u = tb * ntile + tt: S waves share a tile over S parts of the
@@ -843,7 +935,9 @@ __global__ __launch_bounds__((SigCfg<T, Z8>::NT)) void sigma_mfma_kernel(const S
         const FDiv dnn(nn);
         // G chunk of the E product: every wave has passed the barrier after
         // the previous chunk's E, the only reader of bufG; the barrier below
-        // also ends the previous chunk's final product, the reader of bufZ and bufE
+        // also ends the previous chunk's final product, the reader of bufZ and
+        // bufE.  The Z and E jobs that follow it are one phase of the deal.
+        if (tid == 0) *deal.ctr = 0;
         {
           const FDiv dNC(NC);
           for (int e = tid; e < k2 * NC; e += NT) {
@@ -852,6 +946,7 @@ __global__ __launch_bounds__((SigCfg<T, Z8>::NT)) void sigma_mfma_kernel(const S
           }
         }
         __syncthreads();
+        deal.open();
         // Z[(tb,r)][(j,l)] = sum_P conj(P_tb[Q0+r][P]) V[P][J0+j][n0+l], formed as the
         // conjugate of sum_P P_tb[Q0+r][P] conj(V[P][J0+j][n0+l]) with P read
         // straight from memory (rows of the chunk, contiguous in P)
@@ -876,13 +971,13 @@ __global__ __launch_bounds__((SigCfg<T, Z8>::NT)) void sigma_mfma_kernel(const S
           if (Z8 && B.kp > 0) {
             // mixed precision: INT8 slices of P (this slab's first time) and V
             const int64_t p8t = (int64_t)kZ8Comps * kZ8Slices * q * B.kp;
-            nj = z8_jobs<NW>(tbn * nr, zc, 0, z8.P8 + B.p8_off + t0 * p8t, z8.PE + B.pe_off + (int64_t)t0 * 3 * q,
+            nj = z8_jobs(tbn * nr, zc, deal, z8.P8 + B.p8_off + t0 * p8t, z8.PE + B.pe_off + (int64_t)t0 * 3 * q,
                              p8t, 3 * q, q, B.kp, Q0, dnr, nr, z8.V8 + B.v8_off, z8.VE + B.ve_off, K, J0, n0, k2,
                              dnn, nn, Jb, ldz, bufZ);
           } else if (kSigOneLayout<T> || tbn * nr > 16) {
-            nj = gemm_jobs_cs<T, true, kAheadJob<T>, NW>(tbn * nr, zc, q, 0, la, lb, st);
+            nj = gemm_jobs_cs_d<T, true, kAheadJob<T>, NW>(tbn * nr, zc, q, 0, deal, la, lb, st);
           } else {
-            nj = gemm_jobs_rs<T, true, kAheadJob<T>, NW>(tbn * nr, zc, q, 0, la, lb, st);
+            nj = gemm_jobs_rs_d<T, true, kAheadJob<T>, NW>(tbn * nr, zc, q, 0, deal, la, lb, st);
           }
         }
         // E[(r,i)][(tb,l)] = sum_m V[Q0+r][I0+i][m] G2_tb[m][n0+l]  (jobs dealt on after Z's)
@@ -897,8 +992,10 @@ __global__ __launch_bounds__((SigCfg<T, Z8>::NT)) void sigma_mfma_kernel(const S
             const int r = dIb.div(row), ii = row - r * Ib, tb = dnn.div(col), l = col - tb * nn;
             bufE[(tb * Ib + ii) * lde + r * nn + l] = v;
           };
-          if (kSigOneLayout<T> || tbn * nn > 16) gemm_jobs_rs<T, false, kAheadJob<T>, NW>(nr * Ib, tbn * nn, k2, nj, la, lb, st);
-          else gemm_jobs_cs<T, false, kAheadJob<T>, NW>(nr * Ib, tbn * nn, k2, nj, la, lb, st);
+          if (kSigOneLayout<T> || tbn * nn > 16)
+            gemm_jobs_rs_d<T, false, kAheadJob<T>, NW>(nr * Ib, tbn * nn, k2, nj, deal, la, lb, st);
+          else
+            gemm_jobs_cs_d<T, false, kAheadJob<T>, NW>(nr * Ib, tbn * nn, k2, nj, deal, la, lb, st);
         }
         __syncthreads();
         // acc[i][j] += sum_y E_tb[i][y] conj(Z_tb[j][y]) over this wave's part of y

@@ -423,7 +423,7 @@ bool build_worklists(const Layout &L, int eb, WorkLists &W, std::string &err, bo
     return stream ? batch_off[p] + L.V_offsets[p][b] : L.kpair_off[p] + L.V_offsets[p][b];
   };
   const int nk = L.nk, nts = L.nts, ns = L.ns;
-  const int budget = contract_lds(eb) / eb;  // complex elements of LDS per workgroup
+  const int budget = (contract_lds(eb) - kDealBytes) / eb;  // complex elements of LDS per workgroup for data
   flop_model(L, W.flop_p0, W.flop_sig, W.n_p0_contractions, W.n_sig_contractions);
 
   // ---- P0: items per (q_ir, iq, band of <= 16 rows); blocks in (k, V-block) order ----
@@ -509,7 +509,7 @@ bool build_worklists(const Layout &L, int eb, WorkLists &W, std::string &err, bo
   const size_t red_bytes = (size_t)contract_waves(eb) * contract_tiles(eb) * 64 * 8 * (eb / 2);
   // P0 keeps one accumulator per stacked time (at least contract_tiles)
   const size_t red_p0 = (size_t)contract_waves(eb) * std::max(contract_tiles(eb), p0_tb) * 64 * 8 * (eb / 2);
-  W.p0_lds = std::max(p0_lds, red_p0);
+  W.p0_lds = std::max(p0_lds, red_p0) + kDealBytes;
 
   // ---- Sigma: items per (k_ir, ik1, row band, column band); blocks in (q, V-block) order ----
   size_t sig_lds[kNumSigDepths] = {};
@@ -635,7 +635,8 @@ bool build_worklists(const Layout &L, int eb, WorkLists &W, std::string &err, bo
   if (stream && z8) W.v8_batch_begin.push_back((int)W.v8_split.size());
   // the INT8-Z Sigma kernel reduces split-K parts over its own wave / tile shape
   const size_t red_sig = z8 ? (size_t)kZ8Waves * kZ8Tiles * 64 * 8 * (eb / 2) : red_bytes;
-  for (int d = 0; d < kNumSigDepths; ++d) W.sig_lds[d] = W.sig_items[d].empty() ? 0 : std::max(sig_lds[d], red_sig);
+  for (int d = 0; d < kNumSigDepths; ++d)
+    W.sig_lds[d] = W.sig_items[d].empty() ? 0 : std::max(sig_lds[d], red_sig) + kDealBytes;
   for (int d = 0; d < kNumSigDepths; ++d)
     if (W.sig_lds[d] > (size_t)contract_lds(eb)) { err = fmt("Sigma LDS plan exceeds the CU's LDS (bytes)", (int)W.sig_lds[d]); return false; }
   if (W.p0_lds > (size_t)contract_lds(eb)) { err = fmt("P0 LDS plan exceeds the CU's LDS (bytes)", (int)W.p0_lds); return false; }

@@ -51,6 +51,9 @@ bool build_layout(const symgw_layout_desc *d, Layout &L, std::string &err);
 // 12 (three per SIMD, 168 VGPRs) with complex64, whose kernels need the
 // registers.
 constexpr int contract_lds(int elem_bytes) { return elem_bytes > 0 ? 160 * 1024 : 0; }
+// The contraction kernels' LDS starts with the job counter of their dynamic
+// deal; the data (LDS chunks, split-K buffers) follow it.
+constexpr int kDealBytes = 16;
 constexpr int contract_waves(int elem_bytes) { return elem_bytes == 16 ? 16 : elem_bytes > 0 ? 12 : 0; }
 // Accumulator tiles each contraction wave keeps across a slab: three with
 // complex64 operands, two with complex128 (the register file holds no more).
@@ -233,7 +236,7 @@ struct WorkLists {
   int64_t d_scr_group = 0;        // scratch elements per (w, s) (n^2 per block)
   int64_t n_p0_contractions = 0, n_sig_contractions = 0;
   double flop_p0 = 0, flop_sig = 0;
-  size_t p0_lds = 0, sig_lds[kNumSigDepths] = {};  // dynamic LDS bytes of the contraction kernels
+  size_t p0_lds = 0, sig_lds[kNumSigDepths] = {};  // dynamic LDS bytes of the contraction kernels (job counter included)
   // mixed mode: INT8 split tables (V blocks read by Sigma; P blocks of every q)
   std::vector<Split8Blk> v8_split, p8_split;
   // with a streamed V the V splits of each Sigma batch start at
