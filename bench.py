@@ -60,7 +60,7 @@ def setup_ctx(sysm, precision, mu):
     return ctx
 
 
-def run_path(sysm, precision, steps, warmup, mu, G0, keep_P=False, tune=True):
+def run_path(sysm, precision, steps, warmup, mu, G0, keep_P=False, tune=True, plan=None):
     """Time `steps` GW iterations replayed from per-phase hipGraphs.
 
     The first iteration from G0 is untimed (it is warmup step 1) and gives the
@@ -68,11 +68,15 @@ def run_path(sysm, precision, steps, warmup, mu, G0, keep_P=False, tune=True):
     between the phase graphs on the context stream, so the per-phase device
     times are averages over exactly the timed iterations.  keep_P: also return
     that iteration's P0 and P (recomputed from G0 after it, outside timing).
-    tune: pick the work-list plan with the autotuner first (outside timing)."""
+    tune: pick the work-list plan with the autotuner first (outside timing);
+    plan: a fixed plan (dict of symgw_plan rules) instead."""
     ctx = setup_ctx(sysm, precision, mu)
     ctx.upload_G(G0)
     tuned = None
-    if tune:
+    if plan is not None:
+        ctx.replan(**plan)
+        tuned = {"plan": ctx.plan(), "fixed": True}
+    elif tune:
         from mbsymmetry_amd import autotune
         tuned = autotune.tune(ctx, sysm, precision, use_cache=False, log=log)
         tuned = {k: tuned[k] for k in ("plan", "Sigma", "P0", "seconds")}
@@ -253,6 +257,80 @@ def config3_run(tune=True):
     return out
 
 
+def config4_run(shard_points=2, nodes=6):
+    """The precision-scaling system (8x8x8 mesh reduced to 29 irreducible
+    points; nao 256 in 6 blocks, naux 1200 in 6 blocks; beta 200) on a
+    sample, with the three GEMM paths: FP64, FP32 (3M complex products) and
+    mixed (INT8 Z product).  Its V (~5 TB in FP64) is generated on the device
+    per batch of k-pairs inside the sweeps (SYMGW_FLAG_STREAM_V), so the phase
+    times include the generation.  The sample: P0 and Sigma over the first
+    `shard_points` irreducible points (symgw_set_shard) on a `nodes`-node
+    Chebyshev grid; per time the work is that of the 144-point grid, whose
+    full-zone P (287 GB) does not fit one GPU.  P for Sigma comes from the FP64
+    context's W solve (P0 zero outside the shard) and is uploaded to every
+    path, so the three Sigma sweeps see the same inputs; max|dSigma| of FP32
+    and mixed is against FP64 on the shard's rows."""
+    from mbsymmetry_amd import _lib, system as S
+    from mbsymmetry_amd.transforms import chebyshev
+    t0 = time.perf_counter()
+    sysm = S.config4(transforms=chebyshev(nodes, 200.0))
+    log(f"config4: system built in {time.perf_counter() - t0:.1f} s")
+    nk, ink = sysm.nk, sysm.ink
+    out = {"workload": (f"config4: 8x8x8 k-mesh ({ink} iBZ points), nao 256 in blocks {sysm.ao_sizes[0]}, naux 1200 in "
+                        f"blocks {sysm.aux_sizes[0]}, V streamed (generated on the device per batch of k-pairs)"),
+           "sample": (f"P0 and Sigma over iBZ points 0..{shard_points - 1} of {ink} on a {nodes}-node Chebyshev grid "
+                      f"(nts {sysm.nts}), beta 200; phase times include V generation")}
+    ko = sysm.kao_off_irre() + [sysm.flat_ao]
+    rows = slice(ko[0], ko[shard_points])
+    sig = {}
+    G0 = P = None
+    info = None
+    for name, prec in (("fp64", _lib.PREC_FP64), ("fp32", _lib.PREC_FP32), ("mixed", _lib.PREC_MIXED)):
+        ctx = _lib.Context(sysm, 0, prec | _lib.FLAG_STREAM_V)
+        ctx.generate_V(sysm.v_seed, sysm.v_scale, sysm.decay_table())
+        ctx.set_reps(sysm.orep, sysm.auxrep)
+        ctx.set_transforms(sysm.transforms)
+        ctx.set_fock(sysm.F, sysm.S)
+        if G0 is None:
+            ctx.upload_G(sysm.zeros_ts_kij())
+            ctx.dyson(0.0, 0.0)
+            G0 = ctx.download_G((sysm.nts, sysm.ns, sysm.flat_ao))
+        ctx.set_shard(0, shard_points)
+        ctx.upload_G(G0)
+        ctx.upload_P(sysm.zeros_q_t_QP())
+        ctx.set_timing(True)
+        ctx.solve_P0()
+        ctx.sync()
+        p0_ms = float(ctx.last_times()[0])
+        if P is None:
+            ctx.solve_W()
+            P = ctx.copy_P0_to_host(sysm.zeros_q_t_QP())
+        ctx.upload_P(P)
+        ctx.solve_Sigma()
+        ctx.sync()
+        sig_ms = float(ctx.last_times()[2])
+        sig[name] = ctx.copy_Sigma_to_host(sysm.zeros_ts_kij())[:, :, rows] / nk
+        info = ctx.info()
+        ctx.close()
+        f_p0 = info.flop_P0 * shard_points / ink
+        f_sig = info.flop_Sigma * shard_points / ink
+        out[name] = {"P0_ms": p0_ms, "Sigma_ms": sig_ms, "P0_tflops": f_p0 / (p0_ms * 1e-3) / 1e12,
+                     "Sigma_tflops": f_sig / (sig_ms * 1e-3) / 1e12}
+        if name != "fp64":
+            out[name]["max_abs_dSigma_vs_fp64"] = float(np.abs(sig[name] - sig["fp64"]).max())
+        log("config4", name, out[name])
+    out["max_abs_Sigma"] = float(np.abs(sig["fp64"]).max())
+    out["sample_flop_P0"] = info.flop_P0 * shard_points / ink
+    out["sample_flop_Sigma"] = info.flop_Sigma * shard_points / ink
+    # one whole iteration on the 144-point grid at the sample's rates (P0 and
+    # Sigma scale with the number of times: nts/2 and nts)
+    scale = 146.0 / sysm.nts
+    out["full_iteration_estimate_s"] = {
+        n: (info.flop_P0 * scale / (out[n]["P0_tflops"] * 1e12) + info.flop_Sigma * scale / (out[n]["Sigma_tflops"] * 1e12))
+        for n in ("fp64", "fp32", "mixed")}
+    return out
+
+
 def serving_run(window_s, rate, deadline_s, precisions=("fp64", "mixed"), n_systems=32):
     """The serving loop on the 32-system mix (nao in {26, 52, 104, 156, 256},
     meshes 2^3 / 4^3 / 6^3): systems resident on the device (loaded before the
@@ -304,7 +382,11 @@ def main():
     ap.add_argument("--mu", type=float, default=0.0)
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-config3", action="store_true", help="skip the uneven-block (config 3) leg")
+    ap.add_argument("--no-config4", action="store_true", help="skip the sampled precision-scaling (config 4) leg")
     ap.add_argument("--no-tune", action="store_true", help="keep the built-in work-list plan (no autotuner)")
+    ap.add_argument("--plan", default=None,
+                    help="fixed work-list plan for the config-2 legs instead of the autotuner, as JSON "
+                         "(e.g. '{\"sig_max_times\": 2}'; the rules not named keep their defaults)")
     ap.add_argument("--serving-only", action="store_true", help="run only the serving mix and print its JSON")
     ap.add_argument("--serving-window", type=float, default=2.0, help="seconds of Poisson arrivals")
     ap.add_argument("--serving-deadline", type=float, default=30.0,
@@ -338,6 +420,7 @@ def main():
                                                  args.serving_modes.split(","))}),
               flush=True)
         return
+    fixed_plan = json.loads(args.plan) if args.plan else None
     sysm = make_system(args.config)
     log("system", workload_name(sysm))
     peaks = _lib.measure_peaks(0)  # the box's own peaks, beside the datasheet values
@@ -351,14 +434,16 @@ def main():
     log("G0 ready")
 
     plain = run_path(sysm, _lib.PREC_FP64, args.steps, args.warmup, args.mu, G0, keep_P=not args.no_cpu,
-                     tune=not args.no_tune)
+                     tune=not args.no_tune, plan=fixed_plan)
     log("plain fp64", plain["sec_per_iter"], "s/iter; phases ms", plain["phase_ms"][:4])
     mixed = fp32 = None
     if not args.no_mixed:
-        mixed = run_path(sysm, _lib.PREC_MIXED, args.steps, args.warmup, args.mu, G0, tune=not args.no_tune)
+        mixed = run_path(sysm, _lib.PREC_MIXED, args.steps, args.warmup, args.mu, G0, tune=not args.no_tune,
+                         plan=fixed_plan)
         log("mixed (int8 Z)", mixed["sec_per_iter"], "s/iter; phases ms", mixed["phase_ms"][:4])
         if args.fp32:
-            fp32 = run_path(sysm, _lib.PREC_FP32, args.steps, args.warmup, args.mu, G0, tune=not args.no_tune)
+            fp32 = run_path(sysm, _lib.PREC_FP32, args.steps, args.warmup, args.mu, G0, tune=not args.no_tune,
+                            plan=fixed_plan)
             log("fp32", fp32["sec_per_iter"], "s/iter; phases ms", fp32["phase_ms"][:4])
 
     p0_ms, sig_ms = float(plain["phase_ms"][0]), float(plain["phase_ms"][2])
@@ -370,17 +455,23 @@ def main():
     if dom == "Sigma":
         # Sigma runs one launch per slab depth (4 times per slab for ao blocks up
         # to 32 orbitals, 2 or 1 for wider ones); the phase time covers them all
+        # every block of at most 32 orbitals takes the deepest slab the plan allows
         widest = max(b for s in sysm.ao_sizes for b in s)
-        dom_kernel = "sigma_mfma_kernel<double, 4>" if widest <= 32 else "sigma_mfma_kernel<double, 1|2|4>"
+        plan = (plain["tuned"] or {}).get("plan") or _lib.plan_defaults().as_dict()
+        tb = min(4, int(plan["sig_max_times"]))
+        dom_kernel = f"sigma_mfma_kernel<double, {tb}, false>" if widest <= 32 else "sigma_mfma_kernel<double, 1|2|4, false>"
     else:
         dom_kernel = "p0_mfma_kernel<double>"
     traffic = None
+    traffic_source = "no PMC record for this kernel and plan in profiles/"
     tpath = os.path.join(ROOT, "profiles", f"traffic_{dom.lower()}_{args.config}_fp64.json")
     if os.path.exists(tpath):  # HBM bytes per launch from the FETCH_SIZE / WRITE_SIZE runs
         with open(tpath) as f:
             tr = json.load(f)
         if tr.get("kernel") == dom_kernel and tr.get("workload") == args.config:
             traffic = tr["bytes_per_launch"]
+            traffic_source = (f"profiles/{os.path.basename(tpath)}: rocprofv3 FETCH_SIZE / WRITE_SIZE passes of "
+                              f"{tr.get('command', 'bench.py')} at commit {tr.get('commit', '?')}")
     out = {
         "metric": "GW iter/s & P0+Sigma TFLOP/s vs plain FP64 GPU at max|dSigma|<=1e-8 Ha; p99 latency",
         "value": 1.0 / plain["sec_per_iter"],
@@ -400,7 +491,7 @@ def main():
         "phase_ms": {"P0_kernel": p0_ms, "W+P_rotation": float(plain["phase_ms"][1]), "Sigma_kernel": sig_ms,
                      "Dyson+G_rotation": float(plain["phase_ms"][3])},
         "roofline": {"bound": "mfma", "kernel": dom_kernel, "achieved": achieved, "peak": PEAK["fp64"],
-                     "unit": "TFLOP/s", "frac": achieved / PEAK["fp64"], "traffic": traffic,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK["fp64"], "traffic": traffic, "traffic_source": traffic_source,
                      "measured_peak": peaks["fp64_mfma_tflops"],
                      "frac_of_measured_peak": achieved / peaks["fp64_mfma_tflops"]},
         "peaks": {"datasheet": {"fp64_mfma_tflops": PEAK["fp64"], "fp32_mfma_tflops": PEAK["fp32"],
@@ -438,6 +529,8 @@ def main():
         out["fp32"]["frac_fp32_peak"] = {k: v / PEAK["fp32"] for k, v in out["fp32"]["kernel_tflops"].items()}
     if not args.no_config3:
         out["config3"] = config3_run(tune=not args.no_tune)
+    if not args.no_config4:
+        out["config4"] = config4_run()
     if not args.no_serving:
         out["serving"] = serving_run(args.serving_window, args.serving_rate, args.serving_deadline,
                                      args.serving_modes.split(","))
