@@ -532,9 +532,10 @@ typedef int v4i __attribute__((ext_vector_type(4)));
 
 // sum_d 128^-d R_d of the five diagonal accumulators, times 2^ex, as
 // 128^-1 (head + tail / 128): the head 128 R0 + R1 is exact in int32 for
-// K <= 256 (|R_d| <= (d + 1) K 127^2); the tail R2 + R3/128 + R4/128^2 is
-// formed in FP32, whose rounding (~1e-7 of the tail, ~1e-11 of the value)
-// stays below the 128^-5 truncation of the slices.
+// K <= 256 and up to three products summed (|R_0| <= 3 K 127^2, 128 |R_0| +
+// |R_1| < 1.6e9); the tail R2 + R3/128 + R4/128^2 is formed in FP32, whose
+// rounding (~1e-7 of the tail, ~1e-11 of the value) stays below the 128^-5
+// truncation of the slices.
 static_assert(kZ8Slices == 5, "z8_value combines five diagonals");
 __device__ __forceinline__ double z8_value(const v4i (&acc)[kZ8Slices], int e, int ex) {
   const int head = acc[0][e] * 128 + acc[1][e];
@@ -547,7 +548,8 @@ __device__ __forceinline__ double z8_value(const v4i (&acc)[kZ8Slices], int e, i
 // (J0 + jj) k2 + n0 + l.  Single-tile jobs, the first product of the phase
 // (jobs [0, njobs) of the dynamic deal); returns the job count.  Per real product
 // and 64-wide chunk of K the five A and five B slices are loaded once and the
-// 15 slice pairs with a + b < 5 issued from registers.
+// 15 slice pairs with a + b < 5 issued from registers; the three products'
+// sums combine in int32 and each Z element is reconstructed and written once.
 template <typename T>
 __device__ __forceinline__ int z8_jobs(int M, int N, Deal &d, const int8_t *__restrict__ P8,
                                        const int *__restrict__ PE, int64_t p8_tstride, int pe_tstride, int q,
@@ -570,13 +572,15 @@ __device__ __forceinline__ int z8_jobs(int M, int N, Deal &d, const int8_t *__re
     // output element e of this lane: row 4 (lane / 16) + e, column lane % 16
     const int col = nt * 16 + (lane & 15), cc = min(col, N - 1), ojj = dnn.div(cc), ol = cc - ojj * nn;
     const int vcol = (J0 + ojj) * k2 + n0 + ol;
-    // Z = conj(A conj(B)) = (P1 + P2) - i (P3 - P1 + P2), accumulated product by
-    // product in this lane's own Z elements (no other lane touches them)
-#pragma unroll 1
-    for (int c = 0; c < kZ8Comps; ++c) {
-      v4i acc[NS];
+    // Z = conj(A conj(B)) = (P1 + P2) - i (P3 - P1 + P2).  A row (B column)
+    // has one exponent for its three components, so the products' integer
+    // sums combine exactly before the one reconstruction per part:
+    // X = P1 + P2 (the real part) and Y = P3 - P1 + P2 (minus the imaginary
+    // part), Y started as -P1 and the P2 MFMAs issued into both.
+    v4i X[NS], Y[NS];
 #pragma unroll
-      for (int d = 0; d < NS; ++d) acc[d] = v4i{0, 0, 0, 0};
+    for (int dd = 0; dd < NS; ++dd) X[dd] = v4i{0, 0, 0, 0};
+    auto comp = [&](int c, v4i (&A1)[NS], v4i (&A2)[NS], bool two) {
       const int8_t *Ac = Arow + c * NS * ss_a, *Bc = Bcol + c * NS * ss_b;
       for (int ch = 0; ch < nch; ++ch) {
         // unconditional loads (k beyond kp reads the row's first bytes and is
@@ -595,21 +599,25 @@ __device__ __forceinline__ int z8_jobs(int M, int N, Deal &d, const int8_t *__re
 #pragma unroll
         for (int sa = 0; sa < NS; ++sa)
 #pragma unroll
-          for (int sb = 0; sa + sb < NS; ++sb)
-            acc[sa + sb] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[sa], b[sb], acc[sa + sb], 0, 0, 0);
+          for (int sb = 0; sa + sb < NS; ++sb) {
+            A1[sa + sb] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[sa], b[sb], A1[sa + sb], 0, 0, 0);
+            if (two) A2[sa + sb] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[sa], b[sb], A2[sa + sb], 0, 0, 0);
+          }
       }
-      const int eb = VE[c * ncol + vcol];
+    };
+    comp(0, X, Y, false);                       // X = P1
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int row = mt * 16 + 4 * (lane >> 4) + e;
-        if (col >= N || row >= M) continue;
-        const int otb = dnr.div(row), orr = row - otb * nr;
-        const double v = z8_value(acc, e, PE[otb * pe_tstride + c * q + Q0 + orr] + eb);
-        cplx<T> &z = bufZ[(otb * Jb + ojj) * ldz + orr * nn + ol];
-        if (c == 0) z = cplx<T>{(T)v, (T)v};                  // P1: re P1, -im = P1
-        else if (c == 1) z = cplx<T>{z.re + (T)v, z.im - (T)v};  // P2: re + P2, -im - P2
-        else z.im -= (T)v;                                       // P3: -im - P3
-      }
+    for (int dd = 0; dd < NS; ++dd) Y[dd] = v4i{0, 0, 0, 0} - X[dd];  // Y = -P1
+    comp(1, X, Y, true);                        // X = P1 + P2, Y = P2 - P1
+    comp(2, Y, Y, false);                       // Y = P3 - P1 + P2
+    const int eb = VE[vcol];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int row = mt * 16 + 4 * (lane >> 4) + e;
+      if (col >= N || row >= M) continue;
+      const int otb = dnr.div(row), orr = row - otb * nr;
+      const int ex = PE[otb * pe_tstride + Q0 + orr] + eb;
+      bufZ[(otb * Jb + ojj) * ldz + orr * nn + ol] = cplx<T>{(T)z8_value(X, e, ex), (T)-z8_value(Y, e, ex)};
     }
   }
   return njobs;
@@ -618,7 +626,8 @@ __device__ __forceinline__ int z8_jobs(int M, int N, Deal &d, const int8_t *__re
 }  // namespace
 
 // Split of real-matrix rows into INT8 slices (Split8Blk), one wave per row:
-// per component the row maximum m gives e with m 2^-e <= 127, then
+// the row maximum m over its three components gives one e with m 2^-e <= 127
+// for all three (so that the Z product can add their integer sums), then
 // x 2^-e = sum_{s < kZ8Slices} x_s 128^-s + r with |r| <= 128^-(kZ8Slices-1)/2:
 // rint, an exact subtraction and an exact scaling per slice.
 __global__ __launch_bounds__(256) void split8_kernel(const Split8Blk *__restrict__ sb, int nsb,
@@ -648,13 +657,10 @@ __global__ __launch_bounds__(256) void split8_kernel(const Split8Blk *__restrict
 #pragma unroll
     for (int c = 0; c < kZ8Comps; ++c)
       for (int o = 32; o > 0; o >>= 1) m[c] = fmax(m[c], __shfl_xor(m[c], o));
-    int e[kZ8Comps];
-#pragma unroll
-    for (int c = 0; c < kZ8Comps; ++c) {
-      int xe = 0;
-      if (m[c] > 0.0) (void)frexp(m[c] / 127.0, &xe);
-      e[c] = xe;
-    }
+    const double mm = fmax(m[0], fmax(m[1], m[2]));
+    int xe = 0;
+    if (mm > 0.0) (void)frexp(mm / 127.0, &xe);
+    const int e[kZ8Comps] = {xe, xe, xe};
     int8_t *d = dst + S.dst_off + (int64_t)u * kZ8Comps * kZ8Slices * S.nrows * S.kp + (int64_t)r * S.kp;
     if (lane < kZ8Comps) ex[S.e_off + (int64_t)u * 3 * S.nrows + (int64_t)lane * S.nrows + r] = e[lane];
     for (int k = lane; k < S.kp; k += 64) {

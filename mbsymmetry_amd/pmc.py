@@ -46,15 +46,19 @@ def _rows(pattern_dir: str, suffix: str) -> List[dict]:
 
 
 def _short(name: str) -> str:
-    """Kernel name without the argument list."""
+    """Kernel name without its argument list (the parenthesised group that
+    ends the name; "(anonymous namespace)" inside the name stays)."""
+    name = name.strip()
+    if not name.endswith(")"):
+        return name
     depth = 0
-    for i, ch in enumerate(name):
-        if ch == "<":
+    for i in range(len(name) - 1, -1, -1):
+        if name[i] == ")":
             depth += 1
-        elif ch == ">":
+        elif name[i] == "(":
             depth -= 1
-        elif ch == "(" and depth == 0:
-            return name[:i]
+            if depth == 0:
+                return name[:i].rstrip()
     return name
 
 

@@ -4,11 +4,17 @@ z8_value.  CPU only; pins the scheme's error bound, not the kernel."""
 import numpy as np
 
 
-def split_rows(x, nslices=5):
+def exponents(m):
+    """e with m 2^-e <= 127 (0 for m = 0), per row maximum m."""
+    return np.where(m > 0, np.frexp(m / 127.0)[1], 0)
+
+
+def split_rows(x, nslices=5, e=None):
     """x (rows, K) -> (exponents (rows,), slices (nslices, rows, K) int8):
-    x = 2^e sum_s x_s 128^-s + r, |x_0| <= 127, |x_s| <= 64 (s > 0)."""
-    m = np.abs(x).max(axis=1)
-    e = np.where(m > 0, np.frexp(m / 127.0)[1], 0)
+    x = 2^e sum_s x_s 128^-s + r, |x_0| <= 127, |x_s| <= 64 (s > 0); e from
+    the row maxima of x unless given (it must satisfy max|x| 2^-e <= 127)."""
+    if e is None:
+        e = exponents(np.abs(x).max(axis=1))
     t = np.ldexp(x, -e[:, None])
     out = []
     for _ in range(nslices):
@@ -70,3 +76,49 @@ def test_three_real_products_give_a_conj_b():
     got = (p1 + p2) + 1j * (p3 - p1 + p2)
     bound = 4 * np.abs(A).max() * np.abs(B).max() * A.shape[1]   # (|re| + |im|) sums, normwise
     assert np.abs(got - A @ B.conj().T).max() < 2.0 ** -34 * bound
+
+
+def diagonal_sums(A, B, ea, eb, nslices=5):
+    """Integer sums R_d = sum_{a+b=d} sum_k A_a B_b of two real operands
+    split with the given row exponents (the split8_kernel / z8_jobs layout)."""
+    _, sa = split_rows(A, nslices, ea)
+    _, sb = split_rows(B, nslices, eb)
+    return [sum(sa[a].astype(np.int64) @ sb[d - a].astype(np.int64).T for a in range(d + 1)) for d in range(nslices)]
+
+
+def value(R, ex):
+    """z8_value: (128 R0 + R1 + (R2 + R3/128 + R4/128^2) / 128) / 128 * 2^ex, the
+    head formed in int32 (asserted) and the tail in float32."""
+    head = 128 * R[0] + R[1]
+    assert np.abs(head).max() < 2 ** 31
+    tail = (R[4].astype(np.float32) / np.float32(128.0) + R[3].astype(np.float32)) / np.float32(128.0) \
+        + R[2].astype(np.float32)
+    return np.ldexp((head.astype(np.float64) + tail.astype(np.float64) / 128.0) / 128.0, ex)
+
+
+def test_shared_exponent_combines_the_three_products_in_int32():
+    """The Z product's exact integer combination: one exponent per row of A
+    (column of B) for the components re, im and re + im (re - im); X = P1 + P2
+    and Y = P3 - P1 + P2 summed in int32 before one reconstruction each give
+    Z = conj(A conj(B)) = X - i Y to the slices' normwise bound, also at
+    the largest K (256) with every slice at its maximum."""
+    rng = np.random.default_rng(4)
+    for K, big in ((50, False), (256, False), (256, True)):
+        if big:
+            A = np.full((16, K), 127.0) * (1 + 1j)
+            B = np.full((16, K), 127.0) * (1 - 1j)
+        else:
+            A = (rng.standard_normal((16, K)) + 1j * rng.standard_normal((16, K))) * 10.0 ** rng.uniform(-2, 2, (16, 1))
+            B = rng.standard_normal((24, K)) + 1j * rng.standard_normal((24, K))
+        ca = [A.real, A.imag, A.real + A.imag]
+        cb = [B.real, B.imag, B.real - B.imag]
+        ea = exponents(np.max([np.abs(c).max(axis=1) for c in ca], axis=0))
+        eb = exponents(np.max([np.abs(c).max(axis=1) for c in cb], axis=0))
+        R = [diagonal_sums(ca[c], cb[c], ea, eb) for c in range(3)]
+        X = [R[0][d] + R[1][d] for d in range(5)]
+        Y = [R[2][d] - R[0][d] + R[1][d] for d in range(5)]
+        ex = ea[:, None] + eb[None, :]
+        Z = value(X, ex) - 1j * value(Y, ex)
+        exact = np.conj(A @ B.conj().T)
+        bound = 4 * np.abs(A).max() * np.abs(B).max() * K
+        assert np.abs(Z - exact).max() < 2.0 ** -33 * bound, K

@@ -73,3 +73,10 @@ def test_traffic_is_calibrated_by_the_copy(tmp_path):
                      str(tmp_path / "cf"), "--calib-write", str(tmp_path / "cw"), "--kernel", "sigma_mfma",
                      "--calib-kernel", "MulFunctor", "--out", str(out)]) == 0
     assert json.load(open(out))["bytes_per_launch"] == t["bytes_per_launch"]
+
+
+def test_short_names_keep_anonymous_namespaces():
+    assert pmc._short("void symgw::(anonymous namespace)::mfma_peak_kernel<double>(double*, int)") == \
+        "void symgw::(anonymous namespace)::mfma_peak_kernel<double>"
+    assert pmc._short("symgw::tgemm_kernel(symgw::TBatch const*, int)") == "symgw::tgemm_kernel"
+    assert pmc._short("__amd_rocclr_copyBuffer") == "__amd_rocclr_copyBuffer"
