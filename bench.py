@@ -257,7 +257,7 @@ def config3_run(tune=True):
     return out
 
 
-def config4_run(shard_points=2, nodes=6):
+def config4_run(shard_points=1, nodes=2):
     """The precision-scaling system (8x8x8 mesh reduced to 29 irreducible
     points; nao 256 in 6 blocks, naux 1200 in 6 blocks; beta 200) on a
     sample, with the three GEMM paths: FP64, FP32 (3M complex products) and
@@ -299,9 +299,11 @@ def config4_run(shard_points=2, nodes=6):
         ctx.upload_G(G0)
         ctx.upload_P(sysm.zeros_q_t_QP())
         ctx.set_timing(True)
+        log(f"config4 {name}: context ready, P0 sweep of the sample")
         ctx.solve_P0()
         ctx.sync()
         p0_ms = float(ctx.last_times()[0])
+        log(f"config4 {name}: P0 {p0_ms:.1f} ms")
         if P is None:
             ctx.solve_W()
             P = ctx.copy_P0_to_host(sysm.zeros_q_t_QP())
@@ -309,6 +311,7 @@ def config4_run(shard_points=2, nodes=6):
         ctx.solve_Sigma()
         ctx.sync()
         sig_ms = float(ctx.last_times()[2])
+        log(f"config4 {name}: Sigma {sig_ms:.1f} ms")
         sig[name] = ctx.copy_Sigma_to_host(sysm.zeros_ts_kij())[:, :, rows] / nk
         info = ctx.info()
         ctx.close()

@@ -83,6 +83,12 @@ struct symgw_ctx {
   P0Blk *d_p0_blks = nullptr;
   SigItem *d_sig_items[kNumSigDepths] = {};  // slabs of kSigDepths[d] times
   SigBlk *d_sig_blks = nullptr;
+  // split P0 / Sigma items, and the partial sums of their parts (one buffer,
+  // the P0 and Sigma sweeps run one after the other on the context stream)
+  P0Split *d_p0_splits = nullptr;
+  SigSplit *d_sig_splits = nullptr;
+  cplx<double> *part = nullptr;
+  int64_t part_cap = 0;
   RotBlk *d_g_rot = nullptr, *d_p_rot = nullptr;
   SolveBlk *d_w_blocks = nullptr, *d_d_blocks = nullptr;
   int64_t *d_g_src_stride = nullptr, *d_g_src_slice = nullptr, *d_g_dst_slice = nullptr, *d_g_dst_size = nullptr;
@@ -236,6 +242,9 @@ int64_t footprint(const Layout &L, const WorkLists &W, int eb) {
   b += (int64_t)L.nw_b * L.flat_aux * 16 + (int64_t)L.nw * L.ns * L.flat_ao * 16;
   b += (int64_t)L.nw_b * W.w_scr_group * 16 + (int64_t)L.nw * L.ns * W.d_scr_group * 16;
   b += (int64_t)(W.p0_blks.size() * sizeof(P0Blk) + W.sig_blks.size() * sizeof(SigBlk));
+  // split items: their tables and the partial sums of their parts
+  b += std::max(W.p0_part_elems, W.sig_part_elems) * 16 +
+       (int64_t)(W.p0_splits.size() * sizeof(P0Split) + W.sig_splits.size() * sizeof(SigSplit));
   // mixed mode: INT8 slices, exponents and split tables
   b += W.v8_bytes + W.p8_bytes + 4 * (W.ve_count + W.pe_count);
   b += (int64_t)(W.v8_split.size() + W.p8_split.size()) * (sizeof(Split8Blk) + 8);
@@ -336,8 +345,8 @@ int run_p0(symgw_ctx *c) {
   if (!c->stream) {
     // the items of the shard's iBZ q, in one launch
     e = launch_p0<T>(c->d_p0_items + ib[c->shard_first], c->d_p0_blks, ib[c->shard_end] - ib[c->shard_first], L.nts,
-                     L.ns, static_cast<const cplx<T> *>(c->V), static_cast<const cplx<T> *>(c->G_full), c->P, pref,
-                     c->W.p0_lds, c->st, c->W.p0_mode);
+                     L.ns, static_cast<const cplx<T> *>(c->V), static_cast<const cplx<T> *>(c->G_full), c->P, c->part,
+                     pref, c->W.p0_lds, c->st, c->W.p0_mode);
   } else {
     // one batch per group of iBZ q: its V pairs generated, then the items of
     // its points inside the shard contracted in one launch
@@ -348,10 +357,14 @@ int run_p0(symgw_ctx *c) {
       e = gen_batch<T>(c, c->d_p0_gen, c->p0_gen_begin, (int)g);
       if (e == hipSuccess)
         e = launch_p0<T>(c->d_p0_items + ib[a], c->d_p0_blks, ib[b] - ib[a], L.nts, L.ns,
-                         static_cast<const cplx<T> *>(c->V), static_cast<const cplx<T> *>(c->G_full), c->P, pref,
-                         c->W.p0_lds, c->st, c->W.p0_mode);
+                         static_cast<const cplx<T> *>(c->V), static_cast<const cplx<T> *>(c->G_full), c->P, c->part,
+                         pref, c->W.p0_lds, c->st, c->W.p0_mode);
     }
   }
+  // the split items' parts, added in order
+  if (e == hipSuccess)
+    e = launch_p0_reduce(c->d_p0_splits, (int)c->W.p0_splits.size(), c->shard_first, c->shard_end, c->part, c->P,
+                         pref, L.nts, c->st);
   if (e != hipSuccess) return fail(c, SYMGW_ERR_HIP, std::string("P0 kernel: ") + hipGetErrorString(e));
   return SYMGW_OK;
 }
@@ -372,8 +385,8 @@ int run_sigma(symgw_ctx *c) {
       const auto &ib = c->W.sig_batch_items[d];
       e = launch_sigma<T>(c->d_sig_items[d] + ib[c->shard_first], c->d_sig_blks, ib[c->shard_end] - ib[c->shard_first],
                           L.nts, L.ns, static_cast<const cplx<T> *>(c->V), static_cast<const cplx<T> *>(c->G_full),
-                          static_cast<const cplx<T> *>(c->P_full), c->Sig, -1.0, c->W.sig_lds[d], c->st,
-                          kSigDepths[d], pz8);
+                          static_cast<const cplx<T> *>(c->P_full), c->Sig, c->part, -1.0, c->W.sig_lds[d],
+                          c->st, kSigDepths[d], pz8);
     }
   } else {
     // one batch per group of iBZ k: its V pairs generated (and split, in the
@@ -393,11 +406,15 @@ int run_sigma(symgw_ctx *c) {
         if (ib[b] == ib[a]) continue;
         e = launch_sigma<T>(c->d_sig_items[d] + ib[a], c->d_sig_blks, ib[b] - ib[a], L.nts, L.ns,
                             static_cast<const cplx<T> *>(c->V), static_cast<const cplx<T> *>(c->G_full),
-                            static_cast<const cplx<T> *>(c->P_full), c->Sig, -1.0, c->W.sig_lds[d], c->st,
-                            kSigDepths[d], pz8);
+                            static_cast<const cplx<T> *>(c->P_full), c->Sig, c->part, -1.0,
+                            c->W.sig_lds[d], c->st, kSigDepths[d], pz8);
       }
     }
   }
+  // the split items' parts, added in order (every launch above is on the same stream)
+  if (e == hipSuccess)
+    e = launch_sig_reduce(c->d_sig_splits, (int)c->W.sig_splits.size(), c->shard_first, c->shard_end, c->part,
+                          c->Sig, -1.0, L.nts, c->st);
   if (e != hipSuccess) return fail(c, SYMGW_ERR_HIP, std::string("Sigma kernel: ") + hipGetErrorString(e));
   return SYMGW_OK;
 }
@@ -566,6 +583,23 @@ static int upload_plan_tables(symgw_ctx *c) {
   r = r ? r : reupload(c, &c->d_p0_blks, c->W.p0_blks);
   for (int d = 0; d < kNumSigDepths; ++d) r = r ? r : reupload(c, &c->d_sig_items[d], c->W.sig_items[d]);
   r = r ? r : reupload(c, &c->d_sig_blks, c->W.sig_blks);
+  r = r ? r : reupload(c, &c->d_p0_splits, c->W.p0_splits);
+  r = r ? r : reupload(c, &c->d_sig_splits, c->W.sig_splits);
+  const int64_t part_need = std::max(c->W.p0_part_elems, c->W.sig_part_elems);
+  if (!r && part_need > c->part_cap) {
+    // the partial sums of split items: a plan that splits more gets a larger buffer
+    if (c->part) {
+      for (size_t i = 0; i < c->owned.size(); ++i)
+        if (c->owned[i].p == c->part) {
+          c->owned.erase(c->owned.begin() + (long)i);
+          break;
+        }
+      ctx_free(c, c->part);
+      c->part = nullptr;
+    }
+    r = dalloc(c, &c->part, (size_t)part_need);
+    if (!r) c->part_cap = part_need;
+  }
   if (r || c->prec != SYMGW_PREC_MIXED) return r;
   auto prefix = [](const std::vector<Split8Blk> &v, int64_t &tot) {
     std::vector<int64_t> p(v.size() + 1, 0);

@@ -707,7 +707,8 @@ template <typename T, int MODE>
 __global__ __launch_bounds__(Cfg<T>::NT) void p0_mfma_kernel(const P0Item *__restrict__ items, const P0Blk *__restrict__ blks,
                                                     int64_t n_work, int nt_half, int nts, int ns,
                                                     const cplx<T> *__restrict__ V, const cplx<T> *__restrict__ G,
-                                                    cplx<double> *__restrict__ P0, double pref) {
+                                                    cplx<double> *__restrict__ P0, cplx<double> *__restrict__ part,
+                                                    double pref) {
   constexpr int TB = p0_times((int)sizeof(cplx<T>), MODE);
   const int64_t w = xcd_logical(blockIdx.x, gridDim.x);
   if (w >= n_work) return;
@@ -845,11 +846,20 @@ __global__ __launch_bounds__(Cfg<T>::NT) void p0_mfma_kernel(const P0Item *__res
       for (int c = 0; c < MTP; ++c) {
         if (c < nmine) {
           const int t = stack ? ta + c : ta, jt = stack ? tg : tg + TG * c;
-          cplx<double> *out = P0 + it.out_base + (int64_t)t * it.out_tstride;
-          tile_store(acc[c], [&](int i, int j, cplx<T> v) {
-            const int P = jt * 16 + j;
-            if (i < nband && P < q) out[i * q + P] = cplx<double>{pref * (double)v.re, pref * (double)v.im};
-          });
+          if (it.part_out < 0) {
+            cplx<double> *out = P0 + it.out_base + (int64_t)t * it.out_tstride;
+            tile_store(acc[c], [&](int i, int j, cplx<T> v) {
+              const int P = jt * 16 + j;
+              if (i < nband && P < q) out[i * q + P] = cplx<double>{pref * (double)v.re, pref * (double)v.im};
+            });
+          } else {
+            // one part of a split item: the unscaled partial sums of time t of this slab
+            cplx<double> *out = part + it.part_out + ((int64_t)(w % np) * TB + (t - t0)) * nband * q;
+            tile_store(acc[c], [&](int i, int j, cplx<T> v) {
+              const int P = jt * 16 + j;
+              if (i < nband && P < q) out[i * q + P] = cplx<double>{(double)v.re, (double)v.im};
+            });
+          }
         }
       }
     }
@@ -860,6 +870,37 @@ __global__ __launch_bounds__(Cfg<T>::NT) void p0_mfma_kernel(const P0Item *__res
     }
     __syncthreads();
   }
+}
+
+// The parts of every split P0 item of the shard [shard_first, shard_end) of
+// iBZ points: P0 = pref * (part 0 + part 1 + ...), summed in part order, one
+// workgroup per split item (times t < nts/2; the kernel zeroes the others).
+__global__ __launch_bounds__(256) void p0_reduce_kernel(const P0Split *__restrict__ sp, int shard_first,
+                                                        int shard_end, const cplx<double> *__restrict__ part,
+                                                        cplx<double> *__restrict__ P0, double pref, int nt_half) {
+  const P0Split S = sp[blockIdx.x];
+  if (S.iqr < shard_first || S.iqr >= shard_end) return;
+  const int rows = S.nband * S.q;
+  const int64_t n = (int64_t)S.np * S.TB * rows;
+  for (int64_t e = threadIdx.x; e < n; e += 256) {
+    const int t = (int)(e / rows), r = (int)(e - (int64_t)t * rows);
+    if (t >= nt_half) continue;  // past the last time of the last slab: never written
+    double re = 0.0, im = 0.0;
+    for (int p = 0; p < S.nparts; ++p) {
+      const cplx<double> v = part[S.part_out + (int64_t)p * S.part_stride + e];
+      re += v.re;
+      im += v.im;
+    }
+    P0[S.out_base + (int64_t)t * S.out_tstride + r] = cplx<double>{pref * re, pref * im};
+  }
+}
+
+hipError_t launch_p0_reduce(const P0Split *sp, int nsp, int shard_first, int shard_end, const cplx<double> *part,
+                            cplx<double> *P0, double pref, int nts, hipStream_t st) {
+  if (nsp == 0) return hipSuccess;
+  hipLaunchKernelGGL(p0_reduce_kernel, dim3((unsigned)nsp), dim3(256), 0, st, sp, shard_first, shard_end, part, P0,
+                     pref, nts / 2);
+  return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
@@ -890,7 +931,8 @@ __global__ __launch_bounds__((SigCfg<T, Z8>::NT)) void sigma_mfma_kernel(const S
                                                        const SigBlk *__restrict__ blks, int64_t n_work, int nts,
                                                        int ns, const cplx<T> *__restrict__ V,
                                                        const cplx<T> *__restrict__ G, const cplx<T> *__restrict__ Pf,
-                                                       cplx<double> *__restrict__ Sig, double pref, Z8Ops z8) {
+                                                       cplx<double> *__restrict__ Sig, cplx<double> *__restrict__ part,
+                                                       double pref, Z8Ops z8) {
   const int64_t w = xcd_logical(blockIdx.x, gridDim.x);
   if (w >= n_work) return;
   const int np = (nts + TB - 1) / TB, per = np * ns;
@@ -1027,15 +1069,59 @@ __global__ __launch_bounds__((SigCfg<T, Z8>::NT)) void sigma_mfma_kernel(const S
     for (int c = 0; c < MT; ++c) {
       const int u = tg + TG * c, tb = u / ntile, tt = u - tb * ntile;
       if (c < nmine) {
-        cplx<double> *out = Sig + it.out_base + (int64_t)(t0 + tb) * it.out_tstride + (int64_t)s * it.out_sstride;
         const int i0 = (tt / ntj) * 16, j0 = (tt % ntj) * 16;
-        tile_store(acc[c], [&](int i, int j, cplx<T> v) {
-          if (i0 + i < Ib && j0 + j < Jb)
-            out[(int64_t)(i0 + i) * k1 + J0 + j0 + j] = cplx<double>{pref * (double)v.re, pref * (double)v.im};
-        });
+        if (it.part_out < 0) {
+          cplx<double> *out = Sig + it.out_base + (int64_t)(t0 + tb) * it.out_tstride + (int64_t)s * it.out_sstride;
+          tile_store(acc[c], [&](int i, int j, cplx<T> v) {
+            if (i0 + i < Ib && j0 + j < Jb)
+              out[(int64_t)(i0 + i) * k1 + J0 + j0 + j] = cplx<double>{pref * (double)v.re, pref * (double)v.im};
+          });
+        } else {
+          // one part of a split item: the unscaled partial sums of this slab
+          cplx<double> *out = part + it.part_out + ((int64_t)ts * TB + tb) * Ib * Jb;
+          tile_store(acc[c], [&](int i, int j, cplx<T> v) {
+            if (i0 + i < Ib && j0 + j < Jb) out[(i0 + i) * Jb + j0 + j] = cplx<double>{(double)v.re, (double)v.im};
+          });
+        }
       }
     }
   }
+}
+
+// The parts of every split Sigma item of the shard [shard_first, shard_end)
+// of iBZ points: Sigma = pref * (part 0 + part 1 + ...), summed in part order
+// (so the result does not depend on which part finished first), one
+// workgroup per split item.
+__global__ __launch_bounds__(256) void sig_reduce_kernel(const SigSplit *__restrict__ sp, int shard_first,
+                                                         int shard_end, const cplx<double> *__restrict__ part,
+                                                         cplx<double> *__restrict__ Sig, double pref, int nts) {
+  const SigSplit S = sp[blockIdx.x];
+  if (S.ikr < shard_first || S.ikr >= shard_end) return;
+  const int ij = S.Ib * S.Jb;
+  const int64_t n = (int64_t)S.per * S.TB * ij;
+  for (int64_t e = threadIdx.x; e < n; e += 256) {
+    const int64_t st = e / ij;
+    const int r = (int)(e - st * ij), i = r / S.Jb, j = r - i * S.Jb;
+    const int ts = (int)(st / S.TB), tb = (int)(st - (int64_t)ts * S.TB);
+    const int t = (ts / S.ns) * S.TB + tb, s = ts - (ts / S.ns) * S.ns;
+    if (t >= nts) continue;  // past the last time of the last slab: never written
+    double re = 0.0, im = 0.0;
+    for (int p = 0; p < S.nparts; ++p) {
+      const cplx<double> v = part[S.part_out + (int64_t)p * S.part_stride + e];
+      re += v.re;
+      im += v.im;
+    }
+    Sig[S.out_base + (int64_t)t * S.out_tstride + (int64_t)s * S.out_sstride + (int64_t)i * S.k1 + S.J0 + j] =
+        cplx<double>{pref * re, pref * im};
+  }
+}
+
+hipError_t launch_sig_reduce(const SigSplit *sp, int nsp, int shard_first, int shard_end, const cplx<double> *part,
+                             cplx<double> *Sig, double pref, int nts, hipStream_t st) {
+  if (nsp == 0) return hipSuccess;
+  hipLaunchKernelGGL(sig_reduce_kernel, dim3((unsigned)nsp), dim3(256), 0, st, sp, shard_first, shard_end, part, Sig,
+                     pref, nts);
+  return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
@@ -1113,7 +1199,8 @@ template hipError_t launch_rotate<float>(const RotBlk *, int, int, const cplx<do
 
 template <typename T, int MODE>
 hipError_t launch_p0_k(const P0Item *items, const P0Blk *blks, int n_items, int nts, int ns, const cplx<T> *V,
-                       const cplx<T> *G, cplx<double> *P0, double pref, size_t lds, hipStream_t st) {
+                       const cplx<T> *G, cplx<double> *P0, cplx<double> *part, double pref, size_t lds,
+                       hipStream_t st) {
   const int nt_half = nts / 2;
   constexpr int TB = p0_times((int)sizeof(cplx<T>), MODE);
   const int64_t n_work = (int64_t)n_items * ((nt_half + TB - 1) / TB);
@@ -1121,62 +1208,69 @@ hipError_t launch_p0_k(const P0Item *items, const P0Blk *blks, int n_items, int 
   (void)hipFuncSetAttribute((const void *)p0_mfma_kernel<T, MODE>, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)lds);
   hipLaunchKernelGGL((p0_mfma_kernel<T, MODE>), dim3((unsigned)xcd_grid(n_work)), dim3(Cfg<T>::NT), lds, st, items,
-                     blks, n_work, nt_half, nts, ns, V, G, P0, pref);
+                     blks, n_work, nt_half, nts, ns, V, G, P0, part, pref);
   return hipGetLastError();
 }
 
 template <typename T>
 hipError_t launch_p0(const P0Item *items, const P0Blk *blks, int n_items, int nts, int ns, const cplx<T> *V,
-                     const cplx<T> *G, cplx<double> *P0, double pref, size_t lds, hipStream_t st, int mode) {
-  if (mode == 1) return launch_p0_k<T, 1>(items, blks, n_items, nts, ns, V, G, P0, pref, lds, st);
-  if (mode == 0) return launch_p0_k<T, 0>(items, blks, n_items, nts, ns, V, G, P0, pref, lds, st);
-  return launch_p0_k<T, 2>(items, blks, n_items, nts, ns, V, G, P0, pref, lds, st);
+                     const cplx<T> *G, cplx<double> *P0, cplx<double> *part, double pref, size_t lds, hipStream_t st,
+                     int mode) {
+  if (mode == 1) return launch_p0_k<T, 1>(items, blks, n_items, nts, ns, V, G, P0, part, pref, lds, st);
+  if (mode == 0) return launch_p0_k<T, 0>(items, blks, n_items, nts, ns, V, G, P0, part, pref, lds, st);
+  return launch_p0_k<T, 2>(items, blks, n_items, nts, ns, V, G, P0, part, pref, lds, st);
 }
 
 template <typename T, int TB, bool Z8>
 hipError_t launch_sigma_tb(const SigItem *items, const SigBlk *blks, int n_items, int nts, int ns, const cplx<T> *V,
-                           const cplx<T> *G, const cplx<T> *Pf, cplx<double> *Sig, double pref, size_t lds,
-                           hipStream_t st, const Z8Ops &z8) {
+                           const cplx<T> *G, const cplx<T> *Pf, cplx<double> *Sig, cplx<double> *part, double pref,
+                           size_t lds, hipStream_t st, const Z8Ops &z8) {
   const int64_t n_work = (int64_t)n_items * ((nts + TB - 1) / TB) * ns;
   if (n_work == 0) return hipSuccess;
   (void)hipFuncSetAttribute((const void *)sigma_mfma_kernel<T, TB, Z8>, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)lds);
   hipLaunchKernelGGL((sigma_mfma_kernel<T, TB, Z8>), dim3((unsigned)xcd_grid(n_work)), dim3(SigCfg<T, Z8>::NT), lds, st,
-                     items, blks, n_work, nts, ns, V, G, Pf, Sig, pref, z8);
+                     items, blks, n_work, nts, ns, V, G, Pf, Sig, part, pref, z8);
   return hipGetLastError();
 }
 
 template <typename T, bool Z8>
 hipError_t launch_sigma_z(const SigItem *items, const SigBlk *blks, int n_items, int nts, int ns, const cplx<T> *V,
-                          const cplx<T> *G, const cplx<T> *Pf, cplx<double> *Sig, double pref, size_t lds,
-                          hipStream_t st, int times, const Z8Ops &z8) {
-  if (times == 4) return launch_sigma_tb<T, 4, Z8>(items, blks, n_items, nts, ns, V, G, Pf, Sig, pref, lds, st, z8);
-  if (times == 2) return launch_sigma_tb<T, 2, Z8>(items, blks, n_items, nts, ns, V, G, Pf, Sig, pref, lds, st, z8);
-  if (times == 1) return launch_sigma_tb<T, 1, Z8>(items, blks, n_items, nts, ns, V, G, Pf, Sig, pref, lds, st, z8);
+                          const cplx<T> *G, const cplx<T> *Pf, cplx<double> *Sig, cplx<double> *part, double pref,
+                          size_t lds, hipStream_t st, int times, const Z8Ops &z8) {
+  if (times == 4)
+    return launch_sigma_tb<T, 4, Z8>(items, blks, n_items, nts, ns, V, G, Pf, Sig, part, pref, lds, st, z8);
+  if (times == 2)
+    return launch_sigma_tb<T, 2, Z8>(items, blks, n_items, nts, ns, V, G, Pf, Sig, part, pref, lds, st, z8);
+  if (times == 1)
+    return launch_sigma_tb<T, 1, Z8>(items, blks, n_items, nts, ns, V, G, Pf, Sig, part, pref, lds, st, z8);
   return hipErrorInvalidValue;
 }
 
 template <typename T>
 hipError_t launch_sigma(const SigItem *items, const SigBlk *blks, int n_items, int nts, int ns, const cplx<T> *V,
-                        const cplx<T> *G, const cplx<T> *Pf, cplx<double> *Sig, double pref, size_t lds,
-                        hipStream_t st, int times, const Z8Ops *z8) {
+                        const cplx<T> *G, const cplx<T> *Pf, cplx<double> *Sig, cplx<double> *part, double pref,
+                        size_t lds, hipStream_t st, int times, const Z8Ops *z8) {
   if constexpr (sizeof(T) == 8) {
-    if (z8) return launch_sigma_z<T, true>(items, blks, n_items, nts, ns, V, G, Pf, Sig, pref, lds, st, times, *z8);
+    if (z8)
+      return launch_sigma_z<T, true>(items, blks, n_items, nts, ns, V, G, Pf, Sig, part, pref, lds, st, times, *z8);
   } else {
     if (z8) return hipErrorInvalidValue;  // the INT8 Z product reads FP64 P and V
   }
-  return launch_sigma_z<T, false>(items, blks, n_items, nts, ns, V, G, Pf, Sig, pref, lds, st, times, Z8Ops{});
+  return launch_sigma_z<T, false>(items, blks, n_items, nts, ns, V, G, Pf, Sig, part, pref, lds, st, times, Z8Ops{});
 }
 
 template hipError_t launch_p0<double>(const P0Item *, const P0Blk *, int, int, int, const cplx<double> *,
-                                      const cplx<double> *, cplx<double> *, double, size_t, hipStream_t, int);
+                                      const cplx<double> *, cplx<double> *, cplx<double> *, double, size_t,
+                                      hipStream_t, int);
 template hipError_t launch_p0<float>(const P0Item *, const P0Blk *, int, int, int, const cplx<float> *,
-                                     const cplx<float> *, cplx<double> *, double, size_t, hipStream_t, int);
+                                     const cplx<float> *, cplx<double> *, cplx<double> *, double, size_t, hipStream_t,
+                                     int);
 template hipError_t launch_sigma<double>(const SigItem *, const SigBlk *, int, int, int, const cplx<double> *,
-                                         const cplx<double> *, const cplx<double> *, cplx<double> *, double, size_t,
-                                         hipStream_t, int, const Z8Ops *);
+                                         const cplx<double> *, const cplx<double> *, cplx<double> *, cplx<double> *,
+                                         double, size_t, hipStream_t, int, const Z8Ops *);
 template hipError_t launch_sigma<float>(const SigItem *, const SigBlk *, int, int, int, const cplx<float> *,
-                                        const cplx<float> *, const cplx<float> *, cplx<double> *, double, size_t,
-                                        hipStream_t, int, const Z8Ops *);
+                                        const cplx<float> *, const cplx<float> *, cplx<double> *, cplx<double> *,
+                                        double, size_t, hipStream_t, int, const Z8Ops *);
 
 }  // namespace symgw

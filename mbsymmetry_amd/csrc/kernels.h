@@ -28,9 +28,14 @@ struct GenBlk {
   int32_t pair;       // k1 * nk + k2
 };
 
+// Items with part_out >= 0 write unscaled partial sums to `part`; launch_p0_reduce
+// then adds the parts of each split item in order into P0 (times pref).
 template <typename T>
 hipError_t launch_p0(const P0Item *items, const P0Blk *blks, int n_items, int nts, int ns, const cplx<T> *V,
-                     const cplx<T> *G, cplx<double> *P0, double pref, size_t lds, hipStream_t st, int mode);
+                     const cplx<T> *G, cplx<double> *P0, cplx<double> *part, double pref, size_t lds, hipStream_t st,
+                     int mode);
+hipError_t launch_p0_reduce(const P0Split *sp, int nsp, int shard_first, int shard_end, const cplx<double> *part,
+                            cplx<double> *P0, double pref, int nts, hipStream_t st);
 // INT8 slice copies read by the mixed-precision Z product of Sigma.
 struct Z8Ops {
   const int8_t *P8 = nullptr, *V8 = nullptr;
@@ -38,10 +43,14 @@ struct Z8Ops {
 };
 // z8 == nullptr: every Z product on the matrix cores of T; otherwise the
 // blocks with SigBlk::kp > 0 run it on INT8 slices (T = double only).
+// Items with part_out >= 0 write unscaled partial sums to `part`; launch_sig_reduce
+// then adds the parts of each split item in order into Sig (times pref).
 template <typename T>
 hipError_t launch_sigma(const SigItem *items, const SigBlk *blks, int n_items, int nts, int ns, const cplx<T> *V,
-                        const cplx<T> *G, const cplx<T> *Pf, cplx<double> *Sig, double pref, size_t lds,
-                        hipStream_t st, int times, const Z8Ops *z8);
+                        const cplx<T> *G, const cplx<T> *Pf, cplx<double> *Sig, cplx<double> *part, double pref,
+                        size_t lds, hipStream_t st, int times, const Z8Ops *z8);
+hipError_t launch_sig_reduce(const SigSplit *sp, int nsp, int shard_first, int shard_end, const cplx<double> *part,
+                             cplx<double> *Sig, double pref, int nts, hipStream_t st);
 // Split rows of complex128 matrices into INT8 slices (Split8Blk);
 // row_prefix[b] is the first flat row of block b (nsb + 1 entries).
 hipError_t launch_split8(const Split8Blk *sb, int nsb, const int64_t *row_prefix, int64_t nrows_total,

@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <cstdlib>
+#include <cmath>
 #include <map>
 #include <sstream>
 
@@ -351,6 +352,169 @@ static int p0_band(int qs, int band_cap, int budget, int eb, int TB, double chun
   return best_band;
 }
 
+// Contiguous block ranges [b, e) of about equal cost, `parts` of them, each
+// of at least kSplitMinBlocks blocks (parts <= (end - begin) / kSplitMinBlocks).
+static std::vector<int> split_ranges(int begin, int end, int parts, const std::vector<double> &bcost) {
+  double tot = 0;
+  for (int b = begin; b < end; ++b) tot += bcost[b - begin];
+  std::vector<int> cut{begin};
+  double acc = 0;
+  int b = begin;
+  for (int p = 0; p + 1 < parts; ++p) {
+    const int left = parts - 1 - p, start = b;
+    const double goal = tot * (p + 1) / parts;
+    while (b < end - left * kSplitMinBlocks && (b - start < kSplitMinBlocks || acc < goal)) acc += bcost[b++ - begin];
+    cut.push_back(b);
+  }
+  cut.push_back(end);
+  return cut;
+}
+
+// Split the Sigma items over their q where one workgroup per (item, slab)
+// would leave the GPU short of work: a system with few iBZ points, times or
+// blocks (a shard, a sample), or a few items much costlier than the rest
+// (uneven blocks).  An item's cost is the executed work of its blocks for one
+// slab (8 M N K of the Z, E and final products); items above
+// total / kSplitTarget are cut into that many parts of contiguous blocks
+// with balanced cost (at least kSplitMinBlocks blocks each).  The parts of
+// an item write partial sums that the reduction kernel adds in part order.
+// The batch prefixes (items of each iBZ point) are rebuilt afterwards.
+static void split_sigma_items(WorkLists &W, int nts, int ns) {
+  double total = 0;
+  std::vector<double> cost[kNumSigDepths];
+  for (int d = 0; d < kNumSigDepths; ++d) {
+    const int TB = kSigDepths[d], per = (nts + TB - 1) / TB * ns;
+    for (const SigItem &it : W.sig_items[d]) {
+      double c = 0;
+      for (int b = it.blk_begin; b < it.blk_end; ++b) {
+        const SigBlk &B = W.sig_blks[b];
+        c += 8.0 * TB * B.q * B.k2 * ((double)B.q * it.Jb + (double)it.Ib * B.k2 + (double)it.Ib * it.Jb);
+      }
+      cost[d].push_back(c);
+      total += c * per;
+    }
+  }
+  const double cap = total / kSplitTarget;
+  W.sig_splits.clear();
+  W.sig_part_elems = 0;
+  for (int d = 0; d < kNumSigDepths; ++d) {
+    const int TB = kSigDepths[d], per = (nts + TB - 1) / TB * ns;
+    std::vector<SigItem> out;
+    out.reserve(W.sig_items[d].size());
+    for (size_t i = 0; i < W.sig_items[d].size(); ++i) {
+      const SigItem it = W.sig_items[d][i];
+      const int nb = it.blk_end - it.blk_begin;
+      int parts = cap > 0 ? (int)std::ceil(cost[d][i] / cap) : 1;
+      parts = std::max(1, std::min(parts, nb / kSplitMinBlocks));
+      if (parts == 1) {
+        out.push_back(it);
+        continue;
+      }
+      SigSplit sp;
+      sp.part_out = W.sig_part_elems;
+      sp.part_stride = (int64_t)per * TB * it.Ib * it.Jb;
+      sp.out_base = it.out_base;
+      sp.out_tstride = it.out_tstride;
+      sp.out_sstride = it.out_sstride;
+      sp.nparts = parts;
+      sp.TB = TB;
+      sp.per = per;
+      sp.ns = ns;
+      sp.k1 = it.k1;
+      sp.J0 = it.J0;
+      sp.Ib = it.Ib;
+      sp.Jb = it.Jb;
+      sp.ikr = it.ikr;
+      sp.pad = 0;
+      W.sig_splits.push_back(sp);
+      W.sig_part_elems += sp.part_stride * parts;
+      std::vector<double> bcost;
+      for (int b = it.blk_begin; b < it.blk_end; ++b) {
+        const SigBlk &B = W.sig_blks[b];
+        bcost.push_back(8.0 * TB * B.q * B.k2 * ((double)B.q * it.Jb + (double)it.Ib * B.k2 + (double)it.Ib * it.Jb));
+      }
+      const std::vector<int> cut = split_ranges(it.blk_begin, it.blk_end, parts, bcost);
+      for (int p = 0; p < parts; ++p) {
+        SigItem q = it;
+        q.blk_begin = cut[p];
+        q.blk_end = cut[p + 1];
+        q.part_out = sp.part_out + (int64_t)p * sp.part_stride;
+        out.push_back(q);
+      }
+    }
+    W.sig_items[d].swap(out);
+    // items of iBZ point ikr: [prefix[ikr], prefix[ikr + 1])
+    auto &pre = W.sig_batch_items[d];
+    const int ink = (int)pre.size();
+    pre.assign(ink + 1, 0);
+    for (const SigItem &it : W.sig_items[d]) ++pre[it.ikr + 1];
+    for (int k = 0; k < ink; ++k) pre[k + 1] += pre[k];
+  }
+}
+
+// The split of the P0 items over their k, on the rule of split_sigma_items:
+// an item's cost is the work of its blocks for one slab (E, C and the final
+// product of TB times of nband rows).
+static void split_p0_items(WorkLists &W, int nts, int TB) {
+  const int np = (nts / 2 + TB - 1) / TB;
+  auto bc = [&](const P0Item &it, const P0Blk &B) {
+    return 8.0 * TB * it.nband * ((double)B.k1 * B.k2 * B.k2 + (double)B.k1 * B.k1 * B.k2 + (double)it.q * B.k1 * B.k2);
+  };
+  double total = 0;
+  std::vector<double> cost;
+  for (const P0Item &it : W.p0_items) {
+    double c = 0;
+    for (int b = it.blk_begin; b < it.blk_end; ++b) c += bc(it, W.p0_blks[b]);
+    cost.push_back(c);
+    total += c * np;
+  }
+  const double cap = total / kSplitTarget;
+  W.p0_splits.clear();
+  W.p0_part_elems = 0;
+  std::vector<P0Item> out;
+  out.reserve(W.p0_items.size());
+  for (size_t i = 0; i < W.p0_items.size(); ++i) {
+    const P0Item it = W.p0_items[i];
+    const int nb = it.blk_end - it.blk_begin;
+    int parts = cap > 0 ? (int)std::ceil(cost[i] / cap) : 1;
+    parts = std::max(1, std::min(parts, nb / kSplitMinBlocks));
+    if (parts == 1) {
+      out.push_back(it);
+      continue;
+    }
+    P0Split sp;
+    sp.part_out = W.p0_part_elems;
+    sp.part_stride = (int64_t)np * TB * it.nband * it.q;
+    sp.out_base = it.out_base;
+    sp.out_tstride = it.out_tstride;
+    sp.nparts = parts;
+    sp.TB = TB;
+    sp.np = np;
+    sp.nband = it.nband;
+    sp.q = it.q;
+    sp.iqr = it.iqr;
+    sp.pad = 0;
+    W.p0_splits.push_back(sp);
+    W.p0_part_elems += sp.part_stride * parts;
+    std::vector<double> bcost;
+    for (int b = it.blk_begin; b < it.blk_end; ++b) bcost.push_back(bc(it, W.p0_blks[b]));
+    const std::vector<int> cut = split_ranges(it.blk_begin, it.blk_end, parts, bcost);
+    for (int p = 0; p < parts; ++p) {
+      P0Item q = it;
+      q.blk_begin = cut[p];
+      q.blk_end = cut[p + 1];
+      q.part_out = sp.part_out + (int64_t)p * sp.part_stride;
+      out.push_back(q);
+    }
+  }
+  W.p0_items.swap(out);
+  auto &pre = W.p0_batch_items;
+  const int ink = (int)pre.size();
+  pre.assign(ink + 1, 0);
+  for (const P0Item &it : W.p0_items) ++pre[it.iqr + 1];
+  for (int k = 0; k < ink; ++k) pre[k + 1] += pre[k];
+}
+
 bool valid_plan(const Plan &p) {
   return p.sig_chunk_cost >= 0 && p.sig_chunk_cost < 1e12 && p.sig_rows >= 1 && p.sig_rows <= kSigRows &&
          (p.sig_max_times == 1 || p.sig_max_times == 2 || p.sig_max_times == 4) && p.p0_chunk_cost >= 0 &&
@@ -493,16 +657,17 @@ bool build_worklists(const Layout &L, int eb, WorkLists &W, std::string &err, bo
       for (int Q0 = 0; Q0 < qs; Q0 += band) {
         P0Item it;
         it.blk_begin = b0; it.blk_end = b1;
-        it.q = qs; it.Q0 = Q0; it.nband = std::min(band, qs - Q0); it.pad = 0;
+        it.q = qs; it.Q0 = Q0; it.nband = std::min(band, qs - Q0); it.iqr = iqr;
         it.out_base = (int64_t)nts * L.qaux_off_irre[iqr] + L.aux_offsets[q][iq] + (int64_t)Q0 * qs;
         it.out_tstride = (int32_t)L.qaux_size_irre[iqr];
         it.pad2 = 0;
+        it.part_out = -1;
         W.p0_items.push_back(it);
       }
     }
     if (stream && group_ends(W.p0_group_first, iqr)) close_batch(W.p0_batch_pairs, p0_pair_from);
   }
-  W.p0_batch_items.push_back((int)W.p0_items.size());
+  split_p0_items(W, nts, p0_tb);
   if (stream) W.p0_batch_pair_begin.push_back((int)W.p0_batch_pairs.size());
   // the split-K epilogue passes partial tiles through LDS: one slot per wave of
   // contract_tiles tiles x 64 lanes x 8 reals
@@ -621,7 +786,8 @@ bool build_worklists(const Layout &L, int eb, WorkLists &W, std::string &err, bo
           SigItem it;
           it.blk_begin = b0; it.blk_end = b1;
           it.k1 = k1; it.I0 = I0; it.Ib = std::min(band, k1 - I0);
-          it.J0 = J0; it.Jb = std::min(band, k1 - J0); it.pad = 0;
+          it.J0 = J0; it.Jb = std::min(band, k1 - J0); it.ikr = ikr;
+          it.part_out = -1;
           it.out_base = L.kao_off_irre[ikr] * ns * nts + L.ao_offsets[k][i1] + (int64_t)I0 * k1;
           it.out_tstride = (int32_t)(ns * L.kao_size_irre[ikr]);
           it.out_sstride = (int32_t)L.kao_size_irre[ikr];
@@ -630,7 +796,7 @@ bool build_worklists(const Layout &L, int eb, WorkLists &W, std::string &err, bo
     }
     if (stream && group_ends(W.sig_group_first, ikr)) close_batch(W.sig_batch_pairs, sig_pair_from);
   }
-  for (int d = 0; d < kNumSigDepths; ++d) W.sig_batch_items[d].push_back((int)W.sig_items[d].size());
+  split_sigma_items(W, nts, ns);
   if (stream) W.sig_batch_pair_begin.push_back((int)W.sig_batch_pairs.size());
   if (stream && z8) W.v8_batch_begin.push_back((int)W.v8_split.size());
   // the INT8-Z Sigma kernel reduces split-K parts over its own wave / tile shape

@@ -124,13 +124,27 @@ struct P0Blk {
   int32_t nsub;        // orbitals n of k + q per LDS chunk
   int32_t lde;         // LDS row stride of E and C (k1*nsub, odd)
 };
-// Rows [Q0, Q0 + nband) of aux block iq of iBZ point q_ir, summed over all k.
+// Rows [Q0, Q0 + nband) of aux block iq of iBZ point q_ir, summed over the k
+// of its blocks [blk_begin, blk_end): every k, or one part of them when the
+// item is split (part_out >= 0).
 struct P0Item {
   int32_t blk_begin, blk_end;
-  int32_t q, Q0, nband, pad;
+  int32_t q, Q0, nband, iqr;
   int64_t out_base;          // q_t_QP offset of row Q0 at t = 0
   int32_t out_tstride;       // qslice(q_ir)
   int32_t pad2;
+  int64_t part_out;          // -1: the slab writes P0; else this part's partial
+                             // sums, [slab][time][nband][q] complex128, start here
+};
+// One split P0 item: its parts' partial sums are added in part order, scaled
+// and written to P0 by the reduction kernel.
+struct P0Split {
+  int64_t part_out;          // partial buffer of part 0 (complex128 elements)
+  int64_t part_stride;       // elements per part: np * TB * nband * q
+  int64_t out_base;
+  int32_t out_tstride, nparts;
+  int32_t TB, np, nband, q;  // np: time slabs of the item
+  int32_t iqr, pad;
 };
 
 // One V-block contraction that feeds a Sigma output slab
@@ -154,14 +168,34 @@ struct SigBlk {
   int64_t pe_off;      // int32 index: row exponents at t = 0, [t][comp][Q]
 };
 // Rows [I0, I0 + Ib) x columns [J0, J0 + Jb) of ao block ik1 of iBZ point
-// k_ir, summed over all q.
+// k_ir, summed over the q of its blocks [blk_begin, blk_end): every q, or one
+// part of them when the item is split (part_out >= 0).
 struct SigItem {
   int32_t blk_begin, blk_end;
-  int32_t k1, I0, Ib, J0, Jb, pad;
+  int32_t k1, I0, Ib, J0, Jb, ikr;
   int64_t out_base;          // k_ts_ij (iBZ) offset of row I0 at t = 0, s = 0
   int32_t out_tstride;       // ns * kslice(k)
   int32_t out_sstride;       // kslice(k)
+  int64_t part_out;          // -1: the slab writes Sigma; else this part's partial
+                             // sums, [slab][time][Ib][Jb] complex128, start here
 };
+// One split Sigma item: its parts' partial sums are added in part order, scaled
+// and written to Sigma by the reduction kernel.
+struct SigSplit {
+  int64_t part_out;          // partial buffer of part 0 (complex128 elements)
+  int64_t part_stride;       // elements per part: per * TB * Ib * Jb
+  int64_t out_base;
+  int32_t out_tstride, out_sstride;
+  int32_t nparts, TB, per, ns;  // per: slabs of the item (time slabs x spins)
+  int32_t k1, J0, Ib, Jb;
+  int32_t ikr, pad;
+};
+// Sigma items (P0 items) are split over their q (k), i.e. over their blocks,
+// until the sweep has about this many workgroups, and any item above a
+// quarter of one CU's share of the sweep's work is split too; every part
+// holds at least kSplitMinBlocks blocks.
+constexpr int kSplitTarget = 1024;
+constexpr int kSplitMinBlocks = 4;
 
 // Rotation of one full-BZ k (G) or q (P) slice from its iBZ representative:
 // block ia1 of the target = O * X(block ia2 of the source) * O^H, or a copy
@@ -225,9 +259,13 @@ struct SolveBlk {
 struct WorkLists {
   std::vector<P0Blk> p0_blks;
   std::vector<P0Item> p0_items;
+  std::vector<P0Split> p0_splits;                 // split P0 items
+  int64_t p0_part_elems = 0;                      // complex128 elements of their partial sums
   int p0_mode = 0;  // 1: every slab stacked, 0: none, 2: both kinds (chosen per slab)
   std::vector<SigBlk> sig_blks;
   std::vector<SigItem> sig_items[kNumSigDepths];  // slabs of kSigDepths[d] times
+  std::vector<SigSplit> sig_splits;               // split items of every depth
+  int64_t sig_part_elems = 0;                     // complex128 elements of the partial sums
   std::vector<RotBlk> g_rot;      // targets over the full BZ (ao)
   std::vector<RotBlk> p_rot;      // targets over the full BZ (aux)
   std::vector<SolveBlk> w_blocks; // aux blocks of every iBZ q
