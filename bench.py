@@ -75,7 +75,7 @@ def run_path(sysm, precision, steps, warmup, mu, G0, keep_P=False, tune=True, pl
     tuned = None
     if plan is not None:
         ctx.replan(**plan)
-        tuned = {"plan": ctx.plan(), "fixed": True}
+        tuned = {"plan": ctx.plan(), "given": True}
     elif tune:
         from mbsymmetry_amd import autotune
         tuned = autotune.tune(ctx, sysm, precision, use_cache=False, log=log)
@@ -215,12 +215,13 @@ def cpu_baseline(sysm, G0, P0, P, Sigma, max_workers=16):
                 sample_k=len(sub))
 
 
-def config3_run(tune=True):
+def config3_run(tune=True, with_fp32=False):
     """The uneven-batching system (64 irreducible points, each with its own
     2..10 irreps of sizes 1..256; the whole sweep of each contraction in one
     launch per slab depth): P0 and Sigma kernel time and rate from HIP events
-    after one untimed pass, in the three precision modes, with max|dSigma| of
-    FP32 and mixed against FP64."""
+    after one untimed pass, in FP64 and the mixed mode (and FP32 with
+    with_fp32), with max|dSigma| of the others against FP64.  The autotuner
+    picks the plan of the FP64 context; the other modes run it."""
     from mbsymmetry_amd import _lib, system as S
     sysm = S.config3()
     nb = [len(b) for b in sysm.ao_sizes]
@@ -228,17 +229,23 @@ def config3_run(tune=True):
                         f"(ao blocks {min(min(b) for b in sysm.ao_sizes)}..{max(max(b) for b in sysm.ao_sizes)}, "
                         f"mean {np.mean(nb):.1f} per point), nts {sysm.nts}")}
     sig = {}
-    for prec, name in ((_lib.PREC_FP64, "fp64"), (_lib.PREC_FP32, "fp32"), (_lib.PREC_MIXED, "mixed")):
+    modes = [(_lib.PREC_FP64, "fp64"), (_lib.PREC_MIXED, "mixed")] + ([(_lib.PREC_FP32, "fp32")] if with_fp32 else [])
+    plan = None
+    for prec, name in modes:
         ctx = setup_ctx(sysm, prec, 0.0)
         ctx.upload_G(sysm.zeros_ts_kij())
         ctx.dyson(0.0, 0.0)
         G0 = ctx.download_G((sysm.nts, sysm.ns, sysm.flat_ao))
         tuned = None
-        if tune:
+        if plan is not None:
+            ctx.replan(**plan)
+            tuned = {"plan": ctx.plan(), "from": "fp64"}
+        elif tune:
             from mbsymmetry_amd import autotune
             ctx.upload_G(G0)
             rec = autotune.tune(ctx, sysm, prec, sample_s=0.3, use_cache=False, log=log)
             tuned = {k: rec[k] for k in ("plan", "Sigma", "P0", "seconds")}
+            plan = rec["plan"]
         for timed in (False, True):
             ctx.upload_G(G0)
             ctx.set_timing(timed)
@@ -392,7 +399,7 @@ def main():
                          "(e.g. '{\"sig_max_times\": 2}'; the rules not named keep their defaults)")
     ap.add_argument("--serving-only", action="store_true", help="run only the serving mix and print its JSON")
     ap.add_argument("--serving-window", type=float, default=2.0, help="seconds of Poisson arrivals")
-    ap.add_argument("--serving-deadline", type=float, default=30.0,
+    ap.add_argument("--serving-deadline", type=float, default=20.0,
                     help="no serving request starts that is estimated to end later (s)")
     ap.add_argument("--serving-rate", type=float, default=50.0, help="Poisson arrival rate (requests/s)")
     ap.add_argument("--serving-modes", default="fp64,mixed", help="precision modes of the serving leg")
@@ -441,8 +448,9 @@ def main():
     log("plain fp64", plain["sec_per_iter"], "s/iter; phases ms", plain["phase_ms"][:4])
     mixed = fp32 = None
     if not args.no_mixed:
-        mixed = run_path(sysm, _lib.PREC_MIXED, args.steps, args.warmup, args.mu, G0, tune=not args.no_tune,
-                         plan=fixed_plan)
+        # the mixed mode runs the plan the autotuner picked for the plain path
+        mixed = run_path(sysm, _lib.PREC_MIXED, args.steps, args.warmup, args.mu, G0, tune=False,
+                         plan=fixed_plan if fixed_plan is not None else (plain["tuned"] or {}).get("plan"))
         log("mixed (int8 Z)", mixed["sec_per_iter"], "s/iter; phases ms", mixed["phase_ms"][:4])
         if args.fp32:
             fp32 = run_path(sysm, _lib.PREC_FP32, args.steps, args.warmup, args.mu, G0, tune=not args.no_tune,
