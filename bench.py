@@ -122,10 +122,12 @@ def run_path(sysm, precision, steps, warmup, mu, G0, keep_P=False, tune=True, pl
 
 
 def int8_ops(sysm):
-    """INT8 MFMA operations one mixed-mode Sigma sweep executes (executed, with
-    the padding of 16-wide tiles and 64-deep slice-pair chunks): per Z job
-    tile, 3 real products x sum_{d<5} ceil((d + 1) kp / 64) instructions of
-    2 * 16 * 16 * 64 ops, for the blocks above the INT8 threshold."""
+    """INT8 MFMA operations of one mixed-mode Sigma sweep, a lower bound: per
+    16 x 16 Z tile and 64-deep chunk of K (kp, the aux block rounded up to 16),
+    the 60 instructions of 2 * 16 * 16 * 64 ops that the kernel issues (15
+    slice pairs for each of P1, P2 twice and P3), for the blocks above the
+    INT8 threshold, with the tiles counted over whole blocks (the padding of
+    each LDS chunk's rows to 16 is left out)."""
     q_min = int(os.environ.get("SYMGW_Z8_MIN_Q", "32"))
     tot = 0.0
     nts = sysm.nts
@@ -137,7 +139,7 @@ def int8_ops(sysm):
                 if qs <= q_min:
                     continue
                 kp = (qs + 15) // 16 * 16
-                inst = 3 * sum((d * kp + 63) // 64 for d in range(1, 6))
+                inst = 60 * ((kp + 63) // 64)
                 # Z rows: times x aux rows; columns: a1 x a2, one pass per slab band of a1
                 tiles = ((nts * qs + 15) // 16) * ((a1 * a2 + 15) // 16)
                 tot += sysm.ns * tiles * inst * 2 * 16 * 16 * 64
