@@ -38,7 +38,7 @@ from typing import Callable, Dict, List, Optional
 from . import _lib
 
 SIG_CANDIDATES = [dict(sig_chunk_cost=c, sig_rows=r, sig_max_times=t)
-                  for c in (6.25e4, 2.5e5, 1e6, 4e6) for r in (32, 64) for t in (2, 4)]
+                  for c in (6.25e4, 2.5e5, 1e6, 4e6) for r in (32, 64) for t in (1, 2, 4)]
 P0_CANDIDATES = [dict(p0_chunk_cost=c, p0_band_cap=b) for c in (6.25e4, 2.5e5, 1e6, 4e6) for b in (8, 16)]
 
 

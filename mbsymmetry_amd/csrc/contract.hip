@@ -917,8 +917,7 @@ hipError_t launch_p0_reduce(const P0Split *sp, int nsp, int shard_first, int sha
 // and their G^{k-q} blocks as columns of one E product.
 // ---------------------------------------------------------------------------
 // Sigma workgroup shape: the contraction shape of T, or, with the INT8 Z
-// product, kZ8Waves waves of kZ8Tiles accumulator tiles (its slices and
-// integer accumulators need the registers of three waves per SIMD).
+// product, kZ8Waves waves of kZ8Tiles accumulator tiles.
 template <typename T, bool Z8>
 struct SigCfg {
   static constexpr int NW = Z8 ? kZ8Waves : Cfg<T>::NW;

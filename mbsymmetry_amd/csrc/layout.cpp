@@ -370,41 +370,48 @@ static std::vector<int> split_ranges(int begin, int end, int parts, const std::v
   return cut;
 }
 
+// Launch group of an iBZ point: every point is one group with a resident V
+// (one launch covers the sweep); with a streamed V the points of one batch
+// (first: [ngroups + 1] first point of each batch).
+static int launch_group(const std::vector<int> &first, int ir) {
+  if (first.size() < 2) return 0;
+  return (int)(std::upper_bound(first.begin(), first.end(), ir) - first.begin()) - 1;
+}
+
 // Split the Sigma items over their q where one workgroup per (item, slab)
 // would leave the GPU short of work: a system with few iBZ points, times or
-// blocks (a shard, a sample), or a few items much costlier than the rest
-// (uneven blocks).  An item's cost is the executed work of its blocks for one
-// slab (8 M N K of the Z, E and final products); items above
-// total / kSplitTarget are cut into that many parts of contiguous blocks
-// with balanced cost (at least kSplitMinBlocks blocks each).  The parts of
-// an item write partial sums that the reduction kernel adds in part order.
-// The batch prefixes (items of each iBZ point) are rebuilt afterwards.
+// blocks (a sample, a streamed batch of one point), or a few items much
+// costlier than the rest (uneven blocks).  Items are costed per slab (8 M N K
+// of the Z, E and final products of their blocks) and grouped by launch (one
+// slab depth of one streamed batch, or of the whole sweep); an item above
+// its group's total / kSplitTarget is cut into that many parts of
+// contiguous blocks with balanced cost (at least kSplitMinBlocks blocks
+// each).  The parts of an item write partial sums that the reduction kernel
+// adds in part order.  The batch prefixes (items of each iBZ point) are
+// rebuilt afterwards.
 static void split_sigma_items(WorkLists &W, int nts, int ns) {
-  double total = 0;
-  std::vector<double> cost[kNumSigDepths];
-  for (int d = 0; d < kNumSigDepths; ++d) {
-    const int TB = kSigDepths[d], per = (nts + TB - 1) / TB * ns;
-    for (const SigItem &it : W.sig_items[d]) {
-      double c = 0;
-      for (int b = it.blk_begin; b < it.blk_end; ++b) {
-        const SigBlk &B = W.sig_blks[b];
-        c += 8.0 * TB * B.q * B.k2 * ((double)B.q * it.Jb + (double)it.Ib * B.k2 + (double)it.Ib * it.Jb);
-      }
-      cost[d].push_back(c);
-      total += c * per;
-    }
-  }
-  const double cap = total / kSplitTarget;
+  const int ngroups = std::max<int>(1, (int)W.sig_group_first.size() - 1);
   W.sig_splits.clear();
   W.sig_part_elems = 0;
   for (int d = 0; d < kNumSigDepths; ++d) {
     const int TB = kSigDepths[d], per = (nts + TB - 1) / TB * ns;
+    auto bcost = [&](const SigItem &it, const SigBlk &B) {
+      return 8.0 * TB * B.q * B.k2 * ((double)B.q * it.Jb + (double)it.Ib * B.k2 + (double)it.Ib * it.Jb);
+    };
+    std::vector<double> cost, total(ngroups, 0.0);
+    for (const SigItem &it : W.sig_items[d]) {
+      double c = 0;
+      for (int b = it.blk_begin; b < it.blk_end; ++b) c += bcost(it, W.sig_blks[b]);
+      cost.push_back(c);
+      total[launch_group(W.sig_group_first, it.ikr)] += c * per;
+    }
     std::vector<SigItem> out;
     out.reserve(W.sig_items[d].size());
     for (size_t i = 0; i < W.sig_items[d].size(); ++i) {
       const SigItem it = W.sig_items[d][i];
+      const double cap = total[launch_group(W.sig_group_first, it.ikr)] / kSplitTarget;
       const int nb = it.blk_end - it.blk_begin;
-      int parts = cap > 0 ? (int)std::ceil(cost[d][i] / cap) : 1;
+      int parts = cap > 0 ? (int)std::ceil(cost[i] / cap) : 1;
       parts = std::max(1, std::min(parts, nb / kSplitMinBlocks));
       if (parts == 1) {
         out.push_back(it);
@@ -428,12 +435,9 @@ static void split_sigma_items(WorkLists &W, int nts, int ns) {
       sp.pad = 0;
       W.sig_splits.push_back(sp);
       W.sig_part_elems += sp.part_stride * parts;
-      std::vector<double> bcost;
-      for (int b = it.blk_begin; b < it.blk_end; ++b) {
-        const SigBlk &B = W.sig_blks[b];
-        bcost.push_back(8.0 * TB * B.q * B.k2 * ((double)B.q * it.Jb + (double)it.Ib * B.k2 + (double)it.Ib * it.Jb));
-      }
-      const std::vector<int> cut = split_ranges(it.blk_begin, it.blk_end, parts, bcost);
+      std::vector<double> bc;
+      for (int b = it.blk_begin; b < it.blk_end; ++b) bc.push_back(bcost(it, W.sig_blks[b]));
+      const std::vector<int> cut = split_ranges(it.blk_begin, it.blk_end, parts, bc);
       for (int p = 0; p < parts; ++p) {
         SigItem q = it;
         q.blk_begin = cut[p];
@@ -452,29 +456,30 @@ static void split_sigma_items(WorkLists &W, int nts, int ns) {
   }
 }
 
-// The split of the P0 items over their k, on the rule of split_sigma_items:
-// an item's cost is the work of its blocks for one slab (E, C and the final
+// The split of the P0 items over their k, on the rule of split_sigma_items
+// (launch groups: the streamed batches of iBZ q, or the whole sweep): an
+// item's cost is the work of its blocks for one slab (E, C and the final
 // product of TB times of nband rows).
 static void split_p0_items(WorkLists &W, int nts, int TB) {
   const int np = (nts / 2 + TB - 1) / TB;
+  const int ngroups = std::max<int>(1, (int)W.p0_group_first.size() - 1);
   auto bc = [&](const P0Item &it, const P0Blk &B) {
     return 8.0 * TB * it.nband * ((double)B.k1 * B.k2 * B.k2 + (double)B.k1 * B.k1 * B.k2 + (double)it.q * B.k1 * B.k2);
   };
-  double total = 0;
-  std::vector<double> cost;
+  std::vector<double> cost, total(ngroups, 0.0);
   for (const P0Item &it : W.p0_items) {
     double c = 0;
     for (int b = it.blk_begin; b < it.blk_end; ++b) c += bc(it, W.p0_blks[b]);
     cost.push_back(c);
-    total += c * np;
+    total[launch_group(W.p0_group_first, it.iqr)] += c * np;
   }
-  const double cap = total / kSplitTarget;
   W.p0_splits.clear();
   W.p0_part_elems = 0;
   std::vector<P0Item> out;
   out.reserve(W.p0_items.size());
   for (size_t i = 0; i < W.p0_items.size(); ++i) {
     const P0Item it = W.p0_items[i];
+    const double cap = total[launch_group(W.p0_group_first, it.iqr)] / kSplitTarget;
     const int nb = it.blk_end - it.blk_begin;
     int parts = cap > 0 ? (int)std::ceil(cost[i] / cap) : 1;
     parts = std::max(1, std::min(parts, nb / kSplitMinBlocks));

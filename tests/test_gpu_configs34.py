@@ -74,14 +74,21 @@ def test_streamed_V_equals_resident_V(c4small):
         a = _ctx(sysm, prec)
         ra = _sweeps(a, sysm, G0)
         a.close()
-        # all iBZ points in one batch (the default size), and one point per batch
+        # all iBZ points in one batch (the default size): the same launch
+        # groups as the resident sweep, so the same bits; one point per
+        # batch: the planner splits items per launch group, so items can be
+        # cut into other parts and the sums differ in rounding only
         for bb in (None, 1):
             b = _ctx(sysm, prec | _lib.FLAG_STREAM_V, bb)
             assert b.info().device_bytes < _lib.layout_footprint(sysm, prec)
             rb = _sweeps(b, sysm, G0)
             b.close()
             for x, y, name in zip(ra, rb, ("P0", "P", "Sigma", "G")):
-                assert np.array_equal(x, y), (prec, bb, name)
+                if bb is None:
+                    assert np.array_equal(x, y), (prec, bb, name)
+                else:
+                    tol = 1e-12 if prec != _lib.PREC_FP32 else 1e-5
+                    assert np.abs(x - y).max() <= tol * np.abs(x).max(), (prec, bb, name)
 
 
 def test_config4_blocks_against_the_oracle(c4small):
