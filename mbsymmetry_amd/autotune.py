@@ -101,7 +101,7 @@ def _timed(ctx, which: str) -> float:
 
 
 def tune(ctx, sysm, precision: int, sample_s: float = 0.4, use_cache: bool = True,
-         log: Optional[Callable[[str], None]] = None, validate_s: float = 5.0) -> dict:
+         log: Optional[Callable[[str], None]] = None, validate_s: float = 12.0) -> dict:
     """Tune the plan of `ctx` (a context of `sysm` in `precision` with V and G
     set) and leave the best plan applied.  A shard's pick is confirmed on
     whole sweeps when one takes at most `validate_s` seconds.  Returns a
