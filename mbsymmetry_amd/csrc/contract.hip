@@ -123,9 +123,9 @@ __device__ __forceinline__ void mma4(Tile3<T> &acc, const cplx<T> &a, const cplx
   acc.p3 = Mfma<T>::op(a.re + a.im, CB ? b.re - b.im : b.re + b.im, acc.p3);
 }
 
-// The k-loops below keep kAhead k-steps of operands in flight: the fragments
-// of step k+kAhead are loaded while the MFMAs of step k run, in a ring of
-// kAhead + 1 register sets (no copies), and every load is unconditional.  Loaders
+// The k-loops below keep AH k-steps of operands in flight: the fragments of
+// step k+AH are loaded while the MFMAs of step k run, in a ring of AH + 1
+// register sets (no copies), and every load is unconditional.  Loaders
 // therefore clamp their indices and return some element of their operand for
 // any row, column or k; garbage in rows or columns outside the product only
 // reaches accumulator rows or columns that are never stored, and the A
@@ -136,6 +136,39 @@ __device__ __forceinline__ void mma4(Tile3<T> &acc, const cplx<T> &a, const cplx
 // MFMAs (it does so under register pressure, which serialises every k-step on
 // the memory latency).
 __device__ __forceinline__ void prefetch_fence() { __builtin_amdgcn_sched_barrier(0); }
+
+// The k-step loop shared by the products below: ceil((ke - kb) / 4) steps,
+// step s reading register set s % (AH + 1); ld(set, k) loads the fragments of
+// the step at k into a set, mm(set, k) runs that step's MFMAs.  The loop exit
+// is tested once per AH + 1 steps, after the last of them: a branch between a
+// prefetch and the step that consumes it lets the compiler sink the load into
+// the consumer's block, right before its use, which takes the prefetch away
+// (measured: every second step then waited on its own loads).  The last
+// (steps mod (AH + 1)) steps run from sets already loaded.
+template <int AH, typename LD, typename MM>
+__device__ __forceinline__ void kloop(int kb, int ke, const LD &ld, const MM &mm) {
+  constexpr int R = AH + 1;
+  const int nst = (ke - kb + 3) >> 2;
+#pragma unroll
+  for (int d = 0; d < AH; ++d) ld(d, kb + 4 * d);
+  int k0 = kb;
+  for (int g = nst / R; g > 0; --g) {
+#pragma unroll
+    for (int h = 0; h < R; ++h) {
+      ld((h + AH) % R, k0 + 4 * AH);
+      prefetch_fence();
+      mm(h, k0);
+      k0 += 4;
+    }
+  }
+  const int rem = nst - (nst / R) * R;
+#pragma unroll
+  for (int h = 0; h < AH; ++h)
+    if (h < rem) {
+      mm(h, k0);
+      k0 += 4;
+    }
+}
 
 template <typename T>
 __device__ __forceinline__ cplx<T> amask(const cplx<T> &a, bool ok) {
@@ -160,41 +193,35 @@ constexpr bool kSigOneLayout = sizeof(T) == 8;
 
 // acc[C0+c] += A (16 x K) * B_c (K x 16) for c < NG with C0 + c < n, over k in
 // [kb, ke): one A fragment per k-step shared by the tiles.  la(i, k), lb(c, k, j)
-// with c the absolute tile index.  n must be uniform across the wave.
+// with c the absolute tile index.  n must be uniform across the wave; a group
+// with fewer than NG tiles left runs as the next smaller group, so that the
+// k-loop body holds no per-tile branch.
 template <int C0, int NG, bool CB, int AH = kAhead, typename TT, int N, typename LA, typename LB>
 __device__ __forceinline__ void mma_rowshare(TT (&acc)[N], int n, int kb, int ke, bool arow, const LA &la,
                                              const LB &lb) {
   using T = typename TT::elem;
   static_assert(C0 + NG <= N, "tile group outside the accumulator array");
-  constexpr int R = AH + 1;
-  if (kb >= ke || C0 >= n) return;
-  const int lane = threadIdx.x & 63, i = lane & 15, kk = lane >> 4;
-  cplx<T> a[R], b[R][NG];
-#pragma unroll
-  for (int d = 0; d < AH; ++d) {
-    a[d] = la(i, kb + 4 * d + kk);
-#pragma unroll
-    for (int c = 0; c < NG; ++c)
-      if (C0 + c < n) b[d][c] = lb(C0 + c, kb + 4 * d + kk, i);
-  }
-  int k0 = kb;
-  while (true) {
-#pragma unroll
-    for (int h = 0; h < R; ++h) {
-      const int hl = (h + AH) % R;
-      a[hl] = la(i, k0 + 4 * AH + kk);
-#pragma unroll
-      for (int c = 0; c < NG; ++c)
-        if (C0 + c < n) b[hl][c] = lb(C0 + c, k0 + 4 * AH + kk, i);
-      prefetch_fence();
-      const cplx<T> am = amask(a[h], arow && k0 + kk < ke);
-#pragma unroll
-      for (int c = 0; c < NG; ++c)
-        if (C0 + c < n) mma4<CB>(acc[C0 + c], am, b[h][c]);
-      k0 += 4;
-      if (k0 >= ke) return;
+  if constexpr (NG > 1) {
+    if (C0 + NG > n) {
+      mma_rowshare<C0, NG - 1, CB, AH>(acc, n, kb, ke, arow, la, lb);
+      return;
     }
   }
+  if (kb >= ke || C0 >= n) return;
+  const int lane = threadIdx.x & 63, i = lane & 15, kk = lane >> 4;
+  cplx<T> a[AH + 1], b[AH + 1][NG];
+  kloop<AH>(
+      kb, ke,
+      [&](int r, int k) {
+        a[r] = la(i, k + kk);
+#pragma unroll
+        for (int c = 0; c < NG; ++c) b[r][c] = lb(C0 + c, k + kk, i);
+      },
+      [&](int r, int k) {
+        const cplx<T> am = amask(a[r], arow && k + kk < ke);
+#pragma unroll
+        for (int c = 0; c < NG; ++c) mma4<CB>(acc[C0 + c], am, b[r][c]);
+      });
 }
 
 // acc[C0+c] += A_c (16 x K) * B (K x 16): one B fragment shared.  la(c, i, k), lb(k, j).
@@ -202,73 +229,56 @@ template <int C0, int NG, bool CB, int AH = kAhead, typename TT, int N, typename
 __device__ __forceinline__ void mma_colshare(TT (&acc)[N], int n, int kb, int ke, const LA &la, const LB &lb) {
   using T = typename TT::elem;
   static_assert(C0 + NG <= N, "tile group outside the accumulator array");
-  constexpr int R = AH + 1;
-  if (kb >= ke || C0 >= n) return;
-  const int lane = threadIdx.x & 63, i = lane & 15, kk = lane >> 4;
-  cplx<T> a[R][NG], b[R];
-#pragma unroll
-  for (int d = 0; d < AH; ++d) {
-    b[d] = lb(kb + 4 * d + kk, i);
-#pragma unroll
-    for (int c = 0; c < NG; ++c)
-      if (C0 + c < n) a[d][c] = la(C0 + c, i, kb + 4 * d + kk);
-  }
-  int k0 = kb;
-  while (true) {
-#pragma unroll
-    for (int h = 0; h < R; ++h) {
-      const int hl = (h + AH) % R;
-      b[hl] = lb(k0 + 4 * AH + kk, i);
-#pragma unroll
-      for (int c = 0; c < NG; ++c)
-        if (C0 + c < n) a[hl][c] = la(C0 + c, i, k0 + 4 * AH + kk);
-      prefetch_fence();
-      const bool kin = k0 + kk < ke;
-#pragma unroll
-      for (int c = 0; c < NG; ++c)
-        if (C0 + c < n) mma4<CB>(acc[C0 + c], amask(a[h][c], kin), b[h]);
-      k0 += 4;
-      if (k0 >= ke) return;
+  if constexpr (NG > 1) {
+    if (C0 + NG > n) {
+      mma_colshare<C0, NG - 1, CB, AH>(acc, n, kb, ke, la, lb);
+      return;
     }
   }
+  if (kb >= ke || C0 >= n) return;
+  const int lane = threadIdx.x & 63, i = lane & 15, kk = lane >> 4;
+  cplx<T> a[AH + 1][NG], b[AH + 1];
+  kloop<AH>(
+      kb, ke,
+      [&](int r, int k) {
+        b[r] = lb(k + kk, i);
+#pragma unroll
+        for (int c = 0; c < NG; ++c) a[r][c] = la(C0 + c, i, k + kk);
+      },
+      [&](int r, int k) {
+        const bool kin = k + kk < ke;
+#pragma unroll
+        for (int c = 0; c < NG; ++c) mma4<CB>(acc[C0 + c], amask(a[r][c], kin), b[r]);
+      });
 }
 
 // acc[C0+c] += A_c * B_c with independent operands.  la(c, i, k), lb(c, k, j).
 template <int C0, int NG, bool CB, typename T, int N, typename LA, typename LB>
 __device__ __forceinline__ void mma_multi(Tile<T> (&acc)[N], int n, int kb, int ke, const LA &la, const LB &lb) {
   static_assert(C0 + NG <= N, "tile group outside the accumulator array");
-  constexpr int R = kAhead + 1;
-  if (kb >= ke || C0 >= n) return;
-  const int lane = threadIdx.x & 63, i = lane & 15, kk = lane >> 4;
-  cplx<T> a[R][NG], b[R][NG];
-#pragma unroll
-  for (int d = 0; d < kAhead; ++d)
-#pragma unroll
-    for (int c = 0; c < NG; ++c)
-      if (C0 + c < n) {
-        a[d][c] = la(C0 + c, i, kb + 4 * d + kk);
-        b[d][c] = lb(C0 + c, kb + 4 * d + kk, i);
-      }
-  int k0 = kb;
-  while (true) {
-#pragma unroll
-    for (int h = 0; h < R; ++h) {
-      const int hl = (h + kAhead) % R;
-#pragma unroll
-      for (int c = 0; c < NG; ++c)
-        if (C0 + c < n) {
-          a[hl][c] = la(C0 + c, i, k0 + 4 * kAhead + kk);
-          b[hl][c] = lb(C0 + c, k0 + 4 * kAhead + kk, i);
-        }
-      prefetch_fence();
-      const bool kin = k0 + kk < ke;
-#pragma unroll
-      for (int c = 0; c < NG; ++c)
-        if (C0 + c < n) mma4<CB>(acc[C0 + c], amask(a[h][c], kin), b[h][c]);
-      k0 += 4;
-      if (k0 >= ke) return;
+  if constexpr (NG > 1) {
+    if (C0 + NG > n) {
+      mma_multi<C0, NG - 1, CB>(acc, n, kb, ke, la, lb);
+      return;
     }
   }
+  if (kb >= ke || C0 >= n) return;
+  const int lane = threadIdx.x & 63, i = lane & 15, kk = lane >> 4;
+  cplx<T> a[kAhead + 1][NG], b[kAhead + 1][NG];
+  kloop<kAhead>(
+      kb, ke,
+      [&](int r, int k) {
+#pragma unroll
+        for (int c = 0; c < NG; ++c) {
+          a[r][c] = la(C0 + c, i, k + kk);
+          b[r][c] = lb(C0 + c, k + kk, i);
+        }
+      },
+      [&](int r, int k) {
+        const bool kin = k + kk < ke;
+#pragma unroll
+        for (int c = 0; c < NG; ++c) mma4<CB>(acc[C0 + c], amask(a[r][c], kin), b[r][c]);
+      });
 }
 
 // All persistent tiles of a wave, in groups of Group<T>::n.
