@@ -183,7 +183,7 @@ constexpr int kAhead = 2;
 template <typename T>
 constexpr int kAheadJob = sizeof(T) == 8 ? 1 : 2;
 // P0's E and C jobs under the dynamic deal (FP32: P0 +1.6%) or the static
-// round-robin one (FP64: the dynamic deal measured 2.5% slower).
+// round-robin one (FP64: the dynamic deal measured 2.5-3.4% slower).
 template <typename T>
 constexpr bool kP0Dyn = sizeof(T) == 4;
 // Sigma's Z and E products with one job layout each (column- and row-share)
