@@ -28,8 +28,12 @@ PEAK = {"fp64": 78.6, "fp32": 157.3, "int8": 5000.0, "hbm_gbps": 8000.0}  # MI35
 # TFLOP/s (INT8 TOPS), HBM3E GB/s
 
 
+_T0 = time.perf_counter()
+
+
 def log(*a):
-    print("[bench]", *a, file=sys.stderr, flush=True)
+    """Progress line on stderr, stamped with the seconds since the bench started."""
+    print("[bench]", f"{time.perf_counter() - _T0:6.1f}s", *a, file=sys.stderr, flush=True)
 
 
 def make_system(name):
