@@ -458,10 +458,10 @@ def main():
         mixed = run_path(sysm, _lib.PREC_MIXED, args.steps, args.warmup, args.mu, G0, tune=False,
                          plan=fixed_plan if fixed_plan is not None else (plain["tuned"] or {}).get("plan"))
         log("mixed (int8 Z)", mixed["sec_per_iter"], "s/iter; phases ms", mixed["phase_ms"][:4])
-        if args.fp32:
-            fp32 = run_path(sysm, _lib.PREC_FP32, args.steps, args.warmup, args.mu, G0, tune=not args.no_tune,
-                            plan=fixed_plan)
-            log("fp32", fp32["sec_per_iter"], "s/iter; phases ms", fp32["phase_ms"][:4])
+    if args.fp32:
+        fp32 = run_path(sysm, _lib.PREC_FP32, args.steps, args.warmup, args.mu, G0, tune=not args.no_tune,
+                        plan=fixed_plan)
+        log("fp32", fp32["sec_per_iter"], "s/iter; phases ms", fp32["phase_ms"][:4])
 
     p0_ms, sig_ms = float(plain["phase_ms"][0]), float(plain["phase_ms"][2])
     dom = "Sigma" if sig_ms >= p0_ms else "P0"
