@@ -22,8 +22,11 @@ because no HDF5 library is available on the device hosts.
 """
 from __future__ import annotations
 
+import argparse
+import json
+import time
 from dataclasses import dataclass, field
-from typing import List, Optional
+from typing import Callable, List, Optional
 
 import numpy as np
 
@@ -41,13 +44,15 @@ class SCResult:
 def self_consistent(ctx, system, G0: np.ndarray, mu: float, max_iter: int = 20, tol: float = 1e-8,
                     damp: float = 1.0, nel: Optional[float] = None, mu_bracket=(-2.0, 2.0),
                     use_graph: bool = True, keep_sigma: bool = False,
-                    stall_ratio: Optional[float] = None) -> SCResult:
+                    stall_ratio: Optional[float] = None,
+                    on_iteration: Optional[Callable[[dict], None]] = None) -> SCResult:
     """Iterate GW on `ctx` (V, rotations, transforms and Fock already set)
     from G0 until max|G_{n+1} - G_n| < tol or max_iter iterations, or, with
     stall_ratio, until an update shrinks by less than that factor (the loop
     has reached its precision's noise floor; converged stays False).  With
     keep_sigma the last iteration's Sigma is copied back, divided by nk as the
-    reference's caller does (cugw_solver_block.cpp:302-304)."""
+    reference's caller does (cugw_solver_block.cpp:302-304).  on_iteration
+    receives each iteration's history entry as it is made."""
     if not 0.0 < damp <= 1.0:
         raise ValueError("damp must be in (0, 1]")
     shape = (system.nts, system.ns, system.flat_ao)
@@ -68,6 +73,8 @@ def self_consistent(ctx, system, G0: np.ndarray, mu: float, max_iter: int = 20, 
             ctx.upload_G(G_new)
         dG = float(np.abs(G_new - G).max())
         hist.append({"iteration": it, "max_abs_dG": dG, "mu": mu, "nel": n_now})
+        if on_iteration is not None:
+            on_iteration(hist[-1])
         G = G_new
         if dG < tol:
             converged = True
@@ -137,3 +144,66 @@ def load_results(path: str) -> dict:
     """Read an archive written by save_results (no pickled objects)."""
     with np.load(path, allow_pickle=False) as z:
         return {k: z[k] for k in z.files}
+
+
+def main(argv=None) -> int:
+    """Self-consistent GW on a synthetic system from the Sigma = 0 Green's
+    function at mu = 0, printing each iteration; with --compare, the same loop
+    in a second precision from the same G0 and the largest difference between
+    the two final G."""
+    from . import _lib, system as S
+    ap = argparse.ArgumentParser(description=main.__doc__)
+    ap.add_argument("--system", default="config2", choices=["config1", "config2"])
+    precisions = {"fp64": _lib.PREC_FP64, "mixed": _lib.PREC_MIXED, "fp32": _lib.PREC_FP32}
+    ap.add_argument("--precision", default="fp64", choices=sorted(precisions))
+    ap.add_argument("--compare", default=None, choices=sorted(precisions))
+    ap.add_argument("--iters", type=int, default=10)
+    ap.add_argument("--tol", type=float, default=1e-8)
+    ap.add_argument("--damp", type=float, default=1.0)
+    ap.add_argument("--out", default=None, help="save the first run's results (.npz)")
+    args = ap.parse_args(argv)
+    sysm = S.config2() if args.system == "config2" else S.config1()
+
+    def context(prec):
+        ctx = _lib.Context(sysm, 0, prec)
+        ctx.generate_V(sysm.v_seed, sysm.v_scale, sysm.decay_table())
+        ctx.set_reps(sysm.orep, sysm.auxrep)
+        ctx.set_transforms(sysm.transforms)
+        ctx.set_fock(sysm.F, sysm.S)
+        return ctx
+
+    summary = {"system": args.system, "runs": {}}
+    G0 = None
+    finals = {}
+    for name in [args.precision] + ([args.compare] if args.compare else []):
+        ctx = context(precisions[name])
+        if G0 is None:
+            ctx.upload_G(sysm.zeros_ts_kij())
+            ctx.dyson(0.0, 0.0)
+            G0 = ctx.download_G((sysm.nts, sysm.ns, sysm.flat_ao))
+        t0 = time.perf_counter()
+        last = [t0]
+
+        def report(h, name=name, last=last):
+            now = time.perf_counter()
+            print(f"[scloop] {name} iteration {h['iteration']}: max|dG| {h['max_abs_dG']:.3e}, "
+                  f"{now - last[0]:.2f} s", flush=True)
+            last[0] = now
+
+        res = self_consistent(ctx, sysm, G0, 0.0, args.iters, args.tol, args.damp, on_iteration=report)
+        finals[name] = res.G
+        summary["runs"][name] = {"iterations": res.iterations, "converged": res.converged,
+                                 "seconds": time.perf_counter() - t0, "max_abs_G": float(np.abs(res.G).max()),
+                                 "max_abs_dG": [h["max_abs_dG"] for h in res.history]}
+        if args.out and name == args.precision:
+            save_results(args.out, res, sysm, {"damping": args.damp, "tol": args.tol})
+        ctx.close()
+    if args.compare:
+        summary["max_abs_G_difference"] = float(np.abs(finals[args.precision] - finals[args.compare]).max())
+    print(json.dumps(summary), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())
+
