@@ -90,7 +90,8 @@ def self_consistent(ctx, system, G0: np.ndarray, mu: float, max_iter: int = 20, 
 def self_consistent_cleanup(ctx_sp, ctx_dp, system, G0: np.ndarray, mu: float, max_iter: int = 20,
                             tol: float = 1e-8, sp_tol: float = 1e-6, damp: float = 1.0,
                             nel: Optional[float] = None, mu_bracket=(-2.0, 2.0), use_graph: bool = True,
-                            keep_sigma: bool = False) -> SCResult:
+                            keep_sigma: bool = False,
+                            on_iteration: Optional[Callable[[dict], None]] = None) -> SCResult:
     """Single precision with a cleanup in double precision (the reference's
     single_prec = 2, SymmetryMBPT/src/common/params_t.cpp:76, switched per
     solver by solver_block_t::single_prec_run, cugw_solver_block.h:70-77):
@@ -100,15 +101,18 @@ def self_consistent_cleanup(ctx_sp, ctx_dp, system, G0: np.ndarray, mu: float, m
     hold the same system.  History entries carry the precision of their
     iteration.  It saves time only where the loop needs many iterations: the
     FP64 phase still takes the iterations from the FP32 floor down to tol."""
+    def tagged(precision, offset):
+        def f(h):
+            h["precision"] = precision
+            h["iteration"] += offset
+            if on_iteration is not None:
+                on_iteration(h)
+        return f
+
     r1 = self_consistent(ctx_sp, system, G0, mu, max_iter, sp_tol, damp, nel, mu_bracket, use_graph,
-                         stall_ratio=0.5)
-    for h in r1.history:
-        h["precision"] = "fp32"
+                         stall_ratio=0.5, on_iteration=tagged("fp32", 0))
     r2 = self_consistent(ctx_dp, system, r1.G, r1.mu, max_iter - r1.iterations, tol, damp, nel, mu_bracket,
-                         use_graph, keep_sigma)
-    for h in r2.history:
-        h["precision"] = "fp64"
-        h["iteration"] += r1.iterations
+                         use_graph, keep_sigma, on_iteration=tagged("fp64", r1.iterations))
     return SCResult(G=r2.G, mu=r2.mu, iterations=r1.iterations + r2.iterations, converged=r2.converged,
                     history=r1.history + r2.history, Sigma=r2.Sigma)
 
@@ -161,6 +165,9 @@ def main(argv=None) -> int:
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--damp", type=float, default=1.0)
     ap.add_argument("--out", default=None, help="save the first run's results (.npz)")
+    ap.add_argument("--cleanup", action="store_true",
+                    help="run --precision (fp32 or mixed) to its noise floor, then FP64 to --tol "
+                         "(self_consistent_cleanup)")
     args = ap.parse_args(argv)
     sysm = S.config2() if args.system == "config2" else S.config1()
 
@@ -186,15 +193,22 @@ def main(argv=None) -> int:
 
         def report(h, name=name, last=last):
             now = time.perf_counter()
-            print(f"[scloop] {name} iteration {h['iteration']}: max|dG| {h['max_abs_dG']:.3e}, "
-                  f"{now - last[0]:.2f} s", flush=True)
+            print(f"[scloop] {name} iteration {h['iteration']} ({h.get('precision', name)}): "
+                  f"max|dG| {h['max_abs_dG']:.3e}, {now - last[0]:.2f} s", flush=True)
             last[0] = now
 
-        res = self_consistent(ctx, sysm, G0, 0.0, args.iters, args.tol, args.damp, on_iteration=report)
+        if args.cleanup and name == args.precision:
+            ctx_dp = context(_lib.PREC_FP64)
+            res = self_consistent_cleanup(ctx, ctx_dp, sysm, G0, 0.0, args.iters, args.tol, damp=args.damp,
+                                          on_iteration=report)
+            ctx_dp.close()
+        else:
+            res = self_consistent(ctx, sysm, G0, 0.0, args.iters, args.tol, args.damp, on_iteration=report)
         finals[name] = res.G
         summary["runs"][name] = {"iterations": res.iterations, "converged": res.converged,
                                  "seconds": time.perf_counter() - t0, "max_abs_G": float(np.abs(res.G).max()),
-                                 "max_abs_dG": [h["max_abs_dG"] for h in res.history]}
+                                 "max_abs_dG": [h["max_abs_dG"] for h in res.history],
+                                 "precisions": [h.get("precision", name) for h in res.history]}
         if args.out and name == args.precision:
             save_results(args.out, res, sysm, {"damping": args.damp, "tol": args.tol})
         ctx.close()
