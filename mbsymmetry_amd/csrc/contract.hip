@@ -827,9 +827,10 @@ __global__ __launch_bounds__(Cfg<T>::NT) void p0_mfma_kernel(const P0Item *__res
             k_part(X, S, kp, kb, ke);
             const int rc = min(threadIdx.x & 15, nband - 1);   // rows >= nband are never stored
             if (stack) {
-              // one V fragment per k-step for the tbn time tiles of column tile tg
+              // one V fragment per k-step for the tbn time tiles of column tile tg, one step
+              // ahead (with two, 4 registers spilled in FP64 and 8-11 in FP32; FP32 P0 +2%)
               if (nmine > 0)
-                mma_colshare<0, TB, true>(
+                mma_colshare<0, TB, true, 1>(
                     acc, tbn, kb, ke, [&](int c, int i, int x) { return bufC[(c * nband + rc) * lde + min(x, X - 1)]; },
                     [&](int x, int j) {
                       x = min(x, X - 1);
