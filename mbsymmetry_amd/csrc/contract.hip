@@ -178,10 +178,10 @@ __device__ __forceinline__ cplx<T> amask(const cplx<T> &a, bool ok) {
 // Operand k-steps kept in flight ahead of the MFMAs (register sets used in a
 // ring of kAhead + 1).
 constexpr int kAhead = 2;
-// The same for the job products (3M tiles), measured per kernel: one step in
-// FP64 (no spills; Sigma +2.3%) and in P0 (FP32 +3%), two in the FP32 Sigma.
+// The same for the job products (3M tiles): one step in both precisions
+// (measured per kernel: FP64 Sigma +1-2% and FP32 Sigma +3.8% against two).
 template <typename T>
-constexpr int kAheadJob = sizeof(T) == 8 ? 1 : 2;
+constexpr int kAheadJob = 1;
 // P0's E and C jobs under the dynamic deal (FP32: P0 +1.6%) or the static
 // round-robin one (FP64: the dynamic deal measured 2.5-3.4% slower).
 template <typename T>
